@@ -1,0 +1,315 @@
+"""ctypes binding of the gfx950 C ABI (include/ffthom.h) and device arrays.
+
+The HIP library is the only compute path: if it is missing or no gfx950 GPU is
+present, every entry point raises ``FhError``. There is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'lib', 'libffthom.so')
+
+FH_OK, FH_ERR_ARG, FH_ERR_HIP, FH_ERR_NODEV, FH_ERR_UNSUPPORTED, FH_ERR_STATE = range(6)
+FH_SCALAR, FH_ELASTICITY = 0, 1
+FH_PRIMAL, FH_DUAL = 0, 1
+FH_F64, FH_F32 = 0, 1
+FH_SOLVE_FP64, FH_SOLVE_MIXED = 0, 1
+FH_COEF_ISO, FH_COEF_LAME, FH_COEF_FULL = 1, 2, 3
+FH_CG_OK, FH_CG_BREAKDOWN = 0, 1
+
+_c_int_p = ctypes.POINTER(ctypes.c_int)
+_c_double_p = ctypes.POINTER(ctypes.c_double)
+_vp = ctypes.c_void_p
+
+# name -> (argtypes); every function returns int except fh_last_error
+SIGNATURES = {
+    'fh_init': [ctypes.c_char_p, ctypes.c_int],
+    'fh_malloc': [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t],
+    'fh_free': [_vp],
+    'fh_memcpy_htod': [_vp, _vp, ctypes.c_size_t],
+    'fh_memcpy_dtoh': [_vp, _vp, ctypes.c_size_t],
+    'fh_memcpy_dtod': [_vp, _vp, ctypes.c_size_t],
+    'fh_memset_zero': [_vp, ctypes.c_size_t],
+    'fh_plan_create': [ctypes.c_int, _c_int_p, _c_double_p, ctypes.c_int, ctypes.c_int,
+                       ctypes.POINTER(ctypes.c_void_p)],
+    'fh_plan_destroy': [_vp],
+    'fh_plan_ncomp': [_vp],
+    'fh_plan_set_primaldual': [_vp, ctypes.c_int],
+    'fh_set_coeff': [_vp, ctypes.c_int, _c_double_p],
+    'fh_matvec': [_vp, ctypes.c_int, _vp, _vp],
+    'fh_project': [_vp, ctypes.c_int, _vp, _vp],
+    'fh_rfftn': [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp],
+    'fh_irfftn': [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp],
+    'fh_apply_projection_hat': [_vp, ctypes.c_int, _vp, _vp],
+    'fh_cg_solve': [_vp, ctypes.c_int, _vp, _vp, ctypes.c_double, ctypes.c_int, ctypes.c_double,
+                    _c_int_p, _c_double_p, _c_double_p, ctypes.c_int, _c_int_p],
+    'fh_energy_matrix': [_vp, ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, _c_double_p],
+    'fh_axpby': [ctypes.c_int, ctypes.c_size_t, ctypes.c_double, _vp, ctypes.c_double, _vp],
+    'fh_dot': [ctypes.c_int, ctypes.c_size_t, _vp, _vp, _c_double_p],
+    'fh_add_constant': [ctypes.c_int, ctypes.c_size_t, ctypes.c_int, _c_double_p, _vp],
+    'fh_last_timing': [_vp, _c_double_p, _c_double_p],
+}
+
+
+class FhError(RuntimeError):
+    """Failure reported by the HIP library (or the library is missing)."""
+
+    def __init__(self, msg, code=None):
+        super().__init__(msg)
+        self.code = code
+
+
+_LIB = None
+
+
+def load_library():
+    """Load the built library and declare every exported symbol. Raises if absent."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise FhError("HIP library %s is not built; run __graft_entry__.build()" % LIB_PATH)
+    lib = ctypes.CDLL(LIB_PATH)
+    lib.fh_last_error.restype = ctypes.c_char_p
+    lib.fh_last_error.argtypes = []
+    for name, args in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = ctypes.c_int
+    _LIB = lib
+    return lib
+
+
+def call(name, *args):
+    """Call an fh_* entry point; raise FhError with the library's message on failure."""
+    lib = load_library()
+    rc = getattr(lib, name)(*args)
+    if rc != FH_OK:
+        msg = lib.fh_last_error().decode(errors='replace')
+        raise FhError('%s failed (code %d): %s' % (name, rc, msg), rc)
+    return rc
+
+
+_DEVICE_NAME = None
+
+
+def init():
+    """Initialise the GPU (device 0, must be gfx950). Returns the device name."""
+    global _DEVICE_NAME
+    if _DEVICE_NAME is None:
+        buf = ctypes.create_string_buffer(256)
+        call('fh_init', buf, 256)
+        _DEVICE_NAME = buf.value.decode()
+    return _DEVICE_NAME
+
+
+_DTYPE_CODE = {np.dtype(np.float64): FH_F64, np.dtype(np.float32): FH_F32}
+
+
+def dtype_code(dtype):
+    try:
+        return _DTYPE_CODE[np.dtype(dtype)]
+    except KeyError:
+        raise FhError('unsupported dtype %s (float64 or float32)' % dtype)
+
+
+class DeviceArray:
+    """A device allocation with a NumPy-like shape and dtype (float64, float32,
+    complex128, complex64). Freed when garbage collected."""
+
+    def __init__(self, shape, dtype=np.float64):
+        init()
+        self.shape = tuple(int(s) for s in shape)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape, dtype=np.int64)) * self.dtype.itemsize
+        p = ctypes.c_void_p()
+        call('fh_malloc', ctypes.byref(p), ctypes.c_size_t(max(self.nbytes, 1)))
+        self._ptr = p
+
+    @property
+    def ptr(self):
+        return self._ptr
+
+    @property
+    def size(self):
+        return int(np.prod(self.shape, dtype=np.int64))
+
+    @property
+    def ndim(self):
+        return len(self.shape)
+
+    @classmethod
+    def from_host(cls, arr, dtype=None):
+        arr = np.ascontiguousarray(arr if dtype is None else np.asarray(arr, dtype=dtype))
+        out = cls(arr.shape, arr.dtype)
+        call('fh_memcpy_htod', out.ptr, arr.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(arr.nbytes))
+        return out
+
+    @classmethod
+    def zeros(cls, shape, dtype=np.float64):
+        out = cls(shape, dtype)
+        call('fh_memset_zero', out.ptr, ctypes.c_size_t(out.nbytes))
+        return out
+
+    def to_host(self):
+        out = np.empty(self.shape, dtype=self.dtype)
+        call('fh_memcpy_dtoh', out.ctypes.data_as(ctypes.c_void_p), self.ptr, ctypes.c_size_t(self.nbytes))
+        return out
+
+    def copy(self):
+        out = DeviceArray(self.shape, self.dtype)
+        call('fh_memcpy_dtod', out.ptr, self.ptr, ctypes.c_size_t(self.nbytes))
+        return out
+
+    def copy_from(self, other):
+        assert other.nbytes == self.nbytes
+        call('fh_memcpy_dtod', self.ptr, other.ptr, ctypes.c_size_t(self.nbytes))
+        return self
+
+    def reshape(self, shape):
+        """A view with another shape (same memory, keeps the owner alive)."""
+        v = object.__new__(DeviceArray)
+        v.shape = tuple(int(s) for s in shape)
+        assert int(np.prod(v.shape)) == self.size
+        v.dtype = self.dtype
+        v.nbytes = self.nbytes
+        v._ptr = self._ptr
+        v._owner = self
+        return v
+
+    def __del__(self):
+        if getattr(self, '_owner', None) is not None:
+            return
+        p = getattr(self, '_ptr', None)
+        if p is not None and p.value and _LIB is not None:
+            try:
+                _LIB.fh_free(p)
+            except Exception:
+                pass
+            self._ptr = None
+
+    def __repr__(self):
+        return 'DeviceArray(shape=%s, dtype=%s)' % (self.shape, self.dtype)
+
+
+def _ptr(x):
+    return x.ptr if isinstance(x, DeviceArray) else ctypes.c_void_p(int(x))
+
+
+def axpby(a, x, b, y):
+    """y <- a x + b y on device vectors of equal size and dtype."""
+    call('fh_axpby', dtype_code(x.dtype), ctypes.c_size_t(x.size), ctypes.c_double(a), x.ptr,
+         ctypes.c_double(b), y.ptr)
+    return y
+
+
+def dot(x, y):
+    """sum(x*y) over two device vectors, accumulated in float64."""
+    out = ctypes.c_double()
+    call('fh_dot', dtype_code(x.dtype), ctypes.c_size_t(x.size), x.ptr, y.ptr, ctypes.byref(out))
+    return out.value
+
+
+def add_constant(y, E):
+    """y[c] += E[c] on a component-major device field y of shape (ncomp,)+N."""
+    E = np.ascontiguousarray(E, dtype=np.float64)
+    nvox = y.size // E.size
+    call('fh_add_constant', dtype_code(y.dtype), ctypes.c_size_t(nvox), ctypes.c_int(E.size),
+         E.ctypes.data_as(_c_double_p), y.ptr)
+    return y
+
+
+class Plan:
+    """Owner of an fh_plan: grid, cell, physics, projection and coefficient."""
+
+    def __init__(self, N, Y, physics='scalar', primaldual='primal'):
+        init()
+        N = [int(n) for n in np.atleast_1d(N)]
+        Y = [float(y) for y in np.atleast_1d(Y)]
+        self.N = tuple(N)
+        self.Y = tuple(Y)
+        self.dim = len(N)
+        self.physics = physics
+        self.primaldual = primaldual
+        Na = (ctypes.c_int * self.dim)(*N)
+        Ya = (ctypes.c_double * self.dim)(*Y)
+        h = ctypes.c_void_p()
+        call('fh_plan_create', ctypes.c_int(self.dim), Na, Ya,
+             ctypes.c_int(FH_SCALAR if physics == 'scalar' else FH_ELASTICITY),
+             ctypes.c_int(FH_PRIMAL if primaldual == 'primal' else FH_DUAL), ctypes.byref(h))
+        self._h = h
+        self.ncomp = load_library().fh_plan_ncomp(h)
+        self.nvox = int(np.prod(N))
+        self.Nhalf = tuple(N[:-1]) + (N[-1] // 2 + 1,)
+        self.coef_kind = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_primaldual(self, primaldual):
+        call('fh_plan_set_primaldual', self._h, ctypes.c_int(FH_PRIMAL if primaldual == 'primal' else FH_DUAL))
+        self.primaldual = primaldual
+
+    def set_coeff(self, kind, vals):
+        vals = np.ascontiguousarray(vals, dtype=np.float64)
+        code = {'iso': FH_COEF_ISO, 'lame': FH_COEF_LAME, 'full': FH_COEF_FULL}[kind]
+        call('fh_set_coeff', self._h, ctypes.c_int(code), vals.ctypes.data_as(_c_double_p))
+        self.coef_kind = kind
+
+    def matvec(self, x, y):
+        call('fh_matvec', self._h, dtype_code(x.dtype), x.ptr, y.ptr)
+        return y
+
+    def project(self, x, y):
+        call('fh_project', self._h, dtype_code(x.dtype), x.ptr, y.ptr)
+        return y
+
+    def rfftn(self, x, xhat, ncomp):
+        call('fh_rfftn', self._h, dtype_code(x.dtype), ctypes.c_int(ncomp), x.ptr, xhat.ptr)
+        return xhat
+
+    def irfftn(self, xhat, x, ncomp):
+        call('fh_irfftn', self._h, dtype_code(x.dtype), ctypes.c_int(ncomp), xhat.ptr, x.ptr)
+        return x
+
+    def apply_projection_hat(self, xin, xout):
+        real = np.float64 if xin.dtype == np.complex128 else np.float32
+        call('fh_apply_projection_hat', self._h, dtype_code(real), xin.ptr, xout.ptr)
+        return xout
+
+    def cg_solve(self, b, x, tol, maxiter, mode='fp64', delta=0.01, hist_len=0):
+        kit = ctypes.c_int()
+        nr = ctypes.c_double()
+        st = ctypes.c_int()
+        hist = np.zeros(max(hist_len, 1))
+        call('fh_cg_solve', self._h, ctypes.c_int(FH_SOLVE_FP64 if mode == 'fp64' else FH_SOLVE_MIXED),
+             b.ptr, x.ptr, ctypes.c_double(tol), ctypes.c_int(int(maxiter)), ctypes.c_double(delta),
+             ctypes.byref(kit), ctypes.byref(nr), hist.ctypes.data_as(_c_double_p), ctypes.c_int(hist_len),
+             ctypes.byref(st))
+        k = kit.value
+        return {'kit': k, 'norm_res': nr.value, 'status': st.value,
+                'res_log': hist[:min(hist_len, k + 1)].copy() if hist_len > 0 else None}
+
+    def energy_matrix(self, fields):
+        nf = len(fields)
+        arr = (ctypes.c_void_p * nf)(*[f.ptr.value for f in fields])
+        AH = np.zeros((nf, nf))
+        call('fh_energy_matrix', self._h, arr, ctypes.c_int(nf), AH.ctypes.data_as(_c_double_p))
+        return AH
+
+    def last_timing(self):
+        mv = ctypes.c_double()
+        sv = ctypes.c_double()
+        call('fh_last_timing', self._h, ctypes.byref(mv), ctypes.byref(sv))
+        return mv.value, sv.value
+
+    def __del__(self):
+        h = getattr(self, '_h', None)
+        if h is not None and h.value and _LIB is not None:
+            try:
+                _LIB.fh_plan_destroy(h)
+            except Exception:
+                pass
+            self._h = None

@@ -1,0 +1,943 @@
+// C ABI of the gfx950 Fourier-Galerkin operator: plans, passes, CG driver, energies.
+// Declarations and reference citations: include/ffthom.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/ffthom.h"
+#include "fh_kernels.hpp"
+
+using fh::AxisPlan;
+using fh::CGState;
+using fh::cpx;
+using fh::GParams;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(call)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(FH_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));      \
+  } while (0)
+
+#define RETCHK(call)        \
+  do {                      \
+    int r_ = (call);        \
+    if (r_ != FH_OK) return r_; \
+  } while (0)
+
+constexpr size_t kLdsTarget = 48 * 1024;
+constexpr size_t kLdsMax = 160 * 1024;
+
+std::mutex g_init_mu;
+int g_init_state = -1;  // -1 unknown, FH_OK, or an error code
+std::string g_init_msg;
+std::string g_dev_name;
+
+int ensure_device() {
+  std::lock_guard<std::mutex> lk(g_init_mu);
+  if (g_init_state == FH_OK) return FH_OK;
+  if (g_init_state > 0) return fail(g_init_state, g_init_msg);
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n <= 0) {
+    g_init_state = FH_ERR_NODEV;
+    g_init_msg = std::string("no HIP device: ") + (e != hipSuccess ? hipGetErrorString(e) : "count is 0");
+    return fail(g_init_state, g_init_msg);
+  }
+  hipDeviceProp_t prop;
+  e = hipGetDeviceProperties(&prop, 0);
+  if (e != hipSuccess) {
+    g_init_state = FH_ERR_HIP;
+    g_init_msg = std::string("hipGetDeviceProperties: ") + hipGetErrorString(e);
+    return fail(g_init_state, g_init_msg);
+  }
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+    g_init_state = FH_ERR_UNSUPPORTED;
+    g_init_msg = std::string("device arch ") + prop.gcnArchName + " is not gfx950";
+    return fail(g_init_state, g_init_msg);
+  }
+  e = hipSetDevice(0);
+  if (e != hipSuccess) {
+    g_init_state = FH_ERR_HIP;
+    g_init_msg = std::string("hipSetDevice: ") + hipGetErrorString(e);
+    return fail(g_init_state, g_init_msg);
+  }
+  g_dev_name = prop.name;
+  g_init_state = FH_OK;
+  return FH_OK;
+}
+
+// Radix schedule: 4s first, then 2, 3, 5, 7, 11, 13.
+bool factorize(int L, AxisPlan* ap) {
+  std::memset(ap, 0, sizeof(*ap));
+  ap->L = L;
+  int rem = L, st = 0, pre = 1;
+  const int rads[] = {4, 2, 3, 5, 7, 11, 13};
+  for (int R : rads) {
+    while (rem % R == 0) {
+      if (st >= fh::kMaxStages) return false;
+      ap->rad[st] = R;
+      ap->pre[st] = pre;
+      pre *= R;
+      rem /= R;
+      ++st;
+    }
+  }
+  ap->nst = st;
+  return rem == 1;
+}
+
+int fft_threads(const AxisPlan& ap, int npen) {
+  long long need = 64;
+  for (int s = 0; s < ap.nst; ++s) {
+    const int R = ap.rad[s];
+    const long long bf = (long long)npen * (ap.L / R);
+    const long long mb = fh::max_butterflies(R);
+    need = std::max(need, (bf + mb - 1) / mb);
+  }
+  need = ((need + 63) / 64) * 64;
+  return (int)need;
+}
+
+struct RowCfg {
+  int rows_pb = 2, nthr = 64, nblocks = 0;
+  size_t lds = 0;
+};
+
+struct ColCfg {
+  int P = 1, nthr = 64, ntiles = 0;
+  size_t lds = 0;
+};
+
+}  // namespace
+
+struct fh_plan {
+  int dim = 0, physics = 0, primaldual = 0, ncomp = 0;
+  int Nu[3] = {1, 1, 1};
+  double Y[3] = {1, 1, 1};
+  int n0 = 1, n1 = 1, n2 = 1, n2h = 1;
+  long long n = 0, nh = 0;
+  AxisPlan ax[3];
+  void* tw64[3] = {nullptr, nullptr, nullptr};
+  void* tw32[3] = {nullptr, nullptr, nullptr};
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int coef_kind = 0;
+  double* coef64 = nullptr;
+  float* coef32 = nullptr;
+  void* spec = nullptr;          // ncomp*nh complex doubles (fp32 passes reuse it)
+  double* partial = nullptr;     // per-workgroup partial sums
+  size_t partial_cap = 0;
+  CGState* st = nullptr;
+  double* hist = nullptr;
+  int hist_cap = 0;
+  void* work = nullptr;          // CG vectors
+  size_t work_bytes = 0;
+  double last_matvec_ms = 0, last_solve_ms = 0;
+  int last_updates = 0;
+};
+
+namespace {
+
+GParams make_gparams(const fh_plan* pl, double scale, int ncomp_override = 0) {
+  GParams g;
+  g.dim = pl->dim;
+  g.physics = pl->physics;
+  g.dual = pl->primaldual;
+  g.ncomp = ncomp_override ? ncomp_override : pl->ncomp;
+  g.n0 = pl->n0;
+  g.n1 = pl->n1;
+  g.n2 = pl->n2;
+  if (pl->dim == 3) {
+    g.iY[0] = 1.0 / pl->Y[0];
+    g.iY[1] = 1.0 / pl->Y[1];
+    g.iY[2] = 1.0 / pl->Y[2];
+  } else {
+    g.iY[0] = 0.0;
+    g.iY[1] = 1.0 / pl->Y[0];
+    g.iY[2] = 1.0 / pl->Y[1];
+  }
+  g.scale = scale;
+  return g;
+}
+
+template <typename T>
+const cpx<T>* twiddles(const fh_plan* pl, int axis) {
+  return reinterpret_cast<const cpx<T>*>(sizeof(T) == 8 ? pl->tw64[axis] : pl->tw32[axis]);
+}
+
+int row_config(const AxisPlan& ap, int nc, size_t csz, long long nrows, RowCfg* c) {
+  const int L = ap.L;
+  int best = 2;
+  for (int rp = 2; rp <= 32; rp += 2) {
+    const size_t lds = (size_t)(rp / 2) * nc * L * csz;
+    const int thr = fft_threads(ap, (rp / 2) * nc);
+    if (rp > 2 && (lds > kLdsTarget || thr > 1024)) break;
+    best = rp;
+    if (rp >= nrows) break;
+  }
+  c->rows_pb = best;
+  c->lds = (size_t)(best / 2) * nc * L * csz;
+  c->nthr = std::max(fft_threads(ap, (best / 2) * nc), 128);
+  c->nthr = std::min(c->nthr, 1024);
+  if (fft_threads(ap, (best / 2) * nc) > 1024 || c->lds > kLdsMax)
+    return fail(FH_ERR_UNSUPPORTED, "grid row of length " + std::to_string(L) + " does not fit one workgroup");
+  c->nblocks = (int)((nrows + best - 1) / best);
+  return FH_OK;
+}
+
+int col_config(const AxisPlan& ap, int unit, size_t csz, long long inner, ColCfg* c) {
+  const int L = ap.L;
+  long long P = (long long)(kLdsTarget / ((size_t)L * csz)) / unit * unit;
+  if (P < unit) P = unit;
+  const long long cap = (inner + unit - 1) / unit * unit;
+  if (P > cap) P = cap;
+  while (P > unit && fft_threads(ap, (int)P) > 1024) P -= unit;
+  c->P = (int)P;
+  c->lds = (size_t)L * P * csz;
+  const int need = fft_threads(ap, (int)P);
+  if (need > 1024 || c->lds > kLdsMax)
+    return fail(FH_ERR_UNSUPPORTED, "grid axis of length " + std::to_string(L) + " does not fit one workgroup");
+  c->nthr = std::max(need, 128);
+  c->ntiles = (int)((inner + P - 1) / P);
+  return FH_OK;
+}
+
+template <typename T>
+int launch_rows_fwd(fh_plan* pl, int nc, const T* in, const T* coef, int kind, cpx<T>* out,
+                    const int* halt) {
+  RowCfg c;
+  const long long nrows = (long long)pl->n0 * pl->n1;
+  RETCHK(row_config(pl->ax[2], nc, sizeof(cpx<T>), nrows, &c));
+  dim3 grid(c.nblocks), block(c.nthr);
+  const cpx<T>* tw = twiddles<T>(pl, 2);
+  switch (kind) {
+    case fh::COEF_NONE:
+      hipLaunchKernelGGL((fh::k_rows_fwd<T, fh::COEF_NONE>), grid, block, c.lds, pl->stream, in, coef, out,
+                         nrows, nc, pl->n, c.rows_pb, pl->ax[2], tw, halt);
+      break;
+    case fh::COEF_ISO:
+      hipLaunchKernelGGL((fh::k_rows_fwd<T, fh::COEF_ISO>), grid, block, c.lds, pl->stream, in, coef, out,
+                         nrows, nc, pl->n, c.rows_pb, pl->ax[2], tw, halt);
+      break;
+    case fh::COEF_LAME:
+      hipLaunchKernelGGL((fh::k_rows_fwd<T, fh::COEF_LAME>), grid, block, c.lds, pl->stream, in, coef, out,
+                         nrows, nc, pl->n, c.rows_pb, pl->ax[2], tw, halt);
+      break;
+    case fh::COEF_FULL:
+      hipLaunchKernelGGL((fh::k_rows_fwd<T, fh::COEF_FULL>), grid, block, c.lds, pl->stream, in, coef, out,
+                         nrows, nc, pl->n, c.rows_pb, pl->ax[2], tw, halt);
+      break;
+    default:
+      return fail(FH_ERR_ARG, "bad coefficient kind");
+  }
+  HIPCHK(hipGetLastError());
+  return FH_OK;
+}
+
+// Returns the number of workgroups through *nblocks (partial sums written when w != null).
+template <typename T>
+int launch_rows_inv(fh_plan* pl, int nc, const cpx<T>* in, T* out, T scale, const T* w,
+                    double* partial, const int* halt, int* nblocks) {
+  RowCfg c;
+  const long long nrows = (long long)pl->n0 * pl->n1;
+  RETCHK(row_config(pl->ax[2], nc, sizeof(cpx<T>), nrows, &c));
+  if (w && (size_t)c.nblocks > pl->partial_cap) return fail(FH_ERR_STATE, "partial buffer too small");
+  dim3 grid(c.nblocks), block(c.nthr);
+  const cpx<T>* tw = twiddles<T>(pl, 2);
+  if (w)
+    hipLaunchKernelGGL((fh::k_rows_inv<T, 1>), grid, block, c.lds, pl->stream, in, out, nrows, nc, pl->n,
+                       c.rows_pb, pl->ax[2], tw, scale, w, partial, halt);
+  else
+    hipLaunchKernelGGL((fh::k_rows_inv<T, 0>), grid, block, c.lds, pl->stream, in, out, nrows, nc, pl->n,
+                       c.rows_pb, pl->ax[2], tw, scale, w, partial, halt);
+  HIPCHK(hipGetLastError());
+  if (nblocks) *nblocks = c.nblocks;
+  return FH_OK;
+}
+
+template <typename T>
+int launch_cols(fh_plan* pl, int mode, int axis, int nc, cpx<T>* data, const GParams& g,
+                const int* halt) {
+  const AxisPlan& ap = pl->ax[axis];
+  long long outer, ostride, inner;
+  if (axis == 1) {
+    outer = pl->n0;
+    inner = (long long)pl->n2h * nc;
+    ostride = (long long)pl->n1 * inner;
+  } else {
+    outer = 1;
+    inner = (long long)pl->n1 * pl->n2h * nc;
+    ostride = 0;
+  }
+  ColCfg c;
+  RETCHK(col_config(ap, mode == 2 ? nc : 4, sizeof(cpx<T>), inner, &c));
+  const long long nb = outer * c.ntiles;
+  if (nb > 0x7fffffffLL) return fail(FH_ERR_UNSUPPORTED, "grid too large");
+  dim3 grid((unsigned)nb), block(c.nthr);
+  const cpx<T>* tw = twiddles<T>(pl, axis);
+  if (mode == 0)
+    hipLaunchKernelGGL((fh::k_cols<T, 0>), grid, block, c.lds, pl->stream, data, ostride, inner, c.P, c.ntiles,
+                       ap, tw, g, pl->n2h, halt);
+  else if (mode == 1)
+    hipLaunchKernelGGL((fh::k_cols<T, 1>), grid, block, c.lds, pl->stream, data, ostride, inner, c.P, c.ntiles,
+                       ap, tw, g, pl->n2h, halt);
+  else
+    hipLaunchKernelGGL((fh::k_cols<T, 2>), grid, block, c.lds, pl->stream, data, ostride, inner, c.P, c.ntiles,
+                       ap, tw, g, pl->n2h, halt);
+  HIPCHK(hipGetLastError());
+  return FH_OK;
+}
+
+// y = F^-1 G F (A x); kind COEF_NONE gives the bare projection. With w != null the
+// last pass also writes per-workgroup partial sums of y.w into pl->partial.
+template <typename T>
+int run_operator(fh_plan* pl, const T* x, T* y, int kind, const T* coef, const T* w, const int* halt,
+                 int* nblocks_out) {
+  const int nc = pl->ncomp;
+  cpx<T>* spec = reinterpret_cast<cpx<T>*>(pl->spec);
+  const GParams g = make_gparams(pl, 1.0 / (double)pl->n);
+  RETCHK(launch_rows_fwd<T>(pl, nc, x, coef, kind, spec, halt));
+  if (pl->n0 > 1) {
+    RETCHK(launch_cols<T>(pl, 0, 1, nc, spec, g, halt));
+    RETCHK(launch_cols<T>(pl, 2, 0, nc, spec, g, halt));
+    RETCHK(launch_cols<T>(pl, 1, 1, nc, spec, g, halt));
+  } else {
+    RETCHK(launch_cols<T>(pl, 2, 1, nc, spec, g, halt));
+  }
+  RETCHK(launch_rows_inv<T>(pl, nc, spec, y, T(1), w, pl->partial, halt, nblocks_out));
+  return FH_OK;
+}
+
+template <typename T>
+const T* plan_coef(const fh_plan* pl) {
+  if constexpr (sizeof(T) == 8)
+    return reinterpret_cast<const T*>(pl->coef64);
+  else
+    return reinterpret_cast<const T*>(pl->coef32);
+}
+
+int grid_blocks(long long n, int per = 256, int cap = 2048) {
+  long long b = (n + per - 1) / per;
+  if (b < 1) b = 1;
+  if (b > cap) b = cap;
+  return (int)b;
+}
+
+int check_plan(const fh_plan* pl) {
+  if (!pl) return fail(FH_ERR_ARG, "null plan");
+  return FH_OK;
+}
+
+int check_dtype(int dtype) {
+  if (dtype != FH_F64 && dtype != FH_F32) return fail(FH_ERR_ARG, "dtype must be FH_F64 or FH_F32");
+  return FH_OK;
+}
+
+std::mutex g_scratch_mu;
+double* g_scratch = nullptr;
+constexpr int kScratchParts = 2048;
+
+}  // namespace
+
+namespace {
+
+static int matvec_impl(fh_plan* pl, int dtype, const void* x, void* y, bool with_coef) {
+  RETCHK(check_plan(pl));
+  RETCHK(check_dtype(dtype));
+  if (!x || !y) return fail(FH_ERR_ARG, "null field pointer");
+  if (with_coef && !pl->coef_kind) return fail(FH_ERR_STATE, "fh_set_coeff has not been called");
+  HIPCHK(hipEventRecord(pl->ev0, pl->stream));
+  if (dtype == FH_F64) {
+    RETCHK(run_operator<double>(pl, (const double*)x, (double*)y, with_coef ? pl->coef_kind : 0,
+                                with_coef ? pl->coef64 : nullptr, nullptr, nullptr, nullptr));
+  } else {
+    RETCHK(run_operator<float>(pl, (const float*)x, (float*)y, with_coef ? pl->coef_kind : 0,
+                               with_coef ? pl->coef32 : nullptr, nullptr, nullptr, nullptr));
+  }
+  HIPCHK(hipEventRecord(pl->ev1, pl->stream));
+  HIPCHK(hipStreamSynchronize(pl->stream));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, pl->ev0, pl->ev1));
+  pl->last_matvec_ms = ms;
+  return FH_OK;
+}
+
+template <typename T>
+static int rfftn_impl(fh_plan* pl, int ncomp, const T* x, cpx<T>* xhat) {
+  const GParams g = make_gparams(pl, 1.0, 1);
+  for (int c = 0; c < ncomp; ++c) {
+    cpx<T>* out = xhat + (size_t)c * pl->nh;
+    RETCHK(launch_rows_fwd<T>(pl, 1, x + (size_t)c * pl->n, nullptr, fh::COEF_NONE, out, nullptr));
+    if (pl->n0 > 1) {
+      RETCHK(launch_cols<T>(pl, 0, 1, 1, out, g, nullptr));
+      RETCHK(launch_cols<T>(pl, 0, 0, 1, out, g, nullptr));
+    } else {
+      RETCHK(launch_cols<T>(pl, 0, 1, 1, out, g, nullptr));
+    }
+  }
+  return FH_OK;
+}
+
+template <typename T>
+static int irfftn_impl(fh_plan* pl, int ncomp, const cpx<T>* xhat, T* x) {
+  const GParams g = make_gparams(pl, 1.0, 1);
+  cpx<T>* tmp = reinterpret_cast<cpx<T>*>(pl->spec);
+  for (int c = 0; c < ncomp; ++c) {
+    HIPCHK(hipMemcpyAsync(tmp, xhat + (size_t)c * pl->nh, (size_t)pl->nh * sizeof(cpx<T>),
+                          hipMemcpyDeviceToDevice, pl->stream));
+    if (pl->n0 > 1) {
+      RETCHK(launch_cols<T>(pl, 1, 0, 1, tmp, g, nullptr));
+      RETCHK(launch_cols<T>(pl, 1, 1, 1, tmp, g, nullptr));
+    } else {
+      RETCHK(launch_cols<T>(pl, 1, 1, 1, tmp, g, nullptr));
+    }
+    RETCHK(launch_rows_inv<T>(pl, 1, tmp, x + (size_t)c * pl->n, T(1.0 / (double)pl->n), nullptr, nullptr,
+                              nullptr, nullptr));
+  }
+  return FH_OK;
+}
+
+// ---- CG ---------------------------------------------------------------------------------
+
+static int ensure_work(fh_plan* pl, size_t bytes) {
+  if (pl->work_bytes >= bytes) return FH_OK;
+  if (pl->work) HIPCHK(hipFree(pl->work));
+  pl->work = nullptr;
+  pl->work_bytes = 0;
+  HIPCHK(hipMalloc(&pl->work, bytes));
+  pl->work_bytes = bytes;
+  return FH_OK;
+}
+
+static int ensure_hist(fh_plan* pl, int len) {
+  if (pl->hist_cap >= len) return FH_OK;
+  if (pl->hist) HIPCHK(hipFree(pl->hist));
+  pl->hist = nullptr;
+  pl->hist_cap = 0;
+  HIPCHK(hipMalloc(&pl->hist, (size_t)len * sizeof(double)));
+  pl->hist_cap = len;
+  return FH_OK;
+}
+
+// one CG iteration in precision T on vectors (x, r, p, Ap); everything halts on st->halt
+template <typename T>
+static int enqueue_iteration(fh_plan* pl, T* x, T* r, T* p, T* Ap, const T* coef) {
+  const long long nv = (long long)pl->ncomp * pl->n;
+  const int* halt = &pl->st->halt;
+  int nb = 0;
+  RETCHK(run_operator<T>(pl, p, Ap, pl->coef_kind, coef, p, halt, &nb));
+  hipLaunchKernelGGL(fh::k_cg_alpha, dim3(1), dim3(256), 0, pl->stream, pl->st, pl->partial, nb);
+  const int gb = grid_blocks(nv);
+  hipLaunchKernelGGL((fh::k_cg_update<T>), dim3(gb), dim3(256), 0, pl->stream, nv, x, r, p, Ap, pl->st,
+                     pl->partial);
+  hipLaunchKernelGGL(fh::k_cg_beta, dim3(1), dim3(256), 0, pl->stream, pl->st, pl->partial, gb, pl->hist);
+  hipLaunchKernelGGL((fh::k_cg_pupdate<T>), dim3(gb), dim3(256), 0, pl->stream, nv, p, r, pl->st);
+  HIPCHK(hipGetLastError());
+  return FH_OK;
+}
+
+// r = b - A x (fp64 operator); rr, beta, done via k_cg_restart
+template <typename TR>
+static int enqueue_true_residual(fh_plan* pl, const double* b, const double* x, double* Ax, TR* r) {
+  const long long nv = (long long)pl->ncomp * pl->n;
+  RETCHK(run_operator<double>(pl, x, Ax, pl->coef_kind, pl->coef64, nullptr, nullptr, nullptr));
+  const int gb = grid_blocks(nv);
+  hipLaunchKernelGGL((fh::k_residual<TR>), dim3(gb), dim3(256), 0, pl->stream, nv, b, Ax, r, pl->partial);
+  hipLaunchKernelGGL(fh::k_cg_restart, dim3(1), dim3(256), 0, pl->stream, pl->st, pl->partial, gb, pl->hist);
+  HIPCHK(hipGetLastError());
+  return FH_OK;
+}
+
+static int scratch_parts(double** out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch) HIPCHK(hipMalloc(&g_scratch, kScratchParts * sizeof(double) + 64));
+  *out = g_scratch;
+  return FH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* fh_last_error(void) { return g_err.c_str(); }
+
+int fh_init(char* name, int name_len) {
+  RETCHK(ensure_device());
+  if (name && name_len > 0) {
+    std::snprintf(name, (size_t)name_len, "%s", g_dev_name.c_str());
+  }
+  return FH_OK;
+}
+
+int fh_malloc(void** dptr, size_t bytes) {
+  if (!dptr) return fail(FH_ERR_ARG, "null pointer argument");
+  RETCHK(ensure_device());
+  *dptr = nullptr;
+  if (bytes == 0) bytes = 256;
+  HIPCHK(hipMalloc(dptr, bytes));
+  return FH_OK;
+}
+
+int fh_free(void* dptr) {
+  if (!dptr) return FH_OK;
+  RETCHK(ensure_device());
+  HIPCHK(hipFree(dptr));
+  return FH_OK;
+}
+
+int fh_memcpy_htod(void* dst, const void* src, size_t bytes) {
+  RETCHK(ensure_device());
+  if (bytes == 0) return FH_OK;
+  if (!dst || !src) return fail(FH_ERR_ARG, "null pointer argument");
+  HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return FH_OK;
+}
+
+int fh_memcpy_dtoh(void* dst, const void* src, size_t bytes) {
+  RETCHK(ensure_device());
+  if (bytes == 0) return FH_OK;
+  if (!dst || !src) return fail(FH_ERR_ARG, "null pointer argument");
+  HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return FH_OK;
+}
+
+int fh_memcpy_dtod(void* dst, const void* src, size_t bytes) {
+  RETCHK(ensure_device());
+  if (bytes == 0) return FH_OK;
+  if (!dst || !src) return fail(FH_ERR_ARG, "null pointer argument");
+  HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+  return FH_OK;
+}
+
+int fh_memset_zero(void* dst, size_t bytes) {
+  RETCHK(ensure_device());
+  if (bytes == 0) return FH_OK;
+  if (!dst) return fail(FH_ERR_ARG, "null pointer argument");
+  HIPCHK(hipMemset(dst, 0, bytes));
+  HIPCHK(hipDeviceSynchronize());
+  return FH_OK;
+}
+
+int fh_plan_destroy(fh_plan* pl) {
+  if (!pl) return FH_OK;
+  for (int a = 0; a < 3; ++a) {
+    if (pl->tw64[a]) (void)hipFree(pl->tw64[a]);
+    if (pl->tw32[a]) (void)hipFree(pl->tw32[a]);
+  }
+  if (pl->coef64) (void)hipFree(pl->coef64);
+  if (pl->coef32) (void)hipFree(pl->coef32);
+  if (pl->spec) (void)hipFree(pl->spec);
+  if (pl->partial) (void)hipFree(pl->partial);
+  if (pl->st) (void)hipFree(pl->st);
+  if (pl->hist) (void)hipFree(pl->hist);
+  if (pl->work) (void)hipFree(pl->work);
+  if (pl->ev0) (void)hipEventDestroy(pl->ev0);
+  if (pl->ev1) (void)hipEventDestroy(pl->ev1);
+  if (pl->stream) (void)hipStreamDestroy(pl->stream);
+  delete pl;
+  return FH_OK;
+}
+
+int fh_plan_create(int dim, const int* N, const double* Y, int physics, int primaldual, fh_plan** out) {
+  if (!out || !N || !Y) return fail(FH_ERR_ARG, "null pointer argument");
+  *out = nullptr;
+  if (dim != 2 && dim != 3) return fail(FH_ERR_ARG, "dim must be 2 or 3");
+  if (physics != FH_SCALAR && physics != FH_ELASTICITY) return fail(FH_ERR_ARG, "bad physics");
+  if (primaldual != FH_PRIMAL && primaldual != FH_DUAL) return fail(FH_ERR_ARG, "bad primaldual");
+  for (int a = 0; a < dim; ++a) {
+    if (N[a] < 1 || N[a] > (1 << 20)) return fail(FH_ERR_ARG, "grid size out of range");
+    if (!(Y[a] > 0.0) || !std::isfinite(Y[a])) return fail(FH_ERR_ARG, "cell size must be positive");
+  }
+  RETCHK(ensure_device());
+  fh_plan* pl = new fh_plan();
+  pl->dim = dim;
+  pl->physics = physics;
+  pl->primaldual = primaldual;
+  pl->ncomp = physics == FH_SCALAR ? dim : dim * (dim + 1) / 2;
+  for (int a = 0; a < dim; ++a) {
+    pl->Nu[a] = N[a];
+    pl->Y[a] = Y[a];
+  }
+  if (dim == 3) {
+    pl->n0 = N[0];
+    pl->n1 = N[1];
+    pl->n2 = N[2];
+  } else {
+    pl->n0 = 1;
+    pl->n1 = N[0];
+    pl->n2 = N[1];
+  }
+  pl->n2h = pl->n2 / 2 + 1;
+  pl->n = (long long)pl->n0 * pl->n1 * pl->n2;
+  pl->nh = (long long)pl->n0 * pl->n1 * pl->n2h;
+  const int lens[3] = {pl->n0, pl->n1, pl->n2};
+  auto bail = [&](int code) {
+    fh_plan_destroy(pl);
+    return code;
+  };
+  for (int a = 0; a < 3; ++a) {
+    if (!factorize(lens[a], &pl->ax[a])) {
+      fh_plan_destroy(pl);
+      return fail(FH_ERR_UNSUPPORTED,
+                  "grid length " + std::to_string(lens[a]) + " has a prime factor above 13");
+    }
+    const int L = lens[a];
+    std::vector<double> t64(2 * (size_t)L);
+    std::vector<float> t32(2 * (size_t)L);
+    for (int m = 0; m < L; ++m) {
+      const long double ang = -2.0L * 3.14159265358979323846264338327950288L * (long double)m / (long double)L;
+      t64[2 * m] = (double)cosl(ang);
+      t64[2 * m + 1] = (double)sinl(ang);
+      t32[2 * m] = (float)t64[2 * m];
+      t32[2 * m + 1] = (float)t64[2 * m + 1];
+    }
+    if (hipMalloc(&pl->tw64[a], t64.size() * sizeof(double)) != hipSuccess ||
+        hipMalloc(&pl->tw32[a], t32.size() * sizeof(float)) != hipSuccess)
+      return bail(fail(FH_ERR_HIP, "hipMalloc twiddles failed"));
+    if (hipMemcpy(pl->tw64[a], t64.data(), t64.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(pl->tw32[a], t32.data(), t32.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+      return bail(fail(FH_ERR_HIP, "hipMemcpy twiddles failed"));
+  }
+  // validate launch shapes once (fp64 is the largest)
+  {
+    RowCfg rc;
+    ColCfg cc;
+    int r = row_config(pl->ax[2], pl->ncomp, sizeof(cpx<double>), (long long)pl->n0 * pl->n1, &rc);
+    if (r != FH_OK) return bail(r);
+    if (pl->n0 > 1) {
+      r = col_config(pl->ax[1], 4, sizeof(cpx<double>), (long long)pl->n2h * pl->ncomp, &cc);
+      if (r != FH_OK) return bail(r);
+      r = col_config(pl->ax[0], pl->ncomp, sizeof(cpx<double>), (long long)pl->n1 * pl->n2h * pl->ncomp, &cc);
+      if (r != FH_OK) return bail(r);
+    } else {
+      r = col_config(pl->ax[1], pl->ncomp, sizeof(cpx<double>), (long long)pl->n2h * pl->ncomp, &cc);
+      if (r != FH_OK) return bail(r);
+    }
+    RowCfg rc32;
+    row_config(pl->ax[2], pl->ncomp, sizeof(cpx<float>), (long long)pl->n0 * pl->n1, &rc32);
+    pl->partial_cap = (size_t)std::max({rc.nblocks, rc32.nblocks, kScratchParts, 2 * 36 * 1024});
+  }
+  if (hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&pl->ev0) != hipSuccess || hipEventCreate(&pl->ev1) != hipSuccess)
+    return bail(fail(FH_ERR_HIP, "stream/event creation failed"));
+  const size_t spec_bytes = (size_t)pl->ncomp * pl->nh * sizeof(cpx<double>);
+  if (hipMalloc(&pl->spec, spec_bytes) != hipSuccess ||
+      hipMalloc(&pl->partial, pl->partial_cap * sizeof(double)) != hipSuccess ||
+      hipMalloc(&pl->st, sizeof(CGState)) != hipSuccess)
+    return bail(fail(FH_ERR_HIP, "hipMalloc plan buffers failed (out of device memory?)"));
+  *out = pl;
+  return FH_OK;
+}
+
+int fh_plan_ncomp(const fh_plan* pl) { return pl ? pl->ncomp : -1; }
+
+int fh_plan_set_primaldual(fh_plan* pl, int primaldual) {
+  RETCHK(check_plan(pl));
+  if (primaldual != FH_PRIMAL && primaldual != FH_DUAL) return fail(FH_ERR_ARG, "bad primaldual");
+  pl->primaldual = primaldual;
+  return FH_OK;
+}
+
+int fh_set_coeff(fh_plan* pl, int kind, const double* host_vals) {
+  RETCHK(check_plan(pl));
+  if (!host_vals) return fail(FH_ERR_ARG, "null coefficient array");
+  int per;
+  if (kind == FH_COEF_ISO) {
+    per = 1;
+  } else if (kind == FH_COEF_LAME) {
+    if (pl->physics != FH_ELASTICITY) return fail(FH_ERR_ARG, "Lame coefficients need elasticity");
+    per = 2;
+  } else if (kind == FH_COEF_FULL) {
+    per = pl->ncomp * pl->ncomp;
+  } else {
+    return fail(FH_ERR_ARG, "bad coefficient kind");
+  }
+  const size_t cnt = (size_t)per * pl->n;
+  for (size_t i = 0; i < cnt; ++i)
+    if (!std::isfinite(host_vals[i])) return fail(FH_ERR_ARG, "coefficient has a non-finite value");
+  if (pl->coef64) (void)hipFree(pl->coef64);
+  if (pl->coef32) (void)hipFree(pl->coef32);
+  pl->coef64 = nullptr;
+  pl->coef32 = nullptr;
+  pl->coef_kind = 0;
+  HIPCHK(hipMalloc(&pl->coef64, cnt * sizeof(double)));
+  HIPCHK(hipMalloc(&pl->coef32, cnt * sizeof(float)));
+  HIPCHK(hipMemcpy(pl->coef64, host_vals, cnt * sizeof(double), hipMemcpyHostToDevice));
+  hipLaunchKernelGGL((fh::k_convert<double, float>), dim3(grid_blocks((long long)cnt)), dim3(256), 0,
+                     pl->stream, (long long)cnt, pl->coef64, pl->coef32);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(pl->stream));
+  pl->coef_kind = kind;
+  return FH_OK;
+}
+
+int fh_matvec(fh_plan* pl, int dtype, const void* x, void* y) { return matvec_impl(pl, dtype, x, y, true); }
+
+int fh_project(fh_plan* pl, int dtype, const void* x, void* y) { return matvec_impl(pl, dtype, x, y, false); }
+
+int fh_rfftn(fh_plan* pl, int dtype, int ncomp, const void* x, void* xhat) {
+  RETCHK(check_plan(pl));
+  RETCHK(check_dtype(dtype));
+  if (!x || !xhat || ncomp < 1) return fail(FH_ERR_ARG, "bad field arguments");
+  if (dtype == FH_F64)
+    RETCHK(rfftn_impl<double>(pl, ncomp, (const double*)x, (cpx<double>*)xhat));
+  else
+    RETCHK(rfftn_impl<float>(pl, ncomp, (const float*)x, (cpx<float>*)xhat));
+  HIPCHK(hipStreamSynchronize(pl->stream));
+  return FH_OK;
+}
+
+int fh_irfftn(fh_plan* pl, int dtype, int ncomp, const void* xhat, void* x) {
+  RETCHK(check_plan(pl));
+  RETCHK(check_dtype(dtype));
+  if (!x || !xhat || ncomp < 1) return fail(FH_ERR_ARG, "bad field arguments");
+  if (dtype == FH_F64)
+    RETCHK(irfftn_impl<double>(pl, ncomp, (const cpx<double>*)xhat, (double*)x));
+  else
+    RETCHK(irfftn_impl<float>(pl, ncomp, (const cpx<float>*)xhat, (float*)x));
+  HIPCHK(hipStreamSynchronize(pl->stream));
+  return FH_OK;
+}
+
+int fh_apply_projection_hat(fh_plan* pl, int dtype, const void* in_hat, void* out_hat) {
+  RETCHK(check_plan(pl));
+  RETCHK(check_dtype(dtype));
+  if (!in_hat || !out_hat) return fail(FH_ERR_ARG, "null spectrum pointer");
+  const GParams g = make_gparams(pl, 1.0);
+  const dim3 grid((unsigned)((pl->nh + 255) / 256)), block(256);
+  if (dtype == FH_F64)
+    hipLaunchKernelGGL(fh::k_apply_G_soa<double>, grid, block, 0, pl->stream, (const cpx<double>*)in_hat,
+                       (cpx<double>*)out_hat, pl->nh, pl->n2h, g);
+  else
+    hipLaunchKernelGGL(fh::k_apply_G_soa<float>, grid, block, 0, pl->stream, (const cpx<float>*)in_hat,
+                       (cpx<float>*)out_hat, pl->nh, pl->n2h, g);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(pl->stream));
+  return FH_OK;
+}
+
+int fh_cg_solve(fh_plan* pl, int mode, const void* b_, void* x_, double tol, int maxiter, double delta,
+                int* kit, double* norm_res, double* res_hist, int hist_len, int* cg_status) {
+  RETCHK(check_plan(pl));
+  if (!b_ || !x_) return fail(FH_ERR_ARG, "null field pointer");
+  if (mode != FH_SOLVE_FP64 && mode != FH_SOLVE_MIXED) return fail(FH_ERR_ARG, "bad solve mode");
+  if (!pl->coef_kind) return fail(FH_ERR_STATE, "fh_set_coeff has not been called");
+  if (!(tol >= 0.0) || maxiter < 0) return fail(FH_ERR_ARG, "bad tol/maxiter");
+  if (mode == FH_SOLVE_MIXED && !(delta > 0.0 && delta < 1.0)) return fail(FH_ERR_ARG, "delta must be in (0,1)");
+  const double* b = (const double*)b_;
+  double* x = (double*)x_;
+  const long long nv = (long long)pl->ncomp * pl->n;
+  const int hl = std::max(hist_len, 1);
+  RETCHK(ensure_hist(pl, hl));
+  CGState s;
+  std::memset(&s, 0, sizeof(s));
+  s.tol = tol;
+  s.maxiter = maxiter;
+  s.mixed = mode == FH_SOLVE_MIXED;
+  s.delta = delta;
+  s.inv_nvox = 1.0 / (double)pl->n;
+  s.hist_len = hist_len > 0 ? hist_len : 0;
+  HIPCHK(hipMemcpyAsync(pl->st, &s, sizeof(s), hipMemcpyHostToDevice, pl->stream));
+  HIPCHK(hipEventRecord(pl->ev0, pl->stream));
+  const int per_check = 8;
+  if (mode == FH_SOLVE_FP64) {
+    RETCHK(ensure_work(pl, 3 * (size_t)nv * sizeof(double)));
+    double* r = (double*)pl->work;
+    double* p = r + nv;
+    double* Ap = p + nv;
+    RETCHK(enqueue_true_residual<double>(pl, b, x, Ap, r));
+    HIPCHK(hipMemcpyAsync(p, r, (size_t)nv * sizeof(double), hipMemcpyDeviceToDevice, pl->stream));
+    for (;;) {
+      for (int k = 0; k < per_check; ++k) RETCHK(enqueue_iteration<double>(pl, x, r, p, Ap, pl->coef64));
+      HIPCHK(hipMemcpyAsync(&s, pl->st, sizeof(s), hipMemcpyDeviceToHost, pl->stream));
+      HIPCHK(hipStreamSynchronize(pl->stream));
+      if (s.done) break;
+    }
+  } else {
+    RETCHK(ensure_work(pl, 4 * (size_t)nv * sizeof(float) + (size_t)nv * sizeof(double)));
+    double* Ax = (double*)pl->work;
+    float* r = (float*)(Ax + nv);
+    float* p = r + nv;
+    float* Ap = p + nv;
+    float* xc = Ap + nv;
+    HIPCHK(hipMemsetAsync(xc, 0, (size_t)nv * sizeof(float), pl->stream));
+    RETCHK(enqueue_true_residual<float>(pl, b, x, Ax, r));
+    HIPCHK(hipMemcpyAsync(p, r, (size_t)nv * sizeof(float), hipMemcpyDeviceToDevice, pl->stream));
+    for (;;) {
+      for (int k = 0; k < per_check; ++k) RETCHK(enqueue_iteration<float>(pl, xc, r, p, Ap, pl->coef32));
+      HIPCHK(hipMemcpyAsync(&s, pl->st, sizeof(s), hipMemcpyDeviceToHost, pl->stream));
+      HIPCHK(hipStreamSynchronize(pl->stream));
+      if (s.done) break;
+      if (s.pause) {
+        const int gb = grid_blocks(nv);
+        hipLaunchKernelGGL(fh::k_fold_correction, dim3(gb), dim3(256), 0, pl->stream, nv, x, xc);
+        RETCHK(enqueue_true_residual<float>(pl, b, x, Ax, r));
+        hipLaunchKernelGGL((fh::k_cg_pupdate<float>), dim3(gb), dim3(256), 0, pl->stream, nv, p, r, pl->st);
+        HIPCHK(hipGetLastError());
+      }
+    }
+    const int gb = grid_blocks(nv);
+    hipLaunchKernelGGL(fh::k_fold_correction, dim3(gb), dim3(256), 0, pl->stream, nv, x, xc);
+    HIPCHK(hipGetLastError());
+    // final fp64 projection x <- G x removes the non-conforming part left by fp32 rounding
+    RETCHK(run_operator<double>(pl, x, Ax, fh::COEF_NONE, nullptr, nullptr, nullptr, nullptr));
+    HIPCHK(hipMemcpyAsync(x, Ax, (size_t)nv * sizeof(double), hipMemcpyDeviceToDevice, pl->stream));
+  }
+  HIPCHK(hipEventRecord(pl->ev1, pl->stream));
+  HIPCHK(hipMemcpyAsync(&s, pl->st, sizeof(s), hipMemcpyDeviceToHost, pl->stream));
+  HIPCHK(hipStreamSynchronize(pl->stream));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, pl->ev0, pl->ev1));
+  pl->last_solve_ms = ms;
+  pl->last_updates = s.nupdates;
+  if (kit) *kit = s.kit;
+  if (norm_res) *norm_res = (s.kit == 0) ? 0.0 : std::sqrt(s.rr);
+  if (cg_status) *cg_status = s.status;
+  if (res_hist && hist_len > 0) {
+    const int cnt = std::min(hist_len, s.kit + 1);
+    HIPCHK(hipMemcpy(res_hist, pl->hist, (size_t)cnt * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  return FH_OK;
+}
+
+int fh_energy_matrix(fh_plan* pl, const void* const* fields, int nf, double* AH) {
+  RETCHK(check_plan(pl));
+  if (!fields || !AH) return fail(FH_ERR_ARG, "null pointer argument");
+  if (nf != pl->ncomp) return fail(FH_ERR_ARG, "need one field per unit load (nf == ncomp)");
+  if (!pl->coef_kind) return fail(FH_ERR_STATE, "fh_set_coeff has not been called");
+  for (int i = 0; i < nf; ++i)
+    if (!fields[i]) return fail(FH_ERR_ARG, "null field pointer");
+  const int nn = nf * nf;
+  const int nb = std::min<long long>(1024, std::max<long long>(1, (pl->n + 256 * 8 - 1) / (256 * 8)));
+  if ((size_t)2 * nn * nb > pl->partial_cap) return fail(FH_ERR_STATE, "partial buffer too small");
+  void* dptrs = nullptr;
+  double* dAH = nullptr;
+  HIPCHK(hipMalloc(&dptrs, nf * sizeof(void*)));
+  HIPCHK(hipMalloc(&dAH, nn * sizeof(double)));
+  int rc = FH_OK;
+  hipError_t e = hipMemcpy(dptrs, fields, nf * sizeof(void*), hipMemcpyHostToDevice);
+  if (e != hipSuccess) rc = fail(FH_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
+  if (rc == FH_OK) {
+    const double* const* F = (const double* const*)dptrs;
+    const dim3 grid(nb), block(256);
+    const int kind = pl->coef_kind;
+#define FH_ENERGY(NC)                                                                                \
+  do {                                                                                               \
+    if (kind == FH_COEF_ISO)                                                                         \
+      hipLaunchKernelGGL((fh::k_energy<NC, fh::COEF_ISO>), grid, block, 0, pl->stream, F, pl->coef64, \
+                         pl->n, pl->partial);                                                        \
+    else if (kind == FH_COEF_LAME)                                                                   \
+      hipLaunchKernelGGL((fh::k_energy<NC, fh::COEF_LAME>), grid, block, 0, pl->stream, F,            \
+                         pl->coef64, pl->n, pl->partial);                                            \
+    else                                                                                             \
+      hipLaunchKernelGGL((fh::k_energy<NC, fh::COEF_FULL>), grid, block, 0, pl->stream, F,            \
+                         pl->coef64, pl->n, pl->partial);                                            \
+  } while (0)
+    if (nf == 2)
+      FH_ENERGY(2);
+    else if (nf == 3)
+      FH_ENERGY(3);
+    else if (nf == 6)
+      FH_ENERGY(6);
+    else
+      rc = fail(FH_ERR_ARG, "unsupported number of components");
+#undef FH_ENERGY
+    if (rc == FH_OK) {
+      hipLaunchKernelGGL(fh::k_energy_final, dim3(1), dim3(256), 0, pl->stream, pl->partial, nb, nn,
+                         1.0 / (double)pl->n, dAH);
+      e = hipGetLastError();
+      if (e == hipSuccess) e = hipStreamSynchronize(pl->stream);
+      if (e == hipSuccess) e = hipMemcpy(AH, dAH, nn * sizeof(double), hipMemcpyDeviceToHost);
+      if (e != hipSuccess) rc = fail(FH_ERR_HIP, std::string("energy: ") + hipGetErrorString(e));
+    }
+  }
+  (void)hipFree(dptrs);
+  (void)hipFree(dAH);
+  return rc;
+}
+
+int fh_axpby(int dtype, size_t n, double a, const void* x, double b, void* y) {
+  RETCHK(check_dtype(dtype));
+  RETCHK(ensure_device());
+  if (n == 0) return FH_OK;
+  if (!x || !y) return fail(FH_ERR_ARG, "null pointer argument");
+  const dim3 grid(grid_blocks((long long)n)), block(256);
+  if (dtype == FH_F64)
+    hipLaunchKernelGGL(fh::k_axpby<double>, grid, block, 0, 0, (long long)n, a, (const double*)x, b, (double*)y);
+  else
+    hipLaunchKernelGGL(fh::k_axpby<float>, grid, block, 0, 0, (long long)n, a, (const float*)x, b, (float*)y);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipDeviceSynchronize());
+  return FH_OK;
+}
+
+int fh_dot(int dtype, size_t n, const void* x, const void* y, double* out) {
+  RETCHK(check_dtype(dtype));
+  RETCHK(ensure_device());
+  if (!out) return fail(FH_ERR_ARG, "null pointer argument");
+  if (n == 0) {
+    *out = 0.0;
+    return FH_OK;
+  }
+  if (!x || !y) return fail(FH_ERR_ARG, "null pointer argument");
+  double* part = nullptr;
+  RETCHK(scratch_parts(&part));
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  const int gb = std::min(grid_blocks((long long)n), kScratchParts);
+  if (dtype == FH_F64)
+    hipLaunchKernelGGL(fh::k_dot_partial<double>, dim3(gb), dim3(256), 0, 0, (long long)n, (const double*)x,
+                       (const double*)y, part);
+  else
+    hipLaunchKernelGGL(fh::k_dot_partial<float>, dim3(gb), dim3(256), 0, 0, (long long)n, (const float*)x,
+                       (const float*)y, part);
+  hipLaunchKernelGGL(fh::k_sum_partials, dim3(1), dim3(256), 0, 0, part, gb, 1.0, part + kScratchParts);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpy(out, part + kScratchParts, sizeof(double), hipMemcpyDeviceToHost));
+  return FH_OK;
+}
+
+int fh_add_constant(int dtype, size_t nvox, int ncomp, const double* E, void* y) {
+  RETCHK(check_dtype(dtype));
+  RETCHK(ensure_device());
+  if (!E || !y || ncomp < 1 || ncomp > 64) return fail(FH_ERR_ARG, "bad arguments");
+  double* dE = nullptr;
+  HIPCHK(hipMalloc(&dE, ncomp * sizeof(double)));
+  hipError_t e = hipMemcpy(dE, E, ncomp * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    const dim3 grid(grid_blocks((long long)nvox)), block(256);
+    if (dtype == FH_F64)
+      hipLaunchKernelGGL(fh::k_add_const<double>, grid, block, 0, 0, (long long)nvox, ncomp, dE, (double*)y);
+    else
+      hipLaunchKernelGGL(fh::k_add_const<float>, grid, block, 0, 0, (long long)nvox, ncomp, dE, (float*)y);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+  }
+  (void)hipFree(dE);
+  if (e != hipSuccess) return fail(FH_ERR_HIP, std::string("fh_add_constant: ") + hipGetErrorString(e));
+  return FH_OK;
+}
+
+int fh_last_timing(const fh_plan* pl, double* matvec_ms, double* solve_ms) {
+  if (!pl) return fail(FH_ERR_ARG, "null plan");
+  if (matvec_ms) *matvec_ms = pl->last_matvec_ms;
+  if (solve_ms) *solve_ms = pl->last_solve_ms;
+  return FH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,141 @@
+/*
+ * ffthom: C ABI of the MI355X (gfx950) Fourier-Galerkin homogenization operator and solver.
+ *
+ * Drop-in boundary for the GaNi solve of FFTHomPy (vondrejc/FFTHomPy). The reference
+ * builds, in ffthompy/applications.py:24-58,
+ *     Afun = Operator([[FiN, hG, FN], A])     i.e.  y = F^-1 G(xi) F (A(x) v(x))
+ * and hands it to linear_solver('CG', Afun, B, x0, par, callback)
+ * (ffthompy/general/solver.py:8-60). Every entry point below replaces one piece of
+ * that call graph; its comment names the reference interface (file:line).
+ *
+ * Conventions
+ *  - All functions return FH_OK (0) or an FH_ERR_* code; fh_last_error() gives the
+ *    message of the calling thread's last failure. No entry point aborts the process.
+ *  - Fields are device pointers (from fh_malloc) in the reference Tensor layout
+ *    shape+N: component-major, C-order, last axis contiguous, origin at index 0.
+ *  - Spectra in the reference 'r' form: (ncomp,) + N[:-1] + (N[-1]//2+1,), complex
+ *    interleaved (re, im), unnormalised forward transform (numpy.fft.rfftn).
+ *  - A plan owns one HIP stream (default priority). Calls on one plan are not
+ *    re-entrant; use one plan per host thread. Every entry point has finished its
+ *    GPU work when it returns.
+ */
+#ifndef FFTHOM_H
+#define FFTHOM_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  FH_OK = 0,
+  FH_ERR_ARG = 1,          /* invalid argument (shape, pointer, enum) */
+  FH_ERR_HIP = 2,          /* HIP runtime failure */
+  FH_ERR_NODEV = 3,        /* no usable GPU */
+  FH_ERR_UNSUPPORTED = 4,  /* grid length with a prime factor above 13, wrong GPU arch */
+  FH_ERR_STATE = 5         /* call out of order (e.g. solve before coefficients) */
+};
+
+enum { FH_SCALAR = 0, FH_ELASTICITY = 1 };           /* ffthompy/problem.py:33-36 */
+enum { FH_PRIMAL = 0, FH_DUAL = 1 };                 /* ffthompy/applications.py:53-56 */
+enum { FH_F64 = 0, FH_F32 = 1 };                     /* element type of a device field */
+enum { FH_SOLVE_FP64 = 0, FH_SOLVE_MIXED = 1 };      /* CG arithmetic */
+enum { FH_COEF_ISO = 1, FH_COEF_LAME = 2, FH_COEF_FULL = 3 };
+enum { FH_CG_OK = 0, FH_CG_BREAKDOWN = 1 };
+
+typedef struct fh_plan fh_plan;
+
+/* Message of the calling thread's last error ("" if none). */
+const char* fh_last_error(void);
+
+/* Select device 0 and check it is a gfx950 GPU. Writes the device name.
+ * No reference counterpart (the reference is host-only NumPy). */
+int fh_init(char* name, int name_len);
+
+/* Device memory. No reference counterpart: NumPy owns the reference's memory
+ * (ffthompy/tensors/objects.py:117-121 allocates Tensor.val with np.zeros). */
+int fh_malloc(void** dptr, size_t bytes);
+int fh_free(void* dptr);
+int fh_memcpy_htod(void* dst, const void* src, size_t bytes);
+int fh_memcpy_dtoh(void* dst, const void* src, size_t bytes);
+int fh_memcpy_dtod(void* dst, const void* src, size_t bytes);
+int fh_memset_zero(void* dst, size_t bytes);
+
+/* Plan: grid, cell size, physics and which projection the operator uses.
+ * Replaces proj.scalar / proj.elasticity (ffthompy/projections.py:9-112, 114-267)
+ * and the DFT pair FN, FiN (ffthompy/applications.py:33-37, ffthompy/tensors/operators.py:14-64):
+ * G(xi) is never materialised, it is evaluated from the frequency index inside the FFT pass.
+ * dim = 2 or 3, N[dim] grid points per axis (any length whose prime factors are <= 13),
+ * Y[dim] cell size. */
+int fh_plan_create(int dim, const int* N, const double* Y, int physics, int primaldual,
+                   fh_plan** out);
+int fh_plan_destroy(fh_plan* plan);
+/* Components of a field: dim (scalar) or dim*(dim+1)/2 (elasticity, Mandel). */
+int fh_plan_ncomp(const fh_plan* plan);
+/* Switch the projection between G1 (primal) and G2 (dual): applications.py:53-56. */
+int fh_plan_set_primaldual(fh_plan* plan, int primaldual);
+
+/* Coefficient A(x) of the operator, host array in origin-0 voxel order.
+ * Replaces Material.get_A_GaNi (ffthompy/materials.py:115-123) as the right factor of
+ * Afun = [[GN, A]] (ffthompy/applications.py:58). For the dual problem pass the inverse
+ * coefficient, as the reference does (materials.py:121-122).
+ *  FH_COEF_ISO : n values, A = a(x) I
+ *  FH_COEF_LAME: 2n values (lambda then mu), Mandel C = lambda m m^T + 2 mu I (elasticity)
+ *  FH_COEF_FULL: ncomp*ncomp*n values, entry (i,j) at [(i*ncomp+j)*n + voxel] */
+int fh_set_coeff(fh_plan* plan, int kind, const double* host_vals);
+
+/* y = F^-1 G(xi) F (A(x) x): the matvec Afun(x) (ffthompy/tensors/operators.py:136-144)
+ * in fp64 or fp32 arithmetic (dtype of both fields). */
+int fh_matvec(fh_plan* plan, int dtype, const void* x, void* y);
+
+/* y = F^-1 G(xi) F x: the projection operator G1N / G2N (ffthompy/applications.py:36-37). */
+int fh_project(fh_plan* plan, int dtype, const void* x, void* y);
+
+/* Forward / inverse real FFT of an ncomp-component field on the plan grid:
+ * DFT.__call__ with fft_form 'r' (ffthompy/tensors/operators.py:49-58,
+ * ffthompy/tensors/fft.py:39-43). rfftn is unnormalised, irfftn normalised by 1/n. */
+int fh_rfftn(fh_plan* plan, int dtype, int ncomp, const void* x, void* xhat);
+int fh_irfftn(fh_plan* plan, int dtype, int ncomp, const void* xhat, void* x);
+
+/* out_hat = G(xi) in_hat on 'r'-form spectra of plan ncomp components: the einsum
+ * 'ij...,j...->i...' of the materialised hG with a spectrum
+ * (ffthompy/tensors/objects.py:223-232, 599-604). */
+int fh_apply_projection_hat(fh_plan* plan, int dtype, const void* in_hat, void* out_hat);
+
+/* Conjugate gradients on Afun x = b: CG of ffthompy/general/solver.py:80-139 with the
+ * reference's absolute stop sqrt(<r,r>) <= tol, <u,v> = sum(u*v)/prod(N)
+ * (ffthompy/tensors/objects.py:634-635). b and x are fp64 fields; x holds x0 on entry.
+ * mode FH_SOLVE_FP64: plain fp64 CG.
+ * mode FH_SOLVE_MIXED: fp32 CG with fp64 reliable updates (x accumulated in fp64, the
+ *   true residual b - A x recomputed in fp64 whenever the fp32 residual fell by `delta`
+ *   or claims convergence; the stop test always uses the fp64 true residual) and a final
+ *   fp64 projection x <- G x.
+ * kit, norm_res: iterations and final residual norm, as info['kit'], info['norm_res'].
+ * res_hist (host, nullable): sqrt(<r,r>) after every iteration (entry 0: initial),
+ * hist_len entries at most. cg_status: FH_CG_OK or FH_CG_BREAKDOWN (<p,Ap> <= 0). */
+int fh_cg_solve(fh_plan* plan, int mode, const void* b, void* x, double tol, int maxiter,
+                double delta, int* kit, double* norm_res, double* res_hist, int hist_len,
+                int* cg_status);
+
+/* AH[i][j] = <A e_i, e_j> for nf = ncomp fields e_i (fp64, macroscopic part included),
+ * double-double accumulated: assembly_matrix (ffthompy/postprocess.py:53-70). */
+int fh_energy_matrix(fh_plan* plan, const void* const* fields, int nf, double* AH);
+
+/* Helpers on fp64/fp32 device vectors (Tensor arithmetic, ffthompy/tensors/objects.py:188-245):
+ * y = a x + b y; out = sum(x*y) (caller divides by prod(N) for the grid inner product,
+ * ffthompy/tensors/objects.py:618-636); y_c += E_c on an ncomp-component field
+ * (Tensor.add_mean, objects.py:274-287). */
+int fh_axpby(int dtype, size_t n, double a, const void* x, double b, void* y);
+int fh_dot(int dtype, size_t n, const void* x, const void* y, double* out);
+int fh_add_constant(int dtype, size_t nvox, int ncomp, const double* E, void* y);
+
+/* Device-side wall time of the last fh_matvec / fh_cg_solve on this plan, in ms
+ * (HIP events on the plan stream). */
+int fh_last_timing(const fh_plan* plan, double* matvec_ms, double* solve_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FFTHOM_H */

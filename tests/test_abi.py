@@ -1,0 +1,66 @@
+"""CPU checks of the C ABI: the library loads, exports every symbol include/ffthom.h
+declares, and fails cleanly (error code + message, no crash) without a GPU."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from ffthompy_amd import _native as nat
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, 'include', 'ffthom.h')
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r'\b(fh_[a-z0-9_]+)\s*\(', text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = nat.load_library()
+    names = declared_symbols()
+    assert 'fh_cg_solve' in names and 'fh_matvec' in names
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_binding_covers_header():
+    names = set(declared_symbols()) - {'fh_last_error'}
+    assert names == set(nat.SIGNATURES), names ^ set(nat.SIGNATURES)
+
+
+def _have_gpu():
+    try:
+        nat.init()
+        return True
+    except nat.FhError:
+        return False
+
+
+@pytest.mark.skipif(_have_gpu(), reason='checks the no-GPU error path')
+def test_no_device_is_a_clean_error():
+    with pytest.raises(nat.FhError) as ei:
+        nat.init()
+    assert ei.value.code in (nat.FH_ERR_NODEV, nat.FH_ERR_UNSUPPORTED, nat.FH_ERR_HIP)
+    with pytest.raises(nat.FhError):
+        nat.Plan((5, 5, 5), (1., 1., 1.))
+    lib = nat.load_library()
+    h = ctypes.c_void_p()
+    N = (ctypes.c_int * 3)(5, 5, 5)
+    Y = (ctypes.c_double * 3)(1., 1., 1.)
+    assert lib.fh_plan_create(3, N, Y, 0, 0, ctypes.byref(h)) != 0
+    assert len(lib.fh_last_error()) > 0
+
+
+def test_argument_validation_without_device():
+    lib = nat.load_library()
+    h = ctypes.c_void_p()
+    N = (ctypes.c_int * 3)(5, 5, 5)
+    Y = (ctypes.c_double * 3)(1., 1., 1.)
+    assert lib.fh_plan_create(4, N, Y, 0, 0, ctypes.byref(h)) == nat.FH_ERR_ARG
+    assert lib.fh_plan_create(3, N, Y, 7, 0, ctypes.byref(h)) == nat.FH_ERR_ARG
+    Yb = (ctypes.c_double * 3)(1., -1., 1.)
+    assert lib.fh_plan_create(3, N, Yb, 0, 0, ctypes.byref(h)) == nat.FH_ERR_ARG
+    assert lib.fh_matvec(None, 0, None, None) == nat.FH_ERR_ARG
+    assert lib.fh_cg_solve(None, 0, None, None, 1e-6, 10, 0.1, None, None, None, 0, None) == nat.FH_ERR_ARG

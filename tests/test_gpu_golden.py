@@ -1,0 +1,70 @@
+"""End-to-end GaNi solves of the reference's example problems on the GPU.
+
+The coefficient comes from the oracle's material evaluation; the projection, FFTs,
+CG and energies run on the GPU. Homogenized matrices are compared with the golden
+results (tests/golden/reference_results.json) at the reference's 1e-9 absolute and
+the CG iteration counts with the goldens exactly.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import examples
+from oracle.homogenize import Problem as OracleProblem
+
+pytestmark = pytest.mark.gpu
+nat = pytest.importorskip('ffthompy_amd._native')
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), 'golden', 'reference_results.json')))
+GANI = [c for c in examples.problems() if c[2]['solve']['kind'] == 'GaNi']
+
+
+def gpu_gani(conf, mats, mode='fp64'):
+    conf = dict(conf)
+    conf['postprocess'] = [{'kind': 'GaNi'}]
+    ref = OracleProblem(dict(conf), mats())
+    ref.calculate()
+    N = tuple(int(n) for n in conf['solve']['N'])
+    out = {}
+    for pd in conf['solve']['primaldual']:
+        A = ref.output['A_' + pd].val
+        physics = conf['physics']
+        plan = nat.Plan(N, ref.Y, physics, pd)
+        nc = plan.ncomp
+        plan.set_coeff('full', A.reshape((nc * nc,) + N))
+        sols, kits = [], []
+        for iL in range(nc):
+            E = np.zeros(nc)
+            E[iL] = 1.
+            EN = nat.DeviceArray.zeros((nc,) + N)
+            nat.add_constant(EN, -E)
+            B = nat.DeviceArray((nc,) + N)
+            plan.matvec(EN, B)
+            x = nat.DeviceArray.zeros((nc,) + N)
+            info = plan.cg_solve(B, x, tol=conf['solver']['tol'], maxiter=1000, mode=mode)
+            kits.append(info['kit'])
+            nat.add_constant(x, E)
+            sols.append(x)
+        AH = plan.energy_matrix(sols)
+        out[pd] = (AH if pd == 'primal' else np.linalg.inv(AH), kits)
+    return ref, out
+
+
+@pytest.mark.parametrize('case', GANI, ids=['%s_%s' % (c[0], c[1]) for c in GANI])
+def test_gani_golden_on_gpu(case):
+    f, name, conf, mats = case
+    print('case golden', f, name, flush=True)
+    gold = GOLD['problems']['%s_%s' % (f, name)]
+    ref, out = gpu_gani(conf, mats)
+    for pd, (AH, kits) in out.items():
+        assert kits == gold['kit'][pd], (pd, kits)
+        key = 'AH_GaNi_' + pd
+        if 'diag' in gold and key in gold['diag']:
+            assert np.abs(AH - gold['diag'][key] * np.eye(AH.shape[0])).max() < 1e-9
+        if 'entries' in gold and key in gold['entries']:
+            for ij, v in gold['entries'][key].items():
+                i, j = map(int, ij.split(','))
+                assert abs(AH[i, j] - v) < 1e-9
+        assert np.abs(AH - ref.output['mat_' + pd][key]).max() < 1e-12

@@ -1,0 +1,180 @@
+"""GPU parity of the HIP path against the CPU oracle, through the C ABI.
+
+Tolerances (float64 unless stated): FFT and operator results agree with NumPy /
+the oracle to 1e-12 relative to the largest entry; float32 kernels to 2e-5
+relative; CG iteration counts exactly; homogenized matrices to the reference's own
+1e-9 absolute (run_unittests.py:64).
+"""
+import numpy as np
+import pytest
+
+from oracle import projections
+from oracle.fields import DFT, Field, Operator, rfftn as o_rfftn, irfftn as o_irfftn
+from oracle.homogenize import assembly_matrix
+from oracle.solver import linear_solver as o_linear_solver
+
+pytestmark = pytest.mark.gpu
+
+nat = pytest.importorskip('ffthompy_amd._native')
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+FFT_SHAPES = [(5, 5), (9, 15), (4, 6), (26, 21), (45, 45), (5, 5, 5), (9, 7, 11), (4, 6, 8),
+              (13, 10, 9), (15, 15, 15), (21, 33, 25), (63, 45, 27), (99, 81, 105)]
+
+
+@pytest.mark.parametrize('N', FFT_SHAPES, ids=[str(s) for s in FFT_SHAPES])
+def test_rfftn_irfftn_match_numpy(N):
+    print('case rfftn', N, flush=True)
+    rng = np.random.default_rng(10)
+    nc = 2
+    plan = nat.Plan(N, np.ones(len(N)))
+    x = rng.standard_normal((nc,) + N)
+    Nh = N[:-1] + (N[-1] // 2 + 1,)
+    ref = o_rfftn(x, N)
+    for dt, ct, tol in ((np.float64, np.complex128, 1e-12), (np.float32, np.complex64, 2e-5)):
+        dx = nat.DeviceArray.from_host(x.astype(dt))
+        dX = nat.DeviceArray((nc,) + Nh, ct)
+        plan.rfftn(dx, dX, nc)
+        X = dX.to_host()
+        assert rel(X, ref) < tol, (dt, rel(X, ref))
+        # inverse on an arbitrary (non-Hermitian) spectrum, as numpy.fft.irfftn defines it
+        Z = (rng.standard_normal((nc,) + Nh) + 1j * rng.standard_normal((nc,) + Nh)).astype(ct)
+        dZ = nat.DeviceArray.from_host(Z)
+        dz = nat.DeviceArray((nc,) + N, dt)
+        plan.irfftn(dZ, dz, nc)
+        zref = o_irfftn(Z.astype(np.complex128), N)
+        assert rel(dz.to_host(), zref) < tol, (dt, rel(dz.to_host(), zref))
+
+
+def _projections(physics, N, Y):
+    if physics == 'scalar':
+        _, G1, G2 = projections.scalar(N, Y)
+        return G1.val, G2.val
+    _, G1h, G1s, G2h, G2s = projections.elasticity(N, Y)
+    return G1h.val + G1s.val, G2h.val + G2s.val
+
+
+PROJ_CASES = [('scalar', (5, 7)), ('scalar', (6, 8)), ('scalar', (5, 7, 9)), ('scalar', (4, 6, 5)),
+              ('elasticity', (5, 7)), ('elasticity', (5, 7, 9)), ('elasticity', (9, 5, 7))]
+
+
+@pytest.mark.parametrize('physics,N', PROJ_CASES, ids=['%s-%s' % c for c in PROJ_CASES])
+@pytest.mark.parametrize('pd', ['primal', 'dual'])
+def test_projection_matches_materialised(physics, N, pd):
+    print('case projection', physics, N, pd, flush=True)
+    rng = np.random.default_rng(11)
+    Y = np.linspace(1., 2., len(N))
+    G1, G2 = _projections(physics, N, Y)
+    G = G1 if pd == 'primal' else G2
+    plan = nat.Plan(N, Y, physics, pd)
+    nc = plan.ncomp
+    Nh = N[:-1] + (N[-1] // 2 + 1,)
+    X = rng.standard_normal((nc,) + Nh) + 1j * rng.standard_normal((nc,) + Nh)
+    ref = np.einsum('ij...,j...->i...', G, X)
+    dX = nat.DeviceArray.from_host(X)
+    dY = nat.DeviceArray((nc,) + Nh, np.complex128)
+    plan.apply_projection_hat(dX, dY)
+    assert rel(dY.to_host(), ref) < 1e-13
+
+
+def _random_coef(rng, kind, nc, N, physics):
+    n = int(np.prod(N))
+    if kind == 'iso':
+        a = 1. + 10. * rng.random(N)
+        A = np.einsum('ij,...->ij...', np.eye(nc), a)
+        return a, A
+    if kind == 'lame':
+        lam = 1. + rng.random(N)
+        mu = 0.5 + rng.random(N)
+        m = np.zeros(nc)
+        m[:3 if nc == 6 else 2] = 1.
+        A = np.einsum('i,j,...->ij...', m, m, lam) + 2 * np.einsum('ij,...->ij...', np.eye(nc), mu)
+        return np.stack([lam, mu]), A
+    M = rng.standard_normal((nc, nc) + N) * 0.3
+    A = np.einsum('ik...,jk...->ij...', M, M) + 2. * np.eye(nc).reshape((nc, nc) + (1,) * len(N))
+    return A.reshape((nc * nc,) + N), A
+
+
+MV_CASES = [('scalar', 'iso', (5, 5)), ('scalar', 'full', (9, 15)), ('scalar', 'iso', (5, 5, 5)),
+            ('scalar', 'full', (9, 7, 11)), ('scalar', 'iso', (4, 6, 8)), ('elasticity', 'lame', (5, 7, 9)),
+            ('elasticity', 'full', (5, 5, 5)), ('elasticity', 'lame', (7, 9)), ('scalar', 'iso', (45, 45, 45))]
+
+
+@pytest.mark.parametrize('physics,kind,N', MV_CASES, ids=['%s-%s-%s' % c for c in MV_CASES])
+@pytest.mark.parametrize('pd', ['primal', 'dual'])
+def test_matvec_matches_oracle_operator(physics, kind, N, pd):
+    print('case matvec', physics, kind, N, pd, flush=True)
+    rng = np.random.default_rng(12)
+    Y = np.linspace(1., 1.5, len(N))
+    G1, G2 = _projections(physics, N, Y)
+    plan = nat.Plan(N, Y, physics, pd)
+    nc = plan.ncomp
+    vals, A = _random_coef(rng, kind, nc, N, physics)
+    plan.set_coeff(kind, vals)
+    hG = Field(val=G1 if pd == 'primal' else G2, order=2, N=N, multype=21, Fourier=True)
+    Afun = Operator([[DFT(N, inverse=True), hG, DFT(N), Field(val=A, order=2, multype=21)]])
+    x = rng.standard_normal((nc,) + N)
+    ref = Afun(Field(val=x, order=1)).val
+    for dt, tol in ((np.float64, 1e-12), (np.float32, 2e-5)):
+        dx = nat.DeviceArray.from_host(x.astype(dt))
+        dy = nat.DeviceArray((nc,) + N, dt)
+        plan.matvec(dx, dy)
+        assert rel(dy.to_host(), ref) < tol, (dt, rel(dy.to_host(), ref))
+    # bare projection
+    refp = Operator([[DFT(N, inverse=True), hG, DFT(N)]])(Field(val=x, order=1)).val
+    dx = nat.DeviceArray.from_host(x)
+    dy = nat.DeviceArray((nc,) + N)
+    plan.project(dx, dy)
+    assert rel(dy.to_host(), refp) < 1e-12
+
+
+def test_energy_matrix_matches_assembly():
+    print('case energy', flush=True)
+    rng = np.random.default_rng(13)
+    N = (9, 7, 11)
+    for physics, kind in (('scalar', 'iso'), ('scalar', 'full'), ('elasticity', 'lame'), ('elasticity', 'full')):
+        plan = nat.Plan(N, np.ones(3), physics)
+        nc = plan.ncomp
+        vals, A = _random_coef(rng, kind, nc, N, physics)
+        plan.set_coeff(kind, vals)
+        sols = [rng.standard_normal((nc,) + N) for _ in range(nc)]
+        ref = assembly_matrix(Field(val=A, order=2, multype=21), [Field(val=s, order=1) for s in sols])
+        AH = plan.energy_matrix([nat.DeviceArray.from_host(s) for s in sols])
+        assert rel(AH, ref) < 1e-13, (physics, kind)
+
+
+def _oracle_solve(N, Y, physics, pd, A, E, tol):
+    G1, G2 = _projections(physics, N, Y)
+    hG = Field(val=G1 if pd == 'primal' else G2, order=2, N=N, multype=21, Fourier=True)
+    Afun = Operator([[DFT(N, inverse=True), hG, DFT(N), Field(val=A, order=2, multype=21)]])
+    EN = Field(shape=(len(E),), N=N)
+    EN.set_mean(E)
+    B = Afun(-EN)
+    x, info = o_linear_solver(Afun, B, x0=B.zeros_like(), par={'tol': tol, 'maxiter': 1000}, solver='CG')
+    return B.val, x.val, info
+
+
+@pytest.mark.parametrize('physics,kind,N', [('scalar', 'iso', (9, 9, 9)), ('scalar', 'full', (15, 11)),
+                                            ('elasticity', 'lame', (7, 9, 5))])
+def test_cg_fp64_matches_oracle(physics, kind, N):
+    print('case cg', physics, kind, N, flush=True)
+    rng = np.random.default_rng(14)
+    Y = np.ones(len(N))
+    plan = nat.Plan(N, Y, physics, 'primal')
+    nc = plan.ncomp
+    vals, A = _random_coef(rng, kind, nc, N, physics)
+    plan.set_coeff(kind, vals)
+    E = np.zeros(nc)
+    E[0] = 1.
+    B, xref, info = _oracle_solve(N, Y, physics, 'primal', A, E, 1e-10)
+    dB = nat.DeviceArray.from_host(B)
+    dx = nat.DeviceArray.zeros((nc,) + N)
+    out = plan.cg_solve(dB, dx, tol=1e-10, maxiter=1000, hist_len=1001)
+    assert out['status'] == nat.FH_CG_OK
+    assert out['kit'] == info['kit'], (out['kit'], info['kit'])
+    assert np.abs(out['res_log'] - np.array(info['norm_res_log'])).max() < 1e-9 * info['norm_res_log'][0]
+    assert rel(dx.to_host(), xref) < 1e-8
