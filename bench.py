@@ -1,0 +1,208 @@
+#!/usr/bin/env python3
+"""Benchmark of the GaNi homogenization solve on one MI355X.
+
+Workload: 3-D scalar conductivity on a 243^3 grid, 30 non-overlapping spheres
+(seeded centres, radii U[0.08, 0.15]) of conductivity 100 in a matrix of 1, unit
+loads e1, e2, e3. One step = the complete primal + dual homogenization: 6 CG solves
+to the reference's absolute tolerance 1e-8 (grid-mean residual norm) plus the
+energy forms that give the upper (primal) and lower (dual) bound matrices.
+The headline runs the fp32 CG with fp64 reliable updates; the plain fp64 solve is
+timed beside it. Inputs are resident in HBM before the timed region.
+
+Prints one JSON line (rank 0). Progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = 'time-to-tol 1e-8 (s) + matvec HBM GB/s, 3D scalar 243^3; FP32-refined vs FP64'
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print('[bench]', *a, file=sys.stderr, flush=True)
+
+
+class Homogenizer:
+    """Primal and dual plans plus resident fields for the 3 unit loads."""
+
+    def __init__(self, nat, a, Y=(1., 1., 1.)):
+        self.nat = nat
+        N = a.shape
+        self.N = N
+        self.plans = {}
+        for pd, coef in (('primal', a), ('dual', 1. / a)):
+            pl = nat.Plan(N, Y, 'scalar', pd)
+            pl.set_coeff('iso', coef)
+            self.plans[pd] = pl
+        self.B = nat.DeviceArray((3,) + N)
+        self.EN = nat.DeviceArray((3,) + N)
+        self.sols = {pd: [nat.DeviceArray((3,) + N) for _ in range(3)] for pd in ('primal', 'dual')}
+
+    def step(self, mode, tol=1e-8, maxiter=2000):
+        nat = self.nat
+        out = {'kit': {}, 'solve_ms': {}, 'updates': {}}
+        for pd, pl in self.plans.items():
+            kits, times, ups = [], [], []
+            for iL in range(3):
+                E = np.zeros(3)
+                E[iL] = 1.
+                nat.call('fh_memset_zero', self.EN.ptr, self.EN.nbytes)
+                nat.add_constant(self.EN, -E)
+                pl.matvec(self.EN, self.B)
+                x = self.sols[pd][iL]
+                nat.call('fh_memset_zero', x.ptr, x.nbytes)
+                info = pl.cg_solve(self.B, x, tol=tol, maxiter=maxiter, mode=mode, delta=0.01)
+                if info['status'] != nat.FH_CG_OK:
+                    raise RuntimeError('CG breakdown')
+                kits.append(info['kit'])
+                times.append(pl.last_timing()[1])
+                ups.append(pl.last_updates())
+                nat.add_constant(x, E)
+            AH = pl.energy_matrix(self.sols[pd])
+            out['AH_' + pd] = AH if pd == 'primal' else np.linalg.inv(AH)
+            out['kit'][pd] = kits
+            out['solve_ms'][pd] = times
+            out['updates'][pd] = ups
+        return out
+
+
+def timed(fn, steps, warmup, sync):
+    for _ in range(warmup):
+        fn()
+    sync()
+    t0 = time.perf_counter()
+    res = None
+    for _ in range(steps):
+        res = fn()
+    sync()
+    return (time.perf_counter() - t0) / steps, res
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=3)
+    ap.add_argument('--warmup', type=int, default=1)
+    ap.add_argument('--n', type=int, default=243, help='grid points per axis')
+    ap.add_argument('--fp64-steps', type=int, default=None, help='timed fp64 steps (default: --steps)')
+    ap.add_argument('--cpu-iters', type=int, default=2, help='CG iterations in the CPU baseline sample')
+    ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
+    ap.add_argument('--profile-only', action='store_true', help='matvec profile only (for rocprofv3 runs)')
+    args = ap.parse_args()
+    world = int(os.environ.get('WORLD_SIZE', '1'))
+    if args.gpus != 1 or world != 1:
+        print('bench.py runs on one GPU only: this build has no multi-GPU path '
+              '(--gpus %d, WORLD_SIZE %d requested)' % (args.gpus, world), file=sys.stderr)
+        sys.exit(2)
+
+    from ffthompy_amd import _native as nat
+    from ffthompy_amd import microstructure as ms
+    dev = nat.init()
+    N = (args.n,) * 3
+    n = int(np.prod(N))
+    nh = args.n * args.n * (args.n // 2 + 1)
+    log('device', dev, 'grid', N)
+    t = time.perf_counter()
+    a, centres, radii = ms.spheres_conductivity(N, count=30, contrast=100., seed=ms.BASE_SEED)
+    log('microstructure: %d spheres, volume fraction %.4f (%.1fs)' % (len(radii), (a > 1).mean(),
+                                                                        time.perf_counter() - t))
+    H = Homogenizer(nat, a)
+
+    # --- matvec profile (per kernel, HIP events on the plan stream) ---
+    pl = H.plans['primal']
+    prof = {}
+    for dt, name in ((np.float32, 'fp32'), (np.float64, 'fp64')):
+        x = nat.DeviceArray.from_host(np.random.default_rng(1).standard_normal((3,) + N).astype(dt))
+        y = nat.DeviceArray((3,) + N, dt)
+        pl.matvec(x, y)
+        ms_k, by_k = pl.matvec_profile(x, y, nrep=10)
+        sr = 4 if dt == np.float32 else 8
+        model = n * sr * (2 * 3 + 1) + 8 * 3 * nh * 2 * sr
+        prof[name] = {'kernel_ms': ms_k.tolist(), 'kernel_bytes': by_k.tolist(),
+                      'matvec_ms': float(ms_k.sum()), 'model_bytes': model,
+                      'matvec_GBps': model / (ms_k.sum() * 1e-3) / 1e9}
+        log('matvec %s: %.3f ms total, kernels %s ms, %.0f GB/s (model bytes)' % (
+            name, ms_k.sum(), np.round(ms_k, 4).tolist(), prof[name]['matvec_GBps']))
+        del x, y
+    if args.profile_only:
+        return
+    k32 = prof['fp32']
+    dom = int(np.argmax(k32['kernel_ms']))
+    names = ['rows_fwd(A, R2C axis 2)', 'cols axis 1 fwd', 'cols axis 0 fwd+G+inv', 'cols axis 1 inv',
+             'rows_inv(C2R axis 2)']
+    ach = k32['kernel_bytes'][dom] / (k32['kernel_ms'][dom] * 1e-3) / 1e9
+
+    # --- timed steps ---
+    sync = nat.synchronize
+    log('mixed-precision steps: warmup %d, timed %d' % (args.warmup, args.steps))
+    t_mixed, r_mixed = timed(lambda: H.step('mixed'), args.steps, args.warmup, sync)
+    log('mixed step %.3f s, kit %s' % (t_mixed, r_mixed['kit']))
+    fs = args.steps if args.fp64_steps is None else args.fp64_steps
+    t_fp64, r_fp64 = timed(lambda: H.step('fp64'), fs, 1 if args.warmup > 0 else 0, sync)
+    log('fp64 step %.3f s, kit %s' % (t_fp64, r_fp64['kit']))
+    relp = float(np.abs(r_mixed['AH_primal'] - r_fp64['AH_primal']).max() / np.abs(r_fp64['AH_primal']).max())
+    reld = float(np.abs(r_mixed['AH_dual'] - r_fp64['AH_dual']).max() / np.abs(r_fp64['AH_dual']).max())
+    kit64 = sum(sum(v) for v in r_fp64['kit'].values())
+
+    # --- CPU baseline: reference-style NumPy path on this host, bounded sample ---
+    cpu = None
+    if not args.no_cpu and args.cpu_iters > 0:
+        from oracle.baseline import time_cg_iterations
+        log('cpu baseline: %d reference-style CG iterations at %s (fp64, CallBack on)' % (args.cpu_iters, N))
+        s_it = time_cg_iterations(a, np.ones(3), iters=args.cpu_iters, callback=True)
+        cpu = {'value': s_it * kit64, 'unit': 's', 'cores': 1, 'kind': 'port',
+               'sample': ('%d CG iterations of the oracle (reference-style NumPy: materialised hG1 and '
+                          'full A via einsum, numpy.fft, allocating CG, CallBack matvec) on the 243^3 '
+                          'load e1: %.2f s/iteration, extrapolated to the %d fp64 CG iterations of one '
+                          'step; os.cpu_count()=%s, affinity=%d, numpy.fft single-threaded'
+                          % (args.cpu_iters, s_it, kit64, os.cpu_count(), len(os.sched_getaffinity(0)))),
+               'sec_per_iter': s_it}
+        log('cpu baseline: %.2f s/iter -> %.0f s per step (extrapolated)' % (s_it, cpu['value']))
+
+    out = {
+        'metric': METRIC,
+        'value': round(t_mixed, 4),
+        'unit': 's',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': round(t_mixed * 1e3, 2),
+        'higher_is_better': False,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f32+f64',
+        'data': 'synthetic (seeded spheres, seed %d)' % ms.BASE_SEED,
+        'config': {'workload': 'GaNi 3-D scalar %d^3, 30 spheres contrast 100, loads e1-e3, primal+dual, '
+                               'CG tol 1e-8' % args.n,
+                   'grid': list(N), 'volume_fraction': round(float((a > 1).mean()), 4),
+                   'solve': 'fp32 CG + fp64 reliable updates (delta 0.01) + fp64 final projection'},
+        'fp64_time_s': round(t_fp64, 4),
+        'speedup_mixed_vs_fp64': round(t_fp64 / t_mixed, 3),
+        'kit_mixed': r_mixed['kit'], 'kit_fp64': r_fp64['kit'],
+        'reliable_updates': r_mixed['updates'],
+        'solve_ms_mixed': r_mixed['solve_ms'], 'solve_ms_fp64': r_fp64['solve_ms'],
+        'AH_upper_fp64': float(r_fp64['AH_primal'][0, 0]), 'AH_lower_fp64': float(r_fp64['AH_dual'][0, 0]),
+        'bounds_rel_diff_mixed_vs_fp64': [relp, reld],
+        'matvec_GBps_fp32': round(prof['fp32']['matvec_GBps'], 1),
+        'matvec_GBps_fp64': round(prof['fp64']['matvec_GBps'], 1),
+        'matvec_ms_fp32': round(prof['fp32']['matvec_ms'], 4),
+        'matvec_ms_fp64': round(prof['fp64']['matvec_ms'], 4),
+        'matvec_kernels_fp32': {'names': names, 'ms': [round(v, 4) for v in k32['kernel_ms']],
+                                'bytes': k32['kernel_bytes']},
+        'roofline': {'bound': 'hbm', 'kernel': names[dom], 'achieved': round(ach, 1), 'peak': HBM_PEAK_GBS,
+                     'unit': 'GB/s', 'frac': round(ach / HBM_PEAK_GBS, 4), 'traffic': None},
+        'cpu_baseline': cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == '__main__':
+    main()

@@ -49,6 +49,10 @@ SIGNATURES = {
     'fh_dot': [ctypes.c_int, ctypes.c_size_t, _vp, _vp, _c_double_p],
     'fh_add_constant': [ctypes.c_int, ctypes.c_size_t, ctypes.c_int, _c_double_p, _vp],
     'fh_last_timing': [_vp, _c_double_p, _c_double_p],
+    'fh_last_solve_updates': [_vp, _c_int_p],
+    'fh_matvec_profile': [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int, _c_double_p, _c_double_p,
+                          _c_int_p],
+    'fh_synchronize': [],
 }
 
 
@@ -193,6 +197,10 @@ class DeviceArray:
         return 'DeviceArray(shape=%s, dtype=%s)' % (self.shape, self.dtype)
 
 
+def synchronize():
+    call('fh_synchronize')
+
+
 def _ptr(x):
     return x.ptr if isinstance(x, DeviceArray) else ctypes.c_void_p(int(x))
 
@@ -304,6 +312,20 @@ class Plan:
         sv = ctypes.c_double()
         call('fh_last_timing', self._h, ctypes.byref(mv), ctypes.byref(sv))
         return mv.value, sv.value
+
+    def last_updates(self):
+        n = ctypes.c_int()
+        call('fh_last_solve_updates', self._h, ctypes.byref(n))
+        return n.value
+
+    def matvec_profile(self, x, y, nrep=10):
+        """Average ms and algorithmic bytes per kernel of one matvec (HIP events)."""
+        ms = np.zeros(8)
+        by = np.zeros(8)
+        nk = ctypes.c_int()
+        call('fh_matvec_profile', self._h, dtype_code(x.dtype), x.ptr, y.ptr, ctypes.c_int(nrep), ctypes.c_int(8),
+             ms.ctypes.data_as(_c_double_p), by.ctypes.data_as(_c_double_p), ctypes.byref(nk))
+        return ms[:nk.value].copy(), by[:nk.value].copy()
 
     def __del__(self):
         h = getattr(self, '_h', None)

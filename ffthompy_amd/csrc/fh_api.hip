@@ -77,7 +77,7 @@ int ensure_device() {
     g_init_msg = std::string("hipSetDevice: ") + hipGetErrorString(e);
     return fail(g_init_state, g_init_msg);
   }
-  g_dev_name = prop.name;
+  g_dev_name = std::string(prop.name[0] ? prop.name : "AMD GPU") + " (" + prop.gcnArchName + ")";
   g_init_state = FH_OK;
   return FH_OK;
 }
@@ -930,6 +930,92 @@ int fh_add_constant(int dtype, size_t nvox, int ncomp, const double* E, void* y)
   }
   (void)hipFree(dE);
   if (e != hipSuccess) return fail(FH_ERR_HIP, std::string("fh_add_constant: ") + hipGetErrorString(e));
+  return FH_OK;
+}
+
+int fh_last_solve_updates(const fh_plan* pl, int* n_updates) {
+  if (!pl || !n_updates) return fail(FH_ERR_ARG, "null argument");
+  *n_updates = pl->last_updates;
+  return FH_OK;
+}
+
+int fh_synchronize(void) {
+  RETCHK(ensure_device());
+  HIPCHK(hipDeviceSynchronize());
+  return FH_OK;
+}
+
+int fh_matvec_profile(fh_plan* pl, int dtype, const void* x, void* y, int nrep, int max_kernels,
+                      double* kernel_ms, double* kernel_bytes, int* nkernels) {
+  RETCHK(check_plan(pl));
+  RETCHK(check_dtype(dtype));
+  if (!x || !y || !kernel_ms || !kernel_bytes || !nkernels || nrep < 1) return fail(FH_ERR_ARG, "bad arguments");
+  if (!pl->coef_kind) return fail(FH_ERR_STATE, "fh_set_coeff has not been called");
+  const int nk = pl->n0 > 1 ? 5 : 3;
+  if (max_kernels < nk) return fail(FH_ERR_ARG, "max_kernels too small");
+  std::vector<hipEvent_t> ev(nk + 1);
+  for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+  std::vector<double> acc(nk, 0.0);
+  const int nc = pl->ncomp;
+  const size_t sr = dtype == FH_F64 ? 8 : 4, sc = 2 * sr;
+  const double cvals = pl->coef_kind == FH_COEF_ISO ? 1 : (pl->coef_kind == FH_COEF_LAME ? 2 : nc * nc);
+  const double nb_real = (double)nc * pl->n * sr, nb_spec = (double)nc * pl->nh * sc;
+  int rc = FH_OK;
+  for (int it = 0; it < nrep && rc == FH_OK; ++it) {
+    auto step = [&](int k, auto&& launch) {
+      if (rc != FH_OK) return;
+      if (hipEventRecord(ev[k], pl->stream) != hipSuccess) rc = fail(FH_ERR_HIP, "hipEventRecord");
+      if (rc == FH_OK) rc = launch();
+    };
+    if (dtype == FH_F64) {
+      using T = double;
+      cpx<T>* spec = reinterpret_cast<cpx<T>*>(pl->spec);
+      const GParams g = make_gparams(pl, 1.0 / (double)pl->n);
+      int k = 0;
+      step(k++, [&] { return launch_rows_fwd<T>(pl, nc, (const T*)x, pl->coef64, pl->coef_kind, spec, nullptr); });
+      if (pl->n0 > 1) {
+        step(k++, [&] { return launch_cols<T>(pl, 0, 1, nc, spec, g, nullptr); });
+        step(k++, [&] { return launch_cols<T>(pl, 2, 0, nc, spec, g, nullptr); });
+        step(k++, [&] { return launch_cols<T>(pl, 1, 1, nc, spec, g, nullptr); });
+      } else {
+        step(k++, [&] { return launch_cols<T>(pl, 2, 1, nc, spec, g, nullptr); });
+      }
+      step(k++, [&] { return launch_rows_inv<T>(pl, nc, spec, (T*)y, T(1), nullptr, nullptr, nullptr, nullptr); });
+    } else {
+      using T = float;
+      cpx<T>* spec = reinterpret_cast<cpx<T>*>(pl->spec);
+      const GParams g = make_gparams(pl, 1.0 / (double)pl->n);
+      int k = 0;
+      step(k++, [&] { return launch_rows_fwd<T>(pl, nc, (const T*)x, pl->coef32, pl->coef_kind, spec, nullptr); });
+      if (pl->n0 > 1) {
+        step(k++, [&] { return launch_cols<T>(pl, 0, 1, nc, spec, g, nullptr); });
+        step(k++, [&] { return launch_cols<T>(pl, 2, 0, nc, spec, g, nullptr); });
+        step(k++, [&] { return launch_cols<T>(pl, 1, 1, nc, spec, g, nullptr); });
+      } else {
+        step(k++, [&] { return launch_cols<T>(pl, 2, 1, nc, spec, g, nullptr); });
+      }
+      step(k++, [&] { return launch_rows_inv<T>(pl, nc, spec, (T*)y, T(1), nullptr, nullptr, nullptr, nullptr); });
+    }
+    if (rc == FH_OK && hipEventRecord(ev[nk], pl->stream) != hipSuccess) rc = fail(FH_ERR_HIP, "hipEventRecord");
+    if (rc == FH_OK && hipEventSynchronize(ev[nk]) != hipSuccess) rc = fail(FH_ERR_HIP, "hipEventSynchronize");
+    for (int k = 0; k < nk && rc == FH_OK; ++k) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, ev[k], ev[k + 1]) != hipSuccess) rc = fail(FH_ERR_HIP, "hipEventElapsedTime");
+      acc[k] += ms;
+    }
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  if (rc != FH_OK) return rc;
+  for (int k = 0; k < nk; ++k) {
+    kernel_ms[k] = acc[k] / nrep;
+    if (k == 0)
+      kernel_bytes[k] = nb_real + cvals * pl->n * sr + nb_spec;
+    else if (k == nk - 1)
+      kernel_bytes[k] = nb_spec + nb_real;
+    else
+      kernel_bytes[k] = 2.0 * nb_spec;
+  }
+  *nkernels = nk;
   return FH_OK;
 }
 

@@ -131,8 +131,22 @@ int fh_dot(int dtype, size_t n, const void* x, const void* y, double* out);
 int fh_add_constant(int dtype, size_t nvox, int ncomp, const double* E, void* y);
 
 /* Device-side wall time of the last fh_matvec / fh_cg_solve on this plan, in ms
- * (HIP events on the plan stream). */
+ * (HIP events on the plan stream), and the number of fp64 residual recomputations
+ * of the last mixed-precision solve. Replaces the reference's Timer around a solve
+ * (ffthompy/general/base.py:63-81, ffthompy/general/solver.py:13,58-59). */
 int fh_last_timing(const fh_plan* plan, double* matvec_ms, double* solve_ms);
+int fh_last_solve_updates(const fh_plan* plan, int* n_updates);
+
+/* Timing harness for the matvec: runs fh_matvec nrep times and returns, per kernel
+ * of the pass sequence (rows forward, strided passes, rows inverse), the average
+ * duration in ms measured with HIP events on the plan stream, and the algorithmic
+ * HBM bytes each kernel moves per launch. At most max_kernels entries; *nkernels
+ * receives the count. No reference counterpart (measurement only). */
+int fh_matvec_profile(fh_plan* plan, int dtype, const void* x, void* y, int nrep, int max_kernels,
+                      double* kernel_ms, double* kernel_bytes, int* nkernels);
+
+/* Wait for all GPU work of this process. */
+int fh_synchronize(void);
 
 #ifdef __cplusplus
 }
