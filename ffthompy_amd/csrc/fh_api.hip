@@ -12,6 +12,7 @@
 
 #include "../../include/ffthom.h"
 #include "fh_kernels.hpp"
+#include "fh_launch.hpp"
 
 using fh::AxisPlan;
 using fh::CGState;
@@ -150,6 +151,7 @@ struct fh_plan {
   size_t work_bytes = 0;
   double last_matvec_ms = 0, last_solve_ms = 0;
   int last_updates = 0;
+  int generic = 0;               // 1: never use the fixed-length kernels
 };
 
 namespace {
@@ -179,6 +181,25 @@ GParams make_gparams(const fh_plan* pl, double scale, int ncomp_override = 0) {
 template <typename T>
 const cpx<T>* twiddles(const fh_plan* pl, int axis) {
   return reinterpret_cast<const cpx<T>*>(sizeof(T) == 8 ? pl->tw64[axis] : pl->tw32[axis]);
+}
+
+template <typename T>
+fh::LaunchCtx make_ctx(const fh_plan* pl) {
+  fh::LaunchCtx c;
+  c.stream = pl->stream;
+  c.n0 = pl->n0;
+  c.n1 = pl->n1;
+  c.n2 = pl->n2;
+  c.n2h = pl->n2h;
+  c.n = pl->n;
+  c.nh = pl->nh;
+  for (int a = 0; a < 3; ++a) c.tw[a] = sizeof(T) == 8 ? pl->tw64[a] : pl->tw32[a];
+  return c;
+}
+
+int fixed_result(int r, const char* what) {
+  if (r < 0) return fail(FH_ERR_HIP, std::string(what) + ": " + hipGetErrorString((hipError_t)(-r)));
+  return FH_OK;
 }
 
 int row_config(const AxisPlan& ap, int nc, size_t csz, long long nrows, RowCfg* c) {
@@ -221,6 +242,10 @@ int col_config(const AxisPlan& ap, int unit, size_t csz, long long inner, ColCfg
 template <typename T>
 int launch_rows_fwd(fh_plan* pl, int nc, const T* in, const T* coef, int kind, cpx<T>* out,
                     const int* halt) {
+  if (!pl->generic) {
+    const int r = fh::fixed_rows_fwd<T>(make_ctx<T>(pl), nc, in, coef, kind, out, halt);
+    if (r != 0) return fixed_result(r, "rows forward (fixed)");
+  }
   RowCfg c;
   const long long nrows = (long long)pl->n0 * pl->n1;
   RETCHK(row_config(pl->ax[2], nc, sizeof(cpx<T>), nrows, &c));
@@ -254,6 +279,11 @@ int launch_rows_fwd(fh_plan* pl, int nc, const T* in, const T* coef, int kind, c
 template <typename T>
 int launch_rows_inv(fh_plan* pl, int nc, const cpx<T>* in, T* out, T scale, const T* w,
                     double* partial, const int* halt, int* nblocks) {
+  if (!pl->generic) {
+    const int r = fh::fixed_rows_inv<T>(make_ctx<T>(pl), nc, in, out, scale, w, partial, pl->partial_cap, halt,
+                                        nblocks);
+    if (r != 0) return fixed_result(r, "rows inverse (fixed)");
+  }
   RowCfg c;
   const long long nrows = (long long)pl->n0 * pl->n1;
   RETCHK(row_config(pl->ax[2], nc, sizeof(cpx<T>), nrows, &c));
@@ -274,6 +304,11 @@ int launch_rows_inv(fh_plan* pl, int nc, const cpx<T>* in, T* out, T scale, cons
 template <typename T>
 int launch_cols(fh_plan* pl, int mode, int axis, int nc, cpx<T>* data, const GParams& g,
                 const int* halt) {
+  if (!pl->generic) {
+    const int r = mode == 2 ? fh::fixed_cols_g<T>(make_ctx<T>(pl), axis, nc, data, g, halt)
+                            : fh::fixed_cols<T>(make_ctx<T>(pl), mode == 1 ? +1 : -1, axis, nc, data, halt);
+    if (r != 0) return fixed_result(r, "strided pass (fixed)");
+  }
   const AxisPlan& ap = pl->ax[axis];
   long long outer, ostride, inner;
   if (axis == 1) {
@@ -631,7 +666,8 @@ int fh_plan_create(int dim, const int* N, const double* Y, int physics, int prim
     }
     RowCfg rc32;
     row_config(pl->ax[2], pl->ncomp, sizeof(cpx<float>), (long long)pl->n0 * pl->n1, &rc32);
-    pl->partial_cap = (size_t)std::max({rc.nblocks, rc32.nblocks, kScratchParts, 2 * 36 * 1024});
+    pl->partial_cap = (size_t)std::max<long long>({(long long)rc.nblocks, (long long)rc32.nblocks, (long long)kScratchParts,
+                                                       2LL * 36 * 1024, (long long)pl->n0 * pl->n1});
   }
   if (hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&pl->ev0) != hipSuccess || hipEventCreate(&pl->ev1) != hipSuccess)
@@ -646,6 +682,12 @@ int fh_plan_create(int dim, const int* N, const double* Y, int physics, int prim
 }
 
 int fh_plan_ncomp(const fh_plan* pl) { return pl ? pl->ncomp : -1; }
+
+int fh_plan_use_generic(fh_plan* pl, int generic) {
+  RETCHK(check_plan(pl));
+  pl->generic = generic ? 1 : 0;
+  return FH_OK;
+}
 
 int fh_plan_set_primaldual(fh_plan* pl, int primaldual) {
   RETCHK(check_plan(pl));

@@ -1,0 +1,577 @@
+// Compile-time specialised FFT passes for the grid lengths the method uses.
+//
+// For a length L with a fixed radix schedule (e.g. 243 = 9*9*3) every index is a
+// compile-time constant, loops unroll and butterflies live in registers:
+//  * TPP threads own one pencil; in stage s a thread computes ceil((L/R_s)/TPP)
+//    radix-R_s butterflies (Stockham, natural order output).
+//  * The first stage reads HBM straight into registers and the last stage writes
+//    registers straight to HBM; LDS is only used to exchange data between stages.
+//  * In the projection pass the forward FFT along the last axis ends, the projection
+//    G(xi) is applied in registers, and the inverse FFT starts with the same radix, so
+//    the thread already holds its inputs (no LDS round trip around G).
+#pragma once
+#include "fh_kernels.hpp"
+
+namespace fh {
+
+// 9-point DFT as 3x3 with internal twiddles W9^{n2 k1}.
+template <typename T, int DIR>
+__device__ __forceinline__ void dft9(cpx<T> (&v)[9]) {
+  cpx<T> a[3][3];
+#pragma unroll
+  for (int n2 = 0; n2 < 3; ++n2) {
+    cpx<T> t3[3] = {v[n2], v[n2 + 3], v[n2 + 6]};
+    dft<T, 3, DIR>(t3);
+#pragma unroll
+    for (int k1 = 0; k1 < 3; ++k1) a[n2][k1] = t3[k1];
+  }
+  // W9^m, m = 1, 2, 4: cos/sin(2 pi m / 9)
+  const T c1 = T(0.76604444311897803520), s1 = T(0.64278760968653932632);
+  const T c2 = T(0.17364817766693034885), s2 = T(0.98480775301220805936);
+  const T c4 = T(-0.93969262078590838405), s4 = T(0.34202014332566873304);
+  const T sg = DIR < 0 ? T(-1) : T(1);
+  a[1][1] = cmul(a[1][1], mk(c1, sg * s1));
+  a[1][2] = cmul(a[1][2], mk(c2, sg * s2));
+  a[2][1] = cmul(a[2][1], mk(c2, sg * s2));
+  a[2][2] = cmul(a[2][2], mk(c4, sg * s4));
+#pragma unroll
+  for (int k1 = 0; k1 < 3; ++k1) {
+    cpx<T> t3[3] = {a[0][k1], a[1][k1], a[2][k1]};
+    dft<T, 3, DIR>(t3);
+#pragma unroll
+    for (int k2 = 0; k2 < 3; ++k2) v[k1 + 3 * k2] = t3[k2];
+  }
+}
+
+template <typename T, int R, int DIR>
+__device__ __forceinline__ void dft_any(cpx<T> (&v)[R]) {
+  if constexpr (R == 9)
+    dft9<T, DIR>(v);
+  else
+    dft<T, R, DIR>(v);
+}
+
+template <int L_, int TPP_, int... Rs>
+struct FxPlan {
+  static constexpr int L = L_;
+  static constexpr int TPP = TPP_;
+  static constexpr int NST = sizeof...(Rs);
+  static constexpr int R[NST] = {Rs...};
+};
+
+template <class FP>
+constexpr int fx_pre(int s) {
+  int p = 1;
+  for (int i = 0; i < s; ++i) p *= FP::R[i];
+  return p;
+}
+
+template <typename T, class FP, int S, int DIR>
+struct FxStage {
+  static constexpr int R = FP::R[S];
+  static constexpr int p = fx_pre<FP>(S);
+  static constexpr int B = FP::L / R;
+  static constexpr int TPP = FP::TPP;
+  static constexpr int M = (B + TPP - 1) / TPP;
+  static constexpr bool EXACT = (B % TPP) == 0;
+  cpx<T> v[M][R];
+
+  __device__ __forceinline__ static bool valid(int i) { return EXACT || i < B; }
+
+  template <class Ld>
+  __device__ __forceinline__ void load(int t, Ld&& ld) {
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      const int i = t + m * TPP;
+      if (valid(i)) {
+#pragma unroll
+        for (int j = 0; j < R; ++j) v[m][j] = ld(i + j * B);
+      }
+    }
+  }
+
+  __device__ __forceinline__ void compute(int t, const cpx<T>* __restrict__ twl) {
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      const int i = t + m * TPP;
+      if (valid(i)) {
+        if constexpr (p > 1) {
+          const int k = i % p;
+          constexpr int st = FP::L / (p * R);
+#pragma unroll
+          for (int j = 1; j < R; ++j) {
+            cpx<T> w = twl[j * k * st];
+            if (DIR > 0) w.y = -w.y;
+            v[m][j] = cmul(v[m][j], w);
+          }
+        }
+        dft_any<T, R, DIR>(v[m]);
+      }
+    }
+  }
+
+  template <class St>
+  __device__ __forceinline__ void store(int t, St&& st) {
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      const int i = t + m * TPP;
+      if (valid(i)) {
+        const int k = i % p;
+        const int base = (i - k) * R + k;
+#pragma unroll
+        for (int j = 0; j < R; ++j) st(base + j * p, v[m][j]);
+      }
+    }
+  }
+};
+
+// Stages S..SEND-1, LDS to LDS, for NC pencils per thread (accessor lds(j, c)).
+// Barriers are uniform: every thread runs every stage.
+template <typename T, class FP, int DIR, int S, int SEND, int NC, class Lds>
+__device__ __forceinline__ void fx_lds_stages(int t, Lds&& lds, const cpx<T>* __restrict__ twl) {
+  if constexpr (S < SEND) {
+    FxStage<T, FP, S, DIR> st[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) st[c].load(t, [&](int j) { return lds(j, c); });
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      st[c].compute(t, twl);
+      st[c].store(t, [&](int j, cpx<T> v) { lds(j, c) = v; });
+    }
+    __syncthreads();
+    fx_lds_stages<T, FP, DIR, S + 1, SEND, NC>(t, lds, twl);
+  }
+}
+
+// Stage sizing helpers for launch shapes (host and device).
+template <class FP, typename T>
+struct FxShape {
+  // columns per strided tile and frequency points per projection tile
+  static constexpr int P = (sizeof(T) == 4) ? (FP::TPP >= 16 ? 16 : (FP::TPP >= 8 ? 32 : 64))
+                                            : (FP::TPP >= 16 ? 8 : (FP::TPP >= 8 ? 16 : 32));
+  static constexpr int F = (sizeof(T) == 4) ? (FP::TPP >= 16 ? 8 : (FP::TPP >= 8 ? 16 : 32))
+                                            : (FP::TPP >= 16 ? 4 : (FP::TPP >= 8 ? 8 : 16));
+  // rows per workgroup in the contiguous-axis passes (even)
+  static constexpr int RPB = (FP::TPP >= 16) ? 8 : (FP::TPP >= 8 ? 16 : 32);
+};
+
+// ---- strided pass: forward (DIR -1) or inverse (DIR +1) along a non-contiguous axis ----
+template <typename T, class FP, int P, int DIR>
+__global__ void __launch_bounds__(FP::TPP * P)
+k_cols_fx(cpx<T>* __restrict__ data, long long ostride, long long inner, int ntiles,
+          const cpx<T>* __restrict__ twg, const int* __restrict__ halt) {
+  if (halt && *halt) return;
+  constexpr int L = FP::L;
+  __shared__ cpx<T> s[L * P];
+  __shared__ cpx<T> twl[L];
+  const int tid = threadIdx.x;
+  const int q = tid % P, t = tid / P;
+  const int lin = xcd_remap(blockIdx.x, gridDim.x);
+  const int o = lin / ntiles, tile = lin - o * ntiles;
+  const long long col0 = (long long)tile * P;
+  const int ncol = (int)min((long long)P, inner - col0);
+  cpx<T>* base = data + (long long)o * ostride + col0;
+  for (int e = tid; e < L; e += FP::TPP * P) twl[e] = twg[e];
+  const bool act = q < ncol;
+  const cpx<T> zero = mk(T(0), T(0));
+  FxStage<T, FP, 0, DIR> s0;
+  s0.load(t, [&](int j) { return act ? base[(long long)j * inner + q] : zero; });
+  s0.compute(t, twl);
+  if constexpr (FP::NST == 1) {
+    if (act) s0.store(t, [&](int j, cpx<T> v) { base[(long long)j * inner + q] = v; });
+  } else {
+    s0.store(t, [&](int j, cpx<T> v) { s[j * P + q] = v; });
+    __syncthreads();
+    auto at = [&](int j, int) -> cpx<T>& { return s[j * P + q]; };
+    fx_lds_stages<T, FP, DIR, 1, FP::NST - 1, 1>(t, at, twl);
+    FxStage<T, FP, FP::NST - 1, DIR> sl;
+    sl.load(t, [&](int j) { return s[j * P + q]; });
+    sl.compute(t, twl);
+    if (act) sl.store(t, [&](int j, cpx<T> v) { base[(long long)j * inner + q] = v; });
+  }
+}
+
+// G(xi) on NC registers (compile-time component count; same formulas as apply_G).
+template <typename T, int NC>
+__device__ __forceinline__ void apply_G_fx(cpx<T> (&v)[NC], int k0, int k1, int k2, const GParams& g) {
+  bool nyq = false;
+  const int s0 = signed_freq(k0, g.n0, nyq);
+  const int s1 = signed_freq(k1, g.n1, nyq);
+  if ((g.n2 & 1) == 0 && 2 * k2 == g.n2) nyq = true;
+  const T x0 = T(s0 * g.iY[0]), x1 = T(s1 * g.iY[1]), x2 = T(k2 * g.iY[2]);
+  const T sc = T(g.scale);
+  if constexpr (NC == 2 || NC == 3) {
+    if (g.physics == 0) {
+      // scalar problem: NC = dim; 2-D uses the axes (1, 2) of the 3-D view
+      T xi[NC];
+      if constexpr (NC == 3) {
+        xi[0] = x0; xi[1] = x1; xi[2] = x2;
+      } else {
+        xi[0] = x1; xi[1] = x2;
+      }
+      T q = T(0);
+#pragma unroll
+      for (int c = 0; c < NC; ++c) q += xi[c] * xi[c];
+      if (nyq || q == T(0)) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) v[c] = mk(T(0), T(0));
+        return;
+      }
+      cpx<T> dot = mk(T(0), T(0));
+#pragma unroll
+      for (int c = 0; c < NC; ++c) dot = dot + xi[c] * v[c];
+      const T f = sc / q;
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        const cpx<T> w = (xi[c] * f) * dot;
+        v[c] = g.dual ? (sc * v[c] - w) : w;
+      }
+      return;
+    }
+  }
+  // elasticity (NC = 3: 2-D Mandel, NC = 6: 3-D Mandel)
+  if constexpr (NC == 3 || NC == 6) {
+    constexpr int d = NC == 6 ? 3 : 2;
+    T xi[3];
+    if constexpr (d == 3) {
+      xi[0] = x0; xi[1] = x1; xi[2] = x2;
+    } else {
+      xi[0] = x1; xi[1] = x2; xi[2] = T(0);
+    }
+    T q = T(0);
+#pragma unroll
+    for (int a = 0; a < d; ++a) q += xi[a] * xi[a];
+    if (nyq || q == T(0)) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c) v[c] = mk(T(0), T(0));
+      return;
+    }
+    const T r2 = T(1.41421356237309504880), ir2 = T(0.70710678118654752440);
+    cpx<T> S[3][3];
+    if constexpr (d == 3) {
+      S[0][0] = v[0]; S[1][1] = v[1]; S[2][2] = v[2];
+      S[1][2] = S[2][1] = ir2 * v[3];
+      S[0][2] = S[2][0] = ir2 * v[4];
+      S[0][1] = S[1][0] = ir2 * v[5];
+    } else {
+      S[0][0] = v[0]; S[1][1] = v[1];
+      S[0][1] = S[1][0] = ir2 * v[2];
+    }
+    cpx<T> tv[3];
+    cpx<T> qq = mk(T(0), T(0));
+#pragma unroll
+    for (int i = 0; i < d; ++i) {
+      tv[i] = mk(T(0), T(0));
+#pragma unroll
+      for (int j = 0; j < d; ++j) tv[i] = tv[i] + xi[j] * S[i][j];
+      qq = qq + xi[i] * tv[i];
+    }
+    const T iq = T(1) / q, iq2 = iq * iq;
+    auto gam = [&](int i, int j) -> cpx<T> {
+      return iq * (xi[i] * tv[j] + xi[j] * tv[i]) - (xi[i] * xi[j] * iq2) * qq;
+    };
+    cpx<T> o[NC];
+    if constexpr (d == 3) {
+      o[0] = gam(0, 0); o[1] = gam(1, 1); o[2] = gam(2, 2);
+      o[3] = r2 * gam(1, 2); o[4] = r2 * gam(0, 2); o[5] = r2 * gam(0, 1);
+    } else {
+      o[0] = gam(0, 0); o[1] = gam(1, 1); o[2] = r2 * gam(0, 1);
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c) v[c] = g.dual ? sc * (v[c] - o[c]) : sc * o[c];
+  }
+}
+
+// ---- projection pass: forward FFT, G(xi), inverse FFT along the last strided axis ------
+// FP: forward schedule; FPR: the same radices reversed (inverse schedule). A thread owns
+// frequency point f of the tile for all NC components.
+template <typename T, class FP, class FPR, int F, int NC>
+__global__ void __launch_bounds__(FP::TPP * F)
+k_cols_g_fx(cpx<T>* __restrict__ data, long long inner, int ntiles, const cpx<T>* __restrict__ twg,
+            GParams g, int n2h, const int* __restrict__ halt) {
+  static_assert(FP::TPP == FPR::TPP && FP::L == FPR::L, "schedules must match");
+  static_assert(FP::R[FP::NST - 1] == FPR::R[0], "inverse must start with the last forward radix");
+  if (halt && *halt) return;
+  constexpr int L = FP::L;
+  constexpr int P = F * NC;
+  __shared__ cpx<T> s[(FP::NST > 1) ? L * P : 1];
+  __shared__ cpx<T> twl[L];
+  const int tid = threadIdx.x;
+  const int f = tid % F, t = tid / F;
+  const int tile = xcd_remap(blockIdx.x, gridDim.x);
+  const long long col0 = (long long)tile * P;
+  const int ncol = (int)min((long long)P, inner - col0);
+  cpx<T>* base = data + col0;
+  for (int e = tid; e < L; e += FP::TPP * F) twl[e] = twg[e];
+  const bool act = f * NC < ncol;
+  const cpx<T> zero = mk(T(0), T(0));
+  const long long fp = col0 / NC + f;  // frequency point in the inner (plane) grid
+  int ka, kb;                          // the two fixed frequency indices of this pencil
+  if (g.n0 > 1) {
+    ka = (int)(fp / n2h);
+    kb = (int)(fp - (long long)ka * n2h);
+  } else {
+    ka = 0;
+    kb = (int)fp;
+  }
+  auto lds = [&](int j, int c) -> cpx<T>& { return s[(j * NC + c) * F + f]; };
+  // forward stages
+  constexpr int NF = FP::NST;
+  FxStage<T, FP, NF - 1, -1> last[NC];
+  if constexpr (NF == 1) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      last[c].load(t, [&](int j) { return act ? base[(long long)j * inner + f * NC + c] : zero; });
+      last[c].compute(t, twl);
+    }
+  } else {
+    {
+      FxStage<T, FP, 0, -1> s0[NC];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        s0[c].load(t, [&](int j) { return act ? base[(long long)j * inner + f * NC + c] : zero; });
+        s0[c].compute(t, twl);
+      }
+#pragma unroll
+      for (int c = 0; c < NC; ++c) s0[c].store(t, [&](int j, cpx<T> v) { lds(j, c) = v; });
+    }
+    __syncthreads();
+    fx_lds_stages<T, FP, -1, 1, NF - 1, NC>(t, lds, twl);
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      last[c].load(t, [&](int j) { return lds(j, c); });
+      last[c].compute(t, twl);
+    }
+  }
+  // last forward stage outputs sit at i + j*p (p = L/R_last): apply G per (butterfly, j)
+  using SL = FxStage<T, FP, NF - 1, -1>;
+#pragma unroll
+  for (int m = 0; m < SL::M; ++m) {
+    const int i = t + m * SL::TPP;
+    if (SL::valid(i)) {
+#pragma unroll
+      for (int j = 0; j < SL::R; ++j) {
+        const int kk = i + j * SL::p;
+        cpx<T> v[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) v[c] = last[c].v[m][j];
+        if (g.n0 > 1)
+          apply_G_fx<T, NC>(v, kk, ka, kb, g);
+        else
+          apply_G_fx<T, NC>(v, 0, kk, kb, g);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) last[c].v[m][j] = v[c];
+      }
+    }
+  }
+  // inverse stages: the first inverse stage (radix R_last) reads exactly these registers
+  constexpr int NI = FPR::NST;
+  if constexpr (NI == 1) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      FxStage<T, FPR, 0, +1> r0;
+#pragma unroll
+      for (int m = 0; m < SL::M; ++m)
+#pragma unroll
+        for (int j = 0; j < SL::R; ++j) r0.v[m][j] = last[c].v[m][j];
+      r0.compute(t, twl);
+      if (act) r0.store(t, [&](int j, cpx<T> v) { base[(long long)j * inner + f * NC + c] = v; });
+    }
+  } else {
+    if constexpr (NF > 1) __syncthreads();  // LDS reads of the last forward stage are done
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      FxStage<T, FPR, 0, +1> r0;
+#pragma unroll
+      for (int m = 0; m < SL::M; ++m)
+#pragma unroll
+        for (int j = 0; j < SL::R; ++j) r0.v[m][j] = last[c].v[m][j];
+      r0.compute(t, twl);
+      r0.store(t, [&](int j, cpx<T> v) { lds(j, c) = v; });
+    }
+    __syncthreads();
+    fx_lds_stages<T, FPR, +1, 1, NI - 1, NC>(t, lds, twl);
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      FxStage<T, FPR, NI - 1, +1> rl;
+      rl.load(t, [&](int j) { return lds(j, c); });
+      rl.compute(t, twl);
+      if (act) rl.store(t, [&](int j, cpx<T> v) { base[(long long)j * inner + f * NC + c] = v; });
+    }
+  }
+}
+
+
+// value of component c of A(x) v at voxel vox (runtime ncomp), for the contiguous-axis pass
+template <typename T, int COEF>
+__device__ __forceinline__ T coef_apply(const T* __restrict__ in, const T* __restrict__ coef, long long vox,
+                                        int c, int nc, long long n) {
+  if constexpr (COEF == COEF_NONE) {
+    return in[c * n + vox];
+  } else if constexpr (COEF == COEF_ISO) {
+    return coef[vox] * in[c * n + vox];
+  } else if constexpr (COEF == COEF_LAME) {
+    const int dn = (nc == 6) ? 3 : 2;
+    T tr = T(0);
+    for (int d = 0; d < dn; ++d) tr += in[d * n + vox];
+    return T(2) * coef[n + vox] * in[c * n + vox] + (c < dn ? coef[vox] * tr : T(0));
+  } else {
+    T acc = T(0);
+    for (int d = 0; d < nc; ++d) acc += coef[(long long)(c * nc + d) * n + vox] * in[d * n + vox];
+    return acc;
+  }
+}
+
+// ---- pass 1 (fixed length): A(x) v, then R2C along the contiguous axis -------------
+// Workgroup: rpb rows (all components), (rpb/2)*nc complex pencils of paired real rows,
+// TPP threads per pencil (blockDim = (rpb/2)*nc*TPP). Output: AoS half spectrum.
+template <typename T, class FP, int COEF>
+__global__ void __launch_bounds__(1024)
+k_rows_fwd_fx(const T* __restrict__ in, const T* __restrict__ coef, cpx<T>* __restrict__ out,
+              long long nrows, int nc, long long n, int rpb, const cpx<T>* __restrict__ twg,
+              const int* __restrict__ halt) {
+  if (halt && *halt) return;
+  constexpr int L = FP::L, n2h = L / 2 + 1, TPP = FP::TPP;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<T>* s = reinterpret_cast<cpx<T>*>(smem_raw);
+  __shared__ cpx<T> twl[L];
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int t = tid % TPP, pen = tid / TPP;
+  const long long row0 = (long long)xcd_remap(blockIdx.x, gridDim.x) * rpb;
+  const int nr = (int)min((long long)rpb, nrows - row0);
+  for (int e = tid; e < L; e += nthr) twl[e] = twg[e];
+  const int rp = pen / nc, m = pen - rp * nc;
+  const int rra = 2 * m, rrb = 2 * m + 1;
+  const int rowa = 2 * rp + rra / nc, ca = rra % nc;
+  const int rowb = 2 * rp + rrb / nc, cb = rrb % nc;
+  const bool va = rowa < nr, vb = rowb < nr;
+  const long long ba = (row0 + rowa) * L, bb = (row0 + rowb) * L;
+  FxStage<T, FP, 0, -1> s0;
+  s0.load(t, [&](int x) {
+    const T ya = va ? coef_apply<T, COEF>(in, coef, ba + x, ca, nc, n) : T(0);
+    const T yb = vb ? coef_apply<T, COEF>(in, coef, bb + x, cb, nc, n) : T(0);
+    return mk(ya, yb);
+  });
+  s0.compute(t, twl);
+  cpx<T>* sp = s + pen * L;
+  s0.store(t, [&](int j, cpx<T> v) { sp[j] = v; });
+  __syncthreads();
+  auto at = [&](int j, int) -> cpx<T>& { return sp[j]; };
+  fx_lds_stages<T, FP, -1, 1, FP::NST, 1>(t, at, twl);
+  // untangle the paired rows and store the AoS half spectrum (contiguous per block)
+  const int per = n2h * nc;
+  cpx<T>* ob = out + row0 * per;
+  for (int e = tid; e < nr * per; e += nthr) {
+    const int r = e / per;
+    const int rem = e - r * per;
+    const int k = rem / nc, c = rem - k * nc;
+    const int rr = (r & 1) * nc + c;
+    const cpx<T>* z = s + ((r >> 1) * nc + (rr >> 1)) * L;
+    const cpx<T> Z = z[k];
+    const cpx<T> Zc = conj(z[k ? L - k : 0]);
+    cpx<T> Y;
+    if (rr & 1) {
+      const cpx<T> d = Z - Zc;
+      Y = mk(T(0.5) * d.y, T(-0.5) * d.x);
+    } else {
+      Y = T(0.5) * (Z + Zc);
+    }
+    ob[e] = Y;
+  }
+}
+
+// ---- last pass (fixed length): C2R along the contiguous axis ------------------------
+template <typename T, class FP, int EPI>
+__global__ void __launch_bounds__(1024)
+k_rows_inv_fx(const cpx<T>* __restrict__ in, T* __restrict__ out, long long nrows, int nc, long long n,
+              int rpb, const cpx<T>* __restrict__ twg, T scale, const T* __restrict__ w,
+              double* __restrict__ partial, const int* __restrict__ halt) {
+  if (halt && *halt) return;
+  constexpr int L = FP::L, n2h = L / 2 + 1, TPP = FP::TPP;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<T>* s = reinterpret_cast<cpx<T>*>(smem_raw);
+  __shared__ cpx<T> twl[L];
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int t = tid % TPP, pen = tid / TPP;
+  const long long row0 = (long long)xcd_remap(blockIdx.x, gridDim.x) * rpb;
+  const int nr = (int)min((long long)rpb, nrows - row0);
+  for (int e = tid; e < L; e += nthr) twl[e] = twg[e];
+  const int nrp = rpb >> 1;
+  const cpx<T>* ib = in + row0 * (long long)n2h * nc;
+  for (int e = tid; e < nrp * n2h; e += nthr) {
+    const int rp = e / n2h, k = e - rp * n2h;
+    const bool self = (k == 0) || (2 * k == L);
+    for (int mm = 0; mm < nc; ++mm) {
+      int rr = 2 * mm;
+      int row = 2 * rp + rr / nc;
+      cpx<T> Ya = mk(T(0), T(0)), Yb = mk(T(0), T(0));
+      if (row < nr) Ya = ib[((long long)row * n2h + k) * nc + rr % nc];
+      rr = 2 * mm + 1;
+      row = 2 * rp + rr / nc;
+      if (row < nr) Yb = ib[((long long)row * n2h + k) * nc + rr % nc];
+      if (self) {
+        Ya.y = T(0);
+        Yb.y = T(0);
+      }
+      cpx<T>* z = s + (rp * nc + mm) * L;
+      z[k] = mk(Ya.x - Yb.y, Ya.y + Yb.x);
+      if (!self) z[L - k] = mk(Ya.x + Yb.y, Yb.x - Ya.y);
+    }
+  }
+  __syncthreads();
+  cpx<T>* sp = s + pen * L;
+  auto at = [&](int j, int) -> cpx<T>& { return sp[j]; };
+  fx_lds_stages<T, FP, +1, 0, FP::NST - 1, 1>(t, at, twl);
+  const int rp = pen / nc, m = pen - rp * nc;
+  const int rra = 2 * m, rrb = 2 * m + 1;
+  const int rowa = 2 * rp + rra / nc, ca = rra % nc;
+  const int rowb = 2 * rp + rrb / nc, cb = rrb % nc;
+  const bool va = rowa < nr, vb = rowb < nr;
+  T* oa = out + (long long)ca * n + (row0 + rowa) * L;
+  T* obp = out + (long long)cb * n + (row0 + rowb) * L;
+  const T* wa = w ? w + (long long)ca * n + (row0 + rowa) * L : nullptr;
+  const T* wb = w ? w + (long long)cb * n + (row0 + rowb) * L : nullptr;
+  double acc = 0.0;
+  FxStage<T, FP, FP::NST - 1, +1> sl;
+  sl.load(t, [&](int j) { return sp[j]; });
+  sl.compute(t, twl);
+  sl.store(t, [&](int x, cpx<T> v) {
+    const T a = scale * v.x, b = scale * v.y;
+    if (va) {
+      oa[x] = a;
+      if constexpr (EPI == 1) acc += (double)a * (double)wa[x];
+    }
+    if (vb) {
+      obp[x] = b;
+      if constexpr (EPI == 1) acc += (double)b * (double)wb[x];
+    }
+  });
+  if constexpr (EPI == 1) {
+    const double bsum = block_sum(acc);
+    if (tid == 0) partial[blockIdx.x] = bsum;
+  }
+}
+
+// Radix schedules of the specialised lengths (forward / reversed for the inverse).
+using FxP5 = FxPlan<5, 1, 5>;
+using FxP9 = FxPlan<9, 1, 9>;
+using FxP15 = FxPlan<15, 5, 5, 3>;
+using FxP15r = FxPlan<15, 5, 3, 5>;
+using FxP27 = FxPlan<27, 3, 9, 3>;
+using FxP27r = FxPlan<27, 3, 3, 9>;
+using FxP45 = FxPlan<45, 9, 9, 5>;
+using FxP45r = FxPlan<45, 9, 5, 9>;
+using FxP63 = FxPlan<63, 9, 9, 7>;
+using FxP63r = FxPlan<63, 9, 7, 9>;
+using FxP81 = FxPlan<81, 9, 9, 9>;
+using FxP99 = FxPlan<99, 11, 9, 11>;
+using FxP99r = FxPlan<99, 11, 11, 9>;
+using FxP105 = FxPlan<105, 21, 7, 5, 3>;
+using FxP105r = FxPlan<105, 21, 3, 5, 7>;
+using FxP135 = FxPlan<135, 15, 9, 3, 5>;
+using FxP135r = FxPlan<135, 15, 5, 3, 9>;
+using FxP243 = FxPlan<243, 27, 9, 9, 3>;
+using FxP243r = FxPlan<243, 27, 3, 9, 9>;
+
+}  // namespace fh

@@ -1,0 +1,20 @@
+// Fixed-length kernels and launchers, single precision.
+#include "fh_fixed_impl.hpp"
+
+namespace fh {
+template int fixed_rows_fwd<float>(const LaunchCtx&, int, const float*, const float*, int, cpx<float>*, const int*);
+template int fixed_rows_inv<float>(const LaunchCtx&, int, const cpx<float>*, float*, float, const float*, double*,
+                                   size_t, const int*, int*);
+template int fixed_cols<float>(const LaunchCtx&, int, int, int, cpx<float>*, const int*);
+template int fixed_cols_g<float>(const LaunchCtx&, int, int, cpx<float>*, const GParams&, const int*);
+
+bool fixed_length_supported(int L) {
+  switch (L) {
+#define FH_CASE(LL, FP, FPR) case LL:
+    FH_FIXED_LENGTHS(FH_CASE)
+#undef FH_CASE
+    return true;
+    default: return false;
+  }
+}
+}  // namespace fh

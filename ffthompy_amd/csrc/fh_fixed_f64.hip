@@ -1,0 +1,11 @@
+// Fixed-length kernels and launchers, double precision.
+#include "fh_fixed_impl.hpp"
+
+namespace fh {
+template int fixed_rows_fwd<double>(const LaunchCtx&, int, const double*, const double*, int, cpx<double>*,
+                                    const int*);
+template int fixed_rows_inv<double>(const LaunchCtx&, int, const cpx<double>*, double*, double, const double*,
+                                    double*, size_t, const int*, int*);
+template int fixed_cols<double>(const LaunchCtx&, int, int, int, cpx<double>*, const int*);
+template int fixed_cols_g<double>(const LaunchCtx&, int, int, cpx<double>*, const GParams&, const int*);
+}  // namespace fh

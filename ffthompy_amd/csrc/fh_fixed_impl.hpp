@@ -1,0 +1,193 @@
+// Definitions of the fixed-length launchers for one precision FH_T (included by
+// fh_fixed_f32.hip / fh_fixed_f64.hip).
+#pragma once
+#include "fh_fixed.hpp"
+#include "fh_launch.hpp"
+
+namespace fh {
+
+// length -> (forward schedule, reversed schedule)
+#define FH_FIXED_LENGTHS(X)   \
+  X(5, FxP5, FxP5)            \
+  X(9, FxP9, FxP9)            \
+  X(15, FxP15, FxP15r)        \
+  X(27, FxP27, FxP27r)        \
+  X(45, FxP45, FxP45r)        \
+  X(63, FxP63, FxP63r)        \
+  X(81, FxP81, FxP81)         \
+  X(99, FxP99, FxP99r)        \
+  X(105, FxP105, FxP105r)     \
+  X(135, FxP135, FxP135r)     \
+  X(243, FxP243, FxP243r)
+
+template <class K>
+inline void allow_lds(K kernel, size_t bytes) {
+  if (bytes > 64 * 1024) (void)hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                   (int)bytes);
+}
+
+template <typename T, class FP>
+int rows_fwd_fx_launch(const LaunchCtx& c, int nc, const T* in, const T* coef, int kind, cpx<T>* out,
+                       const int* halt) {
+  constexpr int rpb = FxShape<FP, T>::RPB;
+  const int npen = (rpb / 2) * nc;
+  const int nthr = npen * FP::TPP;
+  if (nthr > 1024) return 0;
+  const size_t lds = (size_t)npen * FP::L * sizeof(cpx<T>);
+  if (lds + FP::L * sizeof(cpx<T>) > 160 * 1024) return 0;
+  const long long nrows = (long long)c.n0 * c.n1;
+  const dim3 grid((unsigned)((nrows + rpb - 1) / rpb)), block(nthr);
+  const cpx<T>* tw = reinterpret_cast<const cpx<T>*>(c.tw[2]);
+#define FH_LAUNCH_RF(KIND)                                                                            \
+  do {                                                                                                \
+    auto kf = k_rows_fwd_fx<T, FP, KIND>;                                                             \
+    allow_lds(kf, lds);                                                                               \
+    hipLaunchKernelGGL(kf, grid, block, lds, c.stream, in, coef, out, nrows, nc, c.n, rpb, tw, halt); \
+  } while (0)
+  switch (kind) {
+    case COEF_NONE: FH_LAUNCH_RF(COEF_NONE); break;
+    case COEF_ISO: FH_LAUNCH_RF(COEF_ISO); break;
+    case COEF_LAME: FH_LAUNCH_RF(COEF_LAME); break;
+    default: FH_LAUNCH_RF(COEF_FULL); break;
+  }
+#undef FH_LAUNCH_RF
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 1 : -(int)e;
+}
+
+template <typename T, class FP>
+int rows_inv_fx_launch(const LaunchCtx& c, int nc, const cpx<T>* in, T* out, T scale, const T* w,
+                       double* partial, size_t partial_cap, const int* halt, int* nblocks) {
+  constexpr int rpb = FxShape<FP, T>::RPB;
+  const int npen = (rpb / 2) * nc;
+  const int nthr = npen * FP::TPP;
+  if (nthr > 1024) return 0;
+  const size_t lds = (size_t)npen * FP::L * sizeof(cpx<T>);
+  if (lds + FP::L * sizeof(cpx<T>) > 160 * 1024) return 0;
+  const long long nrows = (long long)c.n0 * c.n1;
+  const long long nb = (nrows + rpb - 1) / rpb;
+  if (w && (size_t)nb > partial_cap) return 0;
+  const dim3 grid((unsigned)nb), block(nthr);
+  const cpx<T>* tw = reinterpret_cast<const cpx<T>*>(c.tw[2]);
+  if (w) {
+    auto kf = k_rows_inv_fx<T, FP, 1>;
+    allow_lds(kf, lds);
+    hipLaunchKernelGGL(kf, grid, block, lds, c.stream, in, out, nrows, nc, c.n, rpb, tw, scale, w, partial, halt);
+  } else {
+    auto kf = k_rows_inv_fx<T, FP, 0>;
+    allow_lds(kf, lds);
+    hipLaunchKernelGGL(kf, grid, block, lds, c.stream, in, out, nrows, nc, c.n, rpb, tw, scale, w, partial, halt);
+  }
+  if (nblocks) *nblocks = (int)nb;
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 1 : -(int)e;
+}
+
+inline void strided_geometry(const LaunchCtx& c, int axis, int nc, long long* outer, long long* ostride,
+                             long long* inner) {
+  if (axis == 1) {
+    *outer = c.n0;
+    *inner = (long long)c.n2h * nc;
+    *ostride = (long long)c.n1 * (*inner);
+  } else {
+    *outer = 1;
+    *inner = (long long)c.n1 * c.n2h * nc;
+    *ostride = 0;
+  }
+}
+
+template <typename T, class FP>
+int cols_fx_launch(const LaunchCtx& c, int dir, int axis, int nc, cpx<T>* data, const int* halt) {
+  constexpr int P = FxShape<FP, T>::P;
+  long long outer, ostride, inner;
+  strided_geometry(c, axis, nc, &outer, &ostride, &inner);
+  const long long ntiles = (inner + P - 1) / P;
+  const long long nb = outer * ntiles;
+  if (nb > 0x7fffffffLL) return 0;
+  const dim3 grid((unsigned)nb), block(FP::TPP * P);
+  const cpx<T>* tw = reinterpret_cast<const cpx<T>*>(c.tw[axis]);
+  if (dir < 0)
+    hipLaunchKernelGGL((k_cols_fx<T, FP, P, -1>), grid, block, 0, c.stream, data, ostride, inner, (int)ntiles, tw,
+                       halt);
+  else
+    hipLaunchKernelGGL((k_cols_fx<T, FP, P, +1>), grid, block, 0, c.stream, data, ostride, inner, (int)ntiles, tw,
+                       halt);
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 1 : -(int)e;
+}
+
+template <typename T, class FP, class FPR, int NC>
+int cols_g_fx_launch_nc(const LaunchCtx& c, int axis, cpx<T>* data, const GParams& g, const int* halt) {
+  constexpr int F = FxShape<FP, T>::F;
+  long long outer, ostride, inner;
+  strided_geometry(c, axis, NC, &outer, &ostride, &inner);
+  if (outer != 1) return 0;
+  const long long ntiles = (inner + F * NC - 1) / (F * NC);
+  if (ntiles > 0x7fffffffLL) return 0;
+  const dim3 grid((unsigned)ntiles), block(FP::TPP * F);
+  const cpx<T>* tw = reinterpret_cast<const cpx<T>*>(c.tw[axis]);
+  hipLaunchKernelGGL((k_cols_g_fx<T, FP, FPR, F, NC>), grid, block, 0, c.stream, data, inner, (int)ntiles, tw, g,
+                     c.n2h, halt);
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 1 : -(int)e;
+}
+
+template <typename T, class FP, class FPR>
+int cols_g_fx_launch(const LaunchCtx& c, int axis, int nc, cpx<T>* data, const GParams& g, const int* halt) {
+  switch (nc) {
+    case 2: return cols_g_fx_launch_nc<T, FP, FPR, 2>(c, axis, data, g, halt);
+    case 3: return cols_g_fx_launch_nc<T, FP, FPR, 3>(c, axis, data, g, halt);
+    case 6: return cols_g_fx_launch_nc<T, FP, FPR, 6>(c, axis, data, g, halt);
+    default: return 0;
+  }
+}
+
+template <typename T>
+int fixed_rows_fwd(const LaunchCtx& c, int nc, const T* in, const T* coef, int kind, cpx<T>* out,
+                   const int* halt) {
+  switch (c.n2) {
+#define FH_CASE(L, FP, FPR) \
+  case L: return rows_fwd_fx_launch<T, FP>(c, nc, in, coef, kind, out, halt);
+    FH_FIXED_LENGTHS(FH_CASE)
+#undef FH_CASE
+    default: return 0;
+  }
+}
+
+template <typename T>
+int fixed_rows_inv(const LaunchCtx& c, int nc, const cpx<T>* in, T* out, T scale, const T* w,
+                   double* partial, size_t partial_cap, const int* halt, int* nblocks) {
+  switch (c.n2) {
+#define FH_CASE(L, FP, FPR) \
+  case L: return rows_inv_fx_launch<T, FP>(c, nc, in, out, scale, w, partial, partial_cap, halt, nblocks);
+    FH_FIXED_LENGTHS(FH_CASE)
+#undef FH_CASE
+    default: return 0;
+  }
+}
+
+template <typename T>
+int fixed_cols(const LaunchCtx& c, int dir, int axis, int nc, cpx<T>* data, const int* halt) {
+  const int L = axis == 0 ? c.n0 : c.n1;
+  switch (L) {
+#define FH_CASE(LL, FP, FPR) \
+  case LL: return cols_fx_launch<T, FP>(c, dir, axis, nc, data, halt);
+    FH_FIXED_LENGTHS(FH_CASE)
+#undef FH_CASE
+    default: return 0;
+  }
+}
+
+template <typename T>
+int fixed_cols_g(const LaunchCtx& c, int axis, int nc, cpx<T>* data, const GParams& g, const int* halt) {
+  const int L = axis == 0 ? c.n0 : c.n1;
+  switch (L) {
+#define FH_CASE(LL, FP, FPR) \
+  case LL: return cols_g_fx_launch<T, FP, FPR>(c, axis, nc, data, g, halt);
+    FH_FIXED_LENGTHS(FH_CASE)
+#undef FH_CASE
+    default: return 0;
+  }
+}
+
+}  // namespace fh

@@ -465,7 +465,7 @@ __global__ void k_dot_partial(long long n, const T* __restrict__ x, const T* __r
   if (threadIdx.x == 0) partial[blockIdx.x] = b;
 }
 
-__global__ void k_sum_partials(const double* __restrict__ partial, int np, double scale,
+static __global__ void k_sum_partials(const double* __restrict__ partial, int np, double scale,
                                double* __restrict__ out) {
   double acc = 0.0;
   for (int i = threadIdx.x; i < np; i += blockDim.x) acc += partial[i];
@@ -489,7 +489,7 @@ __global__ void k_residual(long long n, const double* __restrict__ b, const doub
 }
 
 // (Re)start: rr = <r,r> of the true residual; beta for continuing the direction.
-__global__ void k_cg_restart(CGState* st, const double* __restrict__ partial, int np,
+static __global__ void k_cg_restart(CGState* st, const double* __restrict__ partial, int np,
                              double* __restrict__ hist) {
   double acc = 0.0;
   for (int i = threadIdx.x; i < np; i += blockDim.x) acc += partial[i];
@@ -508,7 +508,7 @@ __global__ void k_cg_restart(CGState* st, const double* __restrict__ partial, in
   }
 }
 
-__global__ void k_cg_alpha(CGState* st, const double* __restrict__ partial, int np) {
+static __global__ void k_cg_alpha(CGState* st, const double* __restrict__ partial, int np) {
   if (st->halt) return;
   double acc = 0.0;
   for (int i = threadIdx.x; i < np; i += blockDim.x) acc += partial[i];
@@ -544,7 +544,7 @@ __global__ void k_cg_update(long long n, T* __restrict__ x, T* __restrict__ r, c
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-__global__ void k_cg_beta(CGState* st, const double* __restrict__ partial, int np,
+static __global__ void k_cg_beta(CGState* st, const double* __restrict__ partial, int np,
                           double* __restrict__ hist) {
   if (st->halt) return;
   double acc = 0.0;
@@ -581,7 +581,7 @@ __global__ void k_cg_pupdate(long long n, T* __restrict__ p, const T* __restrict
 }
 
 // x64 += x32, x32 = 0 (mixed precision: fold the single-precision correction)
-__global__ void k_fold_correction(long long n, double* __restrict__ x64, float* __restrict__ x32) {
+static __global__ void k_fold_correction(long long n, double* __restrict__ x64, float* __restrict__ x32) {
   const long long stride = (long long)gridDim.x * blockDim.x;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     x64[i] += (double)x32[i];
@@ -650,7 +650,7 @@ k_energy(const double* const* __restrict__ F, const double* __restrict__ coef, l
   }
 }
 
-__global__ void k_energy_final(const double* __restrict__ partial, int np, int nn, double inv_n,
+static __global__ void k_energy_final(const double* __restrict__ partial, int np, int nn, double inv_n,
                                double* __restrict__ AH) {
   for (int k = 0; k < nn; ++k) {
     dd t;

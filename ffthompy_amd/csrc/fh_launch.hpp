@@ -1,0 +1,46 @@
+// Launchers of the fixed-length kernels, shared by the per-precision translation units
+// (fh_fixed_f32.hip, fh_fixed_f64.hip) and the host plan code (fh_api.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "fh_kernels.hpp"
+
+namespace fh {
+
+// What a launcher needs from a plan (no host types from fh_api.hip).
+struct LaunchCtx {
+  hipStream_t stream;
+  int n0, n1, n2, n2h;
+  long long n, nh;
+  const void* tw[3];  // per-axis twiddle table W_L^m, m < L, in the launch precision
+};
+
+// Return 1 if a specialised kernel was launched, 0 if the length has none (caller falls
+// back to the generic kernels), or a negative hipError_t on launch failure.
+template <typename T>
+int fixed_rows_fwd(const LaunchCtx& c, int nc, const T* in, const T* coef, int kind, cpx<T>* out,
+                   const int* halt);
+template <typename T>
+int fixed_rows_inv(const LaunchCtx& c, int nc, const cpx<T>* in, T* out, T scale, const T* w,
+                   double* partial, size_t partial_cap, const int* halt, int* nblocks);
+template <typename T>
+int fixed_cols(const LaunchCtx& c, int dir, int axis, int nc, cpx<T>* data, const int* halt);
+template <typename T>
+int fixed_cols_g(const LaunchCtx& c, int axis, int nc, cpx<T>* data, const GParams& g, const int* halt);
+// Lengths with a specialised schedule.
+bool fixed_length_supported(int L);
+
+extern template int fixed_rows_fwd<float>(const LaunchCtx&, int, const float*, const float*, int, cpx<float>*,
+                                          const int*);
+extern template int fixed_rows_fwd<double>(const LaunchCtx&, int, const double*, const double*, int,
+                                           cpx<double>*, const int*);
+extern template int fixed_rows_inv<float>(const LaunchCtx&, int, const cpx<float>*, float*, float, const float*,
+                                          double*, size_t, const int*, int*);
+extern template int fixed_rows_inv<double>(const LaunchCtx&, int, const cpx<double>*, double*, double,
+                                           const double*, double*, size_t, const int*, int*);
+extern template int fixed_cols<float>(const LaunchCtx&, int, int, int, cpx<float>*, const int*);
+extern template int fixed_cols<double>(const LaunchCtx&, int, int, int, cpx<double>*, const int*);
+extern template int fixed_cols_g<float>(const LaunchCtx&, int, int, cpx<float>*, const GParams&, const int*);
+extern template int fixed_cols_g<double>(const LaunchCtx&, int, int, cpx<double>*, const GParams&, const int*);
+
+}  // namespace fh

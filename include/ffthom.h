@@ -75,6 +75,10 @@ int fh_plan_destroy(fh_plan* plan);
 int fh_plan_ncomp(const fh_plan* plan);
 /* Switch the projection between G1 (primal) and G2 (dual): applications.py:53-56. */
 int fh_plan_set_primaldual(fh_plan* plan, int primaldual);
+/* generic = 1 forces the runtime-radix kernels even where a compile-time specialised
+ * schedule exists (lengths 5, 9, 15, 27, 45, 63, 81, 99, 105, 135, 243); used to test
+ * one path against the other. No reference counterpart. */
+int fh_plan_use_generic(fh_plan* plan, int generic);
 
 /* Coefficient A(x) of the operator, host array in origin-0 voxel order.
  * Replaces Material.get_A_GaNi (ffthompy/materials.py:115-123) as the right factor of

@@ -178,3 +178,60 @@ def test_cg_fp64_matches_oracle(physics, kind, N):
     assert out['kit'] == info['kit'], (out['kit'], info['kit'])
     assert np.abs(out['res_log'] - np.array(info['norm_res_log'])).max() < 1e-9 * info['norm_res_log'][0]
     assert rel(dx.to_host(), xref) < 1e-8
+
+
+# Shapes that put every specialised length (5, 9, 15, 27, 45, 63, 81, 99, 105, 135, 243)
+# on the contiguous axis, the middle axis and the outer (projection) axis.
+FIXED_SHAPES = [(243, 9, 243), (5, 243, 27), (27, 45, 63), (81, 99, 105), (135, 15, 9), (9, 5, 15),
+                (45, 63, 81), (99, 105, 135), (63, 81), (105, 135), (243, 45), (15, 99)]
+
+
+@pytest.mark.parametrize('N', FIXED_SHAPES, ids=[str(s) for s in FIXED_SHAPES])
+def test_fixed_length_fft_matches_numpy(N):
+    print('case fixed rfftn', N, flush=True)
+    rng = np.random.default_rng(20)
+    nc = 1
+    plan = nat.Plan(N, np.ones(len(N)))
+    x = rng.standard_normal((nc,) + N)
+    Nh = N[:-1] + (N[-1] // 2 + 1,)
+    ref = o_rfftn(x, N)
+    for dt, ct, tol in ((np.float64, np.complex128, 1e-12), (np.float32, np.complex64, 2e-5)):
+        dx = nat.DeviceArray.from_host(x.astype(dt))
+        dX = nat.DeviceArray((nc,) + Nh, ct)
+        plan.rfftn(dx, dX, nc)
+        assert rel(dX.to_host(), ref) < tol, (dt, rel(dX.to_host(), ref))
+        dz = nat.DeviceArray((nc,) + N, dt)
+        plan.irfftn(dX, dz, nc)
+        assert rel(dz.to_host(), x) < tol
+
+
+FIXED_MV = [('scalar', (243, 9, 45)), ('scalar', (63, 81)), ('scalar', (105, 135)), ('elasticity', (27, 45, 15)),
+            ('elasticity', (81, 9, 5)), ('scalar', (99, 27, 63)), ('elasticity', (15, 45))]
+
+
+@pytest.mark.parametrize('physics,N', FIXED_MV, ids=['%s-%s' % c for c in FIXED_MV])
+@pytest.mark.parametrize('pd', ['primal', 'dual'])
+def test_fixed_matvec_matches_oracle_and_generic(physics, N, pd):
+    print('case fixed matvec', physics, N, pd, flush=True)
+    rng = np.random.default_rng(21)
+    Y = np.linspace(1., 1.3, len(N))
+    G1, G2 = _projections(physics, N, Y)
+    plan = nat.Plan(N, Y, physics, pd)
+    nc = plan.ncomp
+    kind = 'iso' if physics == 'scalar' else 'lame'
+    vals, A = _random_coef(rng, kind, nc, N, physics)
+    plan.set_coeff(kind, vals)
+    hG = Field(val=G1 if pd == 'primal' else G2, order=2, N=N, multype=21, Fourier=True)
+    x = rng.standard_normal((nc,) + N)
+    ref = Operator([[DFT(N, inverse=True), hG, DFT(N), Field(val=A, order=2, multype=21)]])(Field(val=x, order=1)).val
+    for dt, tol in ((np.float64, 1e-12), (np.float32, 2e-5)):
+        dx = nat.DeviceArray.from_host(x.astype(dt))
+        dy = nat.DeviceArray((nc,) + N, dt)
+        plan.use_generic(False)
+        plan.matvec(dx, dy)
+        yf = dy.to_host()
+        plan.use_generic(True)
+        plan.matvec(dx, dy)
+        yg = dy.to_host()
+        assert rel(yf, ref) < tol, (dt, rel(yf, ref))
+        assert rel(yg, ref) < tol, (dt, rel(yg, ref))
