@@ -799,10 +799,11 @@ int fh_cg_solve(fh_plan* pl, int mode, const void* b_, void* x_, double tol, int
   HIPCHK(hipEventRecord(pl->ev0, pl->stream));
   const int per_check = 8;
   if (mode == FH_SOLVE_FP64) {
-    RETCHK(ensure_work(pl, 3 * (size_t)nv * sizeof(double)));
+    const long long nvp = (nv + 63) / 64 * 64;  // 512-byte aligned sub-buffers
+    RETCHK(ensure_work(pl, 3 * (size_t)nvp * sizeof(double)));
     double* r = (double*)pl->work;
-    double* p = r + nv;
-    double* Ap = p + nv;
+    double* p = r + nvp;
+    double* Ap = p + nvp;
     RETCHK(enqueue_true_residual<double>(pl, b, x, Ap, r));
     HIPCHK(hipMemcpyAsync(p, r, (size_t)nv * sizeof(double), hipMemcpyDeviceToDevice, pl->stream));
     for (;;) {
@@ -812,12 +813,13 @@ int fh_cg_solve(fh_plan* pl, int mode, const void* b_, void* x_, double tol, int
       if (s.done) break;
     }
   } else {
-    RETCHK(ensure_work(pl, 4 * (size_t)nv * sizeof(float) + (size_t)nv * sizeof(double)));
+    const long long nvp = (nv + 63) / 64 * 64;  // 256-byte aligned sub-buffers
+    RETCHK(ensure_work(pl, 4 * (size_t)nvp * sizeof(float) + (size_t)nvp * sizeof(double)));
     double* Ax = (double*)pl->work;
-    float* r = (float*)(Ax + nv);
-    float* p = r + nv;
-    float* Ap = p + nv;
-    float* xc = Ap + nv;
+    float* r = (float*)(Ax + nvp);
+    float* p = r + nvp;
+    float* Ap = p + nvp;
+    float* xc = Ap + nvp;
     HIPCHK(hipMemsetAsync(xc, 0, (size_t)nv * sizeof(float), pl->stream));
     RETCHK(enqueue_true_residual<float>(pl, b, x, Ax, r));
     HIPCHK(hipMemcpyAsync(p, r, (size_t)nv * sizeof(float), hipMemcpyDeviceToDevice, pl->stream));

@@ -403,6 +403,113 @@ k_cols_g_fx(cpx<T>* __restrict__ data, long long inner, int ntiles, const cpx<T>
 }
 
 
+
+// ---- projection pass, one thread per column ------------------------------------------
+// Same schedule as k_cols_g_fx, but a thread owns a single column q = f*NC + c of the
+// tile (like the plain strided pass: coalesced rows, few registers, high occupancy).
+// After the last forward stage each thread publishes its values in LDS, reads the NC
+// components of its frequencies, applies G(xi) and keeps its own component.
+template <typename T, class FP, class FPR, int F, int NC>
+__global__ void __launch_bounds__(FP::TPP * F * NC)
+k_cols_gx_fx(cpx<T>* __restrict__ data, long long inner, int ntiles, const cpx<T>* __restrict__ twg,
+             GParams g, int n2h, const int* __restrict__ halt) {
+  static_assert(FP::TPP == FPR::TPP && FP::L == FPR::L, "schedules must match");
+  static_assert(FP::R[FP::NST - 1] == FPR::R[0], "inverse must start with the last forward radix");
+  if (halt && *halt) return;
+  constexpr int L = FP::L;
+  constexpr int P = F * NC;
+  __shared__ cpx<T> s[L * P];
+  __shared__ cpx<T> twl[L];
+  const int tid = threadIdx.x;
+  const int q = tid % P, t = tid / P;
+  const int f = q / NC, c = q - f * NC;
+  const int tile = xcd_remap(blockIdx.x, gridDim.x);
+  const long long col0 = (long long)tile * P;
+  const int ncol = (int)min((long long)P, inner - col0);
+  cpx<T>* base = data + col0;
+  for (int e = tid; e < L; e += FP::TPP * P) twl[e] = twg[e];
+  const bool act = q < ncol;
+  const cpx<T> zero = mk(T(0), T(0));
+  const long long fp = col0 / NC + f;
+  int ka, kb;
+  if (g.n0 > 1) {
+    ka = (int)(fp / n2h);
+    kb = (int)(fp - (long long)ka * n2h);
+  } else {
+    ka = 0;
+    kb = (int)fp;
+  }
+  constexpr int NF = FP::NST;
+  using SL = FxStage<T, FP, NF - 1, -1>;
+  SL last;
+  auto at = [&](int j, int) -> cpx<T>& { return s[j * P + q]; };
+  if constexpr (NF == 1) {
+    last.load(t, [&](int j) { return act ? base[(long long)j * inner + q] : zero; });
+    last.compute(t, twl);
+  } else {
+    FxStage<T, FP, 0, -1> s0;
+    s0.load(t, [&](int j) { return act ? base[(long long)j * inner + q] : zero; });
+    s0.compute(t, twl);
+    s0.store(t, [&](int j, cpx<T> v) { s[j * P + q] = v; });
+    __syncthreads();
+    fx_lds_stages<T, FP, -1, 1, NF - 1, 1>(t, at, twl);
+    last.load(t, [&](int j) { return s[j * P + q]; });
+    last.compute(t, twl);
+    __syncthreads();
+  }
+  // publish, then project: outputs of the last forward stage sit at i + j*p
+#pragma unroll
+  for (int m = 0; m < SL::M; ++m) {
+    const int i = t + m * SL::TPP;
+    if (SL::valid(i)) {
+#pragma unroll
+      for (int j = 0; j < SL::R; ++j) s[(i + j * SL::p) * P + q] = last.v[m][j];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < SL::M; ++m) {
+    const int i = t + m * SL::TPP;
+    if (SL::valid(i)) {
+#pragma unroll
+      for (int j = 0; j < SL::R; ++j) {
+        const int kk = i + j * SL::p;
+        cpx<T> v[NC];
+#pragma unroll
+        for (int cc = 0; cc < NC; ++cc) v[cc] = s[kk * P + f * NC + cc];
+        if (g.n0 > 1)
+          apply_G_fx<T, NC>(v, kk, ka, kb, g);
+        else
+          apply_G_fx<T, NC>(v, 0, kk, kb, g);
+        cpx<T> mine = v[0];
+#pragma unroll
+        for (int cc = 1; cc < NC; ++cc)
+          if (cc == c) mine = v[cc];
+        last.v[m][j] = mine;
+      }
+    }
+  }
+  __syncthreads();
+  constexpr int NI = FPR::NST;
+  FxStage<T, FPR, 0, +1> r0;
+#pragma unroll
+  for (int m = 0; m < SL::M; ++m)
+#pragma unroll
+    for (int j = 0; j < SL::R; ++j) r0.v[m][j] = last.v[m][j];
+  r0.compute(t, twl);
+  if constexpr (NI == 1) {
+    if (act) r0.store(t, [&](int j, cpx<T> v) { base[(long long)j * inner + q] = v; });
+  } else {
+    r0.store(t, [&](int j, cpx<T> v) { s[j * P + q] = v; });
+    __syncthreads();
+    fx_lds_stages<T, FPR, +1, 1, NI - 1, 1>(t, at, twl);
+    FxStage<T, FPR, NI - 1, +1> rl;
+    rl.load(t, [&](int j) { return s[j * P + q]; });
+    rl.compute(t, twl);
+    if (act) rl.store(t, [&](int j, cpx<T> v) { base[(long long)j * inner + q] = v; });
+  }
+}
+
 // value of component c of A(x) v at voxel vox (runtime ncomp), for the contiguous-axis pass
 template <typename T, int COEF>
 __device__ __forceinline__ T coef_apply(const T* __restrict__ in, const T* __restrict__ coef, long long vox,
@@ -426,11 +533,12 @@ __device__ __forceinline__ T coef_apply(const T* __restrict__ in, const T* __res
 // ---- pass 1 (fixed length): A(x) v, then R2C along the contiguous axis -------------
 // Workgroup: rpb rows (all components), (rpb/2)*nc complex pencils of paired real rows,
 // TPP threads per pencil (blockDim = (rpb/2)*nc*TPP). Output: AoS half spectrum.
-template <typename T, class FP, int COEF>
+template <typename T, class FP, int COEF, int NC>
 __global__ void __launch_bounds__(1024)
 k_rows_fwd_fx(const T* __restrict__ in, const T* __restrict__ coef, cpx<T>* __restrict__ out,
-              long long nrows, int nc, long long n, int rpb, const cpx<T>* __restrict__ twg,
+              long long nrows, long long n, int rpb, const cpx<T>* __restrict__ twg,
               const int* __restrict__ halt) {
+  constexpr int nc = NC;
   if (halt && *halt) return;
   constexpr int L = FP::L, n2h = L / 2 + 1, TPP = FP::TPP;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -482,11 +590,12 @@ k_rows_fwd_fx(const T* __restrict__ in, const T* __restrict__ coef, cpx<T>* __re
 }
 
 // ---- last pass (fixed length): C2R along the contiguous axis ------------------------
-template <typename T, class FP, int EPI>
+template <typename T, class FP, int EPI, int NC>
 __global__ void __launch_bounds__(1024)
-k_rows_inv_fx(const cpx<T>* __restrict__ in, T* __restrict__ out, long long nrows, int nc, long long n,
+k_rows_inv_fx(const cpx<T>* __restrict__ in, T* __restrict__ out, long long nrows, long long n,
               int rpb, const cpx<T>* __restrict__ twg, T scale, const T* __restrict__ w,
               double* __restrict__ partial, const int* __restrict__ halt) {
+  constexpr int nc = NC;
   if (halt && *halt) return;
   constexpr int L = FP::L, n2h = L / 2 + 1, TPP = FP::TPP;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -502,6 +611,7 @@ k_rows_inv_fx(const cpx<T>* __restrict__ in, T* __restrict__ out, long long nrow
   for (int e = tid; e < nrp * n2h; e += nthr) {
     const int rp = e / n2h, k = e - rp * n2h;
     const bool self = (k == 0) || (2 * k == L);
+#pragma unroll
     for (int mm = 0; mm < nc; ++mm) {
       int rr = 2 * mm;
       int row = 2 * rp + rr / nc;

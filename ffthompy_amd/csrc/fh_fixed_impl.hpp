@@ -38,17 +38,26 @@ int rows_fwd_fx_launch(const LaunchCtx& c, int nc, const T* in, const T* coef, i
   const long long nrows = (long long)c.n0 * c.n1;
   const dim3 grid((unsigned)((nrows + rpb - 1) / rpb)), block(nthr);
   const cpx<T>* tw = reinterpret_cast<const cpx<T>*>(c.tw[2]);
-#define FH_LAUNCH_RF(KIND)                                                                            \
-  do {                                                                                                \
-    auto kf = k_rows_fwd_fx<T, FP, KIND>;                                                             \
-    allow_lds(kf, lds);                                                                               \
-    hipLaunchKernelGGL(kf, grid, block, lds, c.stream, in, coef, out, nrows, nc, c.n, rpb, tw, halt); \
+#define FH_LAUNCH_RF(KIND, NCC)                                                                   \
+  do {                                                                                            \
+    auto kf = k_rows_fwd_fx<T, FP, KIND, NCC>;                                                    \
+    allow_lds(kf, lds);                                                                           \
+    hipLaunchKernelGGL(kf, grid, block, lds, c.stream, in, coef, out, nrows, c.n, rpb, tw, halt); \
   } while (0)
-  switch (kind) {
-    case COEF_NONE: FH_LAUNCH_RF(COEF_NONE); break;
-    case COEF_ISO: FH_LAUNCH_RF(COEF_ISO); break;
-    case COEF_LAME: FH_LAUNCH_RF(COEF_LAME); break;
-    default: FH_LAUNCH_RF(COEF_FULL); break;
+  const int key = kind * 8 + nc;
+  switch (key) {
+    case COEF_NONE * 8 + 1: FH_LAUNCH_RF(COEF_NONE, 1); break;
+    case COEF_NONE * 8 + 2: FH_LAUNCH_RF(COEF_NONE, 2); break;
+    case COEF_NONE * 8 + 3: FH_LAUNCH_RF(COEF_NONE, 3); break;
+    case COEF_NONE * 8 + 6: FH_LAUNCH_RF(COEF_NONE, 6); break;
+    case COEF_ISO * 8 + 2: FH_LAUNCH_RF(COEF_ISO, 2); break;
+    case COEF_ISO * 8 + 3: FH_LAUNCH_RF(COEF_ISO, 3); break;
+    case COEF_LAME * 8 + 3: FH_LAUNCH_RF(COEF_LAME, 3); break;
+    case COEF_LAME * 8 + 6: FH_LAUNCH_RF(COEF_LAME, 6); break;
+    case COEF_FULL * 8 + 2: FH_LAUNCH_RF(COEF_FULL, 2); break;
+    case COEF_FULL * 8 + 3: FH_LAUNCH_RF(COEF_FULL, 3); break;
+    case COEF_FULL * 8 + 6: FH_LAUNCH_RF(COEF_FULL, 6); break;
+    default: return 0;
   }
 #undef FH_LAUNCH_RF
   const hipError_t e = hipGetLastError();
@@ -69,15 +78,24 @@ int rows_inv_fx_launch(const LaunchCtx& c, int nc, const cpx<T>* in, T* out, T s
   if (w && (size_t)nb > partial_cap) return 0;
   const dim3 grid((unsigned)nb), block(nthr);
   const cpx<T>* tw = reinterpret_cast<const cpx<T>*>(c.tw[2]);
-  if (w) {
-    auto kf = k_rows_inv_fx<T, FP, 1>;
-    allow_lds(kf, lds);
-    hipLaunchKernelGGL(kf, grid, block, lds, c.stream, in, out, nrows, nc, c.n, rpb, tw, scale, w, partial, halt);
-  } else {
-    auto kf = k_rows_inv_fx<T, FP, 0>;
-    allow_lds(kf, lds);
-    hipLaunchKernelGGL(kf, grid, block, lds, c.stream, in, out, nrows, nc, c.n, rpb, tw, scale, w, partial, halt);
+#define FH_LAUNCH_RI(EPI, NCC)                                                                                \
+  do {                                                                                                        \
+    auto kf = k_rows_inv_fx<T, FP, EPI, NCC>;                                                                 \
+    allow_lds(kf, lds);                                                                                       \
+    hipLaunchKernelGGL(kf, grid, block, lds, c.stream, in, out, nrows, c.n, rpb, tw, scale, w, partial, halt); \
+  } while (0)
+  const int key = (w ? 8 : 0) + nc;
+  switch (key) {
+    case 1: FH_LAUNCH_RI(0, 1); break;
+    case 2: FH_LAUNCH_RI(0, 2); break;
+    case 3: FH_LAUNCH_RI(0, 3); break;
+    case 6: FH_LAUNCH_RI(0, 6); break;
+    case 8 + 2: FH_LAUNCH_RI(1, 2); break;
+    case 8 + 3: FH_LAUNCH_RI(1, 3); break;
+    case 8 + 6: FH_LAUNCH_RI(1, 6); break;
+    default: return 0;
   }
+#undef FH_LAUNCH_RI
   if (nblocks) *nblocks = (int)nb;
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? 1 : -(int)e;
@@ -116,17 +134,29 @@ int cols_fx_launch(const LaunchCtx& c, int dir, int axis, int nc, cpx<T>* data, 
   return e == hipSuccess ? 1 : -(int)e;
 }
 
+template <class FP, typename T, int NC>
+struct GxShape {
+  // frequency points per tile: about 16 (fp32) / 8 (fp64) columns of 8-byte / 16-byte
+  // complex values per thread row, and at most 1024 threads
+  static constexpr int cols = (sizeof(T) == 4 ? 24 : 12) / NC > 0 ? (sizeof(T) == 4 ? 24 : 12) / NC : 1;
+  static constexpr int fill = 512 / (FP::TPP * NC) > 0 ? 512 / (FP::TPP * NC) : 1;
+  static constexpr int want = cols > fill ? cols : fill;
+  static constexpr int cap = 1024 / (FP::TPP * NC) > 0 ? 1024 / (FP::TPP * NC) : 1;
+  static constexpr int F = want < cap ? want : cap;
+};
+
 template <typename T, class FP, class FPR, int NC>
 int cols_g_fx_launch_nc(const LaunchCtx& c, int axis, cpx<T>* data, const GParams& g, const int* halt) {
-  constexpr int F = FxShape<FP, T>::F;
+  constexpr int F = GxShape<FP, T, NC>::F;
+  if (FP::TPP * F * NC > 1024) return 0;
   long long outer, ostride, inner;
   strided_geometry(c, axis, NC, &outer, &ostride, &inner);
   if (outer != 1) return 0;
   const long long ntiles = (inner + F * NC - 1) / (F * NC);
   if (ntiles > 0x7fffffffLL) return 0;
-  const dim3 grid((unsigned)ntiles), block(FP::TPP * F);
+  const dim3 grid((unsigned)ntiles), block(FP::TPP * F * NC);
   const cpx<T>* tw = reinterpret_cast<const cpx<T>*>(c.tw[axis]);
-  hipLaunchKernelGGL((k_cols_g_fx<T, FP, FPR, F, NC>), grid, block, 0, c.stream, data, inner, (int)ntiles, tw, g,
+  hipLaunchKernelGGL((k_cols_gx_fx<T, FP, FPR, F, NC>), grid, block, 0, c.stream, data, inner, (int)ntiles, tw, g,
                      c.n2h, halt);
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? 1 : -(int)e;

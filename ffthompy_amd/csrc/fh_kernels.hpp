@@ -526,22 +526,48 @@ static __global__ void k_cg_alpha(CGState* st, const double* __restrict__ partia
   }
 }
 
+// 16-byte vector of T (4 floats or 2 doubles) for the streaming CG kernels.
+template <typename T>
+struct alignas(16) V16 {
+  static constexpr int W = 16 / sizeof(T);
+  T e[W];
+};
+
 template <typename T>
 __global__ void k_cg_update(long long n, T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
                             const T* __restrict__ Ap, const CGState* __restrict__ st,
                             double* __restrict__ partial) {
   if (st->halt) return;
+  constexpr int W = V16<T>::W;
   const T a = T(st->alpha);
   const long long stride = (long long)gridDim.x * blockDim.x;
+  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long nv = n / W;
   double acc = 0.0;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+  V16<T>* xv = reinterpret_cast<V16<T>*>(x);
+  V16<T>* rv = reinterpret_cast<V16<T>*>(r);
+  const V16<T>* pv = reinterpret_cast<const V16<T>*>(p);
+  const V16<T>* av = reinterpret_cast<const V16<T>*>(Ap);
+  for (long long i = gid; i < nv; i += stride) {
+    V16<T> X = xv[i], R = rv[i];
+    const V16<T> Pp = pv[i], AP = av[i];
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+      X.e[k] = X.e[k] + a * Pp.e[k];
+      R.e[k] = R.e[k] - a * AP.e[k];
+      acc += (double)R.e[k] * (double)R.e[k];
+    }
+    xv[i] = X;
+    rv[i] = R;
+  }
+  for (long long i = nv * W + gid; i < n; i += stride) {
     x[i] = x[i] + a * p[i];
     const T rn = r[i] - a * Ap[i];
     r[i] = rn;
     acc += (double)rn * (double)rn;
   }
-  const double s = block_sum(acc);
-  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+  const double sm = block_sum(acc);
+  if (threadIdx.x == 0) partial[blockIdx.x] = sm;
 }
 
 static __global__ void k_cg_beta(CGState* st, const double* __restrict__ partial, int np,
@@ -574,10 +600,21 @@ template <typename T>
 __global__ void k_cg_pupdate(long long n, T* __restrict__ p, const T* __restrict__ r,
                              const CGState* __restrict__ st) {
   if (st->halt) return;
+  constexpr int W = V16<T>::W;
   const T bta = T(st->beta);
   const long long stride = (long long)gridDim.x * blockDim.x;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-    p[i] = r[i] + bta * p[i];
+  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long nv = n / W;
+  V16<T>* pv = reinterpret_cast<V16<T>*>(p);
+  const V16<T>* rv = reinterpret_cast<const V16<T>*>(r);
+  for (long long i = gid; i < nv; i += stride) {
+    V16<T> Pp = pv[i];
+    const V16<T> R = rv[i];
+#pragma unroll
+    for (int k = 0; k < W; ++k) Pp.e[k] = R.e[k] + bta * Pp.e[k];
+    pv[i] = Pp;
+  }
+  for (long long i = nv * W + gid; i < n; i += stride) p[i] = r[i] + bta * p[i];
 }
 
 // x64 += x32, x32 = 0 (mixed precision: fold the single-precision correction)
