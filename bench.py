@@ -133,6 +133,13 @@ def main():
             name, ms_k.sum(), np.round(ms_k, 4).tolist(), prof[name]['matvec_GBps']))
         del x, y
     if args.profile_only:
+        # plain device copy of a known size: the reference point for HBM byte counters
+        src = nat.DeviceArray((1 << 27,), np.float64)   # 1 GiB
+        dst = nat.DeviceArray((1 << 27,), np.float64)
+        for _ in range(3):
+            dst.copy_from(src)
+        nat.synchronize()
+        log('copy probe: 3 x 1 GiB device-to-device copies')
         return
     k32 = prof['fp32']
     dom = int(np.argmax(k32['kernel_ms']))
