@@ -51,6 +51,57 @@ __device__ __forceinline__ void dft_any(cpx<T> (&v)[R]) {
     dft<T, R, DIR>(v);
 }
 
+
+// ---- buffer-resource addressing ------------------------------------------------------
+// A buffer resource holds the 64-bit base in scalar registers; every access then needs
+// only a 32-bit per-lane byte offset, and reads past num_records return 0 (so a ragged
+// tile can load unconditionally). Word 3 = 0x00020000 is the raw-buffer format of the
+// gfx9 family (gfx950 included).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base, unsigned long long bytes) {
+  const unsigned nb = bytes > 0xFFFFFFF0ull ? 0xFFFFFFF0u : (unsigned)bytes;
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)nb, 0x00020000);
+}
+
+template <typename T>
+__device__ __forceinline__ cpx<T> bload_c(__amdgpu_buffer_rsrc_t r, unsigned off) {
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(cpx<T>, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+  } else {
+    return __builtin_bit_cast(cpx<T>, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void bstore_c(__amdgpu_buffer_rsrc_t r, unsigned off, cpx<T> v) {
+  if constexpr (sizeof(T) == 4) {
+    using V = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V, v), r, off, 0, 0);
+  } else {
+    using V = decltype(__builtin_amdgcn_raw_buffer_load_b128(r, 0, 0, 0));
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(V, v), r, off, 0, 0);
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ T bload_r(__amdgpu_buffer_rsrc_t r, unsigned off) {
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+  } else {
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void bstore_r(__amdgpu_buffer_rsrc_t r, unsigned off, T v) {
+  if constexpr (sizeof(T) == 4) {
+    using V = decltype(__builtin_amdgcn_raw_buffer_load_b32(r, 0, 0, 0));
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(V, v), r, off, 0, 0);
+  } else {
+    using V = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V, v), r, off, 0, 0);
+  }
+}
+
 template <int L_, int TPP_, int... Rs>
 struct FxPlan {
   static constexpr int L = L_;
@@ -157,12 +208,15 @@ struct FxShape {
 };
 
 // ---- strided pass: forward (DIR -1) or inverse (DIR +1) along a non-contiguous axis ----
+// total: number of complex values in the array (for the buffer range); all byte
+// offsets of one launch fit in 32 bits (checked by the launcher).
 template <typename T, class FP, int P, int DIR>
 __global__ void __launch_bounds__(FP::TPP * P)
-k_cols_fx(cpx<T>* __restrict__ data, long long ostride, long long inner, int ntiles,
+k_cols_fx(cpx<T>* __restrict__ data, long long ostride, long long inner, int ntiles, long long total,
           const cpx<T>* __restrict__ twg, const int* __restrict__ halt) {
   if (halt && *halt) return;
   constexpr int L = FP::L;
+  constexpr unsigned CS = sizeof(cpx<T>);
   __shared__ cpx<T> s[L * P];
   __shared__ cpx<T> twl[L];
   const int tid = threadIdx.x;
@@ -170,16 +224,18 @@ k_cols_fx(cpx<T>* __restrict__ data, long long ostride, long long inner, int nti
   const int lin = xcd_remap(blockIdx.x, gridDim.x);
   const int o = lin / ntiles, tile = lin - o * ntiles;
   const long long col0 = (long long)tile * P;
+  const long long start = (long long)o * ostride + col0;
   const int ncol = (int)min((long long)P, inner - col0);
-  cpx<T>* base = data + (long long)o * ostride + col0;
+  const __amdgpu_buffer_rsrc_t rs = make_rsrc(data + start, (unsigned long long)(total - start) * CS);
+  const unsigned rowb = (unsigned)inner * CS;
+  const unsigned qb = (unsigned)q * CS;
   for (int e = tid; e < L; e += FP::TPP * P) twl[e] = twg[e];
   const bool act = q < ncol;
-  const cpx<T> zero = mk(T(0), T(0));
   FxStage<T, FP, 0, DIR> s0;
-  s0.load(t, [&](int j) { return act ? base[(long long)j * inner + q] : zero; });
+  s0.load(t, [=](int j) { return bload_c<T>(rs, (unsigned)j * rowb + qb); });
   s0.compute(t, twl);
   if constexpr (FP::NST == 1) {
-    if (act) s0.store(t, [&](int j, cpx<T> v) { base[(long long)j * inner + q] = v; });
+    if (act) s0.store(t, [=](int j, cpx<T> v) { bstore_c<T>(rs, (unsigned)j * rowb + qb, v); });
   } else {
     s0.store(t, [&](int j, cpx<T> v) { s[j * P + q] = v; });
     __syncthreads();
@@ -188,7 +244,7 @@ k_cols_fx(cpx<T>* __restrict__ data, long long ostride, long long inner, int nti
     FxStage<T, FP, FP::NST - 1, DIR> sl;
     sl.load(t, [&](int j) { return s[j * P + q]; });
     sl.compute(t, twl);
-    if (act) sl.store(t, [&](int j, cpx<T> v) { base[(long long)j * inner + q] = v; });
+    if (act) sl.store(t, [=](int j, cpx<T> v) { bstore_c<T>(rs, (unsigned)j * rowb + qb, v); });
   }
 }
 
@@ -288,8 +344,8 @@ __device__ __forceinline__ void apply_G_fx(cpx<T> (&v)[NC], int k0, int k1, int 
 // frequency point f of the tile for all NC components.
 template <typename T, class FP, class FPR, int F, int NC>
 __global__ void __launch_bounds__(FP::TPP * F)
-k_cols_g_fx(cpx<T>* __restrict__ data, long long inner, int ntiles, const cpx<T>* __restrict__ twg,
-            GParams g, int n2h, const int* __restrict__ halt) {
+k_cols_g_fx(cpx<T>* __restrict__ data, long long inner, int ntiles, long long total,
+            const cpx<T>* __restrict__ twg, GParams g, int n2h, const int* __restrict__ halt) {
   static_assert(FP::TPP == FPR::TPP && FP::L == FPR::L, "schedules must match");
   static_assert(FP::R[FP::NST - 1] == FPR::R[0], "inverse must start with the last forward radix");
   if (halt && *halt) return;
@@ -302,10 +358,12 @@ k_cols_g_fx(cpx<T>* __restrict__ data, long long inner, int ntiles, const cpx<T>
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
   const long long col0 = (long long)tile * P;
   const int ncol = (int)min((long long)P, inner - col0);
-  cpx<T>* base = data + col0;
+  constexpr unsigned CS = sizeof(cpx<T>);
+  const __amdgpu_buffer_rsrc_t rs = make_rsrc(data + col0, (unsigned long long)(total - col0) * CS);
+  const unsigned rowb = (unsigned)inner * CS;
+  const unsigned fb = (unsigned)(f * NC) * CS;
   for (int e = tid; e < L; e += FP::TPP * F) twl[e] = twg[e];
   const bool act = f * NC < ncol;
-  const cpx<T> zero = mk(T(0), T(0));
   const long long fp = col0 / NC + f;  // frequency point in the inner (plane) grid
   int ka, kb;                          // the two fixed frequency indices of this pencil
   if (g.n0 > 1) {
@@ -322,7 +380,7 @@ k_cols_g_fx(cpx<T>* __restrict__ data, long long inner, int ntiles, const cpx<T>
   if constexpr (NF == 1) {
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
-      last[c].load(t, [&](int j) { return act ? base[(long long)j * inner + f * NC + c] : zero; });
+      last[c].load(t, [=](int j) { return bload_c<T>(rs, (unsigned)j * rowb + fb + c * CS); });
       last[c].compute(t, twl);
     }
   } else {
@@ -330,7 +388,7 @@ k_cols_g_fx(cpx<T>* __restrict__ data, long long inner, int ntiles, const cpx<T>
       FxStage<T, FP, 0, -1> s0[NC];
 #pragma unroll
       for (int c = 0; c < NC; ++c) {
-        s0[c].load(t, [&](int j) { return act ? base[(long long)j * inner + f * NC + c] : zero; });
+        s0[c].load(t, [=](int j) { return bload_c<T>(rs, (unsigned)j * rowb + fb + c * CS); });
         s0[c].compute(t, twl);
       }
 #pragma unroll
@@ -376,7 +434,7 @@ k_cols_g_fx(cpx<T>* __restrict__ data, long long inner, int ntiles, const cpx<T>
 #pragma unroll
         for (int j = 0; j < SL::R; ++j) r0.v[m][j] = last[c].v[m][j];
       r0.compute(t, twl);
-      if (act) r0.store(t, [&](int j, cpx<T> v) { base[(long long)j * inner + f * NC + c] = v; });
+      if (act) r0.store(t, [=](int j, cpx<T> v) { bstore_c<T>(rs, (unsigned)j * rowb + fb + c * CS, v); });
     }
   } else {
     if constexpr (NF > 1) __syncthreads();  // LDS reads of the last forward stage are done
@@ -397,135 +455,34 @@ k_cols_g_fx(cpx<T>* __restrict__ data, long long inner, int ntiles, const cpx<T>
       FxStage<T, FPR, NI - 1, +1> rl;
       rl.load(t, [&](int j) { return lds(j, c); });
       rl.compute(t, twl);
-      if (act) rl.store(t, [&](int j, cpx<T> v) { base[(long long)j * inner + f * NC + c] = v; });
+      if (act) rl.store(t, [=](int j, cpx<T> v) { bstore_c<T>(rs, (unsigned)j * rowb + fb + c * CS, v); });
     }
   }
 }
 
 
 
-// ---- projection pass, one thread per column ------------------------------------------
-// Same schedule as k_cols_g_fx, but a thread owns a single column q = f*NC + c of the
-// tile (like the plain strided pass: coalesced rows, few registers, high occupancy).
-// After the last forward stage each thread publishes its values in LDS, reads the NC
-// components of its frequencies, applies G(xi) and keeps its own component.
-template <typename T, class FP, class FPR, int F, int NC>
-__global__ void __launch_bounds__(FP::TPP * F * NC)
-k_cols_gx_fx(cpx<T>* __restrict__ data, long long inner, int ntiles, const cpx<T>* __restrict__ twg,
-             GParams g, int n2h, const int* __restrict__ halt) {
-  static_assert(FP::TPP == FPR::TPP && FP::L == FPR::L, "schedules must match");
-  static_assert(FP::R[FP::NST - 1] == FPR::R[0], "inverse must start with the last forward radix");
-  if (halt && *halt) return;
-  constexpr int L = FP::L;
-  constexpr int P = F * NC;
-  __shared__ cpx<T> s[L * P];
-  __shared__ cpx<T> twl[L];
-  const int tid = threadIdx.x;
-  const int q = tid % P, t = tid / P;
-  const int f = q / NC, c = q - f * NC;
-  const int tile = xcd_remap(blockIdx.x, gridDim.x);
-  const long long col0 = (long long)tile * P;
-  const int ncol = (int)min((long long)P, inner - col0);
-  cpx<T>* base = data + col0;
-  for (int e = tid; e < L; e += FP::TPP * P) twl[e] = twg[e];
-  const bool act = q < ncol;
-  const cpx<T> zero = mk(T(0), T(0));
-  const long long fp = col0 / NC + f;
-  int ka, kb;
-  if (g.n0 > 1) {
-    ka = (int)(fp / n2h);
-    kb = (int)(fp - (long long)ka * n2h);
-  } else {
-    ka = 0;
-    kb = (int)fp;
-  }
-  constexpr int NF = FP::NST;
-  using SL = FxStage<T, FP, NF - 1, -1>;
-  SL last;
-  auto at = [&](int j, int) -> cpx<T>& { return s[j * P + q]; };
-  if constexpr (NF == 1) {
-    last.load(t, [&](int j) { return act ? base[(long long)j * inner + q] : zero; });
-    last.compute(t, twl);
-  } else {
-    FxStage<T, FP, 0, -1> s0;
-    s0.load(t, [&](int j) { return act ? base[(long long)j * inner + q] : zero; });
-    s0.compute(t, twl);
-    s0.store(t, [&](int j, cpx<T> v) { s[j * P + q] = v; });
-    __syncthreads();
-    fx_lds_stages<T, FP, -1, 1, NF - 1, 1>(t, at, twl);
-    last.load(t, [&](int j) { return s[j * P + q]; });
-    last.compute(t, twl);
-    __syncthreads();
-  }
-  // publish, then project: outputs of the last forward stage sit at i + j*p
-#pragma unroll
-  for (int m = 0; m < SL::M; ++m) {
-    const int i = t + m * SL::TPP;
-    if (SL::valid(i)) {
-#pragma unroll
-      for (int j = 0; j < SL::R; ++j) s[(i + j * SL::p) * P + q] = last.v[m][j];
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int m = 0; m < SL::M; ++m) {
-    const int i = t + m * SL::TPP;
-    if (SL::valid(i)) {
-#pragma unroll
-      for (int j = 0; j < SL::R; ++j) {
-        const int kk = i + j * SL::p;
-        cpx<T> v[NC];
-#pragma unroll
-        for (int cc = 0; cc < NC; ++cc) v[cc] = s[kk * P + f * NC + cc];
-        if (g.n0 > 1)
-          apply_G_fx<T, NC>(v, kk, ka, kb, g);
-        else
-          apply_G_fx<T, NC>(v, 0, kk, kb, g);
-        cpx<T> mine = v[0];
-#pragma unroll
-        for (int cc = 1; cc < NC; ++cc)
-          if (cc == c) mine = v[cc];
-        last.v[m][j] = mine;
-      }
-    }
-  }
-  __syncthreads();
-  constexpr int NI = FPR::NST;
-  FxStage<T, FPR, 0, +1> r0;
-#pragma unroll
-  for (int m = 0; m < SL::M; ++m)
-#pragma unroll
-    for (int j = 0; j < SL::R; ++j) r0.v[m][j] = last.v[m][j];
-  r0.compute(t, twl);
-  if constexpr (NI == 1) {
-    if (act) r0.store(t, [&](int j, cpx<T> v) { base[(long long)j * inner + q] = v; });
-  } else {
-    r0.store(t, [&](int j, cpx<T> v) { s[j * P + q] = v; });
-    __syncthreads();
-    fx_lds_stages<T, FPR, +1, 1, NI - 1, 1>(t, at, twl);
-    FxStage<T, FPR, NI - 1, +1> rl;
-    rl.load(t, [&](int j) { return s[j * P + q]; });
-    rl.compute(t, twl);
-    if (act) rl.store(t, [&](int j, cpx<T> v) { base[(long long)j * inner + q] = v; });
-  }
-}
-
-// value of component c of A(x) v at voxel vox (runtime ncomp), for the contiguous-axis pass
-template <typename T, int COEF>
-__device__ __forceinline__ T coef_apply(const T* __restrict__ in, const T* __restrict__ coef, long long vox,
-                                        int c, int nc, long long n) {
+// value of component c of A(x) v at voxel vox, for the contiguous-axis pass; byte
+// offsets: vb = vox*sizeof(T), nb = n*sizeof(T) (component / coefficient stride)
+template <typename T, int COEF, int NC>
+__device__ __forceinline__ T coef_apply(__amdgpu_buffer_rsrc_t rin, __amdgpu_buffer_rsrc_t rco, unsigned vb,
+                                        int c, unsigned nb) {
   if constexpr (COEF == COEF_NONE) {
-    return in[c * n + vox];
+    return bload_r<T>(rin, (unsigned)c * nb + vb);
   } else if constexpr (COEF == COEF_ISO) {
-    return coef[vox] * in[c * n + vox];
+    return bload_r<T>(rco, vb) * bload_r<T>(rin, (unsigned)c * nb + vb);
   } else if constexpr (COEF == COEF_LAME) {
-    const int dn = (nc == 6) ? 3 : 2;
+    constexpr int dn = (NC == 6) ? 3 : 2;
     T tr = T(0);
-    for (int d = 0; d < dn; ++d) tr += in[d * n + vox];
-    return T(2) * coef[n + vox] * in[c * n + vox] + (c < dn ? coef[vox] * tr : T(0));
+#pragma unroll
+    for (int d = 0; d < dn; ++d) tr += bload_r<T>(rin, (unsigned)d * nb + vb);
+    return T(2) * bload_r<T>(rco, nb + vb) * bload_r<T>(rin, (unsigned)c * nb + vb) +
+           (c < dn ? bload_r<T>(rco, vb) * tr : T(0));
   } else {
     T acc = T(0);
-    for (int d = 0; d < nc; ++d) acc += coef[(long long)(c * nc + d) * n + vox] * in[d * n + vox];
+#pragma unroll
+    for (int d = 0; d < NC; ++d)
+      acc += bload_r<T>(rco, (unsigned)(c * NC + d) * nb + vb) * bload_r<T>(rin, (unsigned)d * nb + vb);
     return acc;
   }
 }
@@ -554,11 +511,16 @@ k_rows_fwd_fx(const T* __restrict__ in, const T* __restrict__ coef, cpx<T>* __re
   const int rowa = 2 * rp + rra / nc, ca = rra % nc;
   const int rowb = 2 * rp + rrb / nc, cb = rrb % nc;
   const bool va = rowa < nr, vb = rowb < nr;
-  const long long ba = (row0 + rowa) * L, bb = (row0 + rowb) * L;
+  constexpr unsigned SR = sizeof(T);
+  const unsigned nbytes = (unsigned)n * SR;
+  constexpr int kvals = COEF == COEF_NONE ? 0 : (COEF == COEF_ISO ? 1 : (COEF == COEF_LAME ? 2 : NC * NC));
+  const __amdgpu_buffer_rsrc_t rin = make_rsrc(in, (unsigned long long)NC * n * SR);
+  const __amdgpu_buffer_rsrc_t rco = make_rsrc(coef ? coef : in, (unsigned long long)kvals * n * SR);
+  const unsigned ba = (unsigned)((row0 + rowa) * L) * SR, bb = (unsigned)((row0 + rowb) * L) * SR;
   FxStage<T, FP, 0, -1> s0;
-  s0.load(t, [&](int x) {
-    const T ya = va ? coef_apply<T, COEF>(in, coef, ba + x, ca, nc, n) : T(0);
-    const T yb = vb ? coef_apply<T, COEF>(in, coef, bb + x, cb, nc, n) : T(0);
+  s0.load(t, [=](int x) {
+    const T ya = va ? coef_apply<T, COEF, NC>(rin, rco, ba + x * SR, ca, nbytes) : T(0);
+    const T yb = vb ? coef_apply<T, COEF, NC>(rin, rco, bb + x * SR, cb, nbytes) : T(0);
     return mk(ya, yb);
   });
   s0.compute(t, twl);
@@ -638,10 +600,11 @@ k_rows_inv_fx(const cpx<T>* __restrict__ in, T* __restrict__ out, long long nrow
   const int rowa = 2 * rp + rra / nc, ca = rra % nc;
   const int rowb = 2 * rp + rrb / nc, cb = rrb % nc;
   const bool va = rowa < nr, vb = rowb < nr;
-  T* oa = out + (long long)ca * n + (row0 + rowa) * L;
-  T* obp = out + (long long)cb * n + (row0 + rowb) * L;
-  const T* wa = w ? w + (long long)ca * n + (row0 + rowa) * L : nullptr;
-  const T* wb = w ? w + (long long)cb * n + (row0 + rowb) * L : nullptr;
+  constexpr unsigned SR = sizeof(T);
+  const __amdgpu_buffer_rsrc_t rout = make_rsrc(out, (unsigned long long)NC * n * SR);
+  const __amdgpu_buffer_rsrc_t rw = make_rsrc(w ? w : out, (unsigned long long)NC * n * SR);
+  const unsigned oa = (unsigned)(ca * n + (row0 + rowa) * L) * SR;
+  const unsigned ob = (unsigned)(cb * n + (row0 + rowb) * L) * SR;
   double acc = 0.0;
   FxStage<T, FP, FP::NST - 1, +1> sl;
   sl.load(t, [&](int j) { return sp[j]; });
@@ -649,12 +612,12 @@ k_rows_inv_fx(const cpx<T>* __restrict__ in, T* __restrict__ out, long long nrow
   sl.store(t, [&](int x, cpx<T> v) {
     const T a = scale * v.x, b = scale * v.y;
     if (va) {
-      oa[x] = a;
-      if constexpr (EPI == 1) acc += (double)a * (double)wa[x];
+      bstore_r<T>(rout, oa + x * SR, a);
+      if constexpr (EPI == 1) acc += (double)a * (double)bload_r<T>(rw, oa + x * SR);
     }
     if (vb) {
-      obp[x] = b;
-      if constexpr (EPI == 1) acc += (double)b * (double)wb[x];
+      bstore_r<T>(rout, ob + x * SR, b);
+      if constexpr (EPI == 1) acc += (double)b * (double)bload_r<T>(rw, ob + x * SR);
     }
   });
   if constexpr (EPI == 1) {

@@ -20,6 +20,9 @@ namespace fh {
   X(135, FxP135, FxP135r)     \
   X(243, FxP243, FxP243r)
 
+// the buffer addressing of the fixed kernels uses 32-bit byte offsets
+inline bool offsets_fit(long long elems, size_t esize) { return (unsigned long long)elems * esize < 0xFFFFFF00ull; }
+
 template <class K>
 inline void allow_lds(K kernel, size_t bytes) {
   if (bytes > 64 * 1024) (void)hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -35,6 +38,8 @@ int rows_fwd_fx_launch(const LaunchCtx& c, int nc, const T* in, const T* coef, i
   if (nthr > 1024) return 0;
   const size_t lds = (size_t)npen * FP::L * sizeof(cpx<T>);
   if (lds + FP::L * sizeof(cpx<T>) > 160 * 1024) return 0;
+  const int kv = kind == COEF_NONE ? 0 : (kind == COEF_ISO ? 1 : (kind == COEF_LAME ? 2 : nc * nc));
+  if (!offsets_fit(c.n * (nc > kv ? nc : kv), sizeof(T))) return 0;
   const long long nrows = (long long)c.n0 * c.n1;
   const dim3 grid((unsigned)((nrows + rpb - 1) / rpb)), block(nthr);
   const cpx<T>* tw = reinterpret_cast<const cpx<T>*>(c.tw[2]);
@@ -73,6 +78,7 @@ int rows_inv_fx_launch(const LaunchCtx& c, int nc, const cpx<T>* in, T* out, T s
   if (nthr > 1024) return 0;
   const size_t lds = (size_t)npen * FP::L * sizeof(cpx<T>);
   if (lds + FP::L * sizeof(cpx<T>) > 160 * 1024) return 0;
+  if (!offsets_fit(c.n * nc, sizeof(T))) return 0;
   const long long nrows = (long long)c.n0 * c.n1;
   const long long nb = (nrows + rpb - 1) / rpb;
   if (w && (size_t)nb > partial_cap) return 0;
@@ -119,45 +125,38 @@ int cols_fx_launch(const LaunchCtx& c, int dir, int axis, int nc, cpx<T>* data, 
   constexpr int P = FxShape<FP, T>::P;
   long long outer, ostride, inner;
   strided_geometry(c, axis, nc, &outer, &ostride, &inner);
+  const long long total = c.nh * nc;
+  if (!offsets_fit(total, sizeof(cpx<T>))) return 0;
   const long long ntiles = (inner + P - 1) / P;
   const long long nb = outer * ntiles;
   if (nb > 0x7fffffffLL) return 0;
   const dim3 grid((unsigned)nb), block(FP::TPP * P);
   const cpx<T>* tw = reinterpret_cast<const cpx<T>*>(c.tw[axis]);
   if (dir < 0)
-    hipLaunchKernelGGL((k_cols_fx<T, FP, P, -1>), grid, block, 0, c.stream, data, ostride, inner, (int)ntiles, tw,
-                       halt);
+    hipLaunchKernelGGL((k_cols_fx<T, FP, P, -1>), grid, block, 0, c.stream, data, ostride, inner, (int)ntiles,
+                       total, tw, halt);
   else
-    hipLaunchKernelGGL((k_cols_fx<T, FP, P, +1>), grid, block, 0, c.stream, data, ostride, inner, (int)ntiles, tw,
-                       halt);
+    hipLaunchKernelGGL((k_cols_fx<T, FP, P, +1>), grid, block, 0, c.stream, data, ostride, inner, (int)ntiles,
+                       total, tw, halt);
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? 1 : -(int)e;
 }
 
-template <class FP, typename T, int NC>
-struct GxShape {
-  // frequency points per tile: about 16 (fp32) / 8 (fp64) columns of 8-byte / 16-byte
-  // complex values per thread row, and at most 1024 threads
-  static constexpr int cols = (sizeof(T) == 4 ? 24 : 12) / NC > 0 ? (sizeof(T) == 4 ? 24 : 12) / NC : 1;
-  static constexpr int fill = 512 / (FP::TPP * NC) > 0 ? 512 / (FP::TPP * NC) : 1;
-  static constexpr int want = cols > fill ? cols : fill;
-  static constexpr int cap = 1024 / (FP::TPP * NC) > 0 ? 1024 / (FP::TPP * NC) : 1;
-  static constexpr int F = want < cap ? want : cap;
-};
-
 template <typename T, class FP, class FPR, int NC>
 int cols_g_fx_launch_nc(const LaunchCtx& c, int axis, cpx<T>* data, const GParams& g, const int* halt) {
-  constexpr int F = GxShape<FP, T, NC>::F;
-  if (FP::TPP * F * NC > 1024) return 0;
+  constexpr int F = FxShape<FP, T>::F;
+  if (FP::TPP * F > 1024) return 0;
   long long outer, ostride, inner;
   strided_geometry(c, axis, NC, &outer, &ostride, &inner);
   if (outer != 1) return 0;
+  const long long total = c.nh * NC;
+  if (!offsets_fit(total, sizeof(cpx<T>))) return 0;
   const long long ntiles = (inner + F * NC - 1) / (F * NC);
   if (ntiles > 0x7fffffffLL) return 0;
-  const dim3 grid((unsigned)ntiles), block(FP::TPP * F * NC);
+  const dim3 grid((unsigned)ntiles), block(FP::TPP * F);
   const cpx<T>* tw = reinterpret_cast<const cpx<T>*>(c.tw[axis]);
-  hipLaunchKernelGGL((k_cols_gx_fx<T, FP, FPR, F, NC>), grid, block, 0, c.stream, data, inner, (int)ntiles, tw, g,
-                     c.n2h, halt);
+  hipLaunchKernelGGL((k_cols_g_fx<T, FP, FPR, F, NC>), grid, block, 0, c.stream, data, inner, (int)ntiles, total,
+                     tw, g, c.n2h, halt);
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? 1 : -(int)e;
 }
