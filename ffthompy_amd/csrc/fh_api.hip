@@ -239,11 +239,12 @@ int col_config(const AxisPlan& ap, int unit, size_t csz, long long inner, ColCfg
   return FH_OK;
 }
 
+// rvec != null: `in` is the CG direction p, updated in place to r + beta p first.
 template <typename T>
 int launch_rows_fwd(fh_plan* pl, int nc, const T* in, const T* coef, int kind, cpx<T>* out,
-                    const int* halt) {
+                    const int* halt, const T* rvec = nullptr) {
   if (!pl->generic) {
-    const int r = fh::fixed_rows_fwd<T>(make_ctx<T>(pl), nc, in, coef, kind, out, halt);
+    const int r = fh::fixed_rows_fwd<T>(make_ctx<T>(pl), nc, in, coef, kind, out, halt, rvec, pl->st);
     if (r != 0) return fixed_result(r, "rows forward (fixed)");
   }
   RowCfg c;
@@ -251,26 +252,25 @@ int launch_rows_fwd(fh_plan* pl, int nc, const T* in, const T* coef, int kind, c
   RETCHK(row_config(pl->ax[2], nc, sizeof(cpx<T>), nrows, &c));
   dim3 grid(c.nblocks), block(c.nthr);
   const cpx<T>* tw = twiddles<T>(pl, 2);
+  T* io = const_cast<T*>(in);
+#define FH_GEN_RF(KIND)                                                                                        \
+  do {                                                                                                         \
+    if (rvec)                                                                                                  \
+      hipLaunchKernelGGL((fh::k_rows_fwd<T, KIND, 1>), grid, block, c.lds, pl->stream, io, coef, out, nrows, nc, \
+                         pl->n, c.rows_pb, pl->ax[2], tw, halt, rvec, pl->st);                                 \
+    else                                                                                                       \
+      hipLaunchKernelGGL((fh::k_rows_fwd<T, KIND, 0>), grid, block, c.lds, pl->stream, io, coef, out, nrows, nc, \
+                         pl->n, c.rows_pb, pl->ax[2], tw, halt, rvec, pl->st);                                 \
+  } while (0)
   switch (kind) {
-    case fh::COEF_NONE:
-      hipLaunchKernelGGL((fh::k_rows_fwd<T, fh::COEF_NONE>), grid, block, c.lds, pl->stream, in, coef, out,
-                         nrows, nc, pl->n, c.rows_pb, pl->ax[2], tw, halt);
-      break;
-    case fh::COEF_ISO:
-      hipLaunchKernelGGL((fh::k_rows_fwd<T, fh::COEF_ISO>), grid, block, c.lds, pl->stream, in, coef, out,
-                         nrows, nc, pl->n, c.rows_pb, pl->ax[2], tw, halt);
-      break;
-    case fh::COEF_LAME:
-      hipLaunchKernelGGL((fh::k_rows_fwd<T, fh::COEF_LAME>), grid, block, c.lds, pl->stream, in, coef, out,
-                         nrows, nc, pl->n, c.rows_pb, pl->ax[2], tw, halt);
-      break;
-    case fh::COEF_FULL:
-      hipLaunchKernelGGL((fh::k_rows_fwd<T, fh::COEF_FULL>), grid, block, c.lds, pl->stream, in, coef, out,
-                         nrows, nc, pl->n, c.rows_pb, pl->ax[2], tw, halt);
-      break;
+    case fh::COEF_NONE: FH_GEN_RF(fh::COEF_NONE); break;
+    case fh::COEF_ISO: FH_GEN_RF(fh::COEF_ISO); break;
+    case fh::COEF_LAME: FH_GEN_RF(fh::COEF_LAME); break;
+    case fh::COEF_FULL: FH_GEN_RF(fh::COEF_FULL); break;
     default:
       return fail(FH_ERR_ARG, "bad coefficient kind");
   }
+#undef FH_GEN_RF
   HIPCHK(hipGetLastError());
   return FH_OK;
 }
@@ -341,13 +341,14 @@ int launch_cols(fh_plan* pl, int mode, int axis, int nc, cpx<T>* data, const GPa
 
 // y = F^-1 G F (A x); kind COEF_NONE gives the bare projection. With w != null the
 // last pass also writes per-workgroup partial sums of y.w into pl->partial.
+// rvec != null: x is the CG direction, updated in place to rvec + beta x before use.
 template <typename T>
 int run_operator(fh_plan* pl, const T* x, T* y, int kind, const T* coef, const T* w, const int* halt,
-                 int* nblocks_out) {
+                 int* nblocks_out, const T* rvec = nullptr) {
   const int nc = pl->ncomp;
   cpx<T>* spec = reinterpret_cast<cpx<T>*>(pl->spec);
   const GParams g = make_gparams(pl, 1.0 / (double)pl->n);
-  RETCHK(launch_rows_fwd<T>(pl, nc, x, coef, kind, spec, halt));
+  RETCHK(launch_rows_fwd<T>(pl, nc, x, coef, kind, spec, halt, rvec));
   if (pl->n0 > 1) {
     RETCHK(launch_cols<T>(pl, 0, 1, nc, spec, g, halt));
     RETCHK(launch_cols<T>(pl, 2, 0, nc, spec, g, halt));
@@ -470,19 +471,25 @@ static int ensure_hist(fh_plan* pl, int len) {
   return FH_OK;
 }
 
-// one CG iteration in precision T on vectors (x, r, p, Ap); everything halts on st->halt
+// One CG iteration in precision T on vectors (x, r, p, Ap); everything halts on st->halt.
+// The direction update p = r + beta p of the previous iteration happens inside the first
+// pass of the matvec (beta = 0 with p = r on the first iteration).
+// With a Lame or full coefficient a voxel's components are spread over several threads of
+// the first pass, so the in-place update runs as its own kernel there.
 template <typename T>
 static int enqueue_iteration(fh_plan* pl, T* x, T* r, T* p, T* Ap, const T* coef) {
   const long long nv = (long long)pl->ncomp * pl->n;
   const int* halt = &pl->st->halt;
   int nb = 0;
-  RETCHK(run_operator<T>(pl, p, Ap, pl->coef_kind, coef, p, halt, &nb));
+  const bool fused = pl->coef_kind == fh::COEF_ISO;
+  if (!fused)
+    hipLaunchKernelGGL((fh::k_cg_pupdate<T>), dim3(grid_blocks(nv)), dim3(256), 0, pl->stream, nv, p, r, pl->st);
+  RETCHK(run_operator<T>(pl, p, Ap, pl->coef_kind, coef, p, halt, &nb, fused ? r : nullptr));
   hipLaunchKernelGGL(fh::k_cg_alpha, dim3(1), dim3(256), 0, pl->stream, pl->st, pl->partial, nb);
   const int gb = grid_blocks(nv);
   hipLaunchKernelGGL((fh::k_cg_update<T>), dim3(gb), dim3(256), 0, pl->stream, nv, x, r, p, Ap, pl->st,
                      pl->partial);
   hipLaunchKernelGGL(fh::k_cg_beta, dim3(1), dim3(256), 0, pl->stream, pl->st, pl->partial, gb, pl->hist);
-  hipLaunchKernelGGL((fh::k_cg_pupdate<T>), dim3(gb), dim3(256), 0, pl->stream, nv, p, r, pl->st);
   HIPCHK(hipGetLastError());
   return FH_OK;
 }
@@ -832,7 +839,6 @@ int fh_cg_solve(fh_plan* pl, int mode, const void* b_, void* x_, double tol, int
         const int gb = grid_blocks(nv);
         hipLaunchKernelGGL(fh::k_fold_correction, dim3(gb), dim3(256), 0, pl->stream, nv, x, xc);
         RETCHK(enqueue_true_residual<float>(pl, b, x, Ax, r));
-        hipLaunchKernelGGL((fh::k_cg_pupdate<float>), dim3(gb), dim3(256), 0, pl->stream, nv, p, r, pl->st);
         HIPCHK(hipGetLastError());
       }
     }

@@ -464,25 +464,23 @@ k_cols_g_fx(cpx<T>* __restrict__ data, long long inner, int ntiles, long long to
 
 // value of component c of A(x) v at voxel vox, for the contiguous-axis pass; byte
 // offsets: vb = vox*sizeof(T), nb = n*sizeof(T) (component / coefficient stride)
-template <typename T, int COEF, int NC>
-__device__ __forceinline__ T coef_apply(__amdgpu_buffer_rsrc_t rin, __amdgpu_buffer_rsrc_t rco, unsigned vb,
-                                        int c, unsigned nb) {
+// pv(d, vb) gives component d of the operand at that voxel.
+template <typename T, int COEF, int NC, class PV>
+__device__ __forceinline__ T coef_apply(PV&& pv, __amdgpu_buffer_rsrc_t rco, unsigned vb, int c, unsigned nb) {
   if constexpr (COEF == COEF_NONE) {
-    return bload_r<T>(rin, (unsigned)c * nb + vb);
+    return pv(c, vb);
   } else if constexpr (COEF == COEF_ISO) {
-    return bload_r<T>(rco, vb) * bload_r<T>(rin, (unsigned)c * nb + vb);
+    return bload_r<T>(rco, vb) * pv(c, vb);
   } else if constexpr (COEF == COEF_LAME) {
     constexpr int dn = (NC == 6) ? 3 : 2;
     T tr = T(0);
 #pragma unroll
-    for (int d = 0; d < dn; ++d) tr += bload_r<T>(rin, (unsigned)d * nb + vb);
-    return T(2) * bload_r<T>(rco, nb + vb) * bload_r<T>(rin, (unsigned)c * nb + vb) +
-           (c < dn ? bload_r<T>(rco, vb) * tr : T(0));
+    for (int d = 0; d < dn; ++d) tr += pv(d, vb);
+    return T(2) * bload_r<T>(rco, nb + vb) * pv(c, vb) + (c < dn ? bload_r<T>(rco, vb) * tr : T(0));
   } else {
     T acc = T(0);
 #pragma unroll
-    for (int d = 0; d < NC; ++d)
-      acc += bload_r<T>(rco, (unsigned)(c * NC + d) * nb + vb) * bload_r<T>(rin, (unsigned)d * nb + vb);
+    for (int d = 0; d < NC; ++d) acc += bload_r<T>(rco, (unsigned)(c * NC + d) * nb + vb) * pv(d, vb);
     return acc;
   }
 }
@@ -490,11 +488,13 @@ __device__ __forceinline__ T coef_apply(__amdgpu_buffer_rsrc_t rin, __amdgpu_buf
 // ---- pass 1 (fixed length): A(x) v, then R2C along the contiguous axis -------------
 // Workgroup: rpb rows (all components), (rpb/2)*nc complex pencils of paired real rows,
 // TPP threads per pencil (blockDim = (rpb/2)*nc*TPP). Output: AoS half spectrum.
-template <typename T, class FP, int COEF, int NC>
+// UPD = 1: `in` is the CG direction p, updated in place first as p = r + beta p
+// (beta from the CG state); the operator is then applied to the new p.
+template <typename T, class FP, int COEF, int NC, int UPD>
 __global__ void __launch_bounds__(1024)
 k_rows_fwd_fx(const T* __restrict__ in, const T* __restrict__ coef, cpx<T>* __restrict__ out,
               long long nrows, long long n, int rpb, const cpx<T>* __restrict__ twg,
-              const int* __restrict__ halt) {
+              const int* __restrict__ halt, const T* __restrict__ rvec, const CGState* __restrict__ st) {
   constexpr int nc = NC;
   if (halt && *halt) return;
   constexpr int L = FP::L, n2h = L / 2 + 1, TPP = FP::TPP;
@@ -518,11 +518,35 @@ k_rows_fwd_fx(const T* __restrict__ in, const T* __restrict__ coef, cpx<T>* __re
   const __amdgpu_buffer_rsrc_t rco = make_rsrc(coef ? coef : in, (unsigned long long)kvals * n * SR);
   const unsigned ba = (unsigned)((row0 + rowa) * L) * SR, bb = (unsigned)((row0 + rowb) * L) * SR;
   FxStage<T, FP, 0, -1> s0;
-  s0.load(t, [=](int x) {
-    const T ya = va ? coef_apply<T, COEF, NC>(rin, rco, ba + x * SR, ca, nbytes) : T(0);
-    const T yb = vb ? coef_apply<T, COEF, NC>(rin, rco, bb + x * SR, cb, nbytes) : T(0);
-    return mk(ya, yb);
-  });
+  if constexpr (UPD == 0) {
+    auto pv = [=](int d, unsigned vb_) { return bload_r<T>(rin, (unsigned)d * nbytes + vb_); };
+    s0.load(t, [=](int x) {
+      const T ya = va ? coef_apply<T, COEF, NC>(pv, rco, ba + x * SR, ca, nbytes) : T(0);
+      const T yb = vb ? coef_apply<T, COEF, NC>(pv, rco, bb + x * SR, cb, nbytes) : T(0);
+      return mk(ya, yb);
+    });
+  } else {
+    const T bta = T(st->beta);
+    const __amdgpu_buffer_rsrc_t rr = make_rsrc(rvec, (unsigned long long)NC * n * SR);
+    auto pv = [=](int d, unsigned vb_) {
+      const unsigned o = (unsigned)d * nbytes + vb_;
+      return bload_r<T>(rr, o) + bta * bload_r<T>(rin, o);
+    };
+    s0.load(t, [=](int x) {
+      T ya = T(0), yb = T(0);
+      if (va) {
+        const unsigned o = ba + x * SR;
+        ya = coef_apply<T, COEF, NC>(pv, rco, o, ca, nbytes);
+        bstore_r<T>(rin, (unsigned)ca * nbytes + o, pv(ca, o));
+      }
+      if (vb) {
+        const unsigned o = bb + x * SR;
+        yb = coef_apply<T, COEF, NC>(pv, rco, o, cb, nbytes);
+        bstore_r<T>(rin, (unsigned)cb * nbytes + o, pv(cb, o));
+      }
+      return mk(ya, yb);
+    });
+  }
   s0.compute(t, twl);
   cpx<T>* sp = s + pen * L;
   s0.store(t, [&](int j, cpx<T> v) { sp[j] = v; });

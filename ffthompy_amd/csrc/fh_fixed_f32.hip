@@ -2,7 +2,8 @@
 #include "fh_fixed_impl.hpp"
 
 namespace fh {
-template int fixed_rows_fwd<float>(const LaunchCtx&, int, const float*, const float*, int, cpx<float>*, const int*);
+template int fixed_rows_fwd<float>(const LaunchCtx&, int, const float*, const float*, int, cpx<float>*, const int*,
+                                  const float*, const CGState*);
 template int fixed_rows_inv<float>(const LaunchCtx&, int, const cpx<float>*, float*, float, const float*, double*,
                                    size_t, const int*, int*);
 template int fixed_cols<float>(const LaunchCtx&, int, int, int, cpx<float>*, const int*);

@@ -31,7 +31,7 @@ inline void allow_lds(K kernel, size_t bytes) {
 
 template <typename T, class FP>
 int rows_fwd_fx_launch(const LaunchCtx& c, int nc, const T* in, const T* coef, int kind, cpx<T>* out,
-                       const int* halt) {
+                       const int* halt, const T* rvec, const CGState* st) {
   constexpr int rpb = FxShape<FP, T>::RPB;
   const int npen = (rpb / 2) * nc;
   const int nthr = npen * FP::TPP;
@@ -43,12 +43,22 @@ int rows_fwd_fx_launch(const LaunchCtx& c, int nc, const T* in, const T* coef, i
   const long long nrows = (long long)c.n0 * c.n1;
   const dim3 grid((unsigned)((nrows + rpb - 1) / rpb)), block(nthr);
   const cpx<T>* tw = reinterpret_cast<const cpx<T>*>(c.tw[2]);
-#define FH_LAUNCH_RF(KIND, NCC)                                                                   \
-  do {                                                                                            \
-    auto kf = k_rows_fwd_fx<T, FP, KIND, NCC>;                                                    \
-    allow_lds(kf, lds);                                                                           \
-    hipLaunchKernelGGL(kf, grid, block, lds, c.stream, in, coef, out, nrows, c.n, rpb, tw, halt); \
+#define FH_LAUNCH_RF(KIND, NCC)                                                                       \
+  do {                                                                                                \
+    if (rvec) {                                                                                       \
+      auto kf = k_rows_fwd_fx<T, FP, KIND, NCC, 1>;                                                   \
+      allow_lds(kf, lds);                                                                             \
+      hipLaunchKernelGGL(kf, grid, block, lds, c.stream, in, coef, out, nrows, c.n, rpb, tw, halt, rvec, \
+                         st);                                                                         \
+    } else {                                                                                          \
+      auto kf = k_rows_fwd_fx<T, FP, KIND, NCC, 0>;                                                   \
+      allow_lds(kf, lds);                                                                             \
+      hipLaunchKernelGGL(kf, grid, block, lds, c.stream, in, coef, out, nrows, c.n, rpb, tw, halt, rvec, \
+                         st);                                                                         \
+    }                                                                                                 \
   } while (0)
+  // the fused direction update is only race-free when each thread reads its own component
+  if (rvec && kind != COEF_ISO && kind != COEF_NONE) return 0;
   const int key = kind * 8 + nc;
   switch (key) {
     case COEF_NONE * 8 + 1: FH_LAUNCH_RF(COEF_NONE, 1); break;
@@ -173,10 +183,10 @@ int cols_g_fx_launch(const LaunchCtx& c, int axis, int nc, cpx<T>* data, const G
 
 template <typename T>
 int fixed_rows_fwd(const LaunchCtx& c, int nc, const T* in, const T* coef, int kind, cpx<T>* out,
-                   const int* halt) {
+                   const int* halt, const T* rvec, const CGState* st) {
   switch (c.n2) {
 #define FH_CASE(L, FP, FPR) \
-  case L: return rows_fwd_fx_launch<T, FP>(c, nc, in, coef, kind, out, halt);
+  case L: return rows_fwd_fx_launch<T, FP>(c, nc, in, coef, kind, out, halt, rvec, st);
     FH_FIXED_LENGTHS(FH_CASE)
 #undef FH_CASE
     default: return 0;

@@ -16,6 +16,13 @@ constexpr int kMaxComp = 6;
 
 enum CoefKind { COEF_NONE = 0, COEF_ISO = 1, COEF_LAME = 2, COEF_FULL = 3 };
 
+// Conjugate-gradient scalars on the device (see the CG kernels below).
+struct CGState {
+  double rr, rr_old, pAp, alpha, beta;
+  double tol, delta, rmax, inv_nvox;
+  int kit, maxiter, done, pause, halt, status, mixed, hist_len, nupdates;
+};
+
 // Launch-invariant description of the projection G(xi) of the operator.
 struct GParams {
   int dim;      // spatial dimension of the problem (2 or 3)
@@ -221,11 +228,13 @@ __device__ __forceinline__ dd block_sum_dd(dd v) {
 // A workgroup owns rows_pb consecutive grid rows (all components). Real rows are
 // paired into one complex transform (odd lengths have no half-length packing):
 // within a row pair, the 2*ncomp real rows (row, component) are paired in order.
-template <typename T, int COEF>
+// UPD = 1: `in` is the CG direction p, replaced first by r + beta p (beta from *st).
+template <typename T, int COEF, int UPD>
 __global__ void __launch_bounds__(1024)
-k_rows_fwd(const T* __restrict__ in, const T* __restrict__ coef, cpx<T>* __restrict__ out,
+k_rows_fwd(T* __restrict__ in, const T* __restrict__ coef, cpx<T>* __restrict__ out,
            long long nrows, int nc, long long n, int rows_pb, AxisPlan ap,
-           const cpx<T>* __restrict__ tw, const int* __restrict__ halt) {
+           const cpx<T>* __restrict__ tw, const int* __restrict__ halt, const T* __restrict__ rvec,
+           const CGState* __restrict__ st) {
   if (halt && *halt) return;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<T>* s = reinterpret_cast<cpx<T>*>(smem_raw);
@@ -243,6 +252,16 @@ k_rows_fwd(const T* __restrict__ in, const T* __restrict__ coef, cpx<T>* __restr
       T v[kMaxComp];
 #pragma unroll
       for (int c = 0; c < kMaxComp; ++c) v[c] = (c < nc) ? in[c * n + vox] : T(0);
+      if constexpr (UPD == 1) {
+        const T bta = T(st->beta);
+#pragma unroll
+        for (int c = 0; c < kMaxComp; ++c) {
+          if (c < nc) {
+            v[c] = rvec[c * n + vox] + bta * v[c];
+            in[c * n + vox] = v[c];
+          }
+        }
+      }
       apply_coef<T, COEF>(y, v, nc, coef, vox, n);
     } else {
 #pragma unroll
@@ -425,11 +444,7 @@ __global__ void k_apply_G_soa(const cpx<T>* __restrict__ in, cpx<T>* __restrict_
 // Scalars live on the device; every kernel returns at once when the solve has
 // finished (done) or waits for a host-side reliable update (pause), so the host
 // can enqueue several iterations ahead without reading anything back.
-struct CGState {
-  double rr, rr_old, pAp, alpha, beta;
-  double tol, delta, rmax, inv_nvox;
-  int kit, maxiter, done, pause, halt, status, mixed, hist_len, nupdates;
-};
+
 
 enum CGStatus { CG_OK = 0, CG_BREAKDOWN = 1 };
 

@@ -17,9 +17,10 @@ struct LaunchCtx {
 
 // Return 1 if a specialised kernel was launched, 0 if the length has none (caller falls
 // back to the generic kernels), or a negative hipError_t on launch failure.
+// rvec != null: `in` is the CG direction p, replaced by r + beta p (beta in *st) first.
 template <typename T>
 int fixed_rows_fwd(const LaunchCtx& c, int nc, const T* in, const T* coef, int kind, cpx<T>* out,
-                   const int* halt);
+                   const int* halt, const T* rvec, const CGState* st);
 template <typename T>
 int fixed_rows_inv(const LaunchCtx& c, int nc, const cpx<T>* in, T* out, T scale, const T* w,
                    double* partial, size_t partial_cap, const int* halt, int* nblocks);
@@ -31,9 +32,9 @@ int fixed_cols_g(const LaunchCtx& c, int axis, int nc, cpx<T>* data, const GPara
 bool fixed_length_supported(int L);
 
 extern template int fixed_rows_fwd<float>(const LaunchCtx&, int, const float*, const float*, int, cpx<float>*,
-                                          const int*);
+                                          const int*, const float*, const CGState*);
 extern template int fixed_rows_fwd<double>(const LaunchCtx&, int, const double*, const double*, int,
-                                           cpx<double>*, const int*);
+                                           cpx<double>*, const int*, const double*, const CGState*);
 extern template int fixed_rows_inv<float>(const LaunchCtx&, int, const cpx<float>*, float*, float, const float*,
                                           double*, size_t, const int*, int*);
 extern template int fixed_rows_inv<double>(const LaunchCtx&, int, const cpx<double>*, double*, double,

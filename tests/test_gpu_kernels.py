@@ -235,3 +235,56 @@ def test_fixed_matvec_matches_oracle_and_generic(physics, N, pd):
         yg = dy.to_host()
         assert rel(yf, ref) < tol, (dt, rel(yf, ref))
         assert rel(yg, ref) < tol, (dt, rel(yg, ref))
+
+
+MIXED_CASES = [('scalar', 'iso', (45, 45, 45), 100.), ('scalar', 'iso', (63, 81), 100.),
+               ('elasticity', 'lame', (27, 27, 27), 10.), ('scalar', 'full', (21, 25, 27), 10.)]
+
+
+@pytest.mark.parametrize('physics,kind,N,contrast', MIXED_CASES, ids=['%s-%s-%s' % c[:3] for c in MIXED_CASES])
+def test_mixed_precision_solve_matches_fp64(physics, kind, N, contrast):
+    """fp32 CG with fp64 reliable updates and a final fp64 projection: the solution and
+    the energy (AH entry) agree with the plain fp64 solve; the fp64 solve agrees with
+    the oracle's CG iteration count."""
+    print('case mixed', physics, kind, N, flush=True)
+    rng = np.random.default_rng(30)
+    Y = np.ones(len(N))
+    plan = nat.Plan(N, Y, physics, 'primal')
+    nc = plan.ncomp
+    if kind == 'iso':
+        a = np.where(rng.random(N) < 0.3, contrast, 1.)
+        vals, A = a, np.einsum('ij,...->ij...', np.eye(nc), a)
+    elif kind == 'lame':
+        ph = rng.random(N) < 0.3
+        lam = np.where(ph, contrast, 1.) * 0.5
+        mu = np.where(ph, contrast, 1.) * 0.4
+        m = np.zeros(nc)
+        m[:3 if nc == 6 else 2] = 1.
+        vals = np.stack([lam, mu])
+        A = np.einsum('i,j,...->ij...', m, m, lam) + 2 * np.einsum('ij,...->ij...', np.eye(nc), mu)
+    else:
+        vals, A = _random_coef(rng, 'full', nc, N, physics)
+    plan.set_coeff(kind, vals)
+    E = np.zeros(nc)
+    E[0] = 1.
+    EN = nat.DeviceArray.zeros((nc,) + N)
+    nat.add_constant(EN, -E)
+    B = nat.DeviceArray((nc,) + N)
+    plan.matvec(EN, B)
+    sols = {}
+    infos = {}
+    for mode in ('fp64', 'mixed'):
+        x = nat.DeviceArray.zeros((nc,) + N)
+        infos[mode] = plan.cg_solve(B, x, tol=1e-8, maxiter=2000, mode=mode, delta=0.01)
+        assert infos[mode]['status'] == nat.FH_CG_OK
+        nat.add_constant(x, E)
+        sols[mode] = x
+    x64, xmx = sols['fp64'].to_host(), sols['mixed'].to_host()
+    assert rel(xmx, x64) < 1e-6, rel(xmx, x64)
+    e64 = np.einsum('i...,i...', np.einsum('ij...,j...->i...', A, x64), x64).mean()
+    emx = np.einsum('i...,i...', np.einsum('ij...,j...->i...', A, xmx), xmx).mean()
+    assert abs(emx - e64) / abs(e64) < 1e-8, (emx, e64)
+    assert infos['mixed']['norm_res'] <= 1e-8
+    if int(np.prod(N)) <= 30000:
+        Bh, xref, info = _oracle_solve(N, Y, physics, 'primal', A, E, 1e-8)
+        assert infos['fp64']['kit'] == info['kit']
