@@ -1,0 +1,46 @@
+"""Homogenization drivers (GaNi): primal and dual solves over the unit loads.
+
+Mirrors ffthompy/applications.py:11-90 (scalar) and 93-176 (elasticity). The operator
+Afun = F^-1 G F A is one GpuOperator (HIP library); CG runs on the device.
+"""
+import numpy as np
+
+from .materials import Material
+from .postprocess import add_macro2minimizer, postprocess
+from .solver import CallBack, linear_solver
+from .tensors import GpuOperator, Tensor
+
+
+def _homogenize(pb, physics):
+    if pb.solve['kind'] != 'GaNi':
+        raise NotImplementedError("solve kind %r: the GPU path solves the GaNi scheme" % pb.solve['kind'])
+    N = tuple(int(n) for n in pb.solve['N'])
+    nload = pb.dim if physics == 'scalar' else pb.dim * (pb.dim + 1) // 2
+    for primaldual in pb.solve['primaldual']:
+        mat = Material(pb.material)
+        A = mat.get_A_GaNi(N, primaldual)
+        Afun = GpuOperator(N, pb.Y, physics, primaldual, A=A)
+        solutions, results = [], []
+        for iL in range(nload):
+            E = np.zeros(nload)
+            E[iL] = 1.
+            EN = Tensor(name='EN', N=N, shape=(nload,), Y=pb.Y)
+            EN.set_mean(E)
+            x0 = Tensor(name='x0', N=N, shape=(nload,), Y=pb.Y)
+            B = Afun(-EN)
+            cb = CallBack(A=Afun, B=B)
+            X, info = linear_solver(Afun=Afun, B=B, x0=x0, par=dict(pb.solver), solver=pb.solver['kind'],
+                                    callback=cb)
+            solutions.append(add_macro2minimizer(X, E))
+            results.append({'cb': cb, 'info': info})
+        postprocess(pb, Afun, mat, solutions, results, primaldual)
+
+
+def scalar(problem):
+    """Scalar elliptic homogenization (applications.py:11-90)."""
+    _homogenize(problem, 'scalar')
+
+
+def elasticity(problem):
+    """Small-strain linear elasticity homogenization, Mandel notation (applications.py:93-176)."""
+    _homogenize(problem, 'elasticity')
