@@ -96,6 +96,8 @@ def main():
     ap.add_argument('--cpu-iters', type=int, default=2, help='CG iterations in the CPU baseline sample')
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
     ap.add_argument('--profile-only', action='store_true', help='matvec profile only (for rocprofv3 runs)')
+    ap.add_argument('--profile-solve', action='store_true',
+                    help='with --profile-only: also one mixed and one fp64 CG solve (load e1, primal)')
     args = ap.parse_args()
     world = int(os.environ.get('WORLD_SIZE', '1'))
     if args.gpus != 1 or world != 1:
@@ -133,6 +135,16 @@ def main():
             name, ms_k.sum(), np.round(ms_k, 4).tolist(), prof[name]['matvec_GBps']))
         del x, y
     if args.profile_only:
+        if args.profile_solve:
+            for mode in ('mixed', 'fp64'):
+                E = np.array([1., 0., 0.])
+                nat.call('fh_memset_zero', H.EN.ptr, H.EN.nbytes)
+                nat.add_constant(H.EN, -E)
+                pl.matvec(H.EN, H.B)
+                x = H.sols['primal'][0]
+                nat.call('fh_memset_zero', x.ptr, x.nbytes)
+                info = pl.cg_solve(H.B, x, tol=1e-8, maxiter=2000, mode=mode, delta=0.01)
+                log('solve %s: kit %d, %.1f ms' % (mode, info['kit'], pl.last_timing()[1]))
         # plain device copy of a known size: the reference point for HBM byte counters
         src = nat.DeviceArray((1 << 27,), np.float64)   # 1 GiB
         dst = nat.DeviceArray((1 << 27,), np.float64)

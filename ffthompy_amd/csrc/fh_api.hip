@@ -281,7 +281,7 @@ int launch_rows_inv(fh_plan* pl, int nc, const cpx<T>* in, T* out, T scale, cons
                     double* partial, const int* halt, int* nblocks) {
   if (!pl->generic) {
     const int r = fh::fixed_rows_inv<T>(make_ctx<T>(pl), nc, in, out, scale, w, partial, pl->partial_cap, halt,
-                                        nblocks);
+                                        nblocks, pl->st);
     if (r != 0) return fixed_result(r, "rows inverse (fixed)");
   }
   RowCfg c;
@@ -292,10 +292,10 @@ int launch_rows_inv(fh_plan* pl, int nc, const cpx<T>* in, T* out, T scale, cons
   const cpx<T>* tw = twiddles<T>(pl, 2);
   if (w)
     hipLaunchKernelGGL((fh::k_rows_inv<T, 1>), grid, block, c.lds, pl->stream, in, out, nrows, nc, pl->n,
-                       c.rows_pb, pl->ax[2], tw, scale, w, partial, halt);
+                       c.rows_pb, pl->ax[2], tw, scale, w, partial, halt, pl->st);
   else
     hipLaunchKernelGGL((fh::k_rows_inv<T, 0>), grid, block, c.lds, pl->stream, in, out, nrows, nc, pl->n,
-                       c.rows_pb, pl->ax[2], tw, scale, w, partial, halt);
+                       c.rows_pb, pl->ax[2], tw, scale, w, partial, halt, pl->st);
   HIPCHK(hipGetLastError());
   if (nblocks) *nblocks = c.nblocks;
   return FH_OK;
@@ -484,12 +484,13 @@ static int enqueue_iteration(fh_plan* pl, T* x, T* r, T* p, T* Ap, const T* coef
   const bool fused = pl->coef_kind == fh::COEF_ISO;
   if (!fused)
     hipLaunchKernelGGL((fh::k_cg_pupdate<T>), dim3(grid_blocks(nv)), dim3(256), 0, pl->stream, nv, p, r, pl->st);
+  // alpha is finalised by the last workgroup of the matvec's last pass, beta (and the stop
+  // test) by the last workgroup of the update
   RETCHK(run_operator<T>(pl, p, Ap, pl->coef_kind, coef, p, halt, &nb, fused ? r : nullptr));
-  hipLaunchKernelGGL(fh::k_cg_alpha, dim3(1), dim3(256), 0, pl->stream, pl->st, pl->partial, nb);
+  (void)nb;
   const int gb = grid_blocks(nv);
   hipLaunchKernelGGL((fh::k_cg_update<T>), dim3(gb), dim3(256), 0, pl->stream, nv, x, r, p, Ap, pl->st,
                      pl->partial);
-  hipLaunchKernelGGL(fh::k_cg_beta, dim3(1), dim3(256), 0, pl->stream, pl->st, pl->partial, gb, pl->hist);
   HIPCHK(hipGetLastError());
   return FH_OK;
 }
@@ -802,6 +803,7 @@ int fh_cg_solve(fh_plan* pl, int mode, const void* b_, void* x_, double tol, int
   s.delta = delta;
   s.inv_nvox = 1.0 / (double)pl->n;
   s.hist_len = hist_len > 0 ? hist_len : 0;
+  s.hist = pl->hist;
   HIPCHK(hipMemcpyAsync(pl->st, &s, sizeof(s), hipMemcpyHostToDevice, pl->stream));
   HIPCHK(hipEventRecord(pl->ev0, pl->stream));
   const int per_check = 8;

@@ -580,7 +580,7 @@ template <typename T, class FP, int EPI, int NC>
 __global__ void __launch_bounds__(1024)
 k_rows_inv_fx(const cpx<T>* __restrict__ in, T* __restrict__ out, long long nrows, long long n,
               int rpb, const cpx<T>* __restrict__ twg, T scale, const T* __restrict__ w,
-              double* __restrict__ partial, const int* __restrict__ halt) {
+              double* __restrict__ partial, const int* __restrict__ halt, CGState* __restrict__ st) {
   constexpr int nc = NC;
   if (halt && *halt) return;
   constexpr int L = FP::L, n2h = L / 2 + 1, TPP = FP::TPP;
@@ -592,6 +592,32 @@ k_rows_inv_fx(const cpx<T>* __restrict__ in, T* __restrict__ out, long long nrow
   const long long row0 = (long long)xcd_remap(blockIdx.x, gridDim.x) * rpb;
   const int nr = (int)min((long long)rpb, nrows - row0);
   for (int e = tid; e < L; e += nthr) twl[e] = twg[e];
+  // this thread's pencil: real rows (rowa, ca) and (rowb, cb) of the block
+  const int rp_ = pen / nc, m_ = pen - rp_ * nc;
+  const int rowa = 2 * rp_ + (2 * m_) / nc, ca = (2 * m_) % nc;
+  const int rowb = 2 * rp_ + (2 * m_ + 1) / nc, cb = (2 * m_ + 1) % nc;
+  const bool va = rowa < nr, vb = rowb < nr;
+  constexpr unsigned SR = sizeof(T);
+  const __amdgpu_buffer_rsrc_t rout = make_rsrc(out, (unsigned long long)NC * n * SR);
+  const unsigned oa = (unsigned)(ca * n + (row0 + rowa) * L) * SR;
+  const unsigned ob = (unsigned)(cb * n + (row0 + rowb) * L) * SR;
+  using SL = FxStage<T, FP, FP::NST - 1, +1>;
+  // the last stage writes positions i + j*p (k = i); prefetch the weights there now so
+  // their latency hides behind the FFT
+  T wa[SL::M][SL::R], wbv[SL::M][SL::R];
+  if constexpr (EPI == 1) {
+    const __amdgpu_buffer_rsrc_t rw = make_rsrc(w, (unsigned long long)NC * n * SR);
+#pragma unroll
+    for (int m = 0; m < SL::M; ++m) {
+      const int i = t + m * SL::TPP;
+#pragma unroll
+      for (int j = 0; j < SL::R; ++j) {
+        const unsigned x = (unsigned)(i + j * SL::p) * SR;
+        wa[m][j] = (va && SL::valid(i)) ? bload_r<T>(rw, oa + x) : T(0);
+        wbv[m][j] = (vb && SL::valid(i)) ? bload_r<T>(rw, ob + x) : T(0);
+      }
+    }
+  }
   const int nrp = rpb >> 1;
   const cpx<T>* ib = in + row0 * (long long)n2h * nc;
   for (int e = tid; e < nrp * n2h; e += nthr) {
@@ -619,34 +645,38 @@ k_rows_inv_fx(const cpx<T>* __restrict__ in, T* __restrict__ out, long long nrow
   cpx<T>* sp = s + pen * L;
   auto at = [&](int j, int) -> cpx<T>& { return sp[j]; };
   fx_lds_stages<T, FP, +1, 0, FP::NST - 1, 1>(t, at, twl);
-  const int rp = pen / nc, m = pen - rp * nc;
-  const int rra = 2 * m, rrb = 2 * m + 1;
-  const int rowa = 2 * rp + rra / nc, ca = rra % nc;
-  const int rowb = 2 * rp + rrb / nc, cb = rrb % nc;
-  const bool va = rowa < nr, vb = rowb < nr;
-  constexpr unsigned SR = sizeof(T);
-  const __amdgpu_buffer_rsrc_t rout = make_rsrc(out, (unsigned long long)NC * n * SR);
-  const __amdgpu_buffer_rsrc_t rw = make_rsrc(w ? w : out, (unsigned long long)NC * n * SR);
-  const unsigned oa = (unsigned)(ca * n + (row0 + rowa) * L) * SR;
-  const unsigned ob = (unsigned)(cb * n + (row0 + rowb) * L) * SR;
   double acc = 0.0;
-  FxStage<T, FP, FP::NST - 1, +1> sl;
+  SL sl;
   sl.load(t, [&](int j) { return sp[j]; });
   sl.compute(t, twl);
-  sl.store(t, [&](int x, cpx<T> v) {
-    const T a = scale * v.x, b = scale * v.y;
-    if (va) {
-      bstore_r<T>(rout, oa + x * SR, a);
-      if constexpr (EPI == 1) acc += (double)a * (double)bload_r<T>(rw, oa + x * SR);
+#pragma unroll
+  for (int m = 0; m < SL::M; ++m) {
+    const int i = t + m * SL::TPP;
+    if (SL::valid(i)) {
+#pragma unroll
+      for (int j = 0; j < SL::R; ++j) {
+        const unsigned x = (unsigned)(i + j * SL::p) * SR;
+        const T a = scale * sl.v[m][j].x, b = scale * sl.v[m][j].y;
+        if (va) {
+          bstore_r<T>(rout, oa + x, a);
+          if constexpr (EPI == 1) acc += (double)a * (double)wa[m][j];
+        }
+        if (vb) {
+          bstore_r<T>(rout, ob + x, b);
+          if constexpr (EPI == 1) acc += (double)b * (double)wbv[m][j];
+        }
+      }
     }
-    if (vb) {
-      bstore_r<T>(rout, ob + x * SR, b);
-      if constexpr (EPI == 1) acc += (double)b * (double)bload_r<T>(rw, ob + x * SR);
-    }
-  });
+  }
   if constexpr (EPI == 1) {
     const double bsum = block_sum(acc);
-    if (tid == 0) partial[blockIdx.x] = bsum;
+    if (publish_and_arrive(bsum, partial, &st->arrive[0])) {
+      const double tot = sum_published(partial, gridDim.x);
+      if (tid == 0) {
+        finalize_alpha(st, tot);
+        __hip_atomic_store(&st->arrive[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
   }
 }
 

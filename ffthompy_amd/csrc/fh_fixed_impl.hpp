@@ -81,7 +81,7 @@ int rows_fwd_fx_launch(const LaunchCtx& c, int nc, const T* in, const T* coef, i
 
 template <typename T, class FP>
 int rows_inv_fx_launch(const LaunchCtx& c, int nc, const cpx<T>* in, T* out, T scale, const T* w,
-                       double* partial, size_t partial_cap, const int* halt, int* nblocks) {
+                       double* partial, size_t partial_cap, const int* halt, int* nblocks, CGState* st) {
   constexpr int rpb = FxShape<FP, T>::RPB;
   const int npen = (rpb / 2) * nc;
   const int nthr = npen * FP::TPP;
@@ -98,7 +98,8 @@ int rows_inv_fx_launch(const LaunchCtx& c, int nc, const cpx<T>* in, T* out, T s
   do {                                                                                                        \
     auto kf = k_rows_inv_fx<T, FP, EPI, NCC>;                                                                 \
     allow_lds(kf, lds);                                                                                       \
-    hipLaunchKernelGGL(kf, grid, block, lds, c.stream, in, out, nrows, c.n, rpb, tw, scale, w, partial, halt); \
+    hipLaunchKernelGGL(kf, grid, block, lds, c.stream, in, out, nrows, c.n, rpb, tw, scale, w, partial, halt,  \
+                       st);                                                                               \
   } while (0)
   const int key = (w ? 8 : 0) + nc;
   switch (key) {
@@ -195,10 +196,10 @@ int fixed_rows_fwd(const LaunchCtx& c, int nc, const T* in, const T* coef, int k
 
 template <typename T>
 int fixed_rows_inv(const LaunchCtx& c, int nc, const cpx<T>* in, T* out, T scale, const T* w,
-                   double* partial, size_t partial_cap, const int* halt, int* nblocks) {
+                   double* partial, size_t partial_cap, const int* halt, int* nblocks, CGState* st) {
   switch (c.n2) {
 #define FH_CASE(L, FP, FPR) \
-  case L: return rows_inv_fx_launch<T, FP>(c, nc, in, out, scale, w, partial, partial_cap, halt, nblocks);
+  case L: return rows_inv_fx_launch<T, FP>(c, nc, in, out, scale, w, partial, partial_cap, halt, nblocks, st);
     FH_FIXED_LENGTHS(FH_CASE)
 #undef FH_CASE
     default: return 0;

@@ -21,6 +21,8 @@ struct CGState {
   double rr, rr_old, pAp, alpha, beta;
   double tol, delta, rmax, inv_nvox;
   int kit, maxiter, done, pause, halt, status, mixed, hist_len, nupdates;
+  unsigned arrive[2];  // workgroup arrival counters of the alpha / beta reductions
+  double* hist;        // device residual log (nullable)
 };
 
 // Launch-invariant description of the projection G(xi) of the operator.
@@ -224,6 +226,61 @@ __device__ __forceinline__ dd block_sum_dd(dd v) {
   return r;
 }
 
+// ---- last-workgroup reductions -------------------------------------------------------
+// Every workgroup publishes its partial sum, then counts itself in with an agent-scope
+// acquire-release atomic; the last one to arrive sums all partials in a fixed order
+// (bitwise reproducible) and updates the CG scalars, so no separate finalize kernel is
+// launched. The counter is reset by that last workgroup.
+__device__ __forceinline__ bool publish_and_arrive(double blocksum, double* partial, unsigned* counter) {
+  __shared__ int is_last;
+  if (threadIdx.x == 0) {
+    __hip_atomic_store(&partial[blockIdx.x], blocksum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    is_last = (prev == gridDim.x - 1) ? 1 : 0;
+  }
+  __syncthreads();
+  return is_last != 0;
+}
+
+__device__ __forceinline__ double sum_published(const double* partial, int np) {
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x)
+    acc += __hip_atomic_load(&partial[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return block_sum(acc);
+}
+
+// alpha = rr / <p, A p>; breakdown (<p,Ap> <= 0 or not finite) stops the solve.
+__device__ __forceinline__ void finalize_alpha(CGState* st, double psum) {
+  const double pAp = psum * st->inv_nvox;
+  st->pAp = pAp;
+  if (!(pAp > 0.0) || !isfinite(pAp)) {
+    st->status = 1;
+    st->done = 1;
+    st->halt = 1;
+  } else {
+    st->alpha = st->rr / pAp;
+  }
+}
+
+// rr <- <r,r>; beta = rr_new / rr_old; iteration count, residual log and stop test.
+__device__ __forceinline__ void finalize_beta(CGState* st, double rsum) {
+  const double rr = rsum * st->inv_nvox;
+  st->rr_old = st->rr;
+  st->rr = rr;
+  st->beta = rr / st->rr_old;
+  st->kit += 1;
+  const double nr = sqrt(rr);
+  if (st->hist && st->kit < st->hist_len) st->hist[st->kit] = nr;
+  const bool conv = !(nr > st->tol);
+  const bool maxed = st->kit >= st->maxiter;
+  if (st->mixed) {
+    if (conv || nr < st->delta * st->rmax || maxed) st->pause = 1;
+  } else {
+    st->done = conv || maxed;
+  }
+  st->halt = st->done | st->pause;
+}
+
 // ---- pass 1: coefficient multiply + real-to-complex FFT along the contiguous axis ----
 // A workgroup owns rows_pb consecutive grid rows (all components). Real rows are
 // paired into one complex transform (odd lengths have no half-length packing):
@@ -306,7 +363,8 @@ template <typename T, int EPI>
 __global__ void __launch_bounds__(1024)
 k_rows_inv(const cpx<T>* __restrict__ in, T* __restrict__ out, long long nrows, int nc,
            long long n, int rows_pb, AxisPlan ap, const cpx<T>* __restrict__ tw, T scale,
-           const T* __restrict__ w, double* __restrict__ partial, const int* __restrict__ halt) {
+           const T* __restrict__ w, double* __restrict__ partial, const int* __restrict__ halt,
+           CGState* __restrict__ st) {
   if (halt && *halt) return;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<T>* s = reinterpret_cast<cpx<T>*>(smem_raw);
@@ -356,7 +414,13 @@ k_rows_inv(const cpx<T>* __restrict__ in, T* __restrict__ out, long long nrows, 
   }
   if constexpr (EPI == 1) {
     const double b = block_sum(acc);
-    if (tid == 0) partial[blockIdx.x] = b;
+    if (publish_and_arrive(b, partial, &st->arrive[0])) {
+      const double tot = sum_published(partial, gridDim.x);
+      if (tid == 0) {
+        finalize_alpha(st, tot);
+        __hip_atomic_store(&st->arrive[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
   }
 }
 
@@ -523,24 +587,6 @@ static __global__ void k_cg_restart(CGState* st, const double* __restrict__ part
   }
 }
 
-static __global__ void k_cg_alpha(CGState* st, const double* __restrict__ partial, int np) {
-  if (st->halt) return;
-  double acc = 0.0;
-  for (int i = threadIdx.x; i < np; i += blockDim.x) acc += partial[i];
-  const double b = block_sum(acc);
-  if (threadIdx.x == 0) {
-    const double pAp = b * st->inv_nvox;
-    st->pAp = pAp;
-    if (!(pAp > 0.0) || !isfinite(pAp)) {
-      st->status = CG_BREAKDOWN;
-      st->done = 1;
-      st->halt = 1;
-    } else {
-      st->alpha = st->rr / pAp;
-    }
-  }
-}
-
 // 16-byte vector of T (4 floats or 2 doubles) for the streaming CG kernels.
 template <typename T>
 struct alignas(16) V16 {
@@ -550,7 +596,7 @@ struct alignas(16) V16 {
 
 template <typename T>
 __global__ void k_cg_update(long long n, T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
-                            const T* __restrict__ Ap, const CGState* __restrict__ st,
+                            const T* __restrict__ Ap, CGState* __restrict__ st,
                             double* __restrict__ partial) {
   if (st->halt) return;
   constexpr int W = V16<T>::W;
@@ -582,32 +628,12 @@ __global__ void k_cg_update(long long n, T* __restrict__ x, T* __restrict__ r, c
     acc += (double)rn * (double)rn;
   }
   const double sm = block_sum(acc);
-  if (threadIdx.x == 0) partial[blockIdx.x] = sm;
-}
-
-static __global__ void k_cg_beta(CGState* st, const double* __restrict__ partial, int np,
-                          double* __restrict__ hist) {
-  if (st->halt) return;
-  double acc = 0.0;
-  for (int i = threadIdx.x; i < np; i += blockDim.x) acc += partial[i];
-  const double b = block_sum(acc);
-  if (threadIdx.x == 0) {
-    const double rr = b * st->inv_nvox;
-    st->rr_old = st->rr;
-    st->rr = rr;
-    st->beta = rr / st->rr_old;
-    st->kit += 1;
-    const double nr = sqrt(rr);
-    if (hist && st->kit < st->hist_len) hist[st->kit] = nr;
-    const bool conv = !(nr > st->tol);
-    const bool maxed = st->kit >= st->maxiter;
-    if (st->mixed) {
-      // reliable update when the recursive residual fell by delta, or claims convergence
-      if (conv || nr < st->delta * st->rmax || maxed) st->pause = 1;
-    } else {
-      st->done = conv || maxed;
+  if (publish_and_arrive(sm, partial, &st->arrive[1])) {
+    const double tot = sum_published(partial, gridDim.x);
+    if (threadIdx.x == 0) {
+      finalize_beta(st, tot);
+      __hip_atomic_store(&st->arrive[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    st->halt = st->done | st->pause;
   }
 }
 

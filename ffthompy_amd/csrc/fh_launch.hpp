@@ -21,9 +21,10 @@ struct LaunchCtx {
 template <typename T>
 int fixed_rows_fwd(const LaunchCtx& c, int nc, const T* in, const T* coef, int kind, cpx<T>* out,
                    const int* halt, const T* rvec, const CGState* st);
+// w != null: also <out, w> into the CG state (alpha), by the last workgroup.
 template <typename T>
 int fixed_rows_inv(const LaunchCtx& c, int nc, const cpx<T>* in, T* out, T scale, const T* w,
-                   double* partial, size_t partial_cap, const int* halt, int* nblocks);
+                   double* partial, size_t partial_cap, const int* halt, int* nblocks, CGState* st);
 template <typename T>
 int fixed_cols(const LaunchCtx& c, int dir, int axis, int nc, cpx<T>* data, const int* halt);
 template <typename T>
@@ -36,9 +37,9 @@ extern template int fixed_rows_fwd<float>(const LaunchCtx&, int, const float*, c
 extern template int fixed_rows_fwd<double>(const LaunchCtx&, int, const double*, const double*, int,
                                            cpx<double>*, const int*, const double*, const CGState*);
 extern template int fixed_rows_inv<float>(const LaunchCtx&, int, const cpx<float>*, float*, float, const float*,
-                                          double*, size_t, const int*, int*);
+                                          double*, size_t, const int*, int*, CGState*);
 extern template int fixed_rows_inv<double>(const LaunchCtx&, int, const cpx<double>*, double*, double,
-                                           const double*, double*, size_t, const int*, int*);
+                                           const double*, double*, size_t, const int*, int*, CGState*);
 extern template int fixed_cols<float>(const LaunchCtx&, int, int, int, cpx<float>*, const int*);
 extern template int fixed_cols<double>(const LaunchCtx&, int, int, int, cpx<double>*, const int*);
 extern template int fixed_cols_g<float>(const LaunchCtx&, int, int, cpx<float>*, const GParams&, const int*);
