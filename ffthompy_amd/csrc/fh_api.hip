@@ -484,13 +484,12 @@ static int enqueue_iteration(fh_plan* pl, T* x, T* r, T* p, T* Ap, const T* coef
   const bool fused = pl->coef_kind == fh::COEF_ISO;
   if (!fused)
     hipLaunchKernelGGL((fh::k_cg_pupdate<T>), dim3(grid_blocks(nv)), dim3(256), 0, pl->stream, nv, p, r, pl->st);
-  // alpha is finalised by the last workgroup of the matvec's last pass, beta (and the stop
-  // test) by the last workgroup of the update
   RETCHK(run_operator<T>(pl, p, Ap, pl->coef_kind, coef, p, halt, &nb, fused ? r : nullptr));
-  (void)nb;
+  hipLaunchKernelGGL(fh::k_cg_alpha, dim3(1), dim3(256), 0, pl->stream, pl->st, pl->partial, nb);
   const int gb = grid_blocks(nv);
   hipLaunchKernelGGL((fh::k_cg_update<T>), dim3(gb), dim3(256), 0, pl->stream, nv, x, r, p, Ap, pl->st,
                      pl->partial);
+  hipLaunchKernelGGL(fh::k_cg_beta, dim3(1), dim3(256), 0, pl->stream, pl->st, pl->partial, gb);
   HIPCHK(hipGetLastError());
   return FH_OK;
 }

@@ -670,13 +670,7 @@ k_rows_inv_fx(const cpx<T>* __restrict__ in, T* __restrict__ out, long long nrow
   }
   if constexpr (EPI == 1) {
     const double bsum = block_sum(acc);
-    if (publish_and_arrive(bsum, partial, &st->arrive[0])) {
-      const double tot = sum_published(partial, gridDim.x);
-      if (tid == 0) {
-        finalize_alpha(st, tot);
-        __hip_atomic_store(&st->arrive[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
-    }
+    if (tid == 0) partial[blockIdx.x] = bsum;
   }
 }
 

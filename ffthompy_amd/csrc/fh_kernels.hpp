@@ -21,8 +21,7 @@ struct CGState {
   double rr, rr_old, pAp, alpha, beta;
   double tol, delta, rmax, inv_nvox;
   int kit, maxiter, done, pause, halt, status, mixed, hist_len, nupdates;
-  unsigned arrive[2];  // workgroup arrival counters of the alpha / beta reductions
-  double* hist;        // device residual log (nullable)
+  double* hist;  // device residual log (nullable)
 };
 
 // Launch-invariant description of the projection G(xi) of the operator.
@@ -226,29 +225,7 @@ __device__ __forceinline__ dd block_sum_dd(dd v) {
   return r;
 }
 
-// ---- last-workgroup reductions -------------------------------------------------------
-// Every workgroup publishes its partial sum, then counts itself in with an agent-scope
-// acquire-release atomic; the last one to arrive sums all partials in a fixed order
-// (bitwise reproducible) and updates the CG scalars, so no separate finalize kernel is
-// launched. The counter is reset by that last workgroup.
-__device__ __forceinline__ bool publish_and_arrive(double blocksum, double* partial, unsigned* counter) {
-  __shared__ int is_last;
-  if (threadIdx.x == 0) {
-    __hip_atomic_store(&partial[blockIdx.x], blocksum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    is_last = (prev == gridDim.x - 1) ? 1 : 0;
-  }
-  __syncthreads();
-  return is_last != 0;
-}
-
-__device__ __forceinline__ double sum_published(const double* partial, int np) {
-  double acc = 0.0;
-  for (int i = threadIdx.x; i < np; i += blockDim.x)
-    acc += __hip_atomic_load(&partial[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return block_sum(acc);
-}
-
+// ---- CG scalar updates (run by one workgroup after the reduction kernels) -------------
 // alpha = rr / <p, A p>; breakdown (<p,Ap> <= 0 or not finite) stops the solve.
 __device__ __forceinline__ void finalize_alpha(CGState* st, double psum) {
   const double pAp = psum * st->inv_nvox;
@@ -414,13 +391,7 @@ k_rows_inv(const cpx<T>* __restrict__ in, T* __restrict__ out, long long nrows, 
   }
   if constexpr (EPI == 1) {
     const double b = block_sum(acc);
-    if (publish_and_arrive(b, partial, &st->arrive[0])) {
-      const double tot = sum_published(partial, gridDim.x);
-      if (tid == 0) {
-        finalize_alpha(st, tot);
-        __hip_atomic_store(&st->arrive[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
-    }
+    if (tid == 0) partial[blockIdx.x] = b;
   }
 }
 
@@ -587,6 +558,22 @@ static __global__ void k_cg_restart(CGState* st, const double* __restrict__ part
   }
 }
 
+static __global__ void k_cg_alpha(CGState* st, const double* __restrict__ partial, int np) {
+  if (st->halt) return;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) acc += partial[i];
+  const double b = block_sum(acc);
+  if (threadIdx.x == 0) finalize_alpha(st, b);
+}
+
+static __global__ void k_cg_beta(CGState* st, const double* __restrict__ partial, int np) {
+  if (st->halt) return;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) acc += partial[i];
+  const double b = block_sum(acc);
+  if (threadIdx.x == 0) finalize_beta(st, b);
+}
+
 // 16-byte vector of T (4 floats or 2 doubles) for the streaming CG kernels.
 template <typename T>
 struct alignas(16) V16 {
@@ -628,13 +615,7 @@ __global__ void k_cg_update(long long n, T* __restrict__ x, T* __restrict__ r, c
     acc += (double)rn * (double)rn;
   }
   const double sm = block_sum(acc);
-  if (publish_and_arrive(sm, partial, &st->arrive[1])) {
-    const double tot = sum_published(partial, gridDim.x);
-    if (threadIdx.x == 0) {
-      finalize_beta(st, tot);
-      __hip_atomic_store(&st->arrive[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = sm;
 }
 
 template <typename T>
