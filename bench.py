@@ -74,6 +74,18 @@ class Homogenizer:
         return out
 
 
+def measured_traffic(key):
+    """HBM bytes per launch of a kernel, from the committed rocprofv3 counter summary."""
+    path = os.path.join(ROOT, 'profiles', 'matvec_243_hbm_traffic.md')
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if len(parts) == 3 and parts[0] == 'traffic_bytes' and parts[1] == key:
+            return int(parts[2])
+    return None
+
+
 def timed(fn, steps, warmup, sync):
     for _ in range(warmup):
         fn()
@@ -158,6 +170,7 @@ def main():
     names = ['rows_fwd(A, R2C axis 2)', 'cols axis 1 fwd', 'cols axis 0 fwd+G+inv', 'cols axis 1 inv',
              'rows_inv(C2R axis 2)']
     ach = k32['kernel_bytes'][dom] / (k32['kernel_ms'][dom] * 1e-3) / 1e9
+    traffic = measured_traffic(['k_rows_fwd_fx', 'k_cols_fx', 'k_cols_g_fx', 'k_cols_fx', 'k_rows_inv_fx'][dom] + '_fp32')
 
     # --- timed steps ---
     sync = nat.synchronize
@@ -217,7 +230,9 @@ def main():
         'matvec_kernels_fp32': {'names': names, 'ms': [round(v, 4) for v in k32['kernel_ms']],
                                 'bytes': k32['kernel_bytes']},
         'roofline': {'bound': 'hbm', 'kernel': names[dom], 'achieved': round(ach, 1), 'peak': HBM_PEAK_GBS,
-                     'unit': 'GB/s', 'frac': round(ach / HBM_PEAK_GBS, 4), 'traffic': None},
+                     'unit': 'GB/s', 'frac': round(ach / HBM_PEAK_GBS, 4), 'traffic': traffic,
+                     'traffic_source': 'profiles/matvec_243_hbm_traffic.md (rocprofv3 FETCH_SIZE/WRITE_SIZE)'
+                     if traffic else None},
         'cpu_baseline': cpu,
     }
     print(json.dumps(out), flush=True)
