@@ -46,6 +46,7 @@ SIGNATURES = {
     'fh_cg_solve': [_vp, ctypes.c_int, _vp, _vp, ctypes.c_double, ctypes.c_int, ctypes.c_double,
                     _c_int_p, _c_double_p, _c_double_p, ctypes.c_int, _c_int_p],
     'fh_energy_matrix': [_vp, ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, _c_double_p],
+    'fh_energy_gradient': [_vp, ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, _c_double_p, _vp, _vp],
     'fh_axpby': [ctypes.c_int, ctypes.c_size_t, ctypes.c_double, _vp, ctypes.c_double, _vp],
     'fh_dot': [ctypes.c_int, ctypes.c_size_t, _vp, _vp, _c_double_p],
     'fh_add_constant': [ctypes.c_int, ctypes.c_size_t, ctypes.c_int, _c_double_p, _vp],
@@ -310,6 +311,18 @@ class Plan:
         AH = np.zeros((nf, nf))
         call('fh_energy_matrix', self._h, arr, ctypes.c_int(nf), AH.ctypes.data_as(_c_double_p))
         return AH
+
+    def energy_gradient(self, fields, W, chain=None, out=None):
+        """Device gradient of sum_ij W_ij AH_ij w.r.t. the coefficient (see fh_energy_gradient)."""
+        nf = len(fields)
+        arr = (ctypes.c_void_p * nf)(*[f.ptr.value for f in fields])
+        W = np.ascontiguousarray(W, dtype=np.float64)
+        nvals = 1 if self.coef_kind == 'iso' else 2
+        if out is None:
+            out = DeviceArray((nvals,) + self.N if nvals > 1 else self.N, np.float64)
+        call('fh_energy_gradient', self._h, arr, ctypes.c_int(nf), W.ctypes.data_as(_c_double_p),
+             chain.ptr if chain is not None else None, out.ptr)
+        return out
 
     def last_timing(self):
         mv = ctypes.c_double()

@@ -923,6 +923,60 @@ int fh_energy_matrix(fh_plan* pl, const void* const* fields, int nf, double* AH)
   return rc;
 }
 
+int fh_energy_gradient(fh_plan* pl, const void* const* fields, int nf, const double* W, const void* chain,
+                       void* grad) {
+  RETCHK(check_plan(pl));
+  if (!fields || !W || !grad) return fail(FH_ERR_ARG, "null pointer argument");
+  if (nf != pl->ncomp) return fail(FH_ERR_ARG, "need one field per unit load (nf == ncomp)");
+  if (pl->coef_kind != FH_COEF_ISO && pl->coef_kind != FH_COEF_LAME)
+    return fail(FH_ERR_STATE, "the gradient needs an iso or Lame coefficient");
+  for (int i = 0; i < nf; ++i)
+    if (!fields[i]) return fail(FH_ERR_ARG, "null field pointer");
+  const int nn = nf * nf;
+  void* dptrs = nullptr;
+  double* dW = nullptr;
+  HIPCHK(hipMalloc(&dptrs, nf * sizeof(void*)));
+  hipError_t e = hipMalloc(&dW, nn * sizeof(double));
+  if (e == hipSuccess) e = hipMemcpy(dptrs, fields, nf * sizeof(void*), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dW, W, nn * sizeof(double), hipMemcpyHostToDevice);
+  int rc = FH_OK;
+  if (e != hipSuccess) rc = fail(FH_ERR_HIP, std::string("fh_energy_gradient: ") + hipGetErrorString(e));
+  if (rc == FH_OK) {
+    const double* const* F = (const double* const*)dptrs;
+    const dim3 grid(grid_blocks(pl->n, 256, 4096)), block(256);
+    const double inv_n = 1.0 / (double)pl->n;
+    const double* ch = (const double*)chain;
+    double* g = (double*)grad;
+    const bool iso = pl->coef_kind == FH_COEF_ISO;
+#define FH_EGRAD(NC)                                                                                          \
+  do {                                                                                                        \
+    if (iso)                                                                                                  \
+      hipLaunchKernelGGL((fh::k_energy_grad<NC, fh::COEF_ISO>), grid, block, 0, pl->stream, F, pl->n, dW, inv_n, \
+                         ch, g);                                                                              \
+    else                                                                                                      \
+      hipLaunchKernelGGL((fh::k_energy_grad<NC, fh::COEF_LAME>), grid, block, 0, pl->stream, F, pl->n, dW,     \
+                         inv_n, ch, g);                                                                       \
+  } while (0)
+    if (nf == 2)
+      FH_EGRAD(2);
+    else if (nf == 3)
+      FH_EGRAD(3);
+    else if (nf == 6)
+      FH_EGRAD(6);
+    else
+      rc = fail(FH_ERR_ARG, "unsupported number of components");
+#undef FH_EGRAD
+    if (rc == FH_OK) {
+      e = hipGetLastError();
+      if (e == hipSuccess) e = hipStreamSynchronize(pl->stream);
+      if (e != hipSuccess) rc = fail(FH_ERR_HIP, std::string("fh_energy_gradient: ") + hipGetErrorString(e));
+    }
+  }
+  (void)hipFree(dptrs);
+  (void)hipFree(dW);
+  return rc;
+}
+
 int fh_axpby(int dtype, size_t n, double a, const void* x, double b, void* y) {
   RETCHK(check_dtype(dtype));
   RETCHK(ensure_device());

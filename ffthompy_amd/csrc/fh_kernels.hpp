@@ -726,4 +726,54 @@ static __global__ void k_energy_final(const double* __restrict__ partial, int np
   }
 }
 
+// ---- gradient of the energy forms w.r.t. the coefficient (envelope theorem) -----------
+// For the minimisers e_i of the primal energy, dAH_ij/da(x) = (A'(x) e_i . e_j)/n with the
+// fields held fixed. For a loss L(AH) with W = dL/dAH:
+//   ISO : grad[x]            = chain[x] * sum_ij W_ij (e_i(x).e_j(x)) / n
+//   LAME: grad[x], grad[n+x] = sum_ij W_ij tr e_i tr e_j / n,  sum_ij W_ij 2 e_i.e_j / n
+// chain (nullable) multiplies the ISO gradient per voxel (e.g. a = exp(theta) -> da/dtheta = a).
+template <int NC, int COEF>
+__global__ void __launch_bounds__(256)
+k_energy_grad(const double* const* __restrict__ F, long long n, const double* __restrict__ W, double inv_n,
+              const double* __restrict__ chain, double* __restrict__ grad) {
+  double w[NC * NC];
+#pragma unroll
+  for (int k = 0; k < NC * NC; ++k) w[k] = W[k];
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+    double e[NC][NC];
+#pragma unroll
+    for (int i = 0; i < NC; ++i)
+#pragma unroll
+      for (int c = 0; c < NC; ++c) e[i][c] = F[i][c * n + v];
+    double gdot = 0.0, gtr = 0.0;
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+#pragma unroll
+      for (int j = 0; j < NC; ++j) {
+        double d = 0.0;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) d += e[i][c] * e[j][c];
+        gdot += w[i * NC + j] * d;
+        if constexpr (COEF == COEF_LAME) {
+          constexpr int dn = NC == 6 ? 3 : 2;
+          double ti = 0.0, tj = 0.0;
+#pragma unroll
+          for (int c = 0; c < dn; ++c) {
+            ti += e[i][c];
+            tj += e[j][c];
+          }
+          gtr += w[i * NC + j] * ti * tj;
+        }
+      }
+    }
+    if constexpr (COEF == COEF_ISO) {
+      grad[v] = gdot * inv_n * (chain ? chain[v] : 1.0);
+    } else {
+      grad[v] = gtr * inv_n;
+      grad[n + v] = 2.0 * gdot * inv_n;
+    }
+  }
+}
+
 }  // namespace fh

@@ -126,6 +126,18 @@ int fh_cg_solve(fh_plan* plan, int mode, const void* b, void* x, double tol, int
  * double-double accumulated: assembly_matrix (ffthompy/postprocess.py:53-70). */
 int fh_energy_matrix(fh_plan* plan, const void* const* fields, int nf, double* AH);
 
+/* Gradient of a loss L(AH) w.r.t. the coefficient, for the minimisers e_i (fp64 fields,
+ * macroscopic part included) of the solve: with W = dL/dAH (host, nf x nf) and the fields
+ * held fixed (envelope theorem, the e_i are stationary points of the energy),
+ *   FH_COEF_ISO : grad[x] = chain[x] * sum_ij W_ij e_i(x).e_j(x) / n        (n values)
+ *   FH_COEF_LAME: grad[x] = sum_ij W_ij tr e_i tr e_j / n,
+ *                 grad[n+x] = sum_ij W_ij 2 e_i.e_j / n                        (2n values)
+ * chain: optional device per-voxel multiplier (ISO only, e.g. da/dtheta). grad: device fp64.
+ * No reference counterpart (the reference has no derivatives); it differentiates
+ * assembly_matrix (ffthompy/postprocess.py:53-70) through the solve. */
+int fh_energy_gradient(fh_plan* plan, const void* const* fields, int nf, const double* W, const void* chain,
+                       void* grad);
+
 /* Helpers on fp64/fp32 device vectors (Tensor arithmetic, ffthompy/tensors/objects.py:188-245):
  * y = a x + b y; out = sum(x*y) (caller divides by prod(N) for the grid inner product,
  * ffthompy/tensors/objects.py:618-636); y_c += E_c on an ncomp-component field

@@ -1,0 +1,57 @@
+"""Hand-written backward of the homogenized matrix (fh_energy_gradient behind
+torch.autograd.Function) against central finite differences and a float64 PyTorch
+reference that differentiates through its own CG iterations."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip('torch')
+nat = pytest.importorskip('ffthompy_amd._native')
+from ffthompy_amd.autograd import upper_bound  # noqa: E402
+from ffthompy_amd.microstructure import gaussian_random_field, BASE_SEED  # noqa: E402
+from torch_reference import upper_bound_torch  # noqa: E402
+
+
+def loss_of(AH):
+    return torch.sum((AH - 2. * torch.eye(AH.shape[0], dtype=AH.dtype)) ** 2)
+
+
+def test_gradient_matches_torch_reference_15():
+    """15^3, GRF theta (correlation 0.15, std 0.8): loss ||AH - 2I||_F^2."""
+    print('case autograd torch-reference 15^3', flush=True)
+    th = gaussian_random_field((15, 15, 15), 0.15, 0.8, BASE_SEED + 5)
+    t1 = torch.tensor(th, dtype=torch.float64, requires_grad=True)
+    L1 = loss_of(upper_bound(t1, tol=1e-12))
+    L1.backward()
+    t2 = torch.tensor(th, dtype=torch.float64, requires_grad=True)
+    L2 = loss_of(upper_bound_torch(t2, tol=1e-13))
+    L2.backward()
+    assert abs(L1.item() - L2.item()) < 1e-12 * abs(L2.item())
+    g1, g2 = t1.grad.numpy(), t2.grad.numpy()
+    assert np.abs(g1 - g2).max() < 1e-8 * np.abs(g2).max(), np.abs(g1 - g2).max() / np.abs(g2).max()
+
+
+def test_gradient_matches_central_differences_99():
+    """99^3, GRF theta (correlation 0.15, std 0.8): 20 seeded voxels, step 1e-6, fp64."""
+    print('case autograd FD 99^3', flush=True)
+    N = (99, 99, 99)
+    th = gaussian_random_field(N, 0.15, 0.8, BASE_SEED + 5)
+    t = torch.tensor(th, dtype=torch.float64, requires_grad=True)
+    L = loss_of(upper_bound(t, tol=1e-11))
+    L.backward()
+    g = t.grad.numpy()
+    rng = np.random.default_rng(BASE_SEED)
+    vox = [tuple(rng.integers(0, 99, 3)) for _ in range(20)]
+    h = 1e-6
+    errs = []
+    for v in vox:
+        vals = []
+        for s in (+1, -1):
+            tt = th.copy()
+            tt[v] += s * h
+            with torch.no_grad():
+                vals.append(loss_of(upper_bound(torch.tensor(tt), tol=1e-11)).item())
+        fd = (vals[0] - vals[1]) / (2 * h)
+        errs.append(abs(fd - g[v]) / abs(g[v]))
+    print('FD relative errors: max %.2e median %.2e' % (max(errs), float(np.median(errs))), flush=True)
+    assert max(errs) < 1e-3, errs
