@@ -54,6 +54,12 @@ SIGNATURES = {
     'fh_last_solve_updates': [_vp, _c_int_p],
     'fh_matvec_profile': [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int, _c_double_p, _c_double_p,
                           _c_int_p],
+    'fh_batch_create': [ctypes.c_int, _c_int_p, _c_double_p, ctypes.POINTER(ctypes.c_void_p)],
+    'fh_batch_destroy': [_vp],
+    'fh_batch_set_coeff': [_vp, _c_double_p],
+    'fh_batch_solve': [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double, _c_double_p, _c_double_p,
+                       _c_int_p, _c_int_p],
+    'fh_batch_stats': [_vp, _c_int_p, _c_double_p, _c_double_p],
     'fh_synchronize': [],
 }
 
@@ -349,6 +355,61 @@ class Plan:
         if h is not None and h.value and _LIB is not None:
             try:
                 _LIB.fh_plan_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+
+class Batch:
+    """Owner of an fh_batch: many 2-D scalar samples on their own grids, solved together."""
+
+    def __init__(self, dims, Y=None):
+        init()
+        dims = np.ascontiguousarray(np.asarray(dims, dtype=np.int32).reshape(-1, 2))
+        self.dims = [tuple(int(v) for v in d) for d in dims]
+        self.S = len(self.dims)
+        Ya = None
+        if Y is not None:
+            Ya = np.ascontiguousarray(np.broadcast_to(np.asarray(Y, dtype=np.float64), (self.S, 2)))
+            self._Y = Ya
+        h = ctypes.c_void_p()
+        call('fh_batch_create', ctypes.c_int(self.S), dims.ctypes.data_as(_c_int_p),
+             Ya.ctypes.data_as(_c_double_p) if Ya is not None else None, ctypes.byref(h))
+        self._h = h
+        self.nvox = sum(a * b for a, b in self.dims)
+
+    def set_coeff(self, samples):
+        """samples: list of 2-D arrays (shape dims[s]) of positive conductivities, origin 0."""
+        flat = np.concatenate([np.asarray(a, dtype=np.float64).reshape(-1) for a in samples])
+        if flat.size != self.nvox:
+            raise FhError('expected %d voxel values, got %d' % (self.nvox, flat.size), FH_ERR_ARG)
+        flat = np.ascontiguousarray(flat)
+        call('fh_batch_set_coeff', self._h, flat.ctypes.data_as(_c_double_p))
+
+    def solve(self, tol=1e-8, maxiter=10000, mode='mixed', delta=0.01):
+        up = np.zeros((self.S, 2, 2))
+        lo = np.zeros((self.S, 2, 2))
+        kit = np.zeros((self.S, 4), dtype=np.int32)
+        st = np.zeros(self.S, dtype=np.int32)
+        call('fh_batch_solve', self._h, ctypes.c_int(FH_SOLVE_FP64 if mode == 'fp64' else FH_SOLVE_MIXED),
+             ctypes.c_double(tol), ctypes.c_int(int(maxiter)), ctypes.c_double(delta),
+             up.ctypes.data_as(_c_double_p), lo.ctypes.data_as(_c_double_p), kit.ctypes.data_as(_c_int_p),
+             st.ctypes.data_as(_c_int_p))
+        return up, lo, kit, st
+
+    def stats(self):
+        nupd = np.zeros((self.S, 4), dtype=np.int32)
+        nres = np.zeros((self.S, 4))
+        ms = ctypes.c_double()
+        call('fh_batch_stats', self._h, nupd.ctypes.data_as(_c_int_p), nres.ctypes.data_as(_c_double_p),
+             ctypes.byref(ms))
+        return nupd, nres, ms.value
+
+    def __del__(self):
+        h = getattr(self, '_h', None)
+        if h is not None and h.value and _LIB is not None:
+            try:
+                _LIB.fh_batch_destroy(h)
             except Exception:
                 pass
             self._h = None

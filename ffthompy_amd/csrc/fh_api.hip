@@ -13,6 +13,7 @@
 #include "../../include/ffthom.h"
 #include "fh_kernels.hpp"
 #include "fh_launch.hpp"
+#include "fh_batch.hpp"
 
 using fh::AxisPlan;
 using fh::CGState;
@@ -152,6 +153,35 @@ struct fh_plan {
   double last_matvec_ms = 0, last_solve_ms = 0;
   int last_updates = 0;
   int generic = 0;               // 1: never use the fixed-length kernels
+};
+
+struct fh_batch {
+  int S = 0, ntasks = 0, nslots = 0;
+  std::vector<fh::batch::Sample> smp;
+  long long nvox = 0;              // voxels over all samples
+  size_t lds = 0;
+  fh::batch::Args args{};
+  fh::batch::Sample* d_smp = nullptr;
+  int* d_order = nullptr;
+  int* d_counter = nullptr;
+  double* d_a64 = nullptr;
+  float* d_a32 = nullptr;
+  cpx<double>* d_tw64 = nullptr;
+  cpx<float>* d_tw32 = nullptr;
+  double* d_ws64 = nullptr;
+  float* d_ws32 = nullptr;
+  double* d_energy = nullptr;
+  int* d_kit = nullptr;
+  int* d_nupd = nullptr;
+  double* d_normres = nullptr;
+  int* d_status = nullptr;
+  fh::batch::Params* d_par = nullptr;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool coef_set = false;
+  double last_ms = 0.0;
 };
 
 namespace {
@@ -1128,6 +1158,211 @@ int fh_last_timing(const fh_plan* pl, double* matvec_ms, double* solve_ms) {
   if (!pl) return fail(FH_ERR_ARG, "null plan");
   if (matvec_ms) *matvec_ms = pl->last_matvec_ms;
   if (solve_ms) *solve_ms = pl->last_solve_ms;
+  return FH_OK;
+}
+
+// ---- batched 2-D scalar solves -------------------------------------------------------------
+
+int fh_batch_destroy(fh_batch* b) {
+  if (!b) return FH_OK;
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  void* bufs[] = {b->d_smp, b->d_order, b->d_counter, b->d_a64, b->d_a32, b->d_tw64, b->d_tw32, b->d_ws64,
+                  b->d_ws32, b->d_energy, b->d_kit, b->d_nupd, b->d_normres, b->d_status, b->d_par};
+  for (void* p : bufs)
+    if (p) (void)hipFree(p);
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return FH_OK;
+}
+
+int fh_batch_create(int nsamples, const int* dims, const double* Y, fh_batch** out) {
+  if (!out || !dims || nsamples <= 0) return fail(FH_ERR_ARG, "fh_batch_create: bad arguments");
+  *out = nullptr;
+  RETCHK(ensure_device());
+  namespace B = fh::batch;
+  fh_batch* b = new fh_batch();
+  auto bail = [&](int code) {
+    fh_batch_destroy(b);
+    return code;
+  };
+  b->S = nsamples;
+  b->ntasks = 2 * nsamples;
+  b->smp.resize(nsamples);
+  long long nvec = 0, spec = 0, nvox = 0;
+  for (int s = 0; s < nsamples; ++s) {
+    const int n0 = dims[2 * s], n1 = dims[2 * s + 1];
+    if (B::len_id(n0) < 0 || B::len_id(n1) < 0)
+      return bail(fail(FH_ERR_UNSUPPORTED, "sample " + std::to_string(s) + ": grid " + std::to_string(n0) + "x" +
+                                              std::to_string(n1) + " (axis lengths 27, 45, 63, 81, 105, 135)"));
+    const double y0 = Y ? Y[2 * s] : 1.0, y1 = Y ? Y[2 * s + 1] : 1.0;
+    if (!(y0 > 0.0) || !(y1 > 0.0)) return bail(fail(FH_ERR_ARG, "cell sizes must be positive"));
+    B::Sample& sm = b->smp[s];
+    sm.n0 = n0;
+    sm.n1 = n1;
+    sm.off = nvox;
+    sm.iY0 = 1.0 / y0;
+    sm.iY1 = 1.0 / y1;
+    nvox += (long long)n0 * n1;
+    nvec = std::max(nvec, 2LL * n0 * n1);
+    spec = std::max(spec, B::spec_elems(n0, n1));
+  }
+  b->nvox = nvox;
+  nvec = (nvec + 63) / 64 * 64;
+  const long long spec64 = (2 * spec + 63) / 64 * 64;  // doubles
+  b->lds = (size_t)B::OFF_SPEC + (size_t)spec * sizeof(cpx<float>);
+  if (b->lds > kLdsMax) return bail(fail(FH_ERR_UNSUPPORTED, "half spectrum exceeds LDS"));
+  // tasks: largest grids first, primal and dual of a sample adjacent
+  std::vector<int> order(b->ntasks);
+  for (int t = 0; t < b->ntasks; ++t) order[t] = t;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
+    const B::Sample &sa = b->smp[a >> 1], &sc = b->smp[c >> 1];
+    return (long long)sa.n0 * sa.n1 > (long long)sc.n0 * sc.n1;
+  });
+  int ncu = 0;
+  HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, 0));
+  b->nslots = std::max(1, std::min(b->ntasks, ncu));
+  // twiddle tables exp(-2 pi i m / L) of every supported length
+  std::vector<cpx<double>> t64((size_t)B::NLEN * B::TWS);
+  std::vector<cpx<float>> t32((size_t)B::NLEN * B::TWS);
+  for (int id = 0; id < B::NLEN; ++id) {
+    const int L = B::kLens[id];
+    for (int m = 0; m < B::TWS; ++m) {
+      const long double ang = -2.0L * 3.14159265358979323846264338327950288L * (long double)(m % L) / (long double)L;
+      t64[id * B::TWS + m] = fh::mk((double)cosl(ang), (double)sinl(ang));
+      t32[id * B::TWS + m] = fh::mk((float)(double)cosl(ang), (float)(double)sinl(ang));
+    }
+  }
+  const size_t ws64 = (size_t)b->nslots * (6 * nvec + spec64), ws32 = (size_t)b->nslots * 4 * nvec;
+  hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&b->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+  if (e == hipSuccess) e = hipMalloc(&b->d_smp, nsamples * sizeof(B::Sample));
+  if (e == hipSuccess) e = hipMalloc(&b->d_order, b->ntasks * sizeof(int));
+  if (e == hipSuccess) e = hipMalloc(&b->d_counter, 64 * sizeof(int));
+  if (e == hipSuccess) e = hipMalloc(&b->d_a64, std::max<long long>(nvox, 1) * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&b->d_a32, std::max<long long>(nvox, 1) * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&b->d_tw64, t64.size() * sizeof(cpx<double>));
+  if (e == hipSuccess) e = hipMalloc(&b->d_tw32, t32.size() * sizeof(cpx<float>));
+  if (e == hipSuccess) e = hipMalloc(&b->d_ws64, ws64 * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&b->d_ws32, ws32 * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&b->d_energy, (size_t)nsamples * 8 * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&b->d_kit, (size_t)nsamples * 4 * sizeof(int));
+  if (e == hipSuccess) e = hipMalloc(&b->d_nupd, (size_t)nsamples * 4 * sizeof(int));
+  if (e == hipSuccess) e = hipMalloc(&b->d_normres, (size_t)nsamples * 4 * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&b->d_status, (size_t)nsamples * 2 * sizeof(int));
+  if (e == hipSuccess) e = hipMalloc(&b->d_par, sizeof(B::Params));
+  if (e == hipSuccess)
+    e = hipMemcpy(b->d_smp, b->smp.data(), nsamples * sizeof(B::Sample), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(b->d_order, order.data(), b->ntasks * sizeof(int), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(b->d_tw64, t64.data(), t64.size() * sizeof(cpx<double>), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(b->d_tw32, t32.data(), t32.size() * sizeof(cpx<float>), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = B::allow_lds(b->lds);
+  if (e != hipSuccess) return bail(fail(FH_ERR_HIP, std::string("fh_batch_create: ") + hipGetErrorString(e)));
+  B::Args& A = b->args;
+  A.smp = b->d_smp;
+  A.order = b->d_order;
+  A.ntasks = b->ntasks;
+  A.counter = b->d_counter;
+  A.a64 = b->d_a64;
+  A.a32 = b->d_a32;
+  A.tw64 = b->d_tw64;
+  A.tw32 = b->d_tw32;
+  A.ws64 = b->d_ws64;
+  A.ws32 = b->d_ws32;
+  A.ws64_stride = 6 * nvec + spec64;
+  A.ws32_stride = 4 * nvec;
+  A.nvec = nvec;
+  A.energy = b->d_energy;
+  A.kit = b->d_kit;
+  A.nupd = b->d_nupd;
+  A.normres = b->d_normres;
+  A.status = b->d_status;
+  A.par = b->d_par;
+  *out = b;
+  return FH_OK;
+}
+
+int fh_batch_set_coeff(fh_batch* b, const double* a) {
+  if (!b || !a) return fail(FH_ERR_ARG, "fh_batch_set_coeff: null argument");
+  for (long long i = 0; i < b->nvox; ++i)
+    if (!(a[i] > 0.0) || !std::isfinite(a[i]))
+      return fail(FH_ERR_ARG, "conductivity must be positive and finite (voxel " + std::to_string(i) + ")");
+  std::vector<float> a32((size_t)b->nvox);
+  for (long long i = 0; i < b->nvox; ++i) a32[i] = (float)a[i];
+  HIPCHK(hipMemcpy(b->d_a64, a, (size_t)b->nvox * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(b->d_a32, a32.data(), (size_t)b->nvox * sizeof(float), hipMemcpyHostToDevice));
+  b->coef_set = true;
+  return FH_OK;
+}
+
+int fh_batch_solve(fh_batch* b, int mode, double tol, int maxiter, double delta, double* upper, double* lower,
+                   int* kit, int* status) {
+  if (!b) return fail(FH_ERR_ARG, "null batch");
+  if (!b->coef_set) return fail(FH_ERR_STATE, "fh_batch_set_coeff has not been called");
+  if (mode != FH_SOLVE_FP64 && mode != FH_SOLVE_MIXED) return fail(FH_ERR_ARG, "bad solve mode");
+  if (!(tol >= 0.0) || maxiter < 0) return fail(FH_ERR_ARG, "bad tol/maxiter");
+  if (mode == FH_SOLVE_MIXED && !(delta > 0.0 && delta < 1.0)) return fail(FH_ERR_ARG, "delta must be in (0,1)");
+  fh::batch::Params P;
+  P.tol = tol;
+  P.delta = delta;
+  P.maxiter = maxiter;
+  P.mixed = mode == FH_SOLVE_MIXED;
+  HIPCHK(hipMemcpyAsync(b->d_par, &P, sizeof(P), hipMemcpyHostToDevice, b->stream));
+  if (!b->exec) {
+    // one graph: counter reset + the persistent solve kernel; replayed by every call
+    HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = fh::batch::enqueue(b->args, b->nslots, b->lds, b->stream);
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(b->stream, &g);
+    if (e != hipSuccess || e2 != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return fail(FH_ERR_HIP, std::string("batch graph capture: ") + hipGetErrorString(e != hipSuccess ? e : e2));
+    }
+    b->graph = g;
+    HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  }
+  HIPCHK(hipEventRecord(b->ev0, b->stream));
+  HIPCHK(hipGraphLaunch(b->exec, b->stream));
+  HIPCHK(hipEventRecord(b->ev1, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+  b->last_ms = ms;
+  const int S = b->S;
+  std::vector<double> E((size_t)S * 8);
+  std::vector<int> K((size_t)S * 4), St((size_t)S * 2);
+  HIPCHK(hipMemcpy(E.data(), b->d_energy, E.size() * sizeof(double), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(K.data(), b->d_kit, K.size() * sizeof(int), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(St.data(), b->d_status, St.size() * sizeof(int), hipMemcpyDeviceToHost));
+  for (int s = 0; s < S; ++s) {
+    const double* ep = &E[(size_t)s * 8];      // primal energy = upper bound
+    const double* ed = &E[(size_t)s * 8 + 4];  // dual energy; lower bound = its inverse
+    if (upper)
+      for (int k = 0; k < 4; ++k) upper[4 * s + k] = ep[k];
+    if (lower) {
+      const double det = ed[0] * ed[3] - ed[1] * ed[2];
+      lower[4 * s + 0] = ed[3] / det;
+      lower[4 * s + 1] = -ed[1] / det;
+      lower[4 * s + 2] = -ed[2] / det;
+      lower[4 * s + 3] = ed[0] / det;
+    }
+    if (kit)
+      for (int k = 0; k < 4; ++k) kit[4 * s + k] = K[(size_t)s * 4 + k];
+    if (status) status[s] = St[2 * s] | St[2 * s + 1];
+  }
+  return FH_OK;
+}
+
+int fh_batch_stats(const fh_batch* b, int* n_updates, double* norm_res, double* solve_ms) {
+  if (!b) return fail(FH_ERR_ARG, "null batch");
+  if (n_updates) HIPCHK(hipMemcpy(n_updates, b->d_nupd, (size_t)b->S * 4 * sizeof(int), hipMemcpyDeviceToHost));
+  if (norm_res) HIPCHK(hipMemcpy(norm_res, b->d_normres, (size_t)b->S * 4 * sizeof(double), hipMemcpyDeviceToHost));
+  if (solve_ms) *solve_ms = b->last_ms;
   return FH_OK;
 }
 

@@ -146,3 +146,14 @@ def gaussian_random_field(N, corr_len, std, seed, Y=None):
     g = np.exp(-np.pi ** 2 * corr_len ** 2 * xi2)
     g *= std / np.sqrt(np.mean(g ** 2))
     return np.real(np.fft.ifftn(np.fft.fftn(w) * g))
+
+
+def uq_samples(count=512, lengths=(63, 81, 105, 135), corr_len=0.1, std=1.0, seed=BASE_SEED):
+    """Batched uncertainty-quantification workload: `count` 2-D samples whose grid axes are
+    drawn uniformly from `lengths`; each conductivity is exp of a periodic Gaussian random
+    field (squared-exponential covariance, correlation length corr_len, log-std std) on a
+    unit cell. Returns (dims, fields): dims (count, 2) int, fields a list of (n0, n1) arrays."""
+    rng = np.random.default_rng(seed)
+    dims = rng.choice(np.asarray(lengths), size=(count, 2))
+    fields = [np.exp(gaussian_random_field(tuple(d), corr_len, std, seed + 1 + s)) for s, d in enumerate(dims)]
+    return dims, fields

@@ -45,6 +45,7 @@ enum { FH_COEF_ISO = 1, FH_COEF_LAME = 2, FH_COEF_FULL = 3 };
 enum { FH_CG_OK = 0, FH_CG_BREAKDOWN = 1 };
 
 typedef struct fh_plan fh_plan;
+typedef struct fh_batch fh_batch;
 
 /* Message of the calling thread's last error ("" if none). */
 const char* fh_last_error(void);
@@ -160,6 +161,30 @@ int fh_last_solve_updates(const fh_plan* plan, int* n_updates);
  * receives the count. No reference counterpart (measurement only). */
 int fh_matvec_profile(fh_plan* plan, int dtype, const void* x, void* y, int nrep, int max_kernels,
                       double* kernel_ms, double* kernel_bytes, int* nkernels);
+
+/* ---- Batched 2-D scalar homogenization (many samples, uneven grids, one launch) ----------
+ * Each sample s is the reference's scalar GaNi problem on its own grid
+ * (ffthompy/applications.py:24-90 run once per sample): isotropic conductivity a_s(x) on a
+ * grid dims[2s] x dims[2s+1] (axis lengths 27, 45, 63, 81, 105 or 135) of cell
+ * Y[2s] x Y[2s+1] (NULL: unit cells), unit loads E = e1, e2, CG (ffthompy/general/solver.py:80-139)
+ * on the primal (G1, a) and dual (G2, 1/a) problems, and the bounds of
+ * ffthompy/postprocess.py:53-70: upper = <a e_i, e_j>, lower = inv(<a^-1 e_i, e_j>).
+ * All samples are solved by one persistent kernel (one workgroup per sample and primal/dual
+ * task, CG state on the device), launched from a captured HIP graph. */
+int fh_batch_create(int nsamples, const int* dims, const double* Y, fh_batch** out);
+int fh_batch_destroy(fh_batch* batch);
+/* a: host fp64, the samples' voxel values concatenated, each in C order with origin at
+ * index 0 (the layout of Material.get_A_GaNi, ffthompy/materials.py:115-123). */
+int fh_batch_set_coeff(fh_batch* batch, const double* a);
+/* mode FH_SOLVE_FP64 or FH_SOLVE_MIXED (fp32 inner CG, fp64 reliable updates, fp64
+ * projection and energies). Host outputs (any may be NULL): upper[4s..4s+3], lower[4s..],
+ * kit[4s..4s+3] = iterations of (primal e1, primal e2, dual e1, dual e2), status[s] =
+ * FH_CG_OK or FH_CG_BREAKDOWN. Stop rule as fh_cg_solve. */
+int fh_batch_solve(fh_batch* batch, int mode, double tol, int maxiter, double delta, double* upper,
+                   double* lower, int* kit, int* status);
+/* Of the last solve: fp64 residual evaluations and final sqrt(<r,r>) per (sample, primal/dual,
+ * load) (4 per sample, host arrays), and the device time of the whole batch in ms. */
+int fh_batch_stats(const fh_batch* batch, int* n_updates, double* norm_res, double* solve_ms);
 
 /* Wait for all GPU work of this process. */
 int fh_synchronize(void);

@@ -43,7 +43,16 @@ def test_gradient_matches_central_differences_99():
     rng = np.random.default_rng(BASE_SEED)
     vox = [tuple(rng.integers(0, 99, 3)) for _ in range(20)]
     h = 1e-6
-    errs = []
+    L0 = L.item()
+    # Rounding floor of a central difference of FP64 loss values: one evaluation of L
+    # (fields from a CG to 1e-11, then 9 AH entries) reproduces to some tens of ulps of
+    # |L| (printed below: L at tolerance 1e-11 against 1e-12), divided by 2h. At h = 1e-6
+    # and |grad| ~ 5e-6 that is ~1e-3 of the gradient: the FD check resolves no better.
+    with torch.no_grad():
+        L12 = loss_of(upper_bound(torch.tensor(th), tol=1e-12)).item()
+    print('L %.17g, L(tol 1e-12) - L = %.2e (%.1f ulp)' % (L0, L12 - L0, (L12 - L0) / np.spacing(L0)))
+    floor = 64 * np.finfo(np.float64).eps * abs(L0) / (2 * h)
+    rows = []
     for v in vox:
         vals = []
         for s in (+1, -1):
@@ -52,6 +61,10 @@ def test_gradient_matches_central_differences_99():
             with torch.no_grad():
                 vals.append(loss_of(upper_bound(torch.tensor(tt), tol=1e-11)).item())
         fd = (vals[0] - vals[1]) / (2 * h)
-        errs.append(abs(fd - g[v]) / abs(g[v]))
-    print('FD relative errors: max %.2e median %.2e' % (max(errs), float(np.median(errs))), flush=True)
-    assert max(errs) < 1e-3, errs
+        rows.append((v, g[v], fd, abs(fd - g[v])))
+    for v, gv, fd, err in rows:
+        print('voxel %s grad %.6e fd %.6e |diff| %.2e (floor %.2e)' % (v, gv, fd, err, floor), flush=True)
+    rel = [err / abs(gv) for _, gv, _, err in rows]
+    print('FD relative errors: max %.2e median %.2e' % (max(rel), float(np.median(rel))), flush=True)
+    assert all(err <= 1e-5 * abs(gv) + floor for _, gv, _, err in rows)
+    assert float(np.median(rel)) < 1e-3

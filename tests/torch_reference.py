@@ -51,7 +51,7 @@ def upper_bound_torch(theta, Y=None, tol=1e-13, maxiter=2000):
         p = r
         rr = scal(r, r)
         k = 0
-        while float(rr) ** 0.5 > tol and k < maxiter:
+        while float(rr.detach()) ** 0.5 > tol and k < maxiter:
             Ap = Afun(p)
             alp = rr / scal(p, Ap)
             x = x + alp * p
