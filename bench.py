@@ -46,7 +46,7 @@ class Homogenizer:
         self.EN = nat.DeviceArray((3,) + N)
         self.sols = {pd: [nat.DeviceArray((3,) + N) for _ in range(3)] for pd in ('primal', 'dual')}
 
-    def step(self, mode, tol=1e-8, maxiter=2000):
+    def step(self, mode, tol=1e-8, maxiter=2000, delta=0.01):
         nat = self.nat
         out = {'kit': {}, 'solve_ms': {}, 'updates': {}}
         for pd, pl in self.plans.items():
@@ -59,7 +59,7 @@ class Homogenizer:
                 pl.matvec(self.EN, self.B)
                 x = self.sols[pd][iL]
                 nat.call('fh_memset_zero', x.ptr, x.nbytes)
-                info = pl.cg_solve(self.B, x, tol=tol, maxiter=maxiter, mode=mode, delta=0.01)
+                info = pl.cg_solve(self.B, x, tol=tol, maxiter=maxiter, mode=mode, delta=delta)
                 if info['status'] != nat.FH_CG_OK:
                     raise RuntimeError('CG breakdown')
                 kits.append(info['kit'])
@@ -107,6 +107,8 @@ def main():
     ap.add_argument('--fp64-steps', type=int, default=None, help='timed fp64 steps (default: --steps)')
     ap.add_argument('--cpu-iters', type=int, default=2, help='CG iterations in the CPU baseline sample')
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
+    ap.add_argument('--delta', type=float, default=0.01,
+                    help='mixed precision: residual reduction that triggers an fp64 reliable update')
     ap.add_argument('--profile-only', action='store_true', help='matvec profile only (for rocprofv3 runs)')
     ap.add_argument('--profile-solve', action='store_true',
                     help='with --profile-only: also one mixed and one fp64 CG solve (load e1, primal)')
@@ -175,7 +177,7 @@ def main():
     # --- timed steps ---
     sync = nat.synchronize
     log('mixed-precision steps: warmup %d, timed %d' % (args.warmup, args.steps))
-    t_mixed, r_mixed = timed(lambda: H.step('mixed'), args.steps, args.warmup, sync)
+    t_mixed, r_mixed = timed(lambda: H.step('mixed', delta=args.delta), args.steps, args.warmup, sync)
     log('mixed step %.3f s, kit %s' % (t_mixed, r_mixed['kit']))
     fs = args.steps if args.fp64_steps is None else args.fp64_steps
     t_fp64, r_fp64 = timed(lambda: H.step('fp64'), fs, 1 if args.warmup > 0 else 0, sync)

@@ -37,6 +37,7 @@ SIGNATURES = {
     'fh_plan_ncomp': [_vp],
     'fh_plan_set_primaldual': [_vp, ctypes.c_int],
     'fh_plan_use_generic': [_vp, ctypes.c_int],
+    'fh_plan_set_cg_variant': [_vp, ctypes.c_int],
     'fh_set_coeff': [_vp, ctypes.c_int, _c_double_p],
     'fh_matvec': [_vp, ctypes.c_int, _vp, _vp],
     'fh_project': [_vp, ctypes.c_int, _vp, _vp],
@@ -267,6 +268,10 @@ class Plan:
     def set_primaldual(self, primaldual):
         call('fh_plan_set_primaldual', self._h, ctypes.c_int(FH_PRIMAL if primaldual == 'primal' else FH_DUAL))
         self.primaldual = primaldual
+
+    def set_cg_variant(self, fused=True):
+        """Mixed-precision CG: fused iteration (default) or the operator-form denominator."""
+        call('fh_plan_set_cg_variant', self._h, ctypes.c_int(1 if fused else 0))
 
     def use_generic(self, flag=True):
         call('fh_plan_use_generic', self._h, ctypes.c_int(1 if flag else 0))

@@ -153,6 +153,7 @@ struct fh_plan {
   double last_matvec_ms = 0, last_solve_ms = 0;
   int last_updates = 0;
   int generic = 0;               // 1: never use the fixed-length kernels
+  bool fused_cg = true;          // mixed CG: fused iteration where available
 };
 
 struct fh_batch {
@@ -274,7 +275,8 @@ template <typename T>
 int launch_rows_fwd(fh_plan* pl, int nc, const T* in, const T* coef, int kind, cpx<T>* out,
                     const int* halt, const T* rvec = nullptr) {
   if (!pl->generic) {
-    const int r = fh::fixed_rows_fwd<T>(make_ctx<T>(pl), nc, in, coef, kind, out, halt, rvec, pl->st);
+    const int r = fh::fixed_rows_fwd<T>(make_ctx<T>(pl), nc, in, coef, kind, out, halt, rvec, pl->st, nullptr,
+                                        nullptr, 0, nullptr);
     if (r != 0) return fixed_result(r, "rows forward (fixed)");
   }
   RowCfg c;
@@ -310,7 +312,7 @@ template <typename T>
 int launch_rows_inv(fh_plan* pl, int nc, const cpx<T>* in, T* out, T scale, const T* w,
                     double* partial, const int* halt, int* nblocks) {
   if (!pl->generic) {
-    const int r = fh::fixed_rows_inv<T>(make_ctx<T>(pl), nc, in, out, scale, w, partial, pl->partial_cap, halt,
+    const int r = fh::fixed_rows_inv<T>(make_ctx<T>(pl), nc, in, out, scale, w ? 1 : 0, w, partial, pl->partial_cap, halt,
                                         nblocks, pl->st);
     if (r != 0) return fixed_result(r, "rows inverse (fixed)");
   }
@@ -498,6 +500,45 @@ static int ensure_hist(fh_plan* pl, int len) {
   pl->hist_cap = 0;
   HIPCHK(hipMalloc(&pl->hist, (size_t)len * sizeof(double)));
   pl->hist_cap = len;
+  return FH_OK;
+}
+
+// Fused single-precision CG iteration (fixed-length kernels, isotropic coefficient):
+//   rows_fwd : xc += alpha_prev p_old (pending step), p = r + beta p, y = a p, partial <p, a p>
+//   alpha    : alpha = <r,r> / <p, a p>   (for p in the range of G, <p, a p> = <p, G(a p)>)
+//   cols     : F, G, F^-1 along the strided axes
+//   rows_inv : r -= alpha G(a p), partial <r, r>
+//   beta     : beta, stop / reliable-update test
+// Returns 0 when the fused kernels do not cover this plan (nothing was enqueued).
+static int enqueue_fused_iteration(fh_plan* pl, float* xc, float* r, float* p, const float* coef, bool* ok) {
+  *ok = false;
+  if (pl->generic || pl->coef_kind != fh::COEF_ISO) return FH_OK;
+  const int nc = pl->ncomp;
+  const int* halt = &pl->st->halt;
+  cpx<float>* spec = reinterpret_cast<cpx<float>*>(pl->spec);
+  const fh::LaunchCtx ctx = make_ctx<float>(pl);
+  int nb = 0;
+  int rr = fh::fixed_rows_fwd<float>(ctx, nc, p, coef, fh::COEF_ISO, spec, halt, r, pl->st, xc, pl->partial,
+                                     pl->partial_cap, &nb);
+  if (rr == 0) return FH_OK;
+  RETCHK(fixed_result(rr, "rows forward (fused)"));
+  *ok = true;
+  hipLaunchKernelGGL(fh::k_cg_alpha, dim3(1), dim3(256), 0, pl->stream, pl->st, pl->partial, nb);
+  const GParams g = make_gparams(pl, 1.0 / (double)pl->n);
+  if (pl->n0 > 1) {
+    RETCHK(launch_cols<float>(pl, 0, 1, nc, spec, g, halt));
+    RETCHK(launch_cols<float>(pl, 2, 0, nc, spec, g, halt));
+    RETCHK(launch_cols<float>(pl, 1, 1, nc, spec, g, halt));
+  } else {
+    RETCHK(launch_cols<float>(pl, 2, 1, nc, spec, g, halt));
+  }
+  int nb2 = 0;
+  rr = fh::fixed_rows_inv<float>(ctx, nc, spec, r, 1.0f, 2, nullptr, pl->partial, pl->partial_cap, halt, &nb2,
+                                 pl->st);
+  if (rr == 0) return fail(FH_ERR_STATE, "fused rows inverse unavailable");
+  RETCHK(fixed_result(rr, "rows inverse (fused)"));
+  hipLaunchKernelGGL(fh::k_cg_beta, dim3(1), dim3(256), 0, pl->stream, pl->st, pl->partial, nb2);
+  HIPCHK(hipGetLastError());
   return FH_OK;
 }
 
@@ -726,6 +767,12 @@ int fh_plan_use_generic(fh_plan* pl, int generic) {
   return FH_OK;
 }
 
+int fh_plan_set_cg_variant(fh_plan* pl, int fused) {
+  RETCHK(check_plan(pl));
+  pl->fused_cg = fused != 0;
+  return FH_OK;
+}
+
 int fh_plan_set_primaldual(fh_plan* pl, int primaldual) {
   RETCHK(check_plan(pl));
   if (primaldual != FH_PRIMAL && primaldual != FH_DUAL) return fail(FH_ERR_ARG, "bad primaldual");
@@ -861,20 +908,35 @@ int fh_cg_solve(fh_plan* pl, int mode, const void* b_, void* x_, double tol, int
     HIPCHK(hipMemsetAsync(xc, 0, (size_t)nv * sizeof(float), pl->stream));
     RETCHK(enqueue_true_residual<float>(pl, b, x, Ax, r));
     HIPCHK(hipMemcpyAsync(p, r, (size_t)nv * sizeof(float), hipMemcpyDeviceToDevice, pl->stream));
+    // fused iterations where the fixed-length kernels cover the plan (isotropic coefficient)
+    bool fused = pl->fused_cg;
+    const int gb = grid_blocks(nv);
+    auto fold = [&]() {
+      if (fused)
+        hipLaunchKernelGGL(fh::k_fold_pending, dim3(gb), dim3(256), 0, pl->stream, nv, x, xc, p, pl->st);
+      else
+        hipLaunchKernelGGL(fh::k_fold_correction, dim3(gb), dim3(256), 0, pl->stream, nv, x, xc);
+    };
     for (;;) {
-      for (int k = 0; k < per_check; ++k) RETCHK(enqueue_iteration<float>(pl, xc, r, p, Ap, pl->coef32));
+      for (int k = 0; k < per_check; ++k) {
+        if (fused) {
+          bool ok = false;
+          RETCHK(enqueue_fused_iteration(pl, xc, r, p, pl->coef32, &ok));
+          if (ok) continue;
+          fused = false;  // not covered: nothing was enqueued, use the general iteration
+        }
+        RETCHK(enqueue_iteration<float>(pl, xc, r, p, Ap, pl->coef32));
+      }
       HIPCHK(hipMemcpyAsync(&s, pl->st, sizeof(s), hipMemcpyDeviceToHost, pl->stream));
       HIPCHK(hipStreamSynchronize(pl->stream));
       if (s.done) break;
       if (s.pause) {
-        const int gb = grid_blocks(nv);
-        hipLaunchKernelGGL(fh::k_fold_correction, dim3(gb), dim3(256), 0, pl->stream, nv, x, xc);
+        fold();
         RETCHK(enqueue_true_residual<float>(pl, b, x, Ax, r));
         HIPCHK(hipGetLastError());
       }
     }
-    const int gb = grid_blocks(nv);
-    hipLaunchKernelGGL(fh::k_fold_correction, dim3(gb), dim3(256), 0, pl->stream, nv, x, xc);
+    fold();
     HIPCHK(hipGetLastError());
     // final fp64 projection x <- G x removes the non-conforming part left by fp32 rounding
     RETCHK(run_operator<double>(pl, x, Ax, fh::COEF_NONE, nullptr, nullptr, nullptr, nullptr));

@@ -490,11 +490,16 @@ __device__ __forceinline__ T coef_apply(PV&& pv, __amdgpu_buffer_rsrc_t rco, uns
 // TPP threads per pencil (blockDim = (rpb/2)*nc*TPP). Output: AoS half spectrum.
 // UPD = 1: `in` is the CG direction p, updated in place first as p = r + beta p
 // (beta from the CG state); the operator is then applied to the new p.
+// UPD = 2 (fused iteration, COEF_ISO/NONE): as UPD = 1, and before p is overwritten the
+// pending step of the previous iteration is added to the correction, xc += alpha p_old
+// (when st->pend); per-workgroup partial sums of <p, a p> go to partial[blockIdx.x]
+// (for p in the range of G this is <p, G(a p)>, the CG denominator).
 template <typename T, class FP, int COEF, int NC, int UPD>
 __global__ void __launch_bounds__(1024)
 k_rows_fwd_fx(const T* __restrict__ in, const T* __restrict__ coef, cpx<T>* __restrict__ out,
               long long nrows, long long n, int rpb, const cpx<T>* __restrict__ twg,
-              const int* __restrict__ halt, const T* __restrict__ rvec, const CGState* __restrict__ st) {
+              const int* __restrict__ halt, const T* __restrict__ rvec, const CGState* __restrict__ st,
+              T* __restrict__ xc, double* __restrict__ partial) {
   constexpr int nc = NC;
   if (halt && *halt) return;
   constexpr int L = FP::L, n2h = L / 2 + 1, TPP = FP::TPP;
@@ -518,7 +523,30 @@ k_rows_fwd_fx(const T* __restrict__ in, const T* __restrict__ coef, cpx<T>* __re
   const __amdgpu_buffer_rsrc_t rco = make_rsrc(coef ? coef : in, (unsigned long long)kvals * n * SR);
   const unsigned ba = (unsigned)((row0 + rowa) * L) * SR, bb = (unsigned)((row0 + rowb) * L) * SR;
   FxStage<T, FP, 0, -1> s0;
-  if constexpr (UPD == 0) {
+  double acc = 0.0;
+  if constexpr (UPD == 2) {
+    static_assert(COEF == COEF_ISO || COEF == COEF_NONE, "fused iteration: one component per thread");
+    const T bta = T(st->beta);
+    const bool fold = st->pend != 0;
+    const T alp = T(st->alpha);
+    const __amdgpu_buffer_rsrc_t rr = make_rsrc(rvec, (unsigned long long)NC * n * SR);
+    const __amdgpu_buffer_rsrc_t rx = make_rsrc(xc, (unsigned long long)NC * n * SR);
+    auto one = [&](int c, unsigned o) -> T {
+      const unsigned oc = (unsigned)c * nbytes + o;
+      const T pold = bload_r<T>(rin, oc);
+      const T pn = bload_r<T>(rr, oc) + bta * pold;
+      if (fold) bstore_r<T>(rx, oc, bload_r<T>(rx, oc) + alp * pold);
+      bstore_r<T>(rin, oc, pn);
+      const T y = (COEF == COEF_ISO) ? bload_r<T>(rco, o) * pn : pn;
+      acc += (double)y * (double)pn;
+      return y;
+    };
+    s0.load(t, [&](int x) {
+      const T ya = va ? one(ca, ba + x * SR) : T(0);
+      const T yb = vb ? one(cb, bb + x * SR) : T(0);
+      return mk(ya, yb);
+    });
+  } else if constexpr (UPD == 0) {
     auto pv = [=](int d, unsigned vb_) { return bload_r<T>(rin, (unsigned)d * nbytes + vb_); };
     s0.load(t, [=](int x) {
       const T ya = va ? coef_apply<T, COEF, NC>(pv, rco, ba + x * SR, ca, nbytes) : T(0);
@@ -573,9 +601,16 @@ k_rows_fwd_fx(const T* __restrict__ in, const T* __restrict__ coef, cpx<T>* __re
     }
     ob[e] = Y;
   }
+  if constexpr (UPD == 2) {
+    const double bsum = block_sum(acc);
+    if (tid == 0) partial[blockIdx.x] = bsum;
+  }
 }
 
 // ---- last pass (fixed length): C2R along the contiguous axis ------------------------
+// EPI 0: out = y. EPI 1: out = y and per-workgroup partial sums of <y, w>.
+// EPI 2 (fused iteration): `out` is the CG residual r, replaced by r - alpha y (alpha
+// from the CG state), with per-workgroup partial sums of <r, r>.
 template <typename T, class FP, int EPI, int NC>
 __global__ void __launch_bounds__(1024)
 k_rows_inv_fx(const cpx<T>* __restrict__ in, T* __restrict__ out, long long nrows, long long n,
@@ -605,8 +640,8 @@ k_rows_inv_fx(const cpx<T>* __restrict__ in, T* __restrict__ out, long long nrow
   // the last stage writes positions i + j*p (k = i); prefetch the weights there now so
   // their latency hides behind the FFT
   T wa[SL::M][SL::R], wbv[SL::M][SL::R];
-  if constexpr (EPI == 1) {
-    const __amdgpu_buffer_rsrc_t rw = make_rsrc(w, (unsigned long long)NC * n * SR);
+  if constexpr (EPI == 1 || EPI == 2) {
+    const __amdgpu_buffer_rsrc_t rw = make_rsrc(EPI == 1 ? w : out, (unsigned long long)NC * n * SR);
 #pragma unroll
     for (int m = 0; m < SL::M; ++m) {
       const int i = t + m * SL::TPP;
@@ -657,18 +692,32 @@ k_rows_inv_fx(const cpx<T>* __restrict__ in, T* __restrict__ out, long long nrow
       for (int j = 0; j < SL::R; ++j) {
         const unsigned x = (unsigned)(i + j * SL::p) * SR;
         const T a = scale * sl.v[m][j].x, b = scale * sl.v[m][j].y;
-        if (va) {
-          bstore_r<T>(rout, oa + x, a);
-          if constexpr (EPI == 1) acc += (double)a * (double)wa[m][j];
-        }
-        if (vb) {
-          bstore_r<T>(rout, ob + x, b);
-          if constexpr (EPI == 1) acc += (double)b * (double)wbv[m][j];
+        if constexpr (EPI == 2) {
+          const T alp = T(st->alpha);
+          if (va) {
+            const T rn = wa[m][j] - alp * a;
+            bstore_r<T>(rout, oa + x, rn);
+            acc += (double)rn * (double)rn;
+          }
+          if (vb) {
+            const T rn = wbv[m][j] - alp * b;
+            bstore_r<T>(rout, ob + x, rn);
+            acc += (double)rn * (double)rn;
+          }
+        } else {
+          if (va) {
+            bstore_r<T>(rout, oa + x, a);
+            if constexpr (EPI == 1) acc += (double)a * (double)wa[m][j];
+          }
+          if (vb) {
+            bstore_r<T>(rout, ob + x, b);
+            if constexpr (EPI == 1) acc += (double)b * (double)wbv[m][j];
+          }
         }
       }
     }
   }
-  if constexpr (EPI == 1) {
+  if constexpr (EPI == 1 || EPI == 2) {
     const double bsum = block_sum(acc);
     if (tid == 0) partial[blockIdx.x] = bsum;
   }

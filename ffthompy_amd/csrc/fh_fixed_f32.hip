@@ -3,8 +3,8 @@
 
 namespace fh {
 template int fixed_rows_fwd<float>(const LaunchCtx&, int, const float*, const float*, int, cpx<float>*, const int*,
-                                  const float*, const CGState*);
-template int fixed_rows_inv<float>(const LaunchCtx&, int, const cpx<float>*, float*, float, const float*, double*,
+                                   const float*, const CGState*, float*, double*, size_t, int*);
+template int fixed_rows_inv<float>(const LaunchCtx&, int, const cpx<float>*, float*, float, int, const float*, double*,
                                    size_t, const int*, int*, CGState*);
 template int fixed_cols<float>(const LaunchCtx&, int, int, int, cpx<float>*, const int*);
 template int fixed_cols_g<float>(const LaunchCtx&, int, int, cpx<float>*, const GParams&, const int*);

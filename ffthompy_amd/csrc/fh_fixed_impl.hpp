@@ -31,7 +31,8 @@ inline void allow_lds(K kernel, size_t bytes) {
 
 template <typename T, class FP>
 int rows_fwd_fx_launch(const LaunchCtx& c, int nc, const T* in, const T* coef, int kind, cpx<T>* out,
-                       const int* halt, const T* rvec, const CGState* st) {
+                       const int* halt, const T* rvec, const CGState* st, T* xc, double* partial,
+                       size_t partial_cap, int* nblocks) {
   constexpr int rpb = FxShape<FP, T>::RPB;
   const int npen = (rpb / 2) * nc;
   const int nthr = npen * FP::TPP;
@@ -41,7 +42,8 @@ int rows_fwd_fx_launch(const LaunchCtx& c, int nc, const T* in, const T* coef, i
   const int kv = kind == COEF_NONE ? 0 : (kind == COEF_ISO ? 1 : (kind == COEF_LAME ? 2 : nc * nc));
   if (!offsets_fit(c.n * (nc > kv ? nc : kv), sizeof(T))) return 0;
   const long long nrows = (long long)c.n0 * c.n1;
-  const dim3 grid((unsigned)((nrows + rpb - 1) / rpb)), block(nthr);
+  const long long nb = (nrows + rpb - 1) / rpb;
+  const dim3 grid((unsigned)nb), block(nthr);
   const cpx<T>* tw = reinterpret_cast<const cpx<T>*>(c.tw[2]);
 #define FH_LAUNCH_RF(KIND, NCC)                                                                       \
   do {                                                                                                \
@@ -49,16 +51,40 @@ int rows_fwd_fx_launch(const LaunchCtx& c, int nc, const T* in, const T* coef, i
       auto kf = k_rows_fwd_fx<T, FP, KIND, NCC, 1>;                                                   \
       allow_lds(kf, lds);                                                                             \
       hipLaunchKernelGGL(kf, grid, block, lds, c.stream, in, coef, out, nrows, c.n, rpb, tw, halt, rvec, \
-                         st);                                                                         \
+                         st, xc, partial);                                                            \
     } else {                                                                                          \
       auto kf = k_rows_fwd_fx<T, FP, KIND, NCC, 0>;                                                   \
       allow_lds(kf, lds);                                                                             \
       hipLaunchKernelGGL(kf, grid, block, lds, c.stream, in, coef, out, nrows, c.n, rpb, tw, halt, rvec, \
-                         st);                                                                         \
+                         st, xc, partial);                                                            \
     }                                                                                                 \
   } while (0)
   // the fused direction update is only race-free when each thread reads its own component
   if (rvec && kind != COEF_ISO && kind != COEF_NONE) return 0;
+  if (xc) {
+    // fused iteration (k_rows_fwd_fx UPD 2), single precision only
+    if constexpr (sizeof(T) == 4) {
+      if (!rvec || kind != COEF_ISO || (size_t)nb > partial_cap) return 0;
+#define FH_LAUNCH_RF2(NCC)                                                                                \
+  do {                                                                                                    \
+    auto kf = k_rows_fwd_fx<T, FP, COEF_ISO, NCC, 2>;                                                     \
+    allow_lds(kf, lds);                                                                                   \
+    hipLaunchKernelGGL(kf, grid, block, lds, c.stream, in, coef, out, nrows, c.n, rpb, tw, halt, rvec, st,   \
+                       xc, partial);                                                                      \
+  } while (0)
+      switch (nc) {
+        case 2: FH_LAUNCH_RF2(2); break;
+        case 3: FH_LAUNCH_RF2(3); break;
+        default: return 0;
+      }
+#undef FH_LAUNCH_RF2
+      if (nblocks) *nblocks = (int)nb;
+      const hipError_t e = hipGetLastError();
+      return e == hipSuccess ? 1 : -(int)e;
+    } else {
+      return 0;
+    }
+  }
   const int key = kind * 8 + nc;
   switch (key) {
     case COEF_NONE * 8 + 1: FH_LAUNCH_RF(COEF_NONE, 1); break;
@@ -80,7 +106,7 @@ int rows_fwd_fx_launch(const LaunchCtx& c, int nc, const T* in, const T* coef, i
 }
 
 template <typename T, class FP>
-int rows_inv_fx_launch(const LaunchCtx& c, int nc, const cpx<T>* in, T* out, T scale, const T* w,
+int rows_inv_fx_launch(const LaunchCtx& c, int nc, const cpx<T>* in, T* out, T scale, int epi, const T* w,
                        double* partial, size_t partial_cap, const int* halt, int* nblocks, CGState* st) {
   constexpr int rpb = FxShape<FP, T>::RPB;
   const int npen = (rpb / 2) * nc;
@@ -91,7 +117,8 @@ int rows_inv_fx_launch(const LaunchCtx& c, int nc, const cpx<T>* in, T* out, T s
   if (!offsets_fit(c.n * nc, sizeof(T))) return 0;
   const long long nrows = (long long)c.n0 * c.n1;
   const long long nb = (nrows + rpb - 1) / rpb;
-  if (w && (size_t)nb > partial_cap) return 0;
+  if (epi && (size_t)nb > partial_cap) return 0;
+  if (epi == 1 && !w) return 0;
   const dim3 grid((unsigned)nb), block(nthr);
   const cpx<T>* tw = reinterpret_cast<const cpx<T>*>(c.tw[2]);
 #define FH_LAUNCH_RI(EPI, NCC)                                                                                \
@@ -101,7 +128,7 @@ int rows_inv_fx_launch(const LaunchCtx& c, int nc, const cpx<T>* in, T* out, T s
     hipLaunchKernelGGL(kf, grid, block, lds, c.stream, in, out, nrows, c.n, rpb, tw, scale, w, partial, halt,  \
                        st);                                                                               \
   } while (0)
-  const int key = (w ? 8 : 0) + nc;
+  const int key = epi * 8 + nc;
   switch (key) {
     case 1: FH_LAUNCH_RI(0, 1); break;
     case 2: FH_LAUNCH_RI(0, 2); break;
@@ -110,6 +137,8 @@ int rows_inv_fx_launch(const LaunchCtx& c, int nc, const cpx<T>* in, T* out, T s
     case 8 + 2: FH_LAUNCH_RI(1, 2); break;
     case 8 + 3: FH_LAUNCH_RI(1, 3); break;
     case 8 + 6: FH_LAUNCH_RI(1, 6); break;
+    case 16 + 2: FH_LAUNCH_RI(2, 2); break;
+    case 16 + 3: FH_LAUNCH_RI(2, 3); break;
     default: return 0;
   }
 #undef FH_LAUNCH_RI
@@ -184,10 +213,11 @@ int cols_g_fx_launch(const LaunchCtx& c, int axis, int nc, cpx<T>* data, const G
 
 template <typename T>
 int fixed_rows_fwd(const LaunchCtx& c, int nc, const T* in, const T* coef, int kind, cpx<T>* out,
-                   const int* halt, const T* rvec, const CGState* st) {
+                   const int* halt, const T* rvec, const CGState* st, T* xc, double* partial,
+                   size_t partial_cap, int* nblocks) {
   switch (c.n2) {
 #define FH_CASE(L, FP, FPR) \
-  case L: return rows_fwd_fx_launch<T, FP>(c, nc, in, coef, kind, out, halt, rvec, st);
+  case L: return rows_fwd_fx_launch<T, FP>(c, nc, in, coef, kind, out, halt, rvec, st, xc, partial, partial_cap, nblocks);
     FH_FIXED_LENGTHS(FH_CASE)
 #undef FH_CASE
     default: return 0;
@@ -195,11 +225,11 @@ int fixed_rows_fwd(const LaunchCtx& c, int nc, const T* in, const T* coef, int k
 }
 
 template <typename T>
-int fixed_rows_inv(const LaunchCtx& c, int nc, const cpx<T>* in, T* out, T scale, const T* w,
+int fixed_rows_inv(const LaunchCtx& c, int nc, const cpx<T>* in, T* out, T scale, int epi, const T* w,
                    double* partial, size_t partial_cap, const int* halt, int* nblocks, CGState* st) {
   switch (c.n2) {
 #define FH_CASE(L, FP, FPR) \
-  case L: return rows_inv_fx_launch<T, FP>(c, nc, in, out, scale, w, partial, partial_cap, halt, nblocks, st);
+  case L: return rows_inv_fx_launch<T, FP>(c, nc, in, out, scale, epi, w, partial, partial_cap, halt, nblocks, st);
     FH_FIXED_LENGTHS(FH_CASE)
 #undef FH_CASE
     default: return 0;

@@ -21,6 +21,7 @@ struct CGState {
   double rr, rr_old, pAp, alpha, beta;
   double tol, delta, rmax, inv_nvox;
   int kit, maxiter, done, pause, halt, status, mixed, hist_len, nupdates;
+  int pend;      // fused iteration: alpha * p of the last iteration not yet added to x
   double* hist;  // device residual log (nullable)
 };
 
@@ -551,6 +552,7 @@ static __global__ void k_cg_restart(CGState* st, const double* __restrict__ part
     const double nr = sqrt(rr);
     st->rmax = nr;
     st->pause = 0;
+    st->pend = 0;
     if (st->kit == 0 && hist && st->hist_len > 0) hist[0] = nr;
     st->done = (!(nr > st->tol)) || (st->kit >= st->maxiter);
     st->halt = st->done;
@@ -563,7 +565,10 @@ static __global__ void k_cg_alpha(CGState* st, const double* __restrict__ partia
   double acc = 0.0;
   for (int i = threadIdx.x; i < np; i += blockDim.x) acc += partial[i];
   const double b = block_sum(acc);
-  if (threadIdx.x == 0) finalize_alpha(st, b);
+  if (threadIdx.x == 0) {
+    finalize_alpha(st, b);
+    st->pend = st->status == 0 ? 1 : 0;
+  }
 }
 
 static __global__ void k_cg_beta(CGState* st, const double* __restrict__ partial, int np) {
@@ -637,6 +642,17 @@ __global__ void k_cg_pupdate(long long n, T* __restrict__ p, const T* __restrict
     pv[i] = Pp;
   }
   for (long long i = nv * W + gid; i < n; i += stride) p[i] = r[i] + bta * p[i];
+}
+
+// Fused iteration: x64 += x32 + alpha p when a step is pending (st->pend), x32 = 0.
+static __global__ void k_fold_pending(long long n, double* __restrict__ x64, float* __restrict__ x32,
+                                      const float* __restrict__ p, const CGState* __restrict__ st) {
+  const float a = st->pend ? (float)st->alpha : 0.0f;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    x64[i] += (double)(x32[i] + a * p[i]);
+    x32[i] = 0.0f;
+  }
 }
 
 // x64 += x32, x32 = 0 (mixed precision: fold the single-precision correction)

@@ -18,12 +18,16 @@ struct LaunchCtx {
 // Return 1 if a specialised kernel was launched, 0 if the length has none (caller falls
 // back to the generic kernels), or a negative hipError_t on launch failure.
 // rvec != null: `in` is the CG direction p, replaced by r + beta p (beta in *st) first.
+// xc != null (with rvec, COEF_ISO/NONE): fused iteration, see k_rows_fwd_fx UPD 2; the
+// partial sums of <p, a p> go to partial, the workgroup count to *nblocks.
 template <typename T>
 int fixed_rows_fwd(const LaunchCtx& c, int nc, const T* in, const T* coef, int kind, cpx<T>* out,
-                   const int* halt, const T* rvec, const CGState* st);
-// w != null: also <out, w> into the CG state (alpha), by the last workgroup.
+                   const int* halt, const T* rvec, const CGState* st, T* xc, double* partial,
+                   size_t partial_cap, int* nblocks);
+// epi 0: out = y; epi 1: also partial sums of <out, w>; epi 2: out is r, r -= alpha y and
+// partial sums of <r, r> (k_rows_inv_fx).
 template <typename T>
-int fixed_rows_inv(const LaunchCtx& c, int nc, const cpx<T>* in, T* out, T scale, const T* w,
+int fixed_rows_inv(const LaunchCtx& c, int nc, const cpx<T>* in, T* out, T scale, int epi, const T* w,
                    double* partial, size_t partial_cap, const int* halt, int* nblocks, CGState* st);
 template <typename T>
 int fixed_cols(const LaunchCtx& c, int dir, int axis, int nc, cpx<T>* data, const int* halt);
@@ -33,12 +37,14 @@ int fixed_cols_g(const LaunchCtx& c, int axis, int nc, cpx<T>* data, const GPara
 bool fixed_length_supported(int L);
 
 extern template int fixed_rows_fwd<float>(const LaunchCtx&, int, const float*, const float*, int, cpx<float>*,
-                                          const int*, const float*, const CGState*);
+                                          const int*, const float*, const CGState*, float*, double*, size_t,
+                                          int*);
 extern template int fixed_rows_fwd<double>(const LaunchCtx&, int, const double*, const double*, int,
-                                           cpx<double>*, const int*, const double*, const CGState*);
-extern template int fixed_rows_inv<float>(const LaunchCtx&, int, const cpx<float>*, float*, float, const float*,
-                                          double*, size_t, const int*, int*, CGState*);
-extern template int fixed_rows_inv<double>(const LaunchCtx&, int, const cpx<double>*, double*, double,
+                                           cpx<double>*, const int*, const double*, const CGState*, double*,
+                                           double*, size_t, int*);
+extern template int fixed_rows_inv<float>(const LaunchCtx&, int, const cpx<float>*, float*, float, int,
+                                          const float*, double*, size_t, const int*, int*, CGState*);
+extern template int fixed_rows_inv<double>(const LaunchCtx&, int, const cpx<double>*, double*, double, int,
                                            const double*, double*, size_t, const int*, int*, CGState*);
 extern template int fixed_cols<float>(const LaunchCtx&, int, int, int, cpx<float>*, const int*);
 extern template int fixed_cols<double>(const LaunchCtx&, int, int, int, cpx<double>*, const int*);

@@ -81,6 +81,15 @@ int fh_plan_set_primaldual(fh_plan* plan, int primaldual);
  * one path against the other. No reference counterpart. */
 int fh_plan_use_generic(fh_plan* plan, int generic);
 
+/* Iteration variant of the mixed-precision CG (FH_SOLVE_MIXED): fused = 1 (default) runs
+ * each single-precision iteration as rows-forward (direction update, pending solution step,
+ * a p, partial <p, a p>), strided passes, rows-inverse (r -= alpha G(a p), partial <r, r>),
+ * using <p, a p> = <p, G(a p)> for p in the range of G; it applies to isotropic
+ * coefficients on the fixed-length kernels and falls back otherwise. fused = 0 keeps the
+ * reference's operator-form denominator <p, Afun(p)> (ffthompy/general/solver.py:125-126).
+ * Plain FP64 solves always use the operator form. */
+int fh_plan_set_cg_variant(fh_plan* plan, int fused);
+
 /* Coefficient A(x) of the operator, host array in origin-0 voxel order.
  * Replaces Material.get_A_GaNi (ffthompy/materials.py:115-123) as the right factor of
  * Afun = [[GN, A]] (ffthompy/applications.py:58). For the dual problem pass the inverse
