@@ -107,7 +107,7 @@ def main():
     ap.add_argument('--fp64-steps', type=int, default=None, help='timed fp64 steps (default: --steps)')
     ap.add_argument('--cpu-iters', type=int, default=2, help='CG iterations in the CPU baseline sample')
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
-    ap.add_argument('--delta', type=float, default=0.01,
+    ap.add_argument('--delta', type=float, default=1e-4,
                     help='mixed precision: residual reduction that triggers an fp64 reliable update')
     ap.add_argument('--profile-only', action='store_true', help='matvec profile only (for rocprofv3 runs)')
     ap.add_argument('--profile-solve', action='store_true',
@@ -157,7 +157,7 @@ def main():
                 pl.matvec(H.EN, H.B)
                 x = H.sols['primal'][0]
                 nat.call('fh_memset_zero', x.ptr, x.nbytes)
-                info = pl.cg_solve(H.B, x, tol=1e-8, maxiter=2000, mode=mode, delta=0.01)
+                info = pl.cg_solve(H.B, x, tol=1e-8, maxiter=2000, mode=mode, delta=args.delta)
                 log('solve %s: kit %d, %.1f ms' % (mode, info['kit'], pl.last_timing()[1]))
         # plain device copy of a known size: the reference point for HBM byte counters
         src = nat.DeviceArray((1 << 27,), np.float64)   # 1 GiB
@@ -217,7 +217,7 @@ def main():
         'config': {'workload': 'GaNi 3-D scalar %d^3, 30 spheres contrast 100, loads e1-e3, primal+dual, '
                                'CG tol 1e-8' % args.n,
                    'grid': list(N), 'volume_fraction': round(float((a > 1).mean()), 4),
-                   'solve': 'fp32 CG + fp64 reliable updates (delta 0.01) + fp64 final projection'},
+                   'solve': 'fp32 CG (fused iteration) + fp64 reliable updates (delta %g) + fp64 final projection' % args.delta},
         'fp64_time_s': round(t_fp64, 4),
         'speedup_mixed_vs_fp64': round(t_fp64 / t_mixed, 3),
         'kit_mixed': r_mixed['kit'], 'kit_fp64': r_fp64['kit'],

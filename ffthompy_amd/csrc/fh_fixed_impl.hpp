@@ -3,6 +3,7 @@
 #pragma once
 #include "fh_fixed.hpp"
 #include "fh_launch.hpp"
+#include "fh_rows_st.hpp"
 
 namespace fh {
 
@@ -61,6 +62,47 @@ int rows_fwd_fx_launch(const LaunchCtx& c, int nc, const T* in, const T* coef, i
   } while (0)
   // the fused direction update is only race-free when each thread reads its own component
   if (rvec && kind != COEF_ISO && kind != COEF_NONE) return 0;
+  if (kind == COEF_ISO || kind == COEF_NONE) {
+    // staged kernels: 16-byte HBM chunks through LDS
+    const int upd = xc ? 2 : (rvec ? 1 : 0);
+    if (upd == 2 && (!rvec || kind != COEF_ISO || (size_t)nb > partial_cap)) return 0;
+#define FH_LAUNCH_ST(KIND, NCC, UPDV)                                                                         \
+  do {                                                                                                        \
+    const size_t slds = RowStage<FP, T, NCC>::lds_bytes(rpb, KIND == COEF_ISO);                               \
+    if (slds + FP::L * sizeof(cpx<T>) > 160 * 1024) return 0;                                                 \
+    auto kf = k_rows_fwd_st<T, FP, KIND, NCC, UPDV>;                                                          \
+    allow_lds(kf, slds);                                                                                      \
+    hipLaunchKernelGGL(kf, grid, block, slds, c.stream, in, coef, out, nrows, c.n, rpb, tw, halt, rvec, st, xc, \
+                       partial);                                                                              \
+  } while (0)
+#define FH_ST_NC(KIND, UPDV)                 \
+  do {                                       \
+    switch (nc) {                            \
+      case 1: FH_LAUNCH_ST(KIND, 1, UPDV); break; \
+      case 2: FH_LAUNCH_ST(KIND, 2, UPDV); break; \
+      case 3: FH_LAUNCH_ST(KIND, 3, UPDV); break; \
+      case 6: FH_LAUNCH_ST(KIND, 6, UPDV); break; \
+      default: return 0;                     \
+    }                                        \
+  } while (0)
+    if (kind == COEF_NONE) {
+      if (upd == 0) FH_ST_NC(COEF_NONE, 0);
+      else if (upd == 1) FH_ST_NC(COEF_NONE, 1);
+      else return 0;
+    } else {
+      if (upd == 0) FH_ST_NC(COEF_ISO, 0);
+      else if (upd == 1) FH_ST_NC(COEF_ISO, 1);
+      else {
+        if constexpr (sizeof(T) == 4) FH_ST_NC(COEF_ISO, 2);
+        else return 0;
+      }
+    }
+#undef FH_ST_NC
+#undef FH_LAUNCH_ST
+    if (nblocks) *nblocks = (int)nb;
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 1 : -(int)e;
+  }
   if (xc) {
     // fused iteration (k_rows_fwd_fx UPD 2), single precision only
     if constexpr (sizeof(T) == 4) {
@@ -123,9 +165,11 @@ int rows_inv_fx_launch(const LaunchCtx& c, int nc, const cpx<T>* in, T* out, T s
   const cpx<T>* tw = reinterpret_cast<const cpx<T>*>(c.tw[2]);
 #define FH_LAUNCH_RI(EPI, NCC)                                                                                \
   do {                                                                                                        \
-    auto kf = k_rows_inv_fx<T, FP, EPI, NCC>;                                                                 \
-    allow_lds(kf, lds);                                                                                       \
-    hipLaunchKernelGGL(kf, grid, block, lds, c.stream, in, out, nrows, c.n, rpb, tw, scale, w, partial, halt,  \
+    const size_t slds = RowStage<FP, T, NCC>::lds_bytes(rpb, false);                                          \
+    if (slds + FP::L * sizeof(cpx<T>) > 160 * 1024) return 0;                                                 \
+    auto kf = k_rows_inv_st<T, FP, EPI, NCC>;                                                                 \
+    allow_lds(kf, slds);                                                                                      \
+    hipLaunchKernelGGL(kf, grid, block, slds, c.stream, in, out, nrows, c.n, rpb, tw, scale, w, partial, halt, \
                        st);                                                                               \
   } while (0)
   const int key = epi * 8 + nc;
@@ -134,9 +178,11 @@ int rows_inv_fx_launch(const LaunchCtx& c, int nc, const cpx<T>* in, T* out, T s
     case 2: FH_LAUNCH_RI(0, 2); break;
     case 3: FH_LAUNCH_RI(0, 3); break;
     case 6: FH_LAUNCH_RI(0, 6); break;
+    case 8 + 1: FH_LAUNCH_RI(1, 1); break;
     case 8 + 2: FH_LAUNCH_RI(1, 2); break;
     case 8 + 3: FH_LAUNCH_RI(1, 3); break;
     case 8 + 6: FH_LAUNCH_RI(1, 6); break;
+    case 16 + 1: FH_LAUNCH_RI(2, 1); break;
     case 16 + 2: FH_LAUNCH_RI(2, 2); break;
     case 16 + 3: FH_LAUNCH_RI(2, 3); break;
     default: return 0;
