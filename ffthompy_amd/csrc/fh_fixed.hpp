@@ -462,6 +462,70 @@ k_cols_g_fx(cpx<T>* __restrict__ data, long long inner, int ntiles, long long to
 
 
 
+// ---- projection pass, one pencil per thread: forward FFT, G(xi) sweep, inverse FFT --------
+// Same contract as k_cols_g_fx. A thread owns one (frequency point, component) pencil, so a
+// tile of F points uses TPP*F*NC threads (three times the waves of k_cols_g_fx for the
+// same LDS); G is applied by a sweep over the tile in LDS between the transforms.
+template <typename T, class FP, class FPR, int F, int NC>
+__global__ void __launch_bounds__(FP::TPP * F * NC)
+k_cols_g_sw(cpx<T>* __restrict__ data, long long inner, int ntiles, long long total,
+            const cpx<T>* __restrict__ twg, GParams g, int n2h, const int* __restrict__ halt) {
+  if (halt && *halt) return;
+  constexpr int L = FP::L, P = F * NC, TPP = FP::TPP, NT = TPP * P;
+  constexpr int NI = FPR::NST;
+  constexpr unsigned CS = sizeof(cpx<T>);
+  __shared__ cpx<T> s[L * P];
+  __shared__ cpx<T> twl[L];
+  const int tid = threadIdx.x;
+  const int q = tid % P, t = tid / P;
+  const int tile = xcd_remap(blockIdx.x, gridDim.x);
+  const long long col0 = (long long)tile * P;
+  const int ncol = (int)min((long long)P, inner - col0);
+  const __amdgpu_buffer_rsrc_t rs = make_rsrc(data + col0, (unsigned long long)(total - col0) * CS);
+  const unsigned rowb = (unsigned)inner * CS;
+  const unsigned qb = (unsigned)q * CS;
+  FxStage<T, FP, 0, -1> s0;
+  s0.load(t, [=](int j) { return bload_c<T>(rs, (unsigned)j * rowb + qb); });
+  for (int e = tid; e < L; e += NT) twl[e] = twg[e];
+  __syncthreads();
+  s0.compute(t, twl);
+  auto at = [&](int j, int) -> cpx<T>& { return s[j * P + q]; };
+  s0.store(t, [&](int j, cpx<T> v) { s[j * P + q] = v; });
+  __syncthreads();
+  fx_lds_stages<T, FP, -1, 1, FP::NST, 1>(t, at, twl);
+  // G(xi) at every (k0, frequency point) of the tile
+  const long long fp0 = col0 / NC;
+  for (int e = tid; e < L * F; e += NT) {
+    const int k0 = e / F, f = e - k0 * F;
+    if (f * NC < ncol) {
+      const long long fp = fp0 + f;
+      int ka, kb;
+      if (g.n0 > 1) {
+        ka = (int)(fp / n2h);
+        kb = (int)(fp - (long long)ka * n2h);
+      } else {
+        ka = 0;
+        kb = (int)fp;
+      }
+      cpx<T> v[NC];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) v[c] = s[k0 * P + f * NC + c];
+      if (g.n0 > 1)
+        apply_G_fx<T, NC>(v, k0, ka, kb, g);
+      else
+        apply_G_fx<T, NC>(v, 0, k0, kb, g);
+#pragma unroll
+      for (int c = 0; c < NC; ++c) s[k0 * P + f * NC + c] = v[c];
+    }
+  }
+  __syncthreads();
+  fx_lds_stages<T, FPR, +1, 0, NI - 1, 1>(t, at, twl);
+  FxStage<T, FPR, NI - 1, +1> sl;
+  sl.load(t, [&](int j) { return s[j * P + q]; });
+  sl.compute(t, twl);
+  if (q < ncol) sl.store(t, [=](int j, cpx<T> v) { bstore_c<T>(rs, (unsigned)j * rowb + qb, v); });
+}
+
 // value of component c of A(x) v at voxel vox, for the contiguous-axis pass; byte
 // offsets: vb = vox*sizeof(T), nb = n*sizeof(T) (component / coefficient stride)
 // pv(d, vb) gives component d of the operand at that voxel.

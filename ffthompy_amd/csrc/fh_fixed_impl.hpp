@@ -230,8 +230,13 @@ int cols_fx_launch(const LaunchCtx& c, int dir, int axis, int nc, cpx<T>* data, 
 
 template <typename T, class FP, class FPR, int NC>
 int cols_g_fx_launch_nc(const LaunchCtx& c, int axis, cpx<T>* data, const GParams& g, const int* halt) {
-  constexpr int F = FxShape<FP, T>::F;
-  if (FP::TPP * F > 1024) return 0;
+  // frequency points per tile: the shape default, halved until TPP*F*NC threads fit a workgroup
+  constexpr int F = [] {
+    int f = FxShape<FP, T>::F;
+    while (f > 1 && FP::TPP * f * NC > 1024) f /= 2;
+    return f;
+  }();
+  if (FP::TPP * F * NC > 1024) return 0;
   long long outer, ostride, inner;
   strided_geometry(c, axis, NC, &outer, &ostride, &inner);
   if (outer != 1) return 0;
@@ -239,9 +244,9 @@ int cols_g_fx_launch_nc(const LaunchCtx& c, int axis, cpx<T>* data, const GParam
   if (!offsets_fit(total, sizeof(cpx<T>))) return 0;
   const long long ntiles = (inner + F * NC - 1) / (F * NC);
   if (ntiles > 0x7fffffffLL) return 0;
-  const dim3 grid((unsigned)ntiles), block(FP::TPP * F);
+  const dim3 grid((unsigned)ntiles), block(FP::TPP * F * NC);
   const cpx<T>* tw = reinterpret_cast<const cpx<T>*>(c.tw[axis]);
-  hipLaunchKernelGGL((k_cols_g_fx<T, FP, FPR, F, NC>), grid, block, 0, c.stream, data, inner, (int)ntiles, total,
+  hipLaunchKernelGGL((k_cols_g_sw<T, FP, FPR, F, NC>), grid, block, 0, c.stream, data, inner, (int)ntiles, total,
                      tw, g, c.n2h, halt);
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? 1 : -(int)e;
