@@ -213,7 +213,7 @@ struct FxShape {
 template <typename T, class FP, int P, int DIR>
 __global__ void __launch_bounds__(FP::TPP * P)
 k_cols_fx(cpx<T>* __restrict__ data, long long ostride, long long inner, int ntiles, long long total,
-          const cpx<T>* __restrict__ twg, const int* __restrict__ halt) {
+          const cpx<T>* __restrict__ twg, const int* __restrict__ halt, int rev) {
   if (halt && *halt) return;
   constexpr int L = FP::L;
   constexpr unsigned CS = sizeof(cpx<T>);
@@ -221,7 +221,8 @@ k_cols_fx(cpx<T>* __restrict__ data, long long ostride, long long inner, int nti
   __shared__ cpx<T> twl[L];
   const int tid = threadIdx.x;
   const int q = tid % P, t = tid / P;
-  const int lin = xcd_remap(blockIdx.x, gridDim.x);
+  const int lin0 = xcd_remap(blockIdx.x, gridDim.x);
+  const int lin = rev ? (int)gridDim.x - 1 - lin0 : lin0;  // rev: newest data of the previous pass first
   const int o = lin / ntiles, tile = lin - o * ntiles;
   const long long col0 = (long long)tile * P;
   const long long start = (long long)o * ostride + col0;
@@ -246,6 +247,15 @@ k_cols_fx(cpx<T>* __restrict__ data, long long ostride, long long inner, int nti
     sl.compute(t, twl);
     if (act) sl.store(t, [=](int j, cpx<T> v) { bstore_c<T>(rs, (unsigned)j * rowb + qb, v); });
   }
+}
+
+// 1/x: hardware reciprocal (1 ulp) in single precision, IEEE division in double.
+template <typename T>
+__device__ __forceinline__ T recip(T x) {
+  if constexpr (sizeof(T) == 4)
+    return __builtin_amdgcn_rcpf(x);
+  else
+    return T(1) / x;
 }
 
 // G(xi) on NC registers (compile-time component count; same formulas as apply_G).
@@ -277,7 +287,7 @@ __device__ __forceinline__ void apply_G_fx(cpx<T> (&v)[NC], int k0, int k1, int 
       cpx<T> dot = mk(T(0), T(0));
 #pragma unroll
       for (int c = 0; c < NC; ++c) dot = dot + xi[c] * v[c];
-      const T f = sc / q;
+      const T f = sc * recip(q);
 #pragma unroll
       for (int c = 0; c < NC; ++c) {
         const cpx<T> w = (xi[c] * f) * dot;
@@ -323,7 +333,7 @@ __device__ __forceinline__ void apply_G_fx(cpx<T> (&v)[NC], int k0, int k1, int 
       for (int j = 0; j < d; ++j) tv[i] = tv[i] + xi[j] * S[i][j];
       qq = qq + xi[i] * tv[i];
     }
-    const T iq = T(1) / q, iq2 = iq * iq;
+    const T iq = recip(q), iq2 = iq * iq;
     auto gam = [&](int i, int j) -> cpx<T> {
       return iq * (xi[i] * tv[j] + xi[j] * tv[i]) - (xi[i] * xi[j] * iq2) * qq;
     };
@@ -493,20 +503,21 @@ k_cols_g_sw(cpx<T>* __restrict__ data, long long inner, int ntiles, long long to
   s0.store(t, [&](int j, cpx<T> v) { s[j * P + q] = v; });
   __syncthreads();
   fx_lds_stages<T, FP, -1, 1, FP::NST, 1>(t, at, twl);
-  // G(xi) at every (k0, frequency point) of the tile
-  const long long fp0 = col0 / NC;
-  for (int e = tid; e < L * F; e += NT) {
-    const int k0 = e / F, f = e - k0 * F;
+  // G(xi) at every (k0, frequency point) of the tile; NT is a multiple of F, so a thread
+  // keeps one frequency point f and walks k0
+  static_assert(NT % F == 0, "sweep layout");
+  const int f = tid % F;
+  const int fp = (int)(col0 / NC) + f;
+  int ka, kb;
+  if (g.n0 > 1) {
+    ka = fp / n2h;
+    kb = fp - ka * n2h;
+  } else {
+    ka = 0;
+    kb = fp;
+  }
+  for (int k0 = tid / F; k0 < L; k0 += NT / F) {
     if (f * NC < ncol) {
-      const long long fp = fp0 + f;
-      int ka, kb;
-      if (g.n0 > 1) {
-        ka = (int)(fp / n2h);
-        kb = (int)(fp - (long long)ka * n2h);
-      } else {
-        ka = 0;
-        kb = (int)fp;
-      }
       cpx<T> v[NC];
 #pragma unroll
       for (int c = 0; c < NC; ++c) v[c] = s[k0 * P + f * NC + c];

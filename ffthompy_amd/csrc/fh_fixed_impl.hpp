@@ -24,6 +24,11 @@ namespace fh {
 // the buffer addressing of the fixed kernels uses 32-bit byte offsets
 inline bool offsets_fit(long long elems, size_t esize) { return (unsigned long long)elems * esize < 0xFFFFFF00ull; }
 
+// Traversal order of a pass: 1 = workgroups walk the tiles from the end, so the data the
+// previous pass wrote last (most likely still in L2 / the Infinity Cache) is read first.
+// Forward strided pass (after the forward row pass) and the inverse row pass run reversed.
+inline int order_rev(int which) { return which == 0 || which == 1; }
+
 template <class K>
 inline void allow_lds(K kernel, size_t bytes) {
   if (bytes > 64 * 1024) (void)hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -170,7 +175,7 @@ int rows_inv_fx_launch(const LaunchCtx& c, int nc, const cpx<T>* in, T* out, T s
     auto kf = k_rows_inv_st<T, FP, EPI, NCC>;                                                                 \
     allow_lds(kf, slds);                                                                                      \
     hipLaunchKernelGGL(kf, grid, block, slds, c.stream, in, out, nrows, c.n, rpb, tw, scale, w, partial, halt, \
-                       st);                                                                               \
+                       st, order_rev(1));                                                                 \
   } while (0)
   const int key = epi * 8 + nc;
   switch (key) {
@@ -220,10 +225,10 @@ int cols_fx_launch(const LaunchCtx& c, int dir, int axis, int nc, cpx<T>* data, 
   const cpx<T>* tw = reinterpret_cast<const cpx<T>*>(c.tw[axis]);
   if (dir < 0)
     hipLaunchKernelGGL((k_cols_fx<T, FP, P, -1>), grid, block, 0, c.stream, data, ostride, inner, (int)ntiles,
-                       total, tw, halt);
+                       total, tw, halt, order_rev(0));
   else
     hipLaunchKernelGGL((k_cols_fx<T, FP, P, +1>), grid, block, 0, c.stream, data, ostride, inner, (int)ntiles,
-                       total, tw, halt);
+                       total, tw, halt, order_rev(2));
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? 1 : -(int)e;
 }

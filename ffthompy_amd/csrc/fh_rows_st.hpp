@@ -180,25 +180,29 @@ k_rows_fwd_st(const T* __restrict__ in, const T* __restrict__ coef, cpx<T>* __re
     auto at = [&](int j, int) -> cpx<T>& { return sp[j]; };
     fx_lds_stages<T, FP, -1, 1, FP::NST, 1>(t, at, twl);
   }
-  // untangle the paired rows and store the AoS half spectrum (contiguous per block)
-  const int per = n2h * NC;
-  cpx<T>* obp = out + row0 * per;
-  for (int e = tid; e < nr * per; e += nthr) {
-    const int r = e / per;
-    const int rem = e - r * per;
-    const int k = rem / NC, c = rem - k * NC;
-    const int rr2 = (r & 1) * NC + c;
-    const cpx<T>* z = s + ((r >> 1) * NC + (rr2 >> 1)) * L;
-    const cpx<T> Z = z[k];
-    const cpx<T> Zc = conj(z[k ? L - k : 0]);
-    cpx<T> Y;
-    if (rr2 & 1) {
-      const cpx<T> d = Z - Zc;
-      Y = mk(T(0.5) * d.y, T(-0.5) * d.x);
-    } else {
-      Y = T(0.5) * (Z + Zc);
+  // untangle the paired rows and store the AoS half spectrum (contiguous per block): a
+  // thread takes one (row, k) and writes its NC adjacent components
+  cpx<T>* obp = out + row0 * (n2h * NC);
+  for (int e = tid; e < nr * n2h; e += nthr) {
+    const int r = e / n2h, k = e - r * n2h;
+    const int kc = k ? L - k : 0;
+    const cpx<T>* zr = s + (r >> 1) * NC * L;
+    cpx<T> Y[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const int rr2 = (r & 1) * NC + c;
+      const cpx<T>* z = zr + (rr2 >> 1) * L;
+      const cpx<T> Z = z[k];
+      const cpx<T> Zc = conj(z[kc]);
+      if (rr2 & 1) {
+        const cpx<T> d = Z - Zc;
+        Y[c] = mk(T(0.5) * d.y, T(-0.5) * d.x);
+      } else {
+        Y[c] = T(0.5) * (Z + Zc);
+      }
     }
-    obp[e] = Y;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) obp[e * NC + c] = Y[c];
   }
   if constexpr (UPD == 2) {
     const double bsum = block_sum(acc);
@@ -213,7 +217,7 @@ template <typename T, class FP, int EPI, int NC>
 __global__ void __launch_bounds__(1024)
 k_rows_inv_st(const cpx<T>* __restrict__ in, T* __restrict__ out, long long nrows, long long n, int rpb,
               const cpx<T>* __restrict__ twg, T scale, const T* __restrict__ w, double* __restrict__ partial,
-              const int* __restrict__ halt, CGState* __restrict__ st) {
+              const int* __restrict__ halt, CGState* __restrict__ st, int rev) {
   if (halt && *halt) return;
   constexpr int L = FP::L, n2h = L / 2 + 1, TPP = FP::TPP, W = Chunk<T>::W;
   constexpr unsigned SR = sizeof(T);
@@ -225,7 +229,8 @@ k_rows_inv_st(const cpx<T>* __restrict__ in, T* __restrict__ out, long long nrow
   __shared__ cpx<T> twl[L];
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int t = tid % TPP, pen = tid / TPP;
-  const long long row0 = (long long)xcd_remap(blockIdx.x, gridDim.x) * rpb;
+  const int blk0 = xcd_remap(blockIdx.x, gridDim.x);
+  const long long row0 = (long long)(rev ? (int)gridDim.x - 1 - blk0 : blk0) * rpb;
   const int nr = (int)min((long long)rpb, nrows - row0);
   const unsigned len = (unsigned)nr * L;
   const unsigned rbase = (unsigned)(row0 * L);
