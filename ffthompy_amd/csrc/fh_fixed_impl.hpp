@@ -67,8 +67,9 @@ int rows_fwd_fx_launch(const LaunchCtx& c, int nc, const T* in, const T* coef, i
   } while (0)
   // the fused direction update is only race-free when each thread reads its own component
   if (rvec && kind != COEF_ISO && kind != COEF_NONE) return 0;
-  if (kind == COEF_ISO || kind == COEF_NONE) {
-    // staged kernels: 16-byte HBM chunks through LDS
+  // staged kernels (16-byte HBM chunks through LDS) in single precision; in double precision
+  // their LDS footprint halves the occupancy and the direct kernel is faster
+  if ((kind == COEF_ISO || kind == COEF_NONE) && sizeof(T) == 4) {
     const int upd = xc ? 2 : (rvec ? 1 : 0);
     if (upd == 2 && (!rvec || kind != COEF_ISO || (size_t)nb > partial_cap)) return 0;
 #define FH_LAUNCH_ST(KIND, NCC, UPDV)                                                                         \

@@ -9,6 +9,8 @@
 //    stage reads a(x) v(x) from the staging, pairs two real rows into one complex pencil,
 //    and overwrites the staging with pencils (after a barrier); stages run in LDS; the
 //    half spectra are untangled into the AoS spectrum.
+//  * FFT threads: lane q = tid % npen owns pencil q (consecutive lanes, consecutive
+//    pencils at the same element: conflict-free LDS access, shared twiddles).
 //  * inverse: AoS spectrum -> Hermitian-completed complex pencils, stages in LDS, the
 //    last stage writes its two real rows into the real-row staging; the epilogue (store,
 //    dot product or residual update) reads the staging and HBM in 16-byte chunks.
@@ -45,9 +47,13 @@ struct RowStage {
   // starts W + (s0 mod W) values into its region and may be read up to W - 1 values past
   // its end), a whole number of chunks
   __host__ __device__ static constexpr int comp_stride(int rpb) { return (rpb * FP::L + 3 * W + W - 1) / W * W; }
+  // pencils are interleaved, element j of pencil q at j*S + q, with an odd stride S >= the
+  // pencil count: lanes on consecutive pencils hit consecutive banks, lanes walking j of one
+  // pencil hit distinct bank pairs
+  __host__ __device__ static constexpr int pen_stride(int rpb) { return ((rpb / 2) * NC) | 1; }
   // bytes: max(pencils, staging) + coefficient staging (forward pass only)
   __host__ __device__ static constexpr size_t lds_bytes(int rpb, bool coef) {
-    const size_t pen = (size_t)(rpb / 2) * NC * FP::L * sizeof(cpx<T>);
+    const size_t pen = (size_t)pen_stride(rpb) * FP::L * sizeof(cpx<T>);
     const size_t stg = (size_t)NC * comp_stride(rpb) * sizeof(T);
     return (pen > stg ? pen : stg) + (coef ? (size_t)comp_stride(rpb) * sizeof(T) : 0);
   }
@@ -88,62 +94,96 @@ k_rows_fwd_st(const T* __restrict__ in, const T* __restrict__ coef, cpx<T>* __re
   for (int e = tid; e < L; e += nthr) twl[e] = twg[e];
   const __amdgpu_buffer_rsrc_t rin = make_rsrc(in, (unsigned long long)NC * n * SR);
   const unsigned rbase = (unsigned)(row0 * L);
-  // coefficient rows -> LDS
-  if constexpr (COEF == COEF_ISO) {
-    const __amdgpu_buffer_rsrc_t rco = make_rsrc(coef, (unsigned long long)n * SR);
-    const int ab = stage_base<T>(0, rbase);
-    const unsigned q0 = rbase / W, q1 = (rbase + len + W - 1) / W;
-    for (unsigned q = q0 + tid; q < q1; q += nthr) {
-      const Chunk<T> v = bload_chunk<T>(rco, q * 16u);
-      *reinterpret_cast<Chunk<T>*>(sa + ab + (int)(q * W - rbase)) = v;
-    }
-  }
-  // CG vector updates, the operand v -> real-row staging
+  // every HBM load of the block is issued before the first use (CPT chunks per thread per run)
+  constexpr int RPB = FxShape<FP, T>::RPB;
+  constexpr int NT = (RPB / 2) * NC * TPP;
+  constexpr int QMAX = (RPB * L + 2 * W - 2) / W + 1;  // chunks touched by one run of RPB rows
+  constexpr int CPT = (QMAX + NT - 1) / NT;
   const T bta = UPD ? T(st->beta) : T(0);
   const bool fold = UPD == 2 && st->pend != 0;
   const T alp = UPD == 2 ? T(st->alpha) : T(0);
   const __amdgpu_buffer_rsrc_t rr = make_rsrc(UPD ? rvec : in, (unsigned long long)NC * n * SR);
   const __amdgpu_buffer_rsrc_t rx = make_rsrc(UPD == 2 ? xc : in, (unsigned long long)NC * n * SR);
+  Chunk<T> P[NC][CPT], R[UPD ? NC : 1][CPT], X[UPD == 2 ? NC : 1][CPT], A[COEF == COEF_ISO ? CPT : 1];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    const unsigned s0 = (unsigned)(c * n) + rbase;
+    const unsigned q0 = s0 / W, q1 = (s0 + len + W - 1) / W;
+#pragma unroll
+    for (int i = 0; i < CPT; ++i) {
+      const unsigned q = q0 + tid + i * NT;
+      if (q < q1) {
+        P[c][i] = bload_chunk<T>(rin, q * 16u);
+        if constexpr (UPD >= 1) R[c][i] = bload_chunk<T>(rr, q * 16u);
+        if constexpr (UPD == 2) {
+          if (fold) X[c][i] = bload_chunk<T>(rx, q * 16u);
+        }
+      }
+    }
+  }
+  if constexpr (COEF == COEF_ISO) {
+    const __amdgpu_buffer_rsrc_t rco = make_rsrc(coef, (unsigned long long)n * SR);
+    const unsigned q0 = rbase / W, q1 = (rbase + len + W - 1) / W;
+#pragma unroll
+    for (int i = 0; i < CPT; ++i) {
+      const unsigned q = q0 + tid + i * NT;
+      if (q < q1) A[i] = bload_chunk<T>(rco, q * 16u);
+    }
+    const int ab = stage_base<T>(0, rbase);
+#pragma unroll
+    for (int i = 0; i < CPT; ++i) {
+      const unsigned q = q0 + tid + i * NT;
+      if (q < q1) *reinterpret_cast<Chunk<T>*>(sa + ab + (int)(q * W - rbase)) = A[i];
+    }
+  }
+  // CG vector updates, the operand v -> real-row staging
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     const unsigned s0 = (unsigned)(c * n) + rbase;
     const int sb = stage_base<T>(c * cstr, s0);
     const unsigned q0 = s0 / W, q1 = (s0 + len + W - 1) / W;
-    for (unsigned q = q0 + tid; q < q1; q += nthr) {
+#pragma unroll
+    for (int i = 0; i < CPT; ++i) {
+      const unsigned q = q0 + tid + i * NT;
+      if (q >= q1) continue;
       const unsigned ob = q * 16u;
-      Chunk<T> P = bload_chunk<T>(rin, ob);
+      Chunk<T> Pv = P[c][i];
       if constexpr (UPD >= 1) {
-        const Chunk<T> R = bload_chunk<T>(rr, ob);
-        Chunk<T> X;
-        if (fold) X = bload_chunk<T>(rx, ob);
         Chunk<T> Pn;
 #pragma unroll
         for (int e = 0; e < W; ++e) {
-          if (fold) X.e[e] += alp * P.e[e];
-          Pn.e[e] = R.e[e] + bta * P.e[e];
+          if constexpr (UPD == 2) {
+            if (fold) X[c][i].e[e] += alp * Pv.e[e];
+          }
+          Pn.e[e] = R[c][i].e[e] + bta * Pv.e[e];
         }
         const bool full = q * W >= s0 && q * W + W <= s0 + len;
         if (full) {
           bstore_chunk<T>(rin, ob, Pn);
-          if (fold) bstore_chunk<T>(rx, ob, X);
+          if constexpr (UPD == 2) {
+            if (fold) bstore_chunk<T>(rx, ob, X[c][i]);
+          }
         } else {
 #pragma unroll
           for (int e = 0; e < W; ++e) {
             const unsigned f = q * W + e;
             if (f >= s0 && f < s0 + len) {
               bstore_r<T>(rin, ob + e * SR, Pn.e[e]);
-              if (fold) bstore_r<T>(rx, ob + e * SR, X.e[e]);
+              if constexpr (UPD == 2) {
+                if (fold) bstore_r<T>(rx, ob + e * SR, X[c][i].e[e]);
+              }
             }
           }
         }
-        P = Pn;
+        Pv = Pn;
       }
-      *reinterpret_cast<Chunk<T>*>(sr + sb + (int)(q * W - s0)) = P;
+      *reinterpret_cast<Chunk<T>*>(sr + sb + (int)(q * W - s0)) = Pv;
     }
   }
   __syncthreads();
   // first stage: a(x) v(x) of two real rows per complex pencil, from the staging
-  const int t = tid % TPP, pen = tid / TPP;
+  const int npen = (rpb / 2) * NC, S = RS::pen_stride(rpb);
+  const int pen = tid % npen, t = tid / npen;
   const int rp = pen / NC, m = pen - rp * NC;
   const int rra = 2 * m, rrb = 2 * m + 1;
   const int rowa = 2 * rp + rra / NC, ca = rra % NC;
@@ -155,54 +195,51 @@ k_rows_fwd_st(const T* __restrict__ in, const T* __restrict__ coef, cpx<T>* __re
   double acc = 0.0;
   {
     FxStage<T, FP, 0, -1> s0;
+    // rows past nr (last block) read in-bounds staging garbage that the selects discard
     s0.load(t, [&](int x) {
-      T ya = T(0), yb = T(0);
-      if (va) {
-        const T v = sr[pa + x];
-        ya = (COEF == COEF_ISO) ? sa[aa + x] * v : v;
-        if constexpr (UPD == 2) acc += (double)ya * (double)v;
-      }
-      if (vb) {
-        const T v = sr[pb + x];
-        yb = (COEF == COEF_ISO) ? sa[ab2 + x] * v : v;
-        if constexpr (UPD == 2) acc += (double)yb * (double)v;
-      }
+      const T v0 = sr[pa + x], v1 = sr[pb + x];
+      const T y0 = (COEF == COEF_ISO) ? sa[aa + x] * v0 : v0;
+      const T y1 = (COEF == COEF_ISO) ? sa[ab2 + x] * v1 : v1;
+      const T ya = va ? y0 : T(0), yb = vb ? y1 : T(0);
+      if constexpr (UPD == 2) acc += (double)ya * (double)(va ? v0 : T(0)) + (double)yb * (double)(vb ? v1 : T(0));
       return mk(ya, yb);
     });
     __syncthreads();  // the pencils overwrite the staging
     s0.compute(t, twl);
-    cpx<T>* sp = s + pen * L;
-    s0.store(t, [&](int j, cpx<T> v) { sp[j] = v; });
+    s0.store(t, [&](int j, cpx<T> v) { s[j * S + pen] = v; });
   }
   __syncthreads();
   {
-    cpx<T>* sp = s + pen * L;
-    auto at = [&](int j, int) -> cpx<T>& { return sp[j]; };
+    auto at = [&](int j, int) -> cpx<T>& { return s[j * S + pen]; };
     fx_lds_stages<T, FP, -1, 1, FP::NST, 1>(t, at, twl);
   }
   // untangle the paired rows and store the AoS half spectrum (contiguous per block): a
-  // thread takes one (row, k) and writes its NC adjacent components
+  // thread takes one row pair and one k; pencil m holds the real rows 2m and 2m+1 of the
+  // pair's 2*NC (row, component) rows, so two LDS reads give two outputs
   cpx<T>* obp = out + row0 * (n2h * NC);
-  for (int e = tid; e < nr * n2h; e += nthr) {
-    const int r = e / n2h, k = e - r * n2h;
+  const int nrp_v = (nr + 1) >> 1;
+  for (int e = tid; e < nrp_v * n2h; e += nthr) {
+    const int rp = e / n2h, k = e - rp * n2h;
     const int kc = k ? L - k : 0;
-    const cpx<T>* zr = s + (r >> 1) * NC * L;
-    cpx<T> Y[NC];
+    cpx<T> Y[2 * NC];
 #pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      const int rr2 = (r & 1) * NC + c;
-      const cpx<T>* z = zr + (rr2 >> 1) * L;
-      const cpx<T> Z = z[k];
-      const cpx<T> Zc = conj(z[kc]);
-      if (rr2 & 1) {
-        const cpx<T> d = Z - Zc;
-        Y[c] = mk(T(0.5) * d.y, T(-0.5) * d.x);
-      } else {
-        Y[c] = T(0.5) * (Z + Zc);
-      }
+    for (int m = 0; m < NC; ++m) {
+      const int q = rp * NC + m;
+      const cpx<T> Z = s[k * S + q];
+      const cpx<T> Zc = conj(s[kc * S + q]);
+      const cpx<T> d = Z - Zc;
+      Y[2 * m] = T(0.5) * (Z + Zc);
+      Y[2 * m + 1] = mk(T(0.5) * d.y, T(-0.5) * d.x);
     }
+    // Y[h*NC + c] is (row 2rp+h, component c)
+    cpx<T>* o0 = obp + ((long long)(2 * rp) * n2h + k) * NC;
 #pragma unroll
-    for (int c = 0; c < NC; ++c) obp[e * NC + c] = Y[c];
+    for (int c = 0; c < NC; ++c) o0[c] = Y[c];
+    if (2 * rp + 1 < nr) {
+      cpx<T>* o1 = o0 + (long long)n2h * NC;
+#pragma unroll
+      for (int c = 0; c < NC; ++c) o1[c] = Y[NC + c];
+    }
   }
   if constexpr (UPD == 2) {
     const double bsum = block_sum(acc);
@@ -228,40 +265,66 @@ k_rows_inv_st(const cpx<T>* __restrict__ in, T* __restrict__ out, long long nrow
   const int cstr = RS::comp_stride(rpb);
   __shared__ cpx<T> twl[L];
   const int tid = threadIdx.x, nthr = blockDim.x;
-  const int t = tid % TPP, pen = tid / TPP;
+  const int npen = (rpb / 2) * NC, S = RS::pen_stride(rpb);
+  const int pen = tid % npen, t = tid / npen;
   const int blk0 = xcd_remap(blockIdx.x, gridDim.x);
   const long long row0 = (long long)(rev ? (int)gridDim.x - 1 - blk0 : blk0) * rpb;
   const int nr = (int)min((long long)rpb, nrows - row0);
   const unsigned len = (unsigned)nr * L;
   const unsigned rbase = (unsigned)(row0 * L);
   for (int e = tid; e < L; e += nthr) twl[e] = twg[e];
-  // Hermitian completion: pencil (rp, mm) = spectrum of real row a + i spectrum of row b
+  // the epilogue's operand (w or r) is fetched now; it arrives while the transform runs
+  constexpr int RPB = FxShape<FP, T>::RPB;
+  constexpr int NT = (RPB / 2) * NC * TPP;
+  constexpr int QMAX = (RPB * L + 2 * W - 2) / W + 1;
+  constexpr int CPT = (QMAX + NT - 1) / NT;
+  Chunk<T> V[EPI >= 1 ? NC : 1][CPT];
+  if constexpr (EPI >= 1) {
+    const __amdgpu_buffer_rsrc_t rw = make_rsrc(EPI == 1 ? w : out, (unsigned long long)NC * n * SR);
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const unsigned s0 = (unsigned)(c * n) + rbase;
+      const unsigned q0 = s0 / W, q1 = (s0 + len + W - 1) / W;
+#pragma unroll
+      for (int i = 0; i < CPT; ++i) {
+        const unsigned q = q0 + tid + i * NT;
+        if (q < q1) V[c][i] = bload_chunk<T>(rw, q * 16u);
+      }
+    }
+  }
+  // Hermitian completion: pencil q = rp*NC + m holds real rows 2m (real part) and 2m+1
+  // (imaginary part) of row pair rp's 2*NC (row, component) rows. Rows past nr read 0
+  // (buffer range); at k = 0 (and k = L/2) both writes hit one element with one value.
   const int nrp = rpb >> 1;
-  const cpx<T>* ib = in + row0 * (long long)n2h * NC;
+  constexpr unsigned CS = sizeof(cpx<T>);
+  const __amdgpu_buffer_rsrc_t rsp =
+      make_rsrc(in + row0 * (long long)n2h * NC, (unsigned long long)nr * n2h * NC * CS);
   for (int e = tid; e < nrp * n2h; e += nthr) {
     const int rp = e / n2h, k = e - rp * n2h;
     const bool self = (k == 0) || (2 * k == L);
+    const unsigned o0 = (unsigned)(((2 * rp) * n2h + k) * NC) * CS;
+    const unsigned o1 = o0 + (unsigned)(n2h * NC) * CS;
+    cpx<T> Y[2 * NC];
 #pragma unroll
-    for (int mm = 0; mm < NC; ++mm) {
-      int rr = 2 * mm;
-      int row = 2 * rp + rr / NC;
-      cpx<T> Ya = mk(T(0), T(0)), Yb = mk(T(0), T(0));
-      if (row < nr) Ya = ib[((long long)row * n2h + k) * NC + rr % NC];
-      rr = 2 * mm + 1;
-      row = 2 * rp + rr / NC;
-      if (row < nr) Yb = ib[((long long)row * n2h + k) * NC + rr % NC];
+    for (int c = 0; c < NC; ++c) {
+      Y[c] = bload_c<T>(rsp, o0 + c * CS);
+      Y[NC + c] = bload_c<T>(rsp, o1 + c * CS);
+    }
+    const int kc = k ? L - k : 0;
+#pragma unroll
+    for (int m = 0; m < NC; ++m) {
+      cpx<T> Ya = Y[2 * m], Yb = Y[2 * m + 1];
       if (self) {
         Ya.y = T(0);
         Yb.y = T(0);
       }
-      cpx<T>* z = s + (rp * NC + mm) * L;
-      z[k] = mk(Ya.x - Yb.y, Ya.y + Yb.x);
-      if (!self) z[L - k] = mk(Ya.x + Yb.y, Yb.x - Ya.y);
+      const int q = rp * NC + m;
+      s[k * S + q] = mk(Ya.x - Yb.y, Ya.y + Yb.x);
+      s[kc * S + q] = mk(Ya.x + Yb.y, Yb.x - Ya.y);
     }
   }
   __syncthreads();
-  cpx<T>* sp = s + pen * L;
-  auto at = [&](int j, int) -> cpx<T>& { return sp[j]; };
+  auto at = [&](int j, int) -> cpx<T>& { return s[j * S + pen]; };
   fx_lds_stages<T, FP, +1, 0, FP::NST - 1, 1>(t, at, twl);
   // last stage: its two real rows go to the real-row staging (which overlays the pencils)
   {
@@ -271,7 +334,7 @@ k_rows_inv_st(const cpx<T>* __restrict__ in, T* __restrict__ out, long long nrow
     const int pa = stage_base<T>(ca * cstr, (unsigned)(ca * n) + rbase) + rowa * L;
     const int pb = stage_base<T>(cb * cstr, (unsigned)(cb * n) + rbase) + rowb * L;
     FxStage<T, FP, FP::NST - 1, +1> sl;
-    sl.load(t, [&](int j) { return sp[j]; });
+    sl.load(t, [&](int j) { return s[j * S + pen]; });
     __syncthreads();
     sl.compute(t, twl);
     sl.store(t, [&](int j, cpx<T> v) {
@@ -282,7 +345,6 @@ k_rows_inv_st(const cpx<T>* __restrict__ in, T* __restrict__ out, long long nrow
   __syncthreads();
   // epilogue over each component's run of rows, in 16-byte chunks
   const __amdgpu_buffer_rsrc_t rout = make_rsrc(out, (unsigned long long)NC * n * SR);
-  const __amdgpu_buffer_rsrc_t rw = make_rsrc(EPI == 1 ? w : out, (unsigned long long)NC * n * SR);
   const T alp = EPI == 2 ? T(st->alpha) : T(0);
   double acc = 0.0;
 #pragma unroll
@@ -290,21 +352,23 @@ k_rows_inv_st(const cpx<T>* __restrict__ in, T* __restrict__ out, long long nrow
     const unsigned s0 = (unsigned)(c * n) + rbase;
     const int sb = stage_base<T>(c * cstr, s0);
     const unsigned q0 = s0 / W, q1 = (s0 + len + W - 1) / W;
-    for (unsigned q = q0 + tid; q < q1; q += nthr) {
+#pragma unroll
+    for (int i = 0; i < CPT; ++i) {
+      const unsigned q = q0 + tid + i * NT;
+      if (q >= q1) continue;
       const unsigned ob = q * 16u;
       const Chunk<T> Y = *reinterpret_cast<const Chunk<T>*>(sr + sb + (int)(q * W - s0));
       Chunk<T> O = Y;
       const bool full = q * W >= s0 && q * W + W <= s0 + len;
       if constexpr (EPI >= 1) {
-        const Chunk<T> V = bload_chunk<T>(rw, ob);
 #pragma unroll
         for (int e = 0; e < W; ++e) {
           const unsigned f = q * W + e;
           const bool inr = full || (f >= s0 && f < s0 + len);
           if constexpr (EPI == 1) {
-            if (inr) acc += (double)Y.e[e] * (double)V.e[e];
+            if (inr) acc += (double)Y.e[e] * (double)V[c][i].e[e];
           } else {
-            O.e[e] = V.e[e] - alp * Y.e[e];
+            O.e[e] = V[c][i].e[e] - alp * Y.e[e];
             if (inr) acc += (double)O.e[e] * (double)O.e[e];
           }
         }
