@@ -61,6 +61,8 @@ SIGNATURES = {
     'fh_batch_solve': [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double, _c_double_p, _c_double_p,
                        _c_int_p, _c_int_p],
     'fh_batch_stats': [_vp, _c_int_p, _c_double_p, _c_double_p],
+    'fh_batch_set_schedule': [_vp, ctypes.c_int, ctypes.c_int],
+    'fh_batch_autotune': [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, _c_double_p, _c_int_p, _c_int_p],
     'fh_synchronize': [],
 }
 
@@ -401,6 +403,20 @@ class Batch:
              up.ctypes.data_as(_c_double_p), lo.ctypes.data_as(_c_double_p), kit.ctypes.data_as(_c_int_p),
              st.ctypes.data_as(_c_int_p))
         return up, lo, kit, st
+
+    def set_schedule(self, variant, order_mode=0):
+        call('fh_batch_set_schedule', self._h, ctypes.c_int(variant), ctypes.c_int(order_mode))
+
+    def autotune(self, mode='mixed', delta=0.01, iters=12):
+        """Time every kernel schedule on this batch; keep the fastest. Returns (times_ms[order][variant],
+        variant, order_mode)."""
+        t = np.zeros(4)
+        v = ctypes.c_int()
+        o = ctypes.c_int()
+        call('fh_batch_autotune', self._h, ctypes.c_int(FH_SOLVE_FP64 if mode == 'fp64' else FH_SOLVE_MIXED),
+             ctypes.c_double(delta), ctypes.c_int(iters), t.ctypes.data_as(_c_double_p), ctypes.byref(v),
+             ctypes.byref(o))
+        return t.reshape(2, 2), v.value, o.value
 
     def stats(self):
         nupd = np.zeros((self.S, 4), dtype=np.int32)

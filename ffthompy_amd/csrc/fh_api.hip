@@ -183,6 +183,10 @@ struct fh_batch {
   hipGraphExec_t exec = nullptr;
   bool coef_set = false;
   double last_ms = 0.0;
+  int variant = -1;                // radix schedule variant of the solve kernel (-1: autotune)
+  int order_mode = 0;              // 0 largest grids first, 1 sample order
+  std::vector<int> order_size, order_sample;
+  double tune_ms[2 * fh::batch::NVAR] = {0};
 };
 
 namespace {
@@ -1279,10 +1283,12 @@ int fh_batch_create(int nsamples, const int* dims, const double* Y, fh_batch** o
   // tasks: largest grids first, primal and dual of a sample adjacent
   std::vector<int> order(b->ntasks);
   for (int t = 0; t < b->ntasks; ++t) order[t] = t;
+  b->order_sample = order;
   std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
     const B::Sample &sa = b->smp[a >> 1], &sc = b->smp[c >> 1];
     return (long long)sa.n0 * sa.n1 > (long long)sc.n0 * sc.n1;
   });
+  b->order_size = order;
   int ncu = 0;
   HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, 0));
   b->nslots = std::max(1, std::min(b->ntasks, ncu));
@@ -1374,11 +1380,12 @@ int fh_batch_solve(fh_batch* b, int mode, double tol, int maxiter, double delta,
   P.delta = delta;
   P.maxiter = maxiter;
   P.mixed = mode == FH_SOLVE_MIXED;
+  if (b->variant < 0) RETCHK(fh_batch_autotune(b, mode, delta, 12, nullptr, nullptr, nullptr));
   HIPCHK(hipMemcpyAsync(b->d_par, &P, sizeof(P), hipMemcpyHostToDevice, b->stream));
   if (!b->exec) {
     // one graph: counter reset + the persistent solve kernel; replayed by every call
     HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    hipError_t e = fh::batch::enqueue(b->args, b->nslots, b->lds, b->stream);
+    hipError_t e = fh::batch::enqueue(b->args, b->nslots, b->lds, b->stream, b->variant);
     hipGraph_t g = nullptr;
     hipError_t e2 = hipStreamEndCapture(b->stream, &g);
     if (e != hipSuccess || e2 != hipSuccess) {
@@ -1417,6 +1424,69 @@ int fh_batch_solve(fh_batch* b, int mode, double tol, int maxiter, double delta,
       for (int k = 0; k < 4; ++k) kit[4 * s + k] = K[(size_t)s * 4 + k];
     if (status) status[s] = St[2 * s] | St[2 * s + 1];
   }
+  return FH_OK;
+}
+
+static int batch_set_schedule(fh_batch* b, int variant, int order_mode) {
+  const std::vector<int>& ord = order_mode ? b->order_sample : b->order_size;
+  HIPCHK(hipMemcpy(b->d_order, ord.data(), ord.size() * sizeof(int), hipMemcpyHostToDevice));
+  if (variant != b->variant && b->exec) {
+    (void)hipGraphExecDestroy(b->exec);
+    (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+  }
+  b->variant = variant;
+  b->order_mode = order_mode;
+  return FH_OK;
+}
+
+int fh_batch_set_schedule(fh_batch* b, int variant, int order_mode) {
+  if (!b) return fail(FH_ERR_ARG, "null batch");
+  if (variant < 0 || variant >= fh::batch::NVAR || order_mode < 0 || order_mode > 1)
+    return fail(FH_ERR_ARG, "bad schedule");
+  return batch_set_schedule(b, variant, order_mode);
+}
+
+int fh_batch_autotune(fh_batch* b, int mode, double delta, int iters, double* times_ms, int* variant,
+                      int* order_mode) {
+  if (!b) return fail(FH_ERR_ARG, "null batch");
+  if (!b->coef_set) return fail(FH_ERR_STATE, "fh_batch_set_coeff has not been called");
+  if (iters < 1) return fail(FH_ERR_ARG, "iters must be positive");
+  namespace B = fh::batch;
+  // every task runs `iters` CG iterations per load (tolerance 0) with each schedule
+  B::Params P;
+  P.tol = 0.0;
+  P.delta = (delta > 0.0 && delta < 1.0) ? delta : 0.01;
+  P.maxiter = iters;
+  P.mixed = mode == FH_SOLVE_MIXED;
+  HIPCHK(hipMemcpy(b->d_par, &P, sizeof(P), hipMemcpyHostToDevice));
+  double best = 1e300;
+  int bv = 0, bo = 0;
+  for (int o = 0; o < 2; ++o) {
+    for (int v = 0; v < B::NVAR; ++v) {
+      RETCHK(batch_set_schedule(b, v, o));
+      if (o == 0) {  // first launch of each kernel: load its code before timing
+        HIPCHK(B::enqueue(b->args, b->nslots, b->lds, b->stream, v));
+      }
+      HIPCHK(hipEventRecord(b->ev0, b->stream));
+      HIPCHK(B::enqueue(b->args, b->nslots, b->lds, b->stream, v));
+      HIPCHK(hipEventRecord(b->ev1, b->stream));
+      HIPCHK(hipStreamSynchronize(b->stream));
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+      b->tune_ms[o * B::NVAR + v] = ms;
+      if (times_ms) times_ms[o * B::NVAR + v] = ms;
+      if (ms < best) {
+        best = ms;
+        bv = v;
+        bo = o;
+      }
+    }
+  }
+  RETCHK(batch_set_schedule(b, bv, bo));
+  if (variant) *variant = bv;
+  if (order_mode) *order_mode = bo;
   return FH_OK;
 }
 

@@ -81,6 +81,7 @@ struct CG {
 
 }  // namespace
 
+template <int VAR>
 __global__ void __launch_bounds__(NT) k_batch_solve(Args A) {
   __shared__ int sh_task;
   const int tid = threadIdx.x;
@@ -146,7 +147,7 @@ __global__ void __launch_bounds__(NT) k_batch_solve(Args A) {
           OutOp<float> out = out_op<float>(1, nv);
           out.y = s.q32;
           out.p = s.p32;
-          matvec<float, true>(tk, nullptr, in, out);
+          matvec<float, true, VAR>(tk, nullptr, in, out);
           acc = out.acc;
         } else {
           InOp<double> in = in_op<double>(1, coef, nv, a64);
@@ -174,7 +175,7 @@ __global__ void __launch_bounds__(NT) k_batch_solve(Args A) {
             in.v = x;
             out.y = x;
           }
-          matvec<double, false>(tk, s.spec64, in, out);
+          matvec<double, false, VAR>(tk, s.spec64, in, out);
           acc = out.acc;
         }
         // ---- what follows the product
@@ -296,15 +297,24 @@ __global__ void __launch_bounds__(NT) k_batch_solve(Args A) {
   }
 }
 
+template __global__ void k_batch_solve<0>(Args);
+template __global__ void k_batch_solve<1>(Args);
+
 hipError_t allow_lds(size_t lds) {
-  return hipFuncSetAttribute((const void*)k_batch_solve, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipError_t e = hipFuncSetAttribute((const void*)k_batch_solve<0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void*)k_batch_solve<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  return e;
 }
 
-hipError_t enqueue(const Args& A, int nslots, size_t lds, hipStream_t stream) {
+hipError_t enqueue(const Args& A, int nslots, size_t lds, hipStream_t stream, int var) {
   hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, stream, A.counter);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_batch_solve, dim3(nslots), dim3(NT), lds, stream, A);
+  if (var == 0)
+    hipLaunchKernelGGL(k_batch_solve<0>, dim3(nslots), dim3(NT), lds, stream, A);
+  else
+    hipLaunchKernelGGL(k_batch_solve<1>, dim3(nslots), dim3(NT), lds, stream, A);
   return hipGetLastError();
 }
 

@@ -459,39 +459,58 @@ __device__ __forceinline__ void rows_inv(const Task& tk, cpx<T>* gspec, OutOp<T>
   X(105, FxP105, FxP105r)   \
   X(135, FxP135, FxP135r)
 
-template <typename T, bool LDS>
+// Radix schedules: VAR 0 transforms rows with the forward schedule of each length and the
+// columns forward / reversed; VAR 1 swaps the roles (rows reversed, columns reversed /
+// forward). Both are valid for every length; which is faster depends on the mix of grid
+// sizes, so the host picks one by timing (fh_batch_autotune).
+constexpr int NVAR = 2;
+
+template <typename T, bool LDS, int VAR>
 __device__ __forceinline__ void matvec(const Task& tk, cpx<T>* gspec, const InOp<T>& in, OutOp<T>& out) {
   __syncthreads();  // operands written by other threads of the workgroup are complete
   switch (tk.n1) {
-#define X(LEN, FP, FPR) \
-  case LEN:             \
-    rows_fwd<T, FP, LDS>(tk, gspec, in); break;
+#define X(LEN, FP, FPR)                                        \
+  case LEN:                                                    \
+    if constexpr (VAR == 0)                                    \
+      rows_fwd<T, FP, LDS>(tk, gspec, in);                     \
+    else                                                       \
+      rows_fwd<T, FPR, LDS>(tk, gspec, in);                    \
+    break;
     FH_BATCH_LENGTHS(X)
 #undef X
   }
   switch (tk.n0) {
-#define X(LEN, FP, FPR) \
-  case LEN:             \
-    cols_G<T, FP, FPR, LDS, (sizeof(T) == 4)>(tk, gspec); break;
+#define X(LEN, FP, FPR)                                        \
+  case LEN:                                                    \
+    if constexpr (VAR == 0)                                    \
+      cols_G<T, FP, FPR, LDS, (sizeof(T) == 4)>(tk, gspec);    \
+    else                                                       \
+      cols_G<T, FPR, FP, LDS, (sizeof(T) == 4)>(tk, gspec);    \
+    break;
     FH_BATCH_LENGTHS(X)
 #undef X
   }
   switch (tk.n1) {
-#define X(LEN, FP, FPR) \
-  case LEN:             \
-    rows_inv<T, FP, LDS>(tk, gspec, out); break;
+#define X(LEN, FP, FPR)                                        \
+  case LEN:                                                    \
+    if constexpr (VAR == 0)                                    \
+      rows_inv<T, FP, LDS>(tk, gspec, out);                    \
+    else                                                       \
+      rows_inv<T, FPR, LDS>(tk, gspec, out);                   \
+    break;
     FH_BATCH_LENGTHS(X)
 #undef X
   }
 }
 
 __global__ void k_batch_reset(int* counter);
+template <int VAR>
 __global__ void __launch_bounds__(NT) k_batch_solve(Args A);
 
 // Host side (fh_batch.hip): allow `lds` bytes of dynamic LDS, and enqueue counter reset +
 // persistent solve on `stream` (graph-capturable: no memset nodes).
 hipError_t allow_lds(size_t lds);
-hipError_t enqueue(const Args& A, int nslots, size_t lds, hipStream_t stream);
+hipError_t enqueue(const Args& A, int nslots, size_t lds, hipStream_t stream, int var);
 
 }  // namespace batch
 }  // namespace fh

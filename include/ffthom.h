@@ -191,6 +191,15 @@ int fh_batch_set_coeff(fh_batch* batch, const double* a);
  * FH_CG_OK or FH_CG_BREAKDOWN. Stop rule as fh_cg_solve. */
 int fh_batch_solve(fh_batch* batch, int mode, double tol, int maxiter, double delta, double* upper,
                    double* lower, int* kit, int* status);
+/* Schedule of the batched solve kernel: variant 0 or 1 (radix order of the row and column
+ * transforms of every length: forward/reversed or reversed/forward) and the task order (0
+ * largest grids first, 1 sample order). fh_batch_autotune times every (variant, order) on
+ * the loaded batch with `iters` CG iterations per task (times_ms[order * 2 + variant], may be
+ * NULL) and keeps the fastest; fh_batch_solve runs it on first use unless a schedule was set.
+ * No reference counterpart (the reference has no batched or device solve). */
+int fh_batch_set_schedule(fh_batch* batch, int variant, int order_mode);
+int fh_batch_autotune(fh_batch* batch, int mode, double delta, int iters, double* times_ms, int* variant,
+                      int* order_mode);
 /* Of the last solve: fp64 residual evaluations and final sqrt(<r,r>) per (sample, primal/dual,
  * load) (4 per sample, host arrays), and the device time of the whole batch in ms. */
 int fh_batch_stats(const fh_batch* batch, int* n_updates, double* norm_res, double* solve_ms);

@@ -111,3 +111,25 @@ def test_uq_workload_512():
         up, lo = oracle_bounds(fields[s])
         assert rel(out['upper'][s], up) < 1e-8, (s, dims[s], rel(out['upper'][s], up))
         assert rel(out['lower'][s], lo) < 1e-8, (s, dims[s], rel(out['lower'][s], lo))
+
+
+def test_schedules_agree_and_autotune_picks_fastest():
+    """Both radix-schedule variants and both task orders give the same bounds; the
+    autotuner's choice is the fastest of its own timings."""
+    print('case batch schedules', flush=True)
+    shapes = [(63, 81), (135, 45), (105, 105), (27, 63), (81, 81), (45, 135)]
+    fields = grf_samples(shapes, seed=BASE_SEED + 21)
+    res = {}
+    for var in (0, 1):
+        for order in (0, 1):
+            res[(var, order)] = homogenize_batch(fields, precision='fp64', tol=1e-8, schedule=(var, order))
+    ref = res[(0, 0)]
+    for key, r in res.items():
+        assert np.abs(r['upper'] - ref['upper']).max() < 1e-12 * np.abs(ref['upper']).max(), key
+        assert np.abs(r['lower'] - ref['lower']).max() < 1e-12 * np.abs(ref['lower']).max(), key
+    out = homogenize_batch(fields, precision='mixed', tol=1e-8)
+    t = out['tune_ms']
+    var, order = out['schedule']
+    assert t.shape == (2, 2) and (t > 0).all()
+    assert t[order, var] == t.min()
+    assert rel(out['upper'], ref['upper']) < 1e-8
