@@ -98,6 +98,149 @@ def timed(fn, steps, warmup, sync):
     return (time.perf_counter() - t0) / steps, res
 
 
+def bench_uq(args, nat, ms, dev):
+    """512 2-D samples, grids from {63,81,105,135}^2, a = exp(GRF(corr 0.1, log-std 1)); primal + dual
+    bounds of every sample in one persistent graph-launched kernel (fp32 CG + fp64 refinement, tol 1e-8)."""
+    from ffthompy_amd.batch import homogenize_batch
+    dims, fields = ms.uq_samples()
+    log('uq: %d samples, %d voxels' % (len(fields), sum(a.size for a in fields)))
+    b = nat.Batch(dims)
+    b.set_coeff(fields)
+    tune, var, order = b.autotune(mode='mixed', delta=0.01)
+    log('uq: autotune ms %s -> variant %d, order %d' % (np.round(tune, 2).tolist(), var, order))
+    for _ in range(max(args.warmup, 1)):
+        b.solve(tol=1e-8, mode='mixed', delta=0.01)
+    ts = []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        up, lo, kit, st = b.solve(tol=1e-8, mode='mixed', delta=0.01)
+        ts.append(time.perf_counter() - t0)
+    _, _, dev_ms = b.stats()
+    t64 = time.perf_counter()
+    up64, lo64, kit64, _ = b.solve(tol=1e-8, mode='fp64')
+    t64 = time.perf_counter() - t64
+    b.solve(tol=1e-8, mode='mixed', delta=0.01)
+    gap = min(float(np.linalg.eigvalsh(u - l).min()) for u, l in zip(up, lo))
+    it_vox = float(sum(int(k.sum()) * a.size for k, a in zip(kit, fields)))
+    cpu = None
+    if not args.no_cpu:
+        from oracle.baseline import time_cg_iterations
+        pick = [int(i) for i in np.random.default_rng(3).choice(len(fields), 4, replace=False)]
+        sec_vox = []
+        for i in pick:
+            s_it = time_cg_iterations(fields[i], np.ones(2), iters=args.cpu_iters * 5, callback=True)
+            sec_vox.append(s_it / fields[i].size)
+        per = float(np.mean(sec_vox))
+        cpu = {'value': per * it_vox, 'unit': 's', 'cores': 1, 'kind': 'port',
+               'sample': '%d CG iterations each on 4 samples %s of the oracle (reference-style NumPy, CallBack '
+                         'on), seconds per voxel-iteration extrapolated to the batch\'s %.3g voxel-iterations'
+                         % (args.cpu_iters * 5, [tuple(dims[i]) for i in pick], it_vox)}
+    t = float(np.median(ts))
+    out = {'metric': 'time to bounds of 512 2-D UQ samples (s)', 'value': round(t, 4), 'unit': 's',
+           'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup, 'higher_is_better': False,
+           'dtype': 'f32+f64', 'data': 'synthetic (GRF samples, seed %d)' % ms.BASE_SEED,
+           'config': {'workload': 'batched UQ: 512 2-D scalar samples, grids {63,81,105,135}^2, a = exp(GRF '
+                                  'corr 0.1, log-std 1.0), primal+dual, CG tol 1e-8, one persistent launch'},
+           'device_ms': round(dev_ms, 2), 'samples_per_s': round(len(fields) / t, 1),
+           'cg_iterations': int(kit.sum()), 'cg_iter_per_s': round(float(kit.sum()) / t, 0),
+           'schedule': {'variant': var, 'order': order, 'tune_ms': np.round(tune, 3).tolist()},
+           'fp64_time_s': round(t64, 4), 'bounds_rel_diff_mixed_vs_fp64': float(
+               max(np.abs(up - up64).max() / np.abs(up64).max(), np.abs(lo - lo64).max() / np.abs(lo64).max())),
+           'min_gap_upper_minus_lower': gap, 'cpu_baseline': cpu}
+    print(json.dumps(out), flush=True)
+
+
+def bench_elastic(args, nat, ms, dev):
+    """81^3 small-strain elasticity: seeded random ellipsoids (~25 %), matrix (E 1, nu 0.3) vs inclusion
+    (10, 0.2), 6 Mandel unit strains, primal + dual, fp32 CG + fp64 refinement vs plain fp64."""
+    N = (81, 81, 81)
+    ind = ms.ellipsoid_indicator(N)
+    lm0, mu0 = ms.lame_from_young(1.0, 0.3)
+    lm1, mu1 = ms.lame_from_young(10.0, 0.2)
+    lam = np.where(ind, lm1, lm0)
+    mu = np.where(ind, mu1, mu0)
+    log('elastic81: inclusion fraction %.4f' % ind.mean())
+    m = np.array([1., 1., 1., 0., 0., 0.])
+
+    def run(mode):
+        res = {}
+        for pd in ('primal', 'dual'):
+            pl = nat.Plan(N, (1., 1., 1.), 'elasticity', pd)
+            if pd == 'primal':
+                pl.set_coeff('lame', np.stack([lam, mu]))
+            else:
+                # compliance of an isotropic C (Mandel): lambda' = -lambda/(2mu(3lambda+2mu)), mu' = 1/(4mu)
+                pl.set_coeff('lame', np.stack([-lam / (2 * mu * (3 * lam + 2 * mu)), 1. / (4 * mu)]))
+            sols, kits = [], []
+            for iL in range(6):
+                E = np.zeros(6)
+                E[iL] = 1.
+                EN = nat.DeviceArray.zeros((6,) + N)
+                nat.add_constant(EN, -E)
+                B = nat.DeviceArray((6,) + N)
+                pl.matvec(EN, B)
+                x = nat.DeviceArray.zeros((6,) + N)
+                info = pl.cg_solve(B, x, tol=1e-8, maxiter=5000, mode=mode, delta=args.delta)
+                kits.append(info['kit'])
+                nat.add_constant(x, E)
+                sols.append(x)
+            AH = pl.energy_matrix(sols)
+            res[pd] = (AH if pd == 'primal' else np.linalg.inv(AH), kits)
+        return res
+
+    run('mixed')
+    t = time.perf_counter()
+    rm = run('mixed')
+    tm = time.perf_counter() - t
+    t = time.perf_counter()
+    r64 = run('fp64')
+    t64 = time.perf_counter() - t
+    up, lo = rm['primal'][0], rm['dual'][0]
+    out = {'metric': 'time to primal+dual bounds, 3-D elasticity 81^3 (s)', 'value': round(tm, 4), 'unit': 's',
+           'n_gpus': 1, 'higher_is_better': False, 'dtype': 'f32+f64',
+           'data': 'synthetic (seeded ellipsoids, seed %d)' % ms.BASE_SEED,
+           'config': {'workload': '3-D elasticity 81^3, ellipsoids %.3f, (E,nu) (1,0.3) vs (10,0.2), 6 Mandel '
+                                  'loads, primal+dual, CG tol 1e-8' % ind.mean()},
+           'fp64_time_s': round(t64, 4), 'speedup_mixed_vs_fp64': round(t64 / tm, 3),
+           'kit_mixed': {k: v[1] for k, v in rm.items()}, 'kit_fp64': {k: v[1] for k, v in r64.items()},
+           'AH_upper_mandel_diag': np.round(np.diag(up), 6).tolist(),
+           'AH_lower_mandel_diag': np.round(np.diag(lo), 6).tolist(),
+           'bounds_rel_diff_mixed_vs_fp64': float(max(
+               np.abs(up - r64['primal'][0]).max() / np.abs(r64['primal'][0]).max(),
+               np.abs(lo - r64['dual'][0]).max() / np.abs(r64['dual'][0]).max())),
+           'min_gap_upper_minus_lower': float(np.linalg.eigvalsh(0.5 * ((up - lo) + (up - lo).T)).min())}
+    print(json.dumps(out), flush=True)
+
+
+def bench_grad(args, nat, ms, dev):
+    """99^3 differentiable upper bound: theta = GRF(corr 0.15, std 0.8), loss ||AH - 2I||_F^2; forward
+    (3 fp64 solves + energies) and the hand-written backward (one pass over the stored fields)."""
+    import torch
+    from ffthompy_amd.autograd import upper_bound
+    N = (99, 99, 99)
+    th = ms.gaussian_random_field(N, 0.15, 0.8, ms.BASE_SEED + 5)
+    tf, tb = [], []
+    for it in range(args.steps + 1):
+        t = torch.tensor(th, dtype=torch.float64, requires_grad=True)
+        t0 = time.perf_counter()
+        AH = upper_bound(t, tol=1e-10)
+        L = torch.sum((AH - 2. * torch.eye(3, dtype=torch.float64)) ** 2)
+        t1 = time.perf_counter()
+        L.backward()
+        t2 = time.perf_counter()
+        if it:
+            tf.append(t1 - t0)
+            tb.append(t2 - t1)
+    out = {'metric': 'forward + backward time, differentiable 99^3 upper bound (s)',
+           'value': round(float(np.median(tf) + np.median(tb)), 4), 'unit': 's', 'n_gpus': 1,
+           'higher_is_better': False, 'dtype': 'f64', 'data': 'synthetic (GRF, seed %d)' % (ms.BASE_SEED + 5),
+           'config': {'workload': 'a = exp(theta) on 99^3, theta GRF corr 0.15 std 0.8, loss ||AH-2I||_F^2, '
+                                  'fp64 CG tol 1e-10'},
+           'forward_s': round(float(np.median(tf)), 4), 'backward_s': round(float(np.median(tb)), 5),
+           'loss': float(L.item()), 'grad_norm': float(t.grad.norm().item())}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument('--gpus', type=int, default=1)
@@ -112,6 +255,8 @@ def main():
     ap.add_argument('--profile-only', action='store_true', help='matvec profile only (for rocprofv3 runs)')
     ap.add_argument('--profile-solve', action='store_true',
                     help='with --profile-only: also one mixed and one fp64 CG solve (load e1, primal)')
+    ap.add_argument('--workload', default='scalar243', choices=['scalar243', 'uq512', 'elastic81', 'grad99'],
+                    help='scalar243: the headline (default); the others print their own JSON line')
     args = ap.parse_args()
     world = int(os.environ.get('WORLD_SIZE', '1'))
     if args.gpus != 1 or world != 1:
@@ -122,6 +267,9 @@ def main():
     from ffthompy_amd import _native as nat
     from ffthompy_amd import microstructure as ms
     dev = nat.init()
+    if args.workload != 'scalar243':
+        {'uq512': bench_uq, 'elastic81': bench_elastic, 'grad99': bench_grad}[args.workload](args, nat, ms, dev)
+        return
     N = (args.n,) * 3
     n = int(np.prod(N))
     nh = args.n * args.n * (args.n // 2 + 1)
@@ -172,7 +320,7 @@ def main():
     names = ['rows_fwd(A, R2C axis 2)', 'cols axis 1 fwd', 'cols axis 0 fwd+G+inv', 'cols axis 1 inv',
              'rows_inv(C2R axis 2)']
     ach = k32['kernel_bytes'][dom] / (k32['kernel_ms'][dom] * 1e-3) / 1e9
-    traffic = measured_traffic(['k_rows_fwd_fx', 'k_cols_fx', 'k_cols_g_fx', 'k_cols_fx', 'k_rows_inv_fx'][dom] + '_fp32')
+    traffic = measured_traffic(['rows_fwd', 'cols_fwd', 'cols_g', 'cols_inv', 'rows_inv'][dom] + '_fp32')
 
     # --- timed steps ---
     sync = nat.synchronize

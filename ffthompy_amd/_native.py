@@ -48,6 +48,8 @@ SIGNATURES = {
                     _c_int_p, _c_double_p, _c_double_p, ctypes.c_int, _c_int_p],
     'fh_energy_matrix': [_vp, ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, _c_double_p],
     'fh_energy_gradient': [_vp, ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, _c_double_p, _vp, _vp],
+    'fh_field_gradient': [_vp, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, _vp,
+                          _vp],
     'fh_axpby': [ctypes.c_int, ctypes.c_size_t, ctypes.c_double, _vp, ctypes.c_double, _vp],
     'fh_dot': [ctypes.c_int, ctypes.c_size_t, _vp, _vp, _c_double_p],
     'fh_add_constant': [ctypes.c_int, ctypes.c_size_t, ctypes.c_int, _c_double_p, _vp],
@@ -335,6 +337,17 @@ class Plan:
             out = DeviceArray((nvals,) + self.N if nvals > 1 else self.N, np.float64)
         call('fh_energy_gradient', self._h, arr, ctypes.c_int(nf), W.ctypes.data_as(_c_double_p),
              chain.ptr if chain is not None else None, out.ptr)
+        return out
+
+    def field_gradient(self, fields, adjoints, chain=None, out=None):
+        """Device gradient -chain * sum_i adjoints_i . fields_i (see fh_field_gradient)."""
+        nf = len(fields)
+        fa = (ctypes.c_void_p * nf)(*[f.ptr.value for f in fields])
+        aa = (ctypes.c_void_p * nf)(*[f.ptr.value for f in adjoints])
+        if out is None:
+            out = DeviceArray(self.N, np.float64)
+        call('fh_field_gradient', self._h, fa, aa, ctypes.c_int(nf), chain.ptr if chain is not None else None,
+             out.ptr)
         return out
 
     def last_timing(self):

@@ -507,22 +507,22 @@ static int ensure_hist(fh_plan* pl, int len) {
   return FH_OK;
 }
 
-// Fused single-precision CG iteration (fixed-length kernels, isotropic coefficient):
-//   rows_fwd : xc += alpha_prev p_old (pending step), p = r + beta p, y = a p, partial <p, a p>
-//   alpha    : alpha = <r,r> / <p, a p>   (for p in the range of G, <p, a p> = <p, G(a p)>)
+// Fused single-precision CG iteration (fixed-length kernels, isotropic or Lame coefficient):
+//   rows_fwd : xc += alpha_prev p_old (pending step), p = r + beta p, y = A p, partial <p, A p>
+//   alpha    : alpha = <r,r> / <p, A p>   (for p in the range of G, <p, A p> = <p, G(A p)>)
 //   cols     : F, G, F^-1 along the strided axes
-//   rows_inv : r -= alpha G(a p), partial <r, r>
+//   rows_inv : r -= alpha G(A p), partial <r, r>
 //   beta     : beta, stop / reliable-update test
 // Returns 0 when the fused kernels do not cover this plan (nothing was enqueued).
 static int enqueue_fused_iteration(fh_plan* pl, float* xc, float* r, float* p, const float* coef, bool* ok) {
   *ok = false;
-  if (pl->generic || pl->coef_kind != fh::COEF_ISO) return FH_OK;
+  if (pl->generic || (pl->coef_kind != fh::COEF_ISO && pl->coef_kind != fh::COEF_LAME)) return FH_OK;
   const int nc = pl->ncomp;
   const int* halt = &pl->st->halt;
   cpx<float>* spec = reinterpret_cast<cpx<float>*>(pl->spec);
   const fh::LaunchCtx ctx = make_ctx<float>(pl);
   int nb = 0;
-  int rr = fh::fixed_rows_fwd<float>(ctx, nc, p, coef, fh::COEF_ISO, spec, halt, r, pl->st, xc, pl->partial,
+  int rr = fh::fixed_rows_fwd<float>(ctx, nc, p, coef, pl->coef_kind, spec, halt, r, pl->st, xc, pl->partial,
                                      pl->partial_cap, &nb);
   if (rr == 0) return FH_OK;
   RETCHK(fixed_result(rr, "rows forward (fused)"));
@@ -1070,6 +1070,43 @@ int fh_energy_gradient(fh_plan* pl, const void* const* fields, int nf, const dou
   }
   (void)hipFree(dptrs);
   (void)hipFree(dW);
+  return rc;
+}
+
+int fh_field_gradient(fh_plan* pl, const void* const* fields, const void* const* adjoints, int nf,
+                      const void* chain, void* grad) {
+  RETCHK(check_plan(pl));
+  if (!fields || !adjoints || !grad || nf < 1 || nf > 6) return fail(FH_ERR_ARG, "bad field gradient arguments");
+  if (pl->coef_kind != FH_COEF_ISO) return fail(FH_ERR_STATE, "the field gradient needs an iso coefficient");
+  for (int i = 0; i < nf; ++i)
+    if (!fields[i] || !adjoints[i]) return fail(FH_ERR_ARG, "null field pointer");
+  void* dptrs = nullptr;
+  HIPCHK(hipMalloc(&dptrs, 2 * nf * sizeof(void*)));
+  std::vector<const void*> hp(2 * nf);
+  for (int i = 0; i < nf; ++i) {
+    hp[i] = fields[i];
+    hp[nf + i] = adjoints[i];
+  }
+  hipError_t e = hipMemcpy(dptrs, hp.data(), 2 * nf * sizeof(void*), hipMemcpyHostToDevice);
+  int rc = FH_OK;
+  if (e == hipSuccess) {
+    const double* const* F = (const double* const*)dptrs;
+    const double* const* Lm = F + nf;
+    const dim3 grid(grid_blocks(pl->n, 256, 4096)), block(256);
+    const double* ch = (const double*)chain;
+    double* g = (double*)grad;
+    switch (pl->ncomp) {
+      case 2: hipLaunchKernelGGL((fh::k_field_grad<2>), grid, block, 0, pl->stream, F, Lm, nf, pl->n, ch, g); break;
+      case 3: hipLaunchKernelGGL((fh::k_field_grad<3>), grid, block, 0, pl->stream, F, Lm, nf, pl->n, ch, g); break;
+      default: rc = fail(FH_ERR_ARG, "unsupported number of components");
+    }
+    if (rc == FH_OK) {
+      e = hipGetLastError();
+      if (e == hipSuccess) e = hipStreamSynchronize(pl->stream);
+    }
+  }
+  if (rc == FH_OK && e != hipSuccess) rc = fail(FH_ERR_HIP, std::string("fh_field_gradient: ") + hipGetErrorString(e));
+  (void)hipFree(dptrs);
   return rc;
 }
 

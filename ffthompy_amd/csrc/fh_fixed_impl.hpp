@@ -65,16 +65,17 @@ int rows_fwd_fx_launch(const LaunchCtx& c, int nc, const T* in, const T* coef, i
                          st, xc, partial);                                                            \
     }                                                                                                 \
   } while (0)
-  // the fused direction update is only race-free when each thread reads its own component
-  if (rvec && kind != COEF_ISO && kind != COEF_NONE) return 0;
   // staged kernels (16-byte HBM chunks through LDS) in single precision; in double precision
-  // their LDS footprint halves the occupancy and the direct kernel is faster
-  if ((kind == COEF_ISO || kind == COEF_NONE) && sizeof(T) == 4) {
+  // their LDS footprint halves the occupancy and the direct kernel is faster. The staged
+  // kernels finish every direction update before any coefficient product, so the fused
+  // update also serves the Lame coefficient there.
+  const bool lame_ok = kind == COEF_LAME && (nc == 3 || nc == 6);
+  if ((kind == COEF_ISO || kind == COEF_NONE || lame_ok) && sizeof(T) == 4) {
     const int upd = xc ? 2 : (rvec ? 1 : 0);
-    if (upd == 2 && (!rvec || kind != COEF_ISO || (size_t)nb > partial_cap)) return 0;
+    if (upd == 2 && (!rvec || kind == COEF_NONE || (size_t)nb > partial_cap)) return 0;
 #define FH_LAUNCH_ST(KIND, NCC, UPDV)                                                                         \
   do {                                                                                                        \
-    const size_t slds = RowStage<FP, T, NCC>::lds_bytes(rpb, KIND == COEF_ISO);                               \
+    const size_t slds = RowStage<FP, T, NCC>::lds_bytes(rpb, KIND == COEF_ISO ? 1 : (KIND == COEF_LAME ? 2 : 0)); \
     if (slds + FP::L * sizeof(cpx<T>) > 160 * 1024) return 0;                                                 \
     auto kf = k_rows_fwd_st<T, FP, KIND, NCC, UPDV>;                                                          \
     allow_lds(kf, slds);                                                                                      \
@@ -95,6 +96,19 @@ int rows_fwd_fx_launch(const LaunchCtx& c, int nc, const T* in, const T* coef, i
       if (upd == 0) FH_ST_NC(COEF_NONE, 0);
       else if (upd == 1) FH_ST_NC(COEF_NONE, 1);
       else return 0;
+    } else if (kind == COEF_LAME) {
+#define FH_ST_LAME(UPDV)                                \
+  do {                                                  \
+    if (nc == 3) FH_LAUNCH_ST(COEF_LAME, 3, UPDV);      \
+    else FH_LAUNCH_ST(COEF_LAME, 6, UPDV);              \
+  } while (0)
+      if (upd == 0) FH_ST_LAME(0);
+      else if (upd == 1) FH_ST_LAME(1);
+      else {
+        if constexpr (sizeof(T) == 4) FH_ST_LAME(2);
+        else return 0;
+      }
+#undef FH_ST_LAME
     } else {
       if (upd == 0) FH_ST_NC(COEF_ISO, 0);
       else if (upd == 1) FH_ST_NC(COEF_ISO, 1);
@@ -109,6 +123,9 @@ int rows_fwd_fx_launch(const LaunchCtx& c, int nc, const T* in, const T* coef, i
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? 1 : -(int)e;
   }
+  // the direct kernel's fused direction update is only race-free when each thread reads its
+  // own component
+  if (rvec && kind != COEF_ISO && kind != COEF_NONE) return 0;
   if (xc) {
     // fused iteration (k_rows_fwd_fx UPD 2), single precision only
     if constexpr (sizeof(T) == 4) {

@@ -792,4 +792,22 @@ k_energy_grad(const double* const* __restrict__ F, long long n, const double* __
   }
 }
 
+// Adjoint gradient of a loss on the solution fields (isotropic coefficient):
+// grad[x] = -chain[x] * sum_i lam_i(x) . e_i(x), with e_i the fields of the unit loads and
+// lam_i the adjoint fields (solutions of G(a lam_i) = G dL/de_i). All pointers fp64.
+template <int NC>
+__global__ void __launch_bounds__(256)
+k_field_grad(const double* const* __restrict__ F, const double* const* __restrict__ Lm, int nf, long long n,
+             const double* __restrict__ chain, double* __restrict__ grad) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+    double acc = 0.0;
+    for (int i = 0; i < nf; ++i) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c) acc += Lm[i][c * n + v] * F[i][c * n + v];
+    }
+    grad[v] = -acc * (chain ? chain[v] : 1.0);
+  }
+}
+
 }  // namespace fh

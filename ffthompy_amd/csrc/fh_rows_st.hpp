@@ -51,11 +51,11 @@ struct RowStage {
   // pencil count: lanes on consecutive pencils hit consecutive banks, lanes walking j of one
   // pencil hit distinct bank pairs
   __host__ __device__ static constexpr int pen_stride(int rpb) { return ((rpb / 2) * NC) | 1; }
-  // bytes: max(pencils, staging) + coefficient staging (forward pass only)
-  __host__ __device__ static constexpr size_t lds_bytes(int rpb, bool coef) {
+  // bytes: max(pencils, staging) + staging of ncoef coefficient runs (forward pass only)
+  __host__ __device__ static constexpr size_t lds_bytes(int rpb, int ncoef) {
     const size_t pen = (size_t)pen_stride(rpb) * FP::L * sizeof(cpx<T>);
     const size_t stg = (size_t)NC * comp_stride(rpb) * sizeof(T);
-    return (pen > stg ? pen : stg) + (coef ? (size_t)comp_stride(rpb) * sizeof(T) : 0);
+    return (pen > stg ? pen : stg) + (size_t)ncoef * comp_stride(rpb) * sizeof(T);
   }
 };
 
@@ -75,7 +75,10 @@ k_rows_fwd_st(const T* __restrict__ in, const T* __restrict__ coef, cpx<T>* __re
               long long nrows, long long n, int rpb, const cpx<T>* __restrict__ twg,
               const int* __restrict__ halt, const T* __restrict__ rvec, const CGState* __restrict__ st,
               T* __restrict__ xc, double* __restrict__ partial) {
-  static_assert(COEF == COEF_NONE || COEF == COEF_ISO, "staged pass: scalar or isotropic coefficient");
+  static_assert(COEF == COEF_NONE || COEF == COEF_ISO || (COEF == COEF_LAME && (NC == 3 || NC == 6)),
+                "staged pass: no coefficient, isotropic a, or Lame (lambda, mu) in Mandel form");
+  constexpr int KV = COEF == COEF_NONE ? 0 : (COEF == COEF_ISO ? 1 : 2);  // coefficient runs
+  constexpr int DN = NC == 6 ? 3 : 2;                                       // normal components
   if (halt && *halt) return;
   constexpr int L = FP::L, n2h = L / 2 + 1, TPP = FP::TPP, W = Chunk<T>::W;
   constexpr unsigned SR = sizeof(T);
@@ -84,7 +87,7 @@ k_rows_fwd_st(const T* __restrict__ in, const T* __restrict__ coef, cpx<T>* __re
   cpx<T>* s = reinterpret_cast<cpx<T>*>(smem_raw);
   T* sr = reinterpret_cast<T*>(smem_raw);
   const int cstr = RS::comp_stride(rpb);
-  const size_t main_bytes = RS::lds_bytes(rpb, false);
+  const size_t main_bytes = RS::lds_bytes(rpb, 0);
   T* sa = reinterpret_cast<T*>(smem_raw + main_bytes);
   __shared__ cpx<T> twl[L];
   const int tid = threadIdx.x, nthr = blockDim.x;
@@ -104,7 +107,7 @@ k_rows_fwd_st(const T* __restrict__ in, const T* __restrict__ coef, cpx<T>* __re
   const T alp = UPD == 2 ? T(st->alpha) : T(0);
   const __amdgpu_buffer_rsrc_t rr = make_rsrc(UPD ? rvec : in, (unsigned long long)NC * n * SR);
   const __amdgpu_buffer_rsrc_t rx = make_rsrc(UPD == 2 ? xc : in, (unsigned long long)NC * n * SR);
-  Chunk<T> P[NC][CPT], R[UPD ? NC : 1][CPT], X[UPD == 2 ? NC : 1][CPT], A[COEF == COEF_ISO ? CPT : 1];
+  Chunk<T> P[NC][CPT], R[UPD ? NC : 1][CPT], X[UPD == 2 ? NC : 1][CPT], A[KV ? KV : 1][CPT];
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     const unsigned s0 = (unsigned)(c * n) + rbase;
@@ -121,19 +124,29 @@ k_rows_fwd_st(const T* __restrict__ in, const T* __restrict__ coef, cpx<T>* __re
       }
     }
   }
-  if constexpr (COEF == COEF_ISO) {
-    const __amdgpu_buffer_rsrc_t rco = make_rsrc(coef, (unsigned long long)n * SR);
-    const unsigned q0 = rbase / W, q1 = (rbase + len + W - 1) / W;
+  if constexpr (KV > 0) {
+    // coefficient run k (a; or lambda, mu) of the block's rows -> LDS region k of sa
+    const __amdgpu_buffer_rsrc_t rco = make_rsrc(coef, (unsigned long long)KV * n * SR);
 #pragma unroll
-    for (int i = 0; i < CPT; ++i) {
-      const unsigned q = q0 + tid + i * NT;
-      if (q < q1) A[i] = bload_chunk<T>(rco, q * 16u);
+    for (int k = 0; k < KV; ++k) {
+      const unsigned s0 = (unsigned)(k * n) + rbase;
+      const unsigned q0 = s0 / W, q1 = (s0 + len + W - 1) / W;
+#pragma unroll
+      for (int i = 0; i < CPT; ++i) {
+        const unsigned q = q0 + tid + i * NT;
+        if (q < q1) A[k][i] = bload_chunk<T>(rco, q * 16u);
+      }
     }
-    const int ab = stage_base<T>(0, rbase);
 #pragma unroll
-    for (int i = 0; i < CPT; ++i) {
-      const unsigned q = q0 + tid + i * NT;
-      if (q < q1) *reinterpret_cast<Chunk<T>*>(sa + ab + (int)(q * W - rbase)) = A[i];
+    for (int k = 0; k < KV; ++k) {
+      const unsigned s0 = (unsigned)(k * n) + rbase;
+      const unsigned q0 = s0 / W, q1 = (s0 + len + W - 1) / W;
+      const int ab = stage_base<T>(k * cstr, s0);
+#pragma unroll
+      for (int i = 0; i < CPT; ++i) {
+        const unsigned q = q0 + tid + i * NT;
+        if (q < q1) *reinterpret_cast<Chunk<T>*>(sa + ab + (int)(q * W - s0)) = A[k][i];
+      }
     }
   }
   // CG vector updates, the operand v -> real-row staging
@@ -192,14 +205,36 @@ k_rows_fwd_st(const T* __restrict__ in, const T* __restrict__ coef, cpx<T>* __re
   const int pa = stage_base<T>(ca * cstr, (unsigned)(ca * n) + rbase) + rowa * L;
   const int pb = stage_base<T>(cb * cstr, (unsigned)(cb * n) + rbase) + rowb * L;
   const int aa = stage_base<T>(0, rbase) + rowa * L, ab2 = stage_base<T>(0, rbase) + rowb * L;
+  // Lame: mu run and the rows' normal components (for the trace)
+  const int ma = stage_base<T>(cstr, (unsigned)n + rbase) + rowa * L;
+  const int mb = stage_base<T>(cstr, (unsigned)n + rbase) + rowb * L;
+  int tra[DN], trb[DN];
+#pragma unroll
+  for (int d = 0; d < DN; ++d) {
+    tra[d] = stage_base<T>(d * cstr, (unsigned)(d * n) + rbase) + rowa * L;
+    trb[d] = stage_base<T>(d * cstr, (unsigned)(d * n) + rbase) + rowb * L;
+  }
   double acc = 0.0;
   {
     FxStage<T, FP, 0, -1> s0;
     // rows past nr (last block) read in-bounds staging garbage that the selects discard
     s0.load(t, [&](int x) {
       const T v0 = sr[pa + x], v1 = sr[pb + x];
-      const T y0 = (COEF == COEF_ISO) ? sa[aa + x] * v0 : v0;
-      const T y1 = (COEF == COEF_ISO) ? sa[ab2 + x] * v1 : v1;
+      T y0 = v0, y1 = v1;
+      if constexpr (COEF == COEF_ISO) {
+        y0 = sa[aa + x] * v0;
+        y1 = sa[ab2 + x] * v1;
+      } else if constexpr (COEF == COEF_LAME) {
+        // C:e = lambda tr(e) m + 2 mu e (Mandel): lambda run 0, mu run 1 of sa
+        T t0 = T(0), t1 = T(0);
+#pragma unroll
+        for (int d = 0; d < DN; ++d) {
+          t0 += sr[tra[d] + x];
+          t1 += sr[trb[d] + x];
+        }
+        y0 = T(2) * sa[ma + x] * v0 + (ca < DN ? sa[aa + x] * t0 : T(0));
+        y1 = T(2) * sa[mb + x] * v1 + (cb < DN ? sa[ab2 + x] * t1 : T(0));
+      }
       const T ya = va ? y0 : T(0), yb = vb ? y1 : T(0);
       if constexpr (UPD == 2) acc += (double)ya * (double)(va ? v0 : T(0)) + (double)yb * (double)(vb ? v1 : T(0));
       return mk(ya, yb);

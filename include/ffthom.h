@@ -148,6 +148,15 @@ int fh_energy_matrix(fh_plan* plan, const void* const* fields, int nf, double* A
 int fh_energy_gradient(fh_plan* plan, const void* const* fields, int nf, const double* W, const void* chain,
                        void* grad);
 
+/* Adjoint gradient of a loss on the solution fields, isotropic coefficient:
+ * grad[x] = -chain[x] * sum_i adjoints_i(x) . fields_i(x)   (chain nullable: 1),
+ * where fields_i = E_i + x_i are the unit-load solutions and adjoints_i solve
+ * G(a adj_i) = G dL/dfields_i (the same CG, fh_cg_solve on fh_project(dL/dfields_i)).
+ * nf fields and adjoints of ncomp components, fp64 device pointers; grad: n fp64.
+ * No reference counterpart (the reference has no derivatives). */
+int fh_field_gradient(fh_plan* plan, const void* const* fields, const void* const* adjoints, int nf,
+                      const void* chain, void* grad);
+
 /* Helpers on fp64/fp32 device vectors (Tensor arithmetic, ffthompy/tensors/objects.py:188-245):
  * y = a x + b y; out = sum(x*y) (caller divides by prod(N) for the grid inner product,
  * ffthompy/tensors/objects.py:618-636); y_c += E_c on an ncomp-component field

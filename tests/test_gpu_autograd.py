@@ -68,3 +68,40 @@ def test_gradient_matches_central_differences_99():
     print('FD relative errors: max %.2e median %.2e' % (max(rel), float(np.median(rel))), flush=True)
     assert all(err <= 1e-5 * abs(gv) + floor for _, gv, _, err in rows)
     assert float(np.median(rel)) < 1e-3
+
+
+def test_local_field_gradient_adjoint_15():
+    """Loss on the solution fields themselves (needs the adjoint CG): L = sum w (e_0)_0^2 +
+    sum u . e_2 on 15^3, gradient against the float64 torch reference (autograd through its
+    CG) and central differences at 5 voxels."""
+    from ffthompy_amd.autograd import solution_fields
+    from torch_reference import fields_torch
+    print('case autograd local fields 15^3', flush=True)
+    N = (15, 15, 15)
+    th = gaussian_random_field(N, 0.15, 0.8, BASE_SEED + 9)
+    rng = np.random.default_rng(9)
+    w = torch.tensor(rng.random(N))
+    u = torch.tensor(rng.standard_normal((3,) + N))
+
+    def loss(e):
+        return torch.sum(w * e[0][0] ** 2) + torch.sum(u * e[2])
+
+    t1 = torch.tensor(th, requires_grad=True)
+    L1 = loss(solution_fields(t1, tol=1e-12))
+    L1.backward()
+    t2 = torch.tensor(th, requires_grad=True)
+    L2 = loss(fields_torch(t2, tol=1e-13))
+    L2.backward()
+    assert abs(L1.item() - L2.item()) < 1e-10 * abs(L2.item())
+    g1, g2 = t1.grad.numpy(), t2.grad.numpy()
+    assert np.abs(g1 - g2).max() < 1e-8 * np.abs(g2).max(), np.abs(g1 - g2).max() / np.abs(g2).max()
+    h = 1e-6
+    for v in [(0, 0, 0), (3, 7, 11), (14, 2, 5), (8, 8, 8), (1, 13, 6)]:
+        vals = []
+        for s in (1, -1):
+            tt = th.copy()
+            tt[v] += s * h
+            with torch.no_grad():
+                vals.append(loss(solution_fields(torch.tensor(tt), tol=1e-12)).item())
+        fd = (vals[0] - vals[1]) / (2 * h)
+        assert abs(fd - g1[v]) <= 1e-5 * abs(g1[v]) + 1e-7 * np.abs(g1).max(), (v, fd, g1[v])

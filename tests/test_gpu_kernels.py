@@ -321,3 +321,37 @@ def test_fused_mixed_iteration_matches_operator_form(N):
         out[fused] = (info['kit'], float(np.mean(a * np.sum(xh * xh, axis=0))))
     assert abs(out[True][0] - out[False][0]) <= 2, out
     assert abs(out[True][1] - out[False][1]) / out[False][1] < 1e-10, out
+
+
+@pytest.mark.parametrize('N', [(27, 27, 27), (45, 45, 45), (63, 81)])
+def test_fused_mixed_iteration_lame(N):
+    """Fused single-precision iteration with a Lame coefficient (staged row pass computes
+    C:e = lambda tr(e) m + 2 mu e and <p, C:p>) against the operator-form iteration."""
+    print('case fused mixed lame', N, flush=True)
+    rng = np.random.default_rng(32)
+    ph = rng.random(N) < 0.3
+    lam = np.where(ph, 5.0, 0.5)
+    mu = np.where(ph, 4.0, 0.4)
+    plan = nat.Plan(N, np.ones(len(N)), 'elasticity', 'primal')
+    nc = plan.ncomp
+    plan.set_coeff('lame', np.stack([lam, mu]))
+    E = np.zeros(nc)
+    E[nc - 1] = 1.
+    EN = nat.DeviceArray.zeros((nc,) + N)
+    nat.add_constant(EN, -E)
+    B = nat.DeviceArray((nc,) + N)
+    plan.matvec(EN, B)
+    m = np.zeros(nc)
+    m[:3 if nc == 6 else 2] = 1.
+    out = {}
+    for fused in (True, False):
+        plan.set_cg_variant(fused)
+        x = nat.DeviceArray.zeros((nc,) + N)
+        info = plan.cg_solve(B, x, tol=1e-8, maxiter=2000, mode='mixed', delta=0.01)
+        assert info['status'] == nat.FH_CG_OK and info['norm_res'] <= 1e-8
+        nat.add_constant(x, E)
+        xh = x.to_host()
+        tr = np.einsum('i,i...->...', m, xh)
+        out[fused] = (info['kit'], float(np.mean(lam * tr * tr + 2 * mu * np.sum(xh * xh, axis=0))))
+    assert abs(out[True][0] - out[False][0]) <= 2, out
+    assert abs(out[True][1] - out[False][1]) / out[False][1] < 1e-10, out

@@ -45,3 +45,23 @@ def test_autograd_matches_finite_differences():
             vals.append(float(torch.sum((AH - 2. * torch.eye(3, dtype=torch.float64)) ** 2)))
         fd = (vals[0] - vals[1]) / (2 * h)
         assert abs(fd - g[v]) < 1e-6 * abs(g[v]), (v, fd, g[v])
+
+
+def test_fields_match_oracle_solutions():
+    """The reference fields (CG through torch) equal the oracle's GaNi primal solutions."""
+    th = np.random.default_rng(6).standard_normal((5, 5, 5)) * 0.5
+    from torch_reference import fields_torch
+    e = [f.numpy() for f in fields_torch(torch.tensor(th), tol=1e-13)]
+    a = np.exp(th)
+    d = 3
+    conf = {'physics': 'scalar',
+            'material': {'fun': lambda c: np.einsum('ij,...->ij...', np.eye(d), np.fft.fftshift(a)),
+                         'Y': np.ones(d)},
+            'solve': {'kind': 'GaNi', 'N': np.array(a.shape), 'primaldual': ['primal']},
+            'postprocess': [{'kind': 'GaNi'}],
+            'solver': {'kind': 'CG', 'tol': 1e-13, 'maxiter': 1e3}}
+    pb = Problem(conf)
+    pb.calculate()
+    sols = pb.output['sol_primal']
+    for i in range(d):
+        assert np.abs(e[i] - sols[i].val).max() < 1e-12

@@ -26,6 +26,20 @@ def g1_half(N, Y):
 
 
 def upper_bound_torch(theta, Y=None, tol=1e-13, maxiter=2000):
+    a = torch.exp(theta)
+    fields = fields_torch(theta, Y, tol, maxiter)
+    d = len(fields)
+    n = float(np.prod(theta.shape))
+    AH = torch.zeros((d, d), dtype=torch.float64)
+    for i in range(d):
+        for j in range(d):
+            AH[i, j] = torch.sum(a * torch.sum(fields[i] * fields[j], dim=0)) / n
+    return AH
+
+
+def fields_torch(theta, Y=None, tol=1e-13, maxiter=2000):
+    """Unit-load solution fields e_i = E_i + x_i (list of (d,)+N tensors), differentiable
+    through the CG iterations."""
     N = tuple(theta.shape)
     d = len(N)
     Y = np.ones(d) if Y is None else Y
@@ -40,7 +54,6 @@ def upper_bound_torch(theta, Y=None, tol=1e-13, maxiter=2000):
         return torch.fft.irfftn(Z, s=N, dim=dims)
 
     scal = lambda u, v: torch.sum(u * v) / n
-    AH = torch.zeros((d, d), dtype=torch.float64)
     fields = []
     for iL in range(d):
         E = torch.zeros((d,) + N, dtype=torch.float64)
@@ -61,7 +74,4 @@ def upper_bound_torch(theta, Y=None, tol=1e-13, maxiter=2000):
             rr = rrn
             k += 1
         fields.append(x + E)
-    for i in range(d):
-        for j in range(d):
-            AH[i, j] = torch.sum(a * torch.sum(fields[i] * fields[j], dim=0)) / n
-    return AH
+    return fields
