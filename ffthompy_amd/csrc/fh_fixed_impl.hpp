@@ -208,6 +208,7 @@ int rows_inv_fx_launch(const LaunchCtx& c, int nc, const cpx<T>* in, T* out, T s
     case 16 + 1: FH_LAUNCH_RI(2, 1); break;
     case 16 + 2: FH_LAUNCH_RI(2, 2); break;
     case 16 + 3: FH_LAUNCH_RI(2, 3); break;
+    case 16 + 6: FH_LAUNCH_RI(2, 6); break;
     default: return 0;
   }
 #undef FH_LAUNCH_RI
