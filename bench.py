@@ -374,6 +374,7 @@ def main():
         'AH_upper_fp64': float(r_fp64['AH_primal'][0, 0]), 'AH_lower_fp64': float(r_fp64['AH_dual'][0, 0]),
         'bounds_rel_diff_mixed_vs_fp64': [relp, reld],
         'matvec_GBps_fp32': round(prof['fp32']['matvec_GBps'], 1),
+        'matvec_frac_of_hbm_peak_fp32': round(prof['fp32']['matvec_GBps'] / HBM_PEAK_GBS, 4),
         'matvec_GBps_fp64': round(prof['fp64']['matvec_GBps'], 1),
         'matvec_ms_fp32': round(prof['fp32']['matvec_ms'], 4),
         'matvec_ms_fp64': round(prof['fp64']['matvec_ms'], 4),

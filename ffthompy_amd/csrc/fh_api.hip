@@ -154,6 +154,7 @@ struct fh_plan {
   int last_updates = 0;
   int generic = 0;               // 1: never use the fixed-length kernels
   bool fused_cg = true;          // mixed CG: fused iteration where available
+  hipEvent_t prof_t0 = nullptr, prof_t1 = nullptr;  // matvec profile: events of the next launch
 };
 
 struct fh_batch {
@@ -229,6 +230,8 @@ fh::LaunchCtx make_ctx(const fh_plan* pl) {
   c.n = pl->n;
   c.nh = pl->nh;
   for (int a = 0; a < 3; ++a) c.tw[a] = sizeof(T) == 8 ? pl->tw64[a] : pl->tw32[a];
+  c.t0 = pl->prof_t0;
+  c.t1 = pl->prof_t1;
   return c;
 }
 
@@ -1191,58 +1194,49 @@ int fh_matvec_profile(fh_plan* pl, int dtype, const void* x, void* y, int nrep, 
   if (!pl->coef_kind) return fail(FH_ERR_STATE, "fh_set_coeff has not been called");
   const int nk = pl->n0 > 1 ? 5 : 3;
   if (max_kernels < nk) return fail(FH_ERR_ARG, "max_kernels too small");
-  std::vector<hipEvent_t> ev(nk + 1);
-  for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+  // Each launch of the fixed-length kernels records its own start and stop (hipExtLaunchKernel
+  // events: the dispatch's timestamps, no extra packets between kernels). nrep matvecs run
+  // back to back; pass k's duration is averaged over them.
+  std::vector<hipEvent_t> evs(2 * nk);
+  for (auto& e : evs) HIPCHK(hipEventCreate(&e));
+  hipEvent_t ev[2] = {nullptr, nullptr};
   std::vector<double> acc(nk, 0.0);
   const int nc = pl->ncomp;
   const size_t sr = dtype == FH_F64 ? 8 : 4, sc = 2 * sr;
   const double cvals = pl->coef_kind == FH_COEF_ISO ? 1 : (pl->coef_kind == FH_COEF_LAME ? 2 : nc * nc);
   const double nb_real = (double)nc * pl->n * sr, nb_spec = (double)nc * pl->nh * sc;
   int rc = FH_OK;
-  for (int it = 0; it < nrep && rc == FH_OK; ++it) {
-    auto step = [&](int k, auto&& launch) {
-      if (rc != FH_OK) return;
-      if (hipEventRecord(ev[k], pl->stream) != hipSuccess) rc = fail(FH_ERR_HIP, "hipEventRecord");
-      if (rc == FH_OK) rc = launch();
-    };
-    if (dtype == FH_F64) {
-      using T = double;
-      cpx<T>* spec = reinterpret_cast<cpx<T>*>(pl->spec);
-      const GParams g = make_gparams(pl, 1.0 / (double)pl->n);
-      int k = 0;
-      step(k++, [&] { return launch_rows_fwd<T>(pl, nc, (const T*)x, pl->coef64, pl->coef_kind, spec, nullptr); });
-      if (pl->n0 > 1) {
-        step(k++, [&] { return launch_cols<T>(pl, 0, 1, nc, spec, g, nullptr); });
-        step(k++, [&] { return launch_cols<T>(pl, 2, 0, nc, spec, g, nullptr); });
-        step(k++, [&] { return launch_cols<T>(pl, 1, 1, nc, spec, g, nullptr); });
-      } else {
-        step(k++, [&] { return launch_cols<T>(pl, 2, 1, nc, spec, g, nullptr); });
-      }
-      step(k++, [&] { return launch_rows_inv<T>(pl, nc, spec, (T*)y, T(1), nullptr, nullptr, nullptr, nullptr); });
-    } else {
-      using T = float;
-      cpx<T>* spec = reinterpret_cast<cpx<T>*>(pl->spec);
-      const GParams g = make_gparams(pl, 1.0 / (double)pl->n);
-      int k = 0;
-      step(k++, [&] { return launch_rows_fwd<T>(pl, nc, (const T*)x, pl->coef32, pl->coef_kind, spec, nullptr); });
-      if (pl->n0 > 1) {
-        step(k++, [&] { return launch_cols<T>(pl, 0, 1, nc, spec, g, nullptr); });
-        step(k++, [&] { return launch_cols<T>(pl, 2, 0, nc, spec, g, nullptr); });
-        step(k++, [&] { return launch_cols<T>(pl, 1, 1, nc, spec, g, nullptr); });
-      } else {
-        step(k++, [&] { return launch_cols<T>(pl, 2, 1, nc, spec, g, nullptr); });
-      }
-      step(k++, [&] { return launch_rows_inv<T>(pl, nc, spec, (T*)y, T(1), nullptr, nullptr, nullptr, nullptr); });
+  auto pass = [&](auto tag, int k) -> int {
+    using T = decltype(tag);
+    cpx<T>* spec = reinterpret_cast<cpx<T>*>(pl->spec);
+    const T* coef = sizeof(T) == 8 ? (const T*)pl->coef64 : (const T*)pl->coef32;
+    const GParams g = make_gparams(pl, 1.0 / (double)pl->n);
+    if (k == 0) return launch_rows_fwd<T>(pl, nc, (const T*)x, coef, pl->coef_kind, spec, nullptr);
+    if (k == nk - 1) return launch_rows_inv<T>(pl, nc, spec, (T*)y, T(1), nullptr, nullptr, nullptr, nullptr);
+    if (pl->n0 > 1) {
+      if (k == 1) return launch_cols<T>(pl, 0, 1, nc, spec, g, nullptr);
+      if (k == 2) return launch_cols<T>(pl, 2, 0, nc, spec, g, nullptr);
+      return launch_cols<T>(pl, 1, 1, nc, spec, g, nullptr);
     }
-    if (rc == FH_OK && hipEventRecord(ev[nk], pl->stream) != hipSuccess) rc = fail(FH_ERR_HIP, "hipEventRecord");
-    if (rc == FH_OK && hipEventSynchronize(ev[nk]) != hipSuccess) rc = fail(FH_ERR_HIP, "hipEventSynchronize");
+    return launch_cols<T>(pl, 2, 1, nc, spec, g, nullptr);
+  };
+  for (int it = -1; it < nrep && rc == FH_OK; ++it) {  // it = -1: warm-up
     for (int k = 0; k < nk && rc == FH_OK; ++k) {
+      pl->prof_t0 = evs[2 * k];
+      pl->prof_t1 = evs[2 * k + 1];
+      rc = dtype == FH_F64 ? pass(double(0), k) : pass(float(0), k);
+      pl->prof_t0 = pl->prof_t1 = nullptr;
+    }
+    if (rc == FH_OK && hipStreamSynchronize(pl->stream) != hipSuccess) rc = fail(FH_ERR_HIP, "hipStreamSynchronize");
+    for (int k = 0; k < nk && rc == FH_OK && it >= 0; ++k) {
       float ms = 0.f;
-      if (hipEventElapsedTime(&ms, ev[k], ev[k + 1]) != hipSuccess) rc = fail(FH_ERR_HIP, "hipEventElapsedTime");
-      acc[k] += ms;
+      if (hipEventElapsedTime(&ms, evs[2 * k], evs[2 * k + 1]) != hipSuccess)
+        rc = fail(FH_ERR_UNSUPPORTED, "kernel timing needs the fixed-length kernels");
+      acc[k] += (double)ms;
     }
   }
-  for (auto& e : ev) (void)hipEventDestroy(e);
+  for (auto& e : evs) (void)hipEventDestroy(e);
+  (void)ev;
   if (rc != FH_OK) return rc;
   for (int k = 0; k < nk; ++k) {
     kernel_ms[k] = acc[k] / nrep;

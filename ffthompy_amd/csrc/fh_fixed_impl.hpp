@@ -56,12 +56,12 @@ int rows_fwd_fx_launch(const LaunchCtx& c, int nc, const T* in, const T* coef, i
     if (rvec) {                                                                                       \
       auto kf = k_rows_fwd_fx<T, FP, KIND, NCC, 1>;                                                   \
       allow_lds(kf, lds);                                                                             \
-      hipLaunchKernelGGL(kf, grid, block, lds, c.stream, in, coef, out, nrows, c.n, rpb, tw, halt, rvec, \
+      hipExtLaunchKernelGGL(kf, grid, block, lds, c.stream, c.t0, c.t1, 0, in, coef, out, nrows, c.n, rpb, tw, halt, rvec, \
                          st, xc, partial);                                                            \
     } else {                                                                                          \
       auto kf = k_rows_fwd_fx<T, FP, KIND, NCC, 0>;                                                   \
       allow_lds(kf, lds);                                                                             \
-      hipLaunchKernelGGL(kf, grid, block, lds, c.stream, in, coef, out, nrows, c.n, rpb, tw, halt, rvec, \
+      hipExtLaunchKernelGGL(kf, grid, block, lds, c.stream, c.t0, c.t1, 0, in, coef, out, nrows, c.n, rpb, tw, halt, rvec, \
                          st, xc, partial);                                                            \
     }                                                                                                 \
   } while (0)
@@ -79,7 +79,7 @@ int rows_fwd_fx_launch(const LaunchCtx& c, int nc, const T* in, const T* coef, i
     if (slds + FP::L * sizeof(cpx<T>) > 160 * 1024) return 0;                                                 \
     auto kf = k_rows_fwd_st<T, FP, KIND, NCC, UPDV>;                                                          \
     allow_lds(kf, slds);                                                                                      \
-    hipLaunchKernelGGL(kf, grid, block, slds, c.stream, in, coef, out, nrows, c.n, rpb, tw, halt, rvec, st, xc, \
+    hipExtLaunchKernelGGL(kf, grid, block, slds, c.stream, c.t0, c.t1, 0, in, coef, out, nrows, c.n, rpb, tw, halt, rvec, st, xc, \
                        partial);                                                                              \
   } while (0)
 #define FH_ST_NC(KIND, UPDV)                 \
@@ -134,7 +134,7 @@ int rows_fwd_fx_launch(const LaunchCtx& c, int nc, const T* in, const T* coef, i
   do {                                                                                                    \
     auto kf = k_rows_fwd_fx<T, FP, COEF_ISO, NCC, 2>;                                                     \
     allow_lds(kf, lds);                                                                                   \
-    hipLaunchKernelGGL(kf, grid, block, lds, c.stream, in, coef, out, nrows, c.n, rpb, tw, halt, rvec, st,   \
+    hipExtLaunchKernelGGL(kf, grid, block, lds, c.stream, c.t0, c.t1, 0, in, coef, out, nrows, c.n, rpb, tw, halt, rvec, st,   \
                        xc, partial);                                                                      \
   } while (0)
       switch (nc) {
@@ -192,7 +192,7 @@ int rows_inv_fx_launch(const LaunchCtx& c, int nc, const cpx<T>* in, T* out, T s
     if (slds + FP::L * sizeof(cpx<T>) > 160 * 1024) return 0;                                                 \
     auto kf = k_rows_inv_st<T, FP, EPI, NCC>;                                                                 \
     allow_lds(kf, slds);                                                                                      \
-    hipLaunchKernelGGL(kf, grid, block, slds, c.stream, in, out, nrows, c.n, rpb, tw, scale, w, partial, halt, \
+    hipExtLaunchKernelGGL(kf, grid, block, slds, c.stream, c.t0, c.t1, 0, in, out, nrows, c.n, rpb, tw, scale, w, partial, halt, \
                        st, order_rev(1));                                                                 \
   } while (0)
   const int key = epi * 8 + nc;
@@ -243,10 +243,10 @@ int cols_fx_launch(const LaunchCtx& c, int dir, int axis, int nc, cpx<T>* data, 
   const dim3 grid((unsigned)nb), block(FP::TPP * P);
   const cpx<T>* tw = reinterpret_cast<const cpx<T>*>(c.tw[axis]);
   if (dir < 0)
-    hipLaunchKernelGGL((k_cols_fx<T, FP, P, -1>), grid, block, 0, c.stream, data, ostride, inner, (int)ntiles,
+    hipExtLaunchKernelGGL((k_cols_fx<T, FP, P, -1>), grid, block, 0, c.stream, c.t0, c.t1, 0, data, ostride, inner, (int)ntiles,
                        total, tw, halt, order_rev(0));
   else
-    hipLaunchKernelGGL((k_cols_fx<T, FP, P, +1>), grid, block, 0, c.stream, data, ostride, inner, (int)ntiles,
+    hipExtLaunchKernelGGL((k_cols_fx<T, FP, P, +1>), grid, block, 0, c.stream, c.t0, c.t1, 0, data, ostride, inner, (int)ntiles,
                        total, tw, halt, order_rev(2));
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? 1 : -(int)e;
@@ -270,7 +270,7 @@ int cols_g_fx_launch_nc(const LaunchCtx& c, int axis, cpx<T>* data, const GParam
   if (ntiles > 0x7fffffffLL) return 0;
   const dim3 grid((unsigned)ntiles), block(FP::TPP * F * NC);
   const cpx<T>* tw = reinterpret_cast<const cpx<T>*>(c.tw[axis]);
-  hipLaunchKernelGGL((k_cols_g_sw<T, FP, FPR, F, NC>), grid, block, 0, c.stream, data, inner, (int)ntiles, total,
+  hipExtLaunchKernelGGL((k_cols_g_sw<T, FP, FPR, F, NC>), grid, block, 0, c.stream, c.t0, c.t1, 0, data, inner, (int)ntiles, total,
                      tw, g, c.n2h, halt);
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? 1 : -(int)e;

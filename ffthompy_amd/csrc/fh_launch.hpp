@@ -2,6 +2,7 @@
 // (fh_fixed_f32.hip, fh_fixed_f64.hip) and the host plan code (fh_api.hip).
 #pragma once
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include "fh_kernels.hpp"
 
@@ -13,6 +14,7 @@ struct LaunchCtx {
   int n0, n1, n2, n2h;
   long long n, nh;
   const void* tw[3];  // per-axis twiddle table W_L^m, m < L, in the launch precision
+  hipEvent_t t0 = nullptr, t1 = nullptr;  // if set: start/stop time of the launched kernel
 };
 
 // Return 1 if a specialised kernel was launched, 0 if the length has none (caller falls

@@ -172,11 +172,12 @@ int fh_add_constant(int dtype, size_t nvox, int ncomp, const double* E, void* y)
 int fh_last_timing(const fh_plan* plan, double* matvec_ms, double* solve_ms);
 int fh_last_solve_updates(const fh_plan* plan, int* n_updates);
 
-/* Timing harness for the matvec: runs fh_matvec nrep times and returns, per kernel
- * of the pass sequence (rows forward, strided passes, rows inverse), the average
- * duration in ms measured with HIP events on the plan stream, and the algorithmic
- * HBM bytes each kernel moves per launch. At most max_kernels entries; *nkernels
- * receives the count. No reference counterpart (measurement only). */
+/* Timing harness for the matvec passes (rows forward, strided passes, rows inverse): nrep
+ * matvecs on the plan stream, each kernel launched with its own start/stop HIP events
+ * (hipExtLaunchKernel: the dispatch's timestamps); returns each pass's average duration in
+ * ms and the algorithmic HBM bytes it moves per launch. Needs the fixed-length kernels
+ * (FH_ERR_UNSUPPORTED otherwise). y holds the matvec result afterwards. At most max_kernels
+ * entries; *nkernels receives the count. No reference counterpart (measurement only). */
 int fh_matvec_profile(fh_plan* plan, int dtype, const void* x, void* y, int nrep, int max_kernels,
                       double* kernel_ms, double* kernel_bytes, int* nkernels);
 
