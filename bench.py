@@ -134,7 +134,7 @@ def bench_uq(args, nat, ms, dev):
         cpu = {'value': per * it_vox, 'unit': 's', 'cores': 1, 'kind': 'port',
                'sample': '%d CG iterations each on 4 samples %s of the oracle (reference-style NumPy, CallBack '
                          'on), seconds per voxel-iteration extrapolated to the batch\'s %.3g voxel-iterations'
-                         % (args.cpu_iters * 5, [tuple(dims[i]) for i in pick], it_vox)}
+                         % (args.cpu_iters * 5, [tuple(int(v) for v in dims[i]) for i in pick], it_vox)}
     t = float(np.median(ts))
     out = {'metric': 'time to bounds of 512 2-D UQ samples (s)', 'value': round(t, 4), 'unit': 's',
            'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup, 'higher_is_better': False,
@@ -154,12 +154,13 @@ def bench_elastic(args, nat, ms, dev):
     """81^3 small-strain elasticity: seeded random ellipsoids (~25 %), matrix (E 1, nu 0.3) vs inclusion
     (10, 0.2), 6 Mandel unit strains, primal + dual, fp32 CG + fp64 refinement vs plain fp64."""
     N = (81, 81, 81)
-    ind = ms.ellipsoid_indicator(N)
+    ind, n_ell = ms.ellipsoid_indicator(N)
+    ind = ind.astype(bool)
     lm0, mu0 = ms.lame_from_young(1.0, 0.3)
     lm1, mu1 = ms.lame_from_young(10.0, 0.2)
     lam = np.where(ind, lm1, lm0)
     mu = np.where(ind, mu1, mu0)
-    log('elastic81: inclusion fraction %.4f' % ind.mean())
+    log('elastic81: %d ellipsoids, inclusion fraction %.4f' % (n_ell, ind.mean()))
     m = np.array([1., 1., 1., 0., 0., 0.])
 
     def run(mode):
