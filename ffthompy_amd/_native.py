@@ -65,6 +65,14 @@ SIGNATURES = {
     'fh_batch_stats': [_vp, _c_int_p, _c_double_p, _c_double_p],
     'fh_batch_set_schedule': [_vp, ctypes.c_int, ctypes.c_int],
     'fh_batch_autotune': [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, _c_double_p, _c_int_p, _c_int_p],
+    'fh_ga_create': [_vp, ctypes.POINTER(ctypes.c_void_p)],
+    'fh_ga_destroy': [_vp],
+    'fh_ga_grid': [_vp, _c_int_p],
+    'fh_ga_set_inclusions': [_vp, ctypes.c_int, _c_int_p, _c_double_p, _c_double_p, ctypes.c_int, _c_double_p],
+    'fh_ga_set_grid_coeff': [_vp, ctypes.c_int, _c_int_p, ctypes.c_int, _c_double_p],
+    'fh_ga_coefficient': [_vp, _c_double_p],
+    'fh_ga_interpolate': [_vp, _vp, _vp],
+    'fh_ga_energy': [_vp, ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, _c_double_p],
     'fh_synchronize': [],
 }
 
@@ -444,6 +452,88 @@ class Batch:
         if h is not None and h.value and _LIB is not None:
             try:
                 _LIB.fh_batch_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+
+GA_INCLUSION_CODES = {'cube': 1, 'square': 1, 'ball': 2, 'circle': 2, 'bilinear_pyramid': 3, 'pyramid': 3,
+                      'all': 4, 'otherwise': 5}
+_COEF_CODES = {'iso': FH_COEF_ISO, 'lame': FH_COEF_LAME, 'full': FH_COEF_FULL}
+
+
+class Ga:
+    """Owner of an fh_ga: exactly integrated coefficient and interpolated solutions on the
+    quadrature grid of a plan's solutions (Ga post-processing)."""
+
+    def __init__(self, plan):
+        init()
+        self.plan = plan
+        h = ctypes.c_void_p()
+        call('fh_ga_create', plan.handle, ctypes.byref(h))
+        self._h = h
+        M = (ctypes.c_int * plan.dim)()
+        call('fh_ga_grid', h, M)
+        self.M = tuple(int(m) for m in M)
+        self.nvox = int(np.prod(self.M))
+        self.coef_kind = None
+        self.ncoef = 0
+
+    def _ncoef(self, kind):
+        return {'iso': 1, 'lame': 2, 'full': self.plan.ncomp ** 2}[kind]
+
+    def set_inclusions(self, kinds, params, positions, kind, vals):
+        """kinds: inclusion names; params, positions: (nincl, dim) (a ball's diameter in
+        params[:, 0]); vals: (nincl, ncoef) phase coefficients in layout `kind`."""
+        n = len(kinds)
+        codes = np.ascontiguousarray([GA_INCLUSION_CODES[k] for k in kinds], dtype=np.int32)
+        P = np.zeros((n, 3))
+        X = np.zeros((n, 3))
+        d = self.plan.dim
+        P[:, :d] = np.asarray(params, dtype=np.float64).reshape(n, d)
+        X[:, :d] = np.asarray(positions, dtype=np.float64).reshape(n, d)
+        V = np.ascontiguousarray(np.asarray(vals, dtype=np.float64).reshape(n, self._ncoef(kind)))
+        P = np.ascontiguousarray(P)
+        X = np.ascontiguousarray(X)
+        call('fh_ga_set_inclusions', self._h, ctypes.c_int(n), codes.ctypes.data_as(_c_int_p),
+             P.ctypes.data_as(_c_double_p), X.ctypes.data_as(_c_double_p), ctypes.c_int(_COEF_CODES[kind]),
+             V.ctypes.data_as(_c_double_p))
+        self.coef_kind, self.ncoef = kind, self._ncoef(kind)
+
+    def set_grid_coeff(self, order, P, kind, vals):
+        """vals: (ncoef,) + P nodal values in origin-0 order (dual: pointwise inverse)."""
+        P = np.ascontiguousarray(np.asarray(P, dtype=np.int32).reshape(-1))
+        V = np.ascontiguousarray(np.asarray(vals, dtype=np.float64))
+        if V.size != self._ncoef(kind) * int(np.prod(P)):
+            raise FhError('expected %d coefficient values, got %d' % (self._ncoef(kind) * int(np.prod(P)), V.size),
+                          FH_ERR_ARG)
+        call('fh_ga_set_grid_coeff', self._h, ctypes.c_int(int(order)), P.ctypes.data_as(_c_int_p),
+             ctypes.c_int(_COEF_CODES[kind]), V.ctypes.data_as(_c_double_p))
+        self.coef_kind, self.ncoef = kind, self._ncoef(kind)
+
+    def coefficient(self):
+        out = np.zeros((self.ncoef,) + self.M)
+        call('fh_ga_coefficient', self._h, out.ctypes.data_as(_c_double_p))
+        return out
+
+    def interpolate(self, x, out=None):
+        if out is None:
+            out = DeviceArray((self.plan.ncomp,) + self.M)
+        call('fh_ga_interpolate', self._h, x.ptr, out.ptr)
+        return out
+
+    def energy_matrix(self, fields):
+        nf = len(fields)
+        arr = (ctypes.c_void_p * nf)(*[f.ptr for f in fields])
+        AH = np.zeros((nf, nf))
+        call('fh_ga_energy', self._h, arr, ctypes.c_int(nf), AH.ctypes.data_as(_c_double_p))
+        return AH
+
+    def __del__(self):
+        h = getattr(self, '_h', None)
+        if h is not None and h.value and _LIB is not None:
+            try:
+                _LIB.fh_ga_destroy(h)
             except Exception:
                 pass
             self._h = None

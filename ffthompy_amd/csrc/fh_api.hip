@@ -14,6 +14,7 @@
 #include "fh_kernels.hpp"
 #include "fh_launch.hpp"
 #include "fh_batch.hpp"
+#include "fh_ga.hpp"
 
 using fh::AxisPlan;
 using fh::CGState;
@@ -188,6 +189,19 @@ struct fh_batch {
   int order_mode = 0;              // 0 largest grids first, 1 sample order
   std::vector<int> order_size, order_sample;
   double tune_ms[2 * fh::batch::NVAR] = {0};
+};
+
+// Ga post-processing: exactly integrated coefficient and interpolated solutions on a
+// quadrature grid (see fh_ga.hpp for why any grid of at least 2N-1 points per axis is exact
+// for odd N).
+struct fh_ga {
+  fh_plan* coarse = nullptr;  // grid of the solutions (not owned)
+  fh_plan* fine = nullptr;    // quadrature grid (owned): FFT passes, partial sums
+  fh::ga::SpecGrid g{};
+  int coef_kind = 0, ncoef = 0;
+  double* coef = nullptr;     // ncoef * fine->n, origin-0 voxel order
+  double* fields = nullptr;   // interpolated solutions
+  size_t fields_bytes = 0;
 };
 
 namespace {
@@ -424,6 +438,56 @@ int check_dtype(int dtype) {
   return FH_OK;
 }
 
+// AH[i][j] = <A e_i, e_j> on the grid of pl with coefficient `coef` of layout `kind`.
+int energy_impl(fh_plan* pl, const double* coef, int kind, const void* const* fields, int nf, double* AH) {
+  const int nn = nf * nf;
+  const int nb = std::min<long long>(1024, std::max<long long>(1, (pl->n + 256 * 8 - 1) / (256 * 8)));
+  if ((size_t)2 * nn * nb > pl->partial_cap) return fail(FH_ERR_STATE, "partial buffer too small");
+  void* dptrs = nullptr;
+  double* dAH = nullptr;
+  HIPCHK(hipMalloc(&dptrs, nf * sizeof(void*)));
+  HIPCHK(hipMalloc(&dAH, nn * sizeof(double)));
+  int rc = FH_OK;
+  hipError_t e = hipMemcpy(dptrs, fields, nf * sizeof(void*), hipMemcpyHostToDevice);
+  if (e != hipSuccess) rc = fail(FH_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
+  if (rc == FH_OK) {
+    const double* const* F = (const double* const*)dptrs;
+    const dim3 grid(nb), block(256);
+#define FH_ENERGY(NC)                                                                                \
+  do {                                                                                               \
+    if (kind == FH_COEF_ISO)                                                                         \
+      hipLaunchKernelGGL((fh::k_energy<NC, fh::COEF_ISO>), grid, block, 0, pl->stream, F, coef, \
+                         pl->n, pl->partial);                                                        \
+    else if (kind == FH_COEF_LAME)                                                                   \
+      hipLaunchKernelGGL((fh::k_energy<NC, fh::COEF_LAME>), grid, block, 0, pl->stream, F,            \
+                         coef, pl->n, pl->partial);                                            \
+    else                                                                                             \
+      hipLaunchKernelGGL((fh::k_energy<NC, fh::COEF_FULL>), grid, block, 0, pl->stream, F,            \
+                         coef, pl->n, pl->partial);                                            \
+  } while (0)
+    if (nf == 2)
+      FH_ENERGY(2);
+    else if (nf == 3)
+      FH_ENERGY(3);
+    else if (nf == 6)
+      FH_ENERGY(6);
+    else
+      rc = fail(FH_ERR_ARG, "unsupported number of components");
+#undef FH_ENERGY
+    if (rc == FH_OK) {
+      hipLaunchKernelGGL(fh::k_energy_final, dim3(1), dim3(256), 0, pl->stream, pl->partial, nb, nn,
+                         1.0 / (double)pl->n, dAH);
+      e = hipGetLastError();
+      if (e == hipSuccess) e = hipStreamSynchronize(pl->stream);
+      if (e == hipSuccess) e = hipMemcpy(AH, dAH, nn * sizeof(double), hipMemcpyDeviceToHost);
+      if (e != hipSuccess) rc = fail(FH_ERR_HIP, std::string("energy: ") + hipGetErrorString(e));
+    }
+  }
+  (void)hipFree(dptrs);
+  (void)hipFree(dAH);
+  return rc;
+}
+
 std::mutex g_scratch_mu;
 double* g_scratch = nullptr;
 constexpr int kScratchParts = 2048;
@@ -588,6 +652,70 @@ static int scratch_parts(double** out) {
   std::lock_guard<std::mutex> lk(g_scratch_mu);
   if (!g_scratch) HIPCHK(hipMalloc(&g_scratch, kScratchParts * sizeof(double) + 64));
   *out = g_scratch;
+  return FH_OK;
+}
+
+
+// Quadrature grid length for the Ga energy on an axis of N points: 2N-1 (the reference's
+// Nbar, ffthompy/postprocess.py:24) when its prime factors are <= 13, else for odd N the next
+// such length (the energy is exact on any length >= 2N-1 there); -1 when none applies.
+int quadrature_length(int N) {
+  const int nb = 2 * N - 1;
+  AxisPlan ap;
+  if (factorize(nb, &ap)) return nb;
+  if ((N & 1) == 0) return -1;
+  for (int L = nb + 1; L < 2 * nb + 16; ++L)
+    if (factorize(L, &ap)) return L;
+  return -1;
+}
+
+// Inverse real FFT of one spectrum component held in `spec` (the plan's half-spectrum layout,
+// transformed in place) to `out`, scaled by `scale`.
+int spec_to_real(fh_plan* pl, cpx<double>* spec, double* out, double scale) {
+  const GParams g = make_gparams(pl, 1.0, 1);
+  if (pl->n0 > 1) {
+    RETCHK(launch_cols<double>(pl, 1, 0, 1, spec, g, nullptr));
+    RETCHK(launch_cols<double>(pl, 1, 1, 1, spec, g, nullptr));
+  } else {
+    RETCHK(launch_cols<double>(pl, 1, 1, 1, spec, g, nullptr));
+  }
+  return launch_rows_inv<double>(pl, 1, spec, out, scale, nullptr, nullptr, nullptr, nullptr);
+}
+
+int ga_ncoef(const fh_ga* ga, int kind) {
+  if (kind == FH_COEF_ISO) return 1;
+  if (kind == FH_COEF_LAME) return ga->coarse->physics == FH_ELASTICITY ? 2 : -1;
+  if (kind == FH_COEF_FULL) return ga->coarse->ncomp * ga->coarse->ncomp;
+  return -1;
+}
+
+int ga_alloc_coef(fh_ga* ga, int kind, int ncoef) {
+  if (ga->coef) (void)hipFree(ga->coef);
+  ga->coef = nullptr;
+  ga->coef_kind = 0;
+  ga->ncoef = 0;
+  HIPCHK(hipMalloc(&ga->coef, (size_t)ncoef * ga->fine->n * sizeof(double)));
+  ga->ncoef = ncoef;
+  (void)kind;
+  return FH_OK;
+}
+
+// Trigonometric interpolation of one coarse field (ncomp components) onto the quadrature grid.
+int ga_interpolate(fh_ga* ga, const double* x, double* out) {
+  fh_plan* c = ga->coarse;
+  fh_plan* f = ga->fine;
+  cpx<double>* cs = reinterpret_cast<cpx<double>*>(c->spec);
+  cpx<double>* fs = reinterpret_cast<cpx<double>*>(f->spec);
+  const int nb = grid_blocks(f->nh, 256, 8192);
+  for (int k = 0; k < c->ncomp; ++k) {
+    RETCHK(rfftn_impl<double>(c, 1, x + (size_t)k * c->n, cs));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    hipLaunchKernelGGL(fh::ga::k_ga_pad, dim3(nb), dim3(256), 0, f->stream, cs, c->n0, c->n1, c->n2,
+                       1.0 / (double)c->n, ga->g, fs);
+    HIPCHK(hipGetLastError());
+    RETCHK(spec_to_real(f, fs, out + (size_t)k * f->n, 1.0));
+    HIPCHK(hipStreamSynchronize(f->stream));
+  }
   return FH_OK;
 }
 
@@ -973,53 +1101,7 @@ int fh_energy_matrix(fh_plan* pl, const void* const* fields, int nf, double* AH)
   if (!pl->coef_kind) return fail(FH_ERR_STATE, "fh_set_coeff has not been called");
   for (int i = 0; i < nf; ++i)
     if (!fields[i]) return fail(FH_ERR_ARG, "null field pointer");
-  const int nn = nf * nf;
-  const int nb = std::min<long long>(1024, std::max<long long>(1, (pl->n + 256 * 8 - 1) / (256 * 8)));
-  if ((size_t)2 * nn * nb > pl->partial_cap) return fail(FH_ERR_STATE, "partial buffer too small");
-  void* dptrs = nullptr;
-  double* dAH = nullptr;
-  HIPCHK(hipMalloc(&dptrs, nf * sizeof(void*)));
-  HIPCHK(hipMalloc(&dAH, nn * sizeof(double)));
-  int rc = FH_OK;
-  hipError_t e = hipMemcpy(dptrs, fields, nf * sizeof(void*), hipMemcpyHostToDevice);
-  if (e != hipSuccess) rc = fail(FH_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
-  if (rc == FH_OK) {
-    const double* const* F = (const double* const*)dptrs;
-    const dim3 grid(nb), block(256);
-    const int kind = pl->coef_kind;
-#define FH_ENERGY(NC)                                                                                \
-  do {                                                                                               \
-    if (kind == FH_COEF_ISO)                                                                         \
-      hipLaunchKernelGGL((fh::k_energy<NC, fh::COEF_ISO>), grid, block, 0, pl->stream, F, pl->coef64, \
-                         pl->n, pl->partial);                                                        \
-    else if (kind == FH_COEF_LAME)                                                                   \
-      hipLaunchKernelGGL((fh::k_energy<NC, fh::COEF_LAME>), grid, block, 0, pl->stream, F,            \
-                         pl->coef64, pl->n, pl->partial);                                            \
-    else                                                                                             \
-      hipLaunchKernelGGL((fh::k_energy<NC, fh::COEF_FULL>), grid, block, 0, pl->stream, F,            \
-                         pl->coef64, pl->n, pl->partial);                                            \
-  } while (0)
-    if (nf == 2)
-      FH_ENERGY(2);
-    else if (nf == 3)
-      FH_ENERGY(3);
-    else if (nf == 6)
-      FH_ENERGY(6);
-    else
-      rc = fail(FH_ERR_ARG, "unsupported number of components");
-#undef FH_ENERGY
-    if (rc == FH_OK) {
-      hipLaunchKernelGGL(fh::k_energy_final, dim3(1), dim3(256), 0, pl->stream, pl->partial, nb, nn,
-                         1.0 / (double)pl->n, dAH);
-      e = hipGetLastError();
-      if (e == hipSuccess) e = hipStreamSynchronize(pl->stream);
-      if (e == hipSuccess) e = hipMemcpy(AH, dAH, nn * sizeof(double), hipMemcpyDeviceToHost);
-      if (e != hipSuccess) rc = fail(FH_ERR_HIP, std::string("energy: ") + hipGetErrorString(e));
-    }
-  }
-  (void)hipFree(dptrs);
-  (void)hipFree(dAH);
-  return rc;
+  return energy_impl(pl, pl->coef64, pl->coef_kind, fields, nf, AH);
 }
 
 int fh_energy_gradient(fh_plan* pl, const void* const* fields, int nf, const double* W, const void* chain,
@@ -1527,6 +1609,197 @@ int fh_batch_stats(const fh_batch* b, int* n_updates, double* norm_res, double* 
   if (norm_res) HIPCHK(hipMemcpy(norm_res, b->d_normres, (size_t)b->S * 4 * sizeof(double), hipMemcpyDeviceToHost));
   if (solve_ms) *solve_ms = b->last_ms;
   return FH_OK;
+}
+
+
+// ---- Ga post-processing ------------------------------------------------------------------
+
+int fh_ga_create(fh_plan* coarse, fh_ga** out) {
+  RETCHK(check_plan(coarse));
+  if (!out) return fail(FH_ERR_ARG, "null pointer argument");
+  *out = nullptr;
+  const int dim = coarse->dim;
+  int M[3] = {1, 1, 1};
+  for (int a = 0; a < dim; ++a) {
+    M[a] = quadrature_length(coarse->Nu[a]);
+    if (M[a] < 0)
+      return fail(FH_ERR_UNSUPPORTED, "Ga quadrature grid: 2N-1 = " + std::to_string(2 * coarse->Nu[a] - 1) +
+                                          " has a prime factor above 13 and N is even");
+  }
+  fh_plan* fine = nullptr;
+  RETCHK(fh_plan_create(dim, M, coarse->Y, coarse->physics, coarse->primaldual, &fine));
+  fh_ga* ga = new fh_ga();
+  ga->coarse = coarse;
+  ga->fine = fine;
+  fh::ga::SpecGrid& g = ga->g;
+  g.m0 = fine->n0;
+  g.m1 = fine->n1;
+  g.m2 = fine->n2;
+  g.m2h = fine->n2h;
+  g.vol = 1.0;
+  for (int a = 0; a < dim; ++a) g.vol *= coarse->Y[a];
+  const int off = 3 - dim;  // view axis of problem axis 0
+  for (int v = 0; v < 3; ++v) {
+    const int a = v - off;
+    const int N = a >= 0 ? coarse->Nu[a] : 1;
+    g.band[v] = N - 1;
+    g.nbar[v] = 2 * N - 1;
+    g.Y[v] = a >= 0 ? coarse->Y[a] : 1.0;
+  }
+  g.scale = 1.0;  // spec_to_real is the unnormalised inverse transform
+  *out = ga;
+  return FH_OK;
+}
+
+int fh_ga_destroy(fh_ga* ga) {
+  if (!ga) return FH_OK;
+  if (ga->coef) (void)hipFree(ga->coef);
+  if (ga->fields) (void)hipFree(ga->fields);
+  fh_plan_destroy(ga->fine);
+  delete ga;
+  return FH_OK;
+}
+
+int fh_ga_grid(const fh_ga* ga, int* M) {
+  if (!ga || !M) return fail(FH_ERR_ARG, "null pointer argument");
+  for (int a = 0; a < ga->coarse->dim; ++a) M[a] = ga->fine->Nu[a];
+  return FH_OK;
+}
+
+int fh_ga_set_inclusions(fh_ga* ga, int nincl, const int* kinds, const double* params, const double* positions,
+                         int kind, const double* vals) {
+  if (!ga || !kinds || !params || !positions || !vals) return fail(FH_ERR_ARG, "null pointer argument");
+  const int ncoef = ga_ncoef(ga, kind);
+  if (ncoef < 0) return fail(FH_ERR_ARG, "bad coefficient kind");
+  if (nincl < 1 || nincl > 4096) return fail(FH_ERR_ARG, "number of inclusions out of range");
+  const int dim = ga->coarse->dim, off = 3 - dim;
+  std::vector<fh::ga::Incl> h(nincl);
+  for (int ii = 0; ii < nincl; ++ii) {
+    fh::ga::Incl& in = h[ii];
+    std::memset(&in, 0, sizeof(in));
+    in.kind = kinds[ii];
+    if (in.kind < fh::ga::INCL_CUBE || in.kind > fh::ga::INCL_OTHERWISE)
+      return fail(FH_ERR_ARG, "unknown inclusion kind");
+    if (in.kind == fh::ga::INCL_OTHERWISE && ii != nincl - 1)
+      return fail(FH_ERR_ARG, "'otherwise' must be the last inclusion");
+    for (int v = 0; v < 3; ++v) {
+      const int a = v - off;
+      in.p[v] = a >= 0 ? params[3 * ii + a] : 1.0;
+      in.pos[v] = a >= 0 ? positions[3 * ii + a] : 0.0;
+      if (!std::isfinite(in.p[v]) || !std::isfinite(in.pos[v])) return fail(FH_ERR_ARG, "non-finite inclusion");
+    }
+    if (in.kind == fh::ga::INCL_BALL) in.p[0] = params[3 * ii];
+    for (int c = 0; c < ncoef; ++c) {
+      in.v[c] = vals[(size_t)ii * ncoef + c];
+      if (!std::isfinite(in.v[c])) return fail(FH_ERR_ARG, "coefficient has a non-finite value");
+    }
+  }
+  fh::ga::Incl* d = nullptr;
+  HIPCHK(hipMalloc(&d, nincl * sizeof(fh::ga::Incl)));
+  int rc = FH_OK;
+  hipError_t e = hipMemcpy(d, h.data(), nincl * sizeof(fh::ga::Incl), hipMemcpyHostToDevice);
+  if (e != hipSuccess) rc = fail(FH_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
+  if (rc == FH_OK) rc = ga_alloc_coef(ga, kind, ncoef);
+  fh_plan* f = ga->fine;
+  cpx<double>* fs = reinterpret_cast<cpx<double>*>(f->spec);
+  const int nb = grid_blocks(f->nh, 256, 8192);
+  for (int c = 0; c < ncoef && rc == FH_OK; ++c) {
+    hipLaunchKernelGGL(fh::ga::k_ga_incl_spec, dim3(nb), dim3(256), 0, f->stream, d, nincl, c, ga->g, fs);
+    e = hipGetLastError();
+    if (e != hipSuccess) rc = fail(FH_ERR_HIP, std::string("Ga coefficient: ") + hipGetErrorString(e));
+    if (rc == FH_OK) rc = spec_to_real(f, fs, ga->coef + (size_t)c * f->n, 1.0);
+  }
+  e = hipStreamSynchronize(f->stream);
+  if (rc == FH_OK && e != hipSuccess) rc = fail(FH_ERR_HIP, std::string("Ga coefficient: ") + hipGetErrorString(e));
+  (void)hipFree(d);
+  if (rc == FH_OK) ga->coef_kind = kind;
+  return rc;
+}
+
+int fh_ga_set_grid_coeff(fh_ga* ga, int order, const int* P, int kind, const double* vals) {
+  if (!ga || !P || !vals) return fail(FH_ERR_ARG, "null pointer argument");
+  if (order != 0 && order != 1) return fail(FH_ERR_ARG, "order must be 0 or 1");
+  const int ncoef = ga_ncoef(ga, kind);
+  if (ncoef < 0) return fail(FH_ERR_ARG, "bad coefficient kind");
+  const int dim = ga->coarse->dim;
+  fh_plan* pp = nullptr;
+  RETCHK(fh_plan_create(dim, P, ga->coarse->Y, FH_SCALAR, FH_PRIMAL, &pp));
+  const size_t np = (size_t)pp->n;
+  for (size_t i = 0; i < (size_t)ncoef * np; ++i)
+    if (!std::isfinite(vals[i])) {
+      fh_plan_destroy(pp);
+      return fail(FH_ERR_ARG, "coefficient has a non-finite value");
+    }
+  double* dv = nullptr;
+  cpx<double>* dh = nullptr;
+  int rc = FH_OK;
+  hipError_t e = hipMalloc(&dv, np * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&dh, (size_t)pp->nh * sizeof(cpx<double>));
+  if (e != hipSuccess) rc = fail(FH_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+  if (rc == FH_OK) rc = ga_alloc_coef(ga, kind, ncoef);
+  fh_plan* f = ga->fine;
+  cpx<double>* fs = reinterpret_cast<cpx<double>*>(f->spec);
+  const int nb = grid_blocks(f->nh, 256, 8192);
+  for (int c = 0; c < ncoef && rc == FH_OK; ++c) {
+    e = hipMemcpy(dv, vals + (size_t)c * np, np * sizeof(double), hipMemcpyHostToDevice);
+    if (e != hipSuccess) rc = fail(FH_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
+    if (rc == FH_OK) rc = rfftn_impl<double>(pp, 1, dv, dh);
+    if (rc == FH_OK) {
+      e = hipStreamSynchronize(pp->stream);
+      if (e == hipSuccess) {
+        hipLaunchKernelGGL(fh::ga::k_ga_grid_spec, dim3(nb), dim3(256), 0, f->stream, dh, pp->n0, pp->n1, pp->n2,
+                           order + 1, ga->g, fs);
+        e = hipGetLastError();
+      }
+      if (e != hipSuccess) rc = fail(FH_ERR_HIP, std::string("Ga coefficient: ") + hipGetErrorString(e));
+    }
+    if (rc == FH_OK) rc = spec_to_real(f, fs, ga->coef + (size_t)c * f->n, 1.0);
+    if (rc == FH_OK) {
+      e = hipStreamSynchronize(f->stream);
+      if (e != hipSuccess) rc = fail(FH_ERR_HIP, std::string("Ga coefficient: ") + hipGetErrorString(e));
+    }
+  }
+  if (dv) (void)hipFree(dv);
+  if (dh) (void)hipFree(dh);
+  fh_plan_destroy(pp);
+  if (rc == FH_OK) ga->coef_kind = kind;
+  return rc;
+}
+
+int fh_ga_coefficient(const fh_ga* ga, double* host_out) {
+  if (!ga || !host_out) return fail(FH_ERR_ARG, "null pointer argument");
+  if (!ga->coef_kind) return fail(FH_ERR_STATE, "Ga coefficient not set");
+  HIPCHK(hipMemcpy(host_out, ga->coef, (size_t)ga->ncoef * ga->fine->n * sizeof(double), hipMemcpyDeviceToHost));
+  return FH_OK;
+}
+
+int fh_ga_interpolate(fh_ga* ga, const void* x, void* out) {
+  if (!ga || !x || !out) return fail(FH_ERR_ARG, "null pointer argument");
+  return ga_interpolate(ga, (const double*)x, (double*)out);
+}
+
+int fh_ga_energy(fh_ga* ga, const void* const* fields, int nf, double* AH) {
+  if (!ga || !fields || !AH) return fail(FH_ERR_ARG, "null pointer argument");
+  const int nc = ga->coarse->ncomp;
+  if (nf != nc) return fail(FH_ERR_ARG, "need one field per unit load (nf == ncomp)");
+  if (!ga->coef_kind) return fail(FH_ERR_STATE, "Ga coefficient not set");
+  for (int i = 0; i < nf; ++i)
+    if (!fields[i]) return fail(FH_ERR_ARG, "null field pointer");
+  const size_t per = (size_t)nc * ga->fine->n;
+  const size_t bytes = (size_t)nf * per * sizeof(double);
+  if (ga->fields_bytes < bytes) {
+    if (ga->fields) (void)hipFree(ga->fields);
+    ga->fields = nullptr;
+    ga->fields_bytes = 0;
+    HIPCHK(hipMalloc(&ga->fields, bytes));
+    ga->fields_bytes = bytes;
+  }
+  std::vector<const void*> ptr(nf);
+  for (int i = 0; i < nf; ++i) {
+    RETCHK(ga_interpolate(ga, (const double*)fields[i], ga->fields + i * per));
+    ptr[i] = ga->fields + i * per;
+  }
+  return energy_impl(ga->fine, ga->coef, ga->coef_kind, ptr.data(), nf, AH);
 }
 
 }  // extern "C"

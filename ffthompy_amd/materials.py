@@ -8,13 +8,14 @@ follows the reference's rules: nodes x = Y*k/N on the centred index set
 (materials.py:226-275), inclusion membership > -a/2 and <= a/2 per axis for boxes and
 < r for balls; the dual problem uses the pointwise inverse (materials.py:121-122) and the
 result is shifted to origin 0 (objects.py:169-186). Exact-integration ('Ga')
-coefficients are not provided on this path.
+coefficients (materials.py:54-113) are described here and evaluated on the device
+(GaCoefficient, ffthompy_amd._native.Ga).
 """
 import itertools
 
 import numpy as np
 
-from .tensors import Tensor
+from .tensors import Tensor, classify_coefficient
 
 INCLUSION_KEYS = {'ball': ('ball', 'circle'), 'cube': ('cube', 'square'),
                   'pyramid': ('bilinear_pyramid', 'pyramid')}
@@ -127,8 +128,88 @@ class Material:
                 raise NotImplementedError("Inclusion (%s) is not implemented." % kind)
         return out
 
-    def get_A_Ga(self, *args, **kwargs):
-        raise NotImplementedError('exact-integration (Ga) coefficients are not available on the GPU path')
+    def get_A_Ga(self, Nbar, primaldual='primal', order=-1, P=None):
+        """Exactly integrated coefficient on the grid Nbar (materials.py:54-113), as a
+        GaCoefficient evaluated on the device. order None/'exact': closed-form integrals of the
+        inclusions' shape functions; order 0 / 1: box / tent interpolation of the nodal values
+        on the grid P (default conf['P']). The dual problem inverts each phase (order None) or
+        the nodal values (order 0/1), as materials.py:73, 84."""
+        if order == -1:
+            if 'order' not in self.conf:
+                raise ValueError('The material order is undefined!')
+            order = self.conf['order']
+        elif order not in (None, 'exact', 0, 1):
+            raise ValueError('Wrong material order (%s)!' % str(order))
+        Nbar = tuple(int(n) for n in np.atleast_1d(Nbar))
+        d = len(self.Y)
+        if order in (None, 'exact'):
+            if 'inclusions' not in self.conf:
+                raise NotImplementedError('exact integration needs a material defined by inclusions')
+            kinds = list(self.conf['inclusions'])
+            params = np.zeros((len(kinds), d))
+            positions = np.zeros((len(kinds), d))
+            mats = []
+            for ii, kind in enumerate(kinds):
+                if kind not in GA_KINDS:
+                    raise NotImplementedError('The inclusion (%s) is not supported!' % kind)
+                if kind in INCLUSION_KEYS['ball']:
+                    params[ii, 0] = float(np.asarray(self.conf['params'][ii]).reshape(-1)[0])
+                elif kind not in ('all', 'otherwise'):
+                    params[ii] = np.asarray(self.conf['params'][ii], dtype=np.float64)
+                if kind not in ('all', 'otherwise'):
+                    positions[ii] = np.asarray(self.conf['positions'][ii], dtype=np.float64)
+                Ai = np.asarray(self.conf['vals'][ii], dtype=np.float64)
+                mats.append(np.linalg.inv(Ai) if primaldual == 'dual' else Ai)
+            kind, vals = classify_coefficient(np.stack(mats, axis=-1), _physics_of(mats[0].shape[0], d))
+            return GaCoefficient(Nbar, self.Y, 'A_Ga', kind, vals, inclusions=(kinds, params, positions))
+        if P is None and 'P' in self.conf:
+            P = self.conf['P']
+        P = tuple(int(p) for p in np.broadcast_to(np.asarray(P, dtype=np.int64), (d,)))
+        A = self.evaluate(node_coordinates(P, self.Y))
+        if primaldual == 'dual':
+            A = pointwise_inverse(A)
+        A = np.fft.ifftshift(A, axes=tuple(range(2, A.ndim)))
+        kind, vals = classify_coefficient(A, _physics_of(A.shape[0], d))
+        return GaCoefficient(Nbar, self.Y, 'A_Ga_o%s_P%d' % (order, max(P)), kind, vals, order=int(order), P=P)
+
+
+GA_KINDS = INCLUSION_KEYS['ball'] + INCLUSION_KEYS['cube'] + INCLUSION_KEYS['pyramid'] + ('all', 'otherwise')
+
+
+def _physics_of(k, d):
+    return 'elasticity' if k == d * (d + 1) // 2 and k != d else 'scalar'
+
+
+class GaCoefficient:
+    """Exactly integrated coefficient A_Ga of materials.py:54-113 on the grid Nbar, held as its
+    description (inclusions, or nodal values on P with interpolation order 0/1) and evaluated
+    on the device by an ffthompy_amd._native.Ga, whose quadrature grid may be finer than
+    Nbar (the energies are the same, see fh_ga_create). kind/vals: compact phase or nodal
+    coefficients ('iso', 'lame' or 'full' layout)."""
+
+    def __init__(self, Nbar, Y, name, kind, vals, inclusions=None, order=None, P=None):
+        self.N = tuple(Nbar)
+        self.Y = np.asarray(Y, dtype=np.float64)
+        self.name = name
+        self.kind = kind
+        self.vals = np.asarray(vals, dtype=np.float64)
+        self.inclusions = inclusions
+        self.order = order
+        self.P = P
+
+    def load(self, ga):
+        """Evaluate this coefficient on the quadrature grid of `ga`."""
+        if self.inclusions is not None:
+            kinds, params, positions = self.inclusions
+            nincl = len(kinds)
+            vals = self.vals.reshape(-1, nincl).T
+            ga.set_inclusions(kinds, params, positions, self.kind, vals)
+        else:
+            ga.set_grid_coeff(self.order, self.P, self.kind, self.vals)
+        return ga
+
+    def __repr__(self):
+        return 'GaCoefficient(%s, Nbar=%s, %s)' % (self.name, self.N, self.kind)
 
 
 def pointwise_inverse(A):

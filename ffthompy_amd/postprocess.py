@@ -2,8 +2,9 @@
 
 Mirrors ffthompy/postprocess.py:6-86. The energy forms AH_ij = <A e_i, e_j> run on the
 GPU in fp64 with double-double accumulation (fh_energy_matrix); the dual matrix is
-the inverse of the dual energy matrix. GaNi post-processing is provided; the 'Ga'
-kinds (exact-integration coefficient on the 2N-1 grid) are not on this path.
+the inverse of the dual energy matrix. The 'Ga' kinds (exactly integrated coefficient on
+the 2N-1 grid, postprocess.py:20-41) run on the device as well (fh_ga_*): coefficient
+spectra from closed forms, solutions interpolated by zero-padded spectra.
 """
 import numpy as np
 
@@ -23,6 +24,22 @@ def assembly_matrix(A, solutions):
         kind, vals = classify_coefficient(A.val if isinstance(A, Tensor) else A, physics)
         plan.set_coeff(kind, vals)
     return plan.energy_matrix([f.val for f in fields])
+
+
+def ga_assembly_matrix(A_ga, Afun, solutions):
+    """AH[i,j] = <A_Ga P(e_i), P(e_j)>: assembly_matrix with the solutions projected onto the
+    coefficient's grid (postprocess.py:53-70, Tensor.project objects.py:488-511), on the
+    device. A_ga: materials.GaCoefficient; Afun: the GpuOperator of the solve (its plan
+    holds the solutions' grid)."""
+    fields = [s.to_device() for s in solutions]
+    if isinstance(Afun, GpuOperator):
+        plan = Afun.plan
+    else:
+        N = fields[0].N
+        physics = 'scalar' if len(fields) == len(N) else 'elasticity'
+        plan = nat.Plan(N, A_ga.Y, physics)
+    ga = A_ga.load(nat.Ga(plan))
+    return ga.energy_matrix([f.val for f in fields])
 
 
 def add_macro2minimizer(X, E):
@@ -47,7 +64,20 @@ def postprocess(pb, A, mat, solutions, results, primaldual):
             AH = assembly_matrix(A, solutions)
             matrices[name] = AH if primaldual == 'primal' else np.linalg.inv(AH)
         elif pp['kind'] in ('Ga', 'ga'):
-            raise NotImplementedError("'Ga' post-processing is not available on the GPU path")
+            order_name = Nname = ''
+            if 'order' in pp:
+                Nbar = tuple(2 * np.asarray(pb.solve['N']) - 1)
+                if pp['order'] is None:
+                    A_ga = mat.get_A_Ga(Nbar=Nbar, primaldual=primaldual, order=None)
+                else:
+                    order_name = '_o' + str(pp['order'])
+                    Nname = '_P%d' % np.mean(pp['P'])
+                    A_ga = mat.get_A_Ga(Nbar=Nbar, primaldual=primaldual, order=pp['order'], P=pp['P'])
+                AH = ga_assembly_matrix(A_ga, A, solutions)
+            else:
+                AH = assembly_matrix(A, solutions)
+            name = 'AH_%s%s%s_%s' % (pp['kind'], order_name, Nname, primaldual)
+            matrices[name] = AH if primaldual == 'primal' else np.linalg.inv(AH)
         else:
             raise ValueError(pp['kind'])
     pb.output.update({'sol_' + primaldual: solutions, 'res_' + primaldual: results,

@@ -46,6 +46,7 @@ enum { FH_CG_OK = 0, FH_CG_BREAKDOWN = 1 };
 
 typedef struct fh_plan fh_plan;
 typedef struct fh_batch fh_batch;
+typedef struct fh_ga fh_ga;
 
 /* Message of the calling thread's last error ("" if none). */
 const char* fh_last_error(void);
@@ -213,6 +214,40 @@ int fh_batch_autotune(fh_batch* batch, int mode, double delta, int iters, double
 /* Of the last solve: fp64 residual evaluations and final sqrt(<r,r>) per (sample, primal/dual,
  * load) (4 per sample, host arrays), and the device time of the whole batch in ms. */
 int fh_batch_stats(const fh_batch* batch, int* n_updates, double* norm_res, double* solve_ms);
+
+/* ---- Ga post-processing: bounds from the exactly integrated coefficient ---------------------
+ * The 'Ga' entries of ffthompy/postprocess.py:20-41: AH_ij = <A_Ga P(e_i), P(e_j)> on the grid
+ * Nbar = 2N-1, with A_Ga the exactly integrated coefficient (Material.get_A_Ga,
+ * ffthompy/materials.py:54-113) and P(e) the trigonometric interpolation of a GaNi solution
+ * (Tensor.project, ffthompy/tensors/objects.py:488-511, inside assembly_matrix,
+ * ffthompy/postprocess.py:53-70); the dual matrix is the inverse (postprocess.py:43-46).
+ * The quadrature grid is Nbar where its prime factors are <= 13; for odd N with a larger prime
+ * in 2N-1 it is the next such length (the integrand is a trigonometric polynomial of degree
+ * <= 2N-2 per axis, so every grid of at least 2N-1 points integrates it exactly); for even N
+ * without such an Nbar the call fails with FH_ERR_UNSUPPORTED. */
+int fh_ga_create(fh_plan* coarse, fh_ga** out);
+int fh_ga_destroy(fh_ga* ga);
+/* Quadrature grid actually used, dim entries. */
+int fh_ga_grid(const fh_ga* ga, int* M);
+/* Coefficient from inclusions (get_A_Ga with order None, materials.py:54-73; shape functions
+ * materials.py:125-169; phase and weights materials.py:317-425): kinds[ii] 1 box ('cube'/'square'),
+ * 2 ball ('ball'/'circle'; params[3ii] = diameter), 3 pyramid, 4 'all', 5 'otherwise' (last);
+ * params/positions: 3 doubles per inclusion (problem axes first); vals: the phase coefficient
+ * per inclusion in layout kind (FH_COEF_ISO 1, FH_COEF_LAME 2, FH_COEF_FULL ncomp^2 values),
+ * already inverted for the dual problem (materials.py:73). */
+int fh_ga_set_inclusions(fh_ga* ga, int nincl, const int* kinds, const double* params, const double* positions,
+                         int kind, const double* vals);
+/* Coefficient from nodal values on a grid P (get_A_Ga with order 0 or 1 and P,
+ * materials.py:74-113): vals host fp64, component-major, each a P-grid array in origin-0
+ * voxel order (dual: the pointwise inverse), layout kind as above. */
+int fh_ga_set_grid_coeff(fh_ga* ga, int order, const int* P, int kind, const double* vals);
+/* A_Ga on the quadrature grid to host (ncoef arrays of prod(M) values, origin 0). */
+int fh_ga_coefficient(const fh_ga* ga, double* host_out);
+/* P(x) on the quadrature grid: x a device fp64 field of the coarse plan (ncomp components),
+ * out a device fp64 field of ncomp * prod(M) values (Tensor.project, objects.py:488-511). */
+int fh_ga_interpolate(fh_ga* ga, const void* x, void* out);
+/* AH[i][j] = <A_Ga P(e_i), P(e_j)> for nf = ncomp fp64 device fields of the coarse plan. */
+int fh_ga_energy(fh_ga* ga, const void* const* fields, int nf, double* AH);
 
 /* Wait for all GPU work of this process. */
 int fh_synchronize(void);

@@ -1,0 +1,185 @@
+"""Ga post-processing on the GPU: the exactly integrated coefficient on the 2N-1 grid and the
+solutions' trigonometric interpolation onto it, then the energies.
+
+* The reference's example problems (GaNi solves with 'Ga' post-processing) end to end
+  through the drop-in Problem, against the golden bounds (1e-9 absolute) and the oracle.
+* Each piece against the oracle: the coefficient (inclusions of every kind, off-centre
+  positions, 3-D balls, order 0/1 on grids P below, at and above 2N-1, even P), the
+  interpolation (odd and even N) and the energy.
+* Grids where 2N-1 has a prime factor above 13 (N = 45 in 2-D: 89; N = 15 in 3-D: 29): the
+  device integrates on the next 13-smooth grid (90, 30); the bounds equal the oracle's
+  (computed on 89 / 29) to rounding.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import examples
+from oracle.fields import Field
+from oracle.homogenize import Problem as OracleProblem
+from oracle.homogenize import assembly_matrix as o_assembly_matrix
+from oracle.materials import Material as OMaterial
+
+pytestmark = pytest.mark.gpu
+ffa = pytest.importorskip('ffthompy_amd')
+from ffthompy_amd import Problem  # noqa: E402
+from ffthompy_amd import _native as nat  # noqa: E402
+from ffthompy_amd.materials import Material  # noqa: E402
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), 'golden', 'reference_results.json')))
+GANI_GA = [c for c in examples.problems()
+           if c[2]['solve']['kind'] == 'GaNi' and any(pp['kind'] == 'Ga' for pp in c[2]['postprocess'])]
+
+
+@pytest.mark.parametrize('case', GANI_GA, ids=['%s_%s' % (c[0], c[1]) for c in GANI_GA])
+def test_ga_postprocess_goldens(case):
+    f, name, conf, mats = case
+    print('case ga golden', f, name, flush=True)
+    pb = Problem(dict(conf), type('conf', (), {'materials': mats()}))
+    pb.calculate()
+    ref = OracleProblem(dict(conf), mats())
+    ref.calculate()
+    gold = GOLD['problems']['%s_%s' % (f, name)]
+    checked = 0
+    for pd in pb.solve['primaldual']:
+        for key, AH in pb.output['mat_' + pd].items():
+            if key in gold.get('diag', {}):
+                assert np.abs(AH - gold['diag'][key] * np.eye(AH.shape[0])).max() < 1e-9, key
+                checked += 1
+            for ij, v in gold.get('entries', {}).get(key, {}).items():
+                i, j = map(int, ij.split(','))
+                assert abs(AH[i, j] - v) < 1e-9, (key, ij)
+                checked += 1
+            assert np.abs(AH - ref.output['mat_' + pd][key]).max() < 1e-10 * max(1., np.abs(AH).max()), key
+    assert checked >= 4
+
+
+def _expand(kind, vals, k):
+    """Compact coefficient layout -> (k, k) + grid."""
+    grid = vals.shape[1:]
+    if kind == 'iso':
+        return np.einsum('ij,...->ij...', np.eye(k), vals[0])
+    if kind == 'lame':
+        dn = 3 if k == 6 else 2
+        m = np.zeros(k)
+        m[:dn] = 1.
+        return np.einsum('i,j,...->ij...', m, m, vals[0]) + 2. * np.einsum('ij,...->ij...', np.eye(k), vals[1])
+    return vals.reshape((k, k) + grid)
+
+
+def _scalar_material(dim, kinds, params, positions, Y=None):
+    I = np.eye(dim)
+    vals = [(3. + 4. * ii) * I for ii in range(len(kinds) - 1)] + [1. * I]
+    return {'inclusions': list(kinds), 'params': list(params), 'positions': list(positions), 'vals': vals,
+            'Y': np.ones(dim) if Y is None else np.asarray(Y, dtype=np.float64)}
+
+
+COEF_CASES = {
+    'square_offcentre': (2, (5, 5), _scalar_material(2, ['square', 'otherwise'], [np.array([0.4, 0.6]), ''],
+                                                      [np.array([0.1, 0.25]), ''])),
+    'circle_pyramid': (2, (7, 5), _scalar_material(2, ['circle', 'pyramid', 'otherwise'],
+                                                    [0.5, np.array([0.3, 0.4]), ''],
+                                                    [np.array([0.5, 0.5]), np.array([0.2, 0.0]), ''],
+                                                    Y=[1., 1.5])),
+    'pyramid_all': (2, (5, 5), _scalar_material(2, ['pyramid', 'all'], [np.ones(2), ''], [np.zeros(2), ''])),
+    'cube_ball_3d': (3, (5, 5, 5), _scalar_material(3, ['cube', 'ball', 'otherwise'],
+                                                     [np.array([0.3, 0.4, 0.5]), 0.4, ''],
+                                                     [np.zeros(3), np.array([0.5, 0.3, 0.7]), ''])),
+}
+
+
+@pytest.mark.parametrize('name', sorted(COEF_CASES))
+@pytest.mark.parametrize('pd', ['primal', 'dual'])
+def test_ga_inclusion_coefficient_matches_oracle(name, pd):
+    dim, N, conf = COEF_CASES[name]
+    print('case ga coef', name, pd, flush=True)
+    Nbar = tuple(2 * n - 1 for n in N)
+    ref = OMaterial({k: (list(v) if isinstance(v, list) else v) for k, v in conf.items()}).get_A_Ga(
+        Nbar, pd, order=None).val
+    A = Material({k: (list(v) if isinstance(v, list) else v) for k, v in conf.items()}).get_A_Ga(
+        Nbar, pd, order=None)
+    ga = nat.Ga(nat.Plan(N, conf['Y']))
+    assert ga.M == Nbar
+    got = _expand(A.kind, A.load(ga).coefficient(), dim)
+    assert np.abs(got - ref).max() < 1e-12 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize('order', [0, 1])
+@pytest.mark.parametrize('P', [(5, 5), (9, 9), (15, 12), (6, 8)])
+def test_ga_grid_coefficient_matches_oracle(order, P):
+    print('case ga grid coef', order, P, flush=True)
+    conf = COEF_CASES['square_offcentre'][2]
+    N = (5, 5)
+    Nbar = (9, 9)
+    for pd in ('primal', 'dual'):
+        ref = OMaterial(dict(conf, positions=list(conf['positions']))).get_A_Ga(Nbar, pd, order=order, P=P).val
+        A = Material(dict(conf, positions=list(conf['positions']))).get_A_Ga(Nbar, pd, order=order, P=P)
+        ga = nat.Ga(nat.Plan(N, conf['Y']))
+        got = _expand(A.kind, A.load(ga).coefficient(), 2)
+        assert np.abs(got - ref).max() < 1e-12 * np.abs(ref).max(), pd
+
+
+def test_ga_elastic_coefficient_matches_oracle():
+    conf = examples.linelas_3d_materials()['square']
+    for pd in ('primal', 'dual'):
+        ref = OMaterial(dict(conf, positions=list(conf['positions']))).get_A_Ga((9, 9, 9), pd, order=None).val
+        A = Material(dict(conf, positions=list(conf['positions']))).get_A_Ga((9, 9, 9), pd, order=None)
+        assert A.kind == 'lame'
+        ga = nat.Ga(nat.Plan((5, 5, 5), conf['Y'], 'elasticity'))
+        got = _expand(A.kind, A.load(ga).coefficient(), 6)
+        assert np.abs(got - ref).max() < 1e-12 * np.abs(ref).max(), pd
+
+
+@pytest.mark.parametrize('N', [(5, 5), (4, 6), (8, 7), (5, 4, 6), (7, 5, 5)])
+def test_ga_interpolation_matches_oracle_project(N):
+    print('case ga interp', N, flush=True)
+    rng = np.random.default_rng(sum(N))
+    d = len(N)
+    x = rng.standard_normal((d,) + N)
+    plan = nat.Plan(N, np.ones(d))
+    ga = nat.Ga(plan)
+    M = tuple(2 * n - 1 for n in N)
+    assert ga.M == M
+    got = ga.interpolate(nat.DeviceArray.from_host(x)).to_host()
+    ref = Field(val=x.copy(), order=1, Y=np.ones(d)).project(M).val
+    assert np.abs(got - ref).max() < 1e-13 * np.abs(ref).max()
+
+
+def _random_fields(N, nf, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(nf):
+        v = 0.3 * rng.standard_normal((nf,) + N)
+        v[i] += 1.
+        out.append(v)
+    return out
+
+
+@pytest.mark.parametrize('dim,N,M', [(2, (45, 45), (90, 90)), (3, (15, 15, 15), (30, 30, 30))])
+def test_ga_energy_on_smooth_quadrature_grid(dim, N, M):
+    """2N-1 prime: the device integrates on M > 2N-1, the oracle on 2N-1; same energies."""
+    print('case ga smooth grid', N, flush=True)
+    conf = _scalar_material(dim, ['cube' if dim == 3 else 'square', 'otherwise'],
+                            [0.55 * np.ones(dim), ''], [np.zeros(dim), ''])
+    Nbar = tuple(2 * n - 1 for n in N)
+    fields = _random_fields(N, dim, 7)
+    plan = nat.Plan(N, np.ones(dim))
+    ga = nat.Ga(plan)
+    assert ga.M == M
+    for order, P in ((None, None), (0, N), (1, N)):
+        for pd in ('primal', 'dual'):
+            c = dict(conf, positions=list(conf['positions']))
+            Aref = OMaterial(dict(c)).get_A_Ga(Nbar, pd, order=order, P=P)
+            ref = o_assembly_matrix(Aref, [Field(val=v.copy(), order=1, Y=np.ones(dim)) for v in fields])
+            Material(dict(c)).get_A_Ga(Nbar, pd, order=order, P=P).load(ga)
+            got = ga.energy_matrix([nat.DeviceArray.from_host(v) for v in fields])
+            assert np.abs(got - ref).max() < 1e-11 * np.abs(ref).max(), (order, pd)
+
+
+def test_ga_even_grid_without_smooth_nbar_is_rejected():
+    plan = nat.Plan((10, 10), (1., 1.))  # 2N-1 = 19, prime; even N needs exactly 19
+    with pytest.raises(nat.FhError) as ei:
+        nat.Ga(plan)
+    assert ei.value.code == nat.FH_ERR_UNSUPPORTED
