@@ -65,7 +65,10 @@ SIGNATURES = {
     'fh_batch_stats': [_vp, _c_int_p, _c_double_p, _c_double_p],
     'fh_batch_set_schedule': [_vp, ctypes.c_int, ctypes.c_int],
     'fh_batch_autotune': [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, _c_double_p, _c_int_p, _c_int_p],
-    'fh_ga_create': [_vp, ctypes.POINTER(ctypes.c_void_p)],
+    'fh_ga_create': [_vp, _c_int_p, ctypes.POINTER(ctypes.c_void_p)],
+    'fh_ga_set_plan_coeff': [_vp, _vp],
+    'fh_ga_energy_on_grid': [_vp, ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, _c_double_p],
+    'fh_plan_set_band': [_vp, _c_int_p, ctypes.c_double],
     'fh_ga_destroy': [_vp],
     'fh_ga_grid': [_vp, _c_int_p],
     'fh_ga_set_inclusions': [_vp, ctypes.c_int, _c_int_p, _c_double_p, _c_double_p, ctypes.c_int, _c_double_p],
@@ -285,6 +288,12 @@ class Plan:
         """Mixed-precision CG: fused iteration (default) or the operator-form denominator."""
         call('fh_plan_set_cg_variant', self._h, ctypes.c_int(1 if fused else 0))
 
+    def set_band(self, band, scale=1.0):
+        """Projection restricted to |k_a| <= band[a] (None: unrestricted); `scale` multiplies
+        the operator G A (see fh_plan_set_band)."""
+        B = None if band is None else (ctypes.c_int * self.dim)(*[int(b) for b in band])
+        call('fh_plan_set_band', self._h, B, ctypes.c_double(scale))
+
     def use_generic(self, flag=True):
         call('fh_plan_use_generic', self._h, ctypes.c_int(1 if flag else 0))
 
@@ -466,11 +475,12 @@ class Ga:
     """Owner of an fh_ga: exactly integrated coefficient and interpolated solutions on the
     quadrature grid of a plan's solutions (Ga post-processing)."""
 
-    def __init__(self, plan):
+    def __init__(self, plan, grid=None):
         init()
         self.plan = plan
         h = ctypes.c_void_p()
-        call('fh_ga_create', plan.handle, ctypes.byref(h))
+        G = None if grid is None else (ctypes.c_int * plan.dim)(*[int(m) for m in grid])
+        call('fh_ga_create', plan.handle, G, ctypes.byref(h))
         self._h = h
         M = (ctypes.c_int * plan.dim)()
         call('fh_ga_grid', h, M)
@@ -522,12 +532,20 @@ class Ga:
         call('fh_ga_interpolate', self._h, x.ptr, out.ptr)
         return out
 
-    def energy_matrix(self, fields):
+    def energy_matrix(self, fields, on_grid=False):
+        """fields: solutions on the plan's grid (interpolated here), or with on_grid=True
+        already on the quadrature grid."""
         nf = len(fields)
         arr = (ctypes.c_void_p * nf)(*[f.ptr for f in fields])
         AH = np.zeros((nf, nf))
-        call('fh_ga_energy', self._h, arr, ctypes.c_int(nf), AH.ctypes.data_as(_c_double_p))
+        call('fh_ga_energy_on_grid' if on_grid else 'fh_ga_energy', self._h, arr, ctypes.c_int(nf),
+             AH.ctypes.data_as(_c_double_p))
         return AH
+
+    def set_plan_coeff(self, plan):
+        """Make A_Ga the coefficient of `plan` (a plan on the quadrature grid)."""
+        call('fh_ga_set_plan_coeff', self._h, plan.handle)
+        plan.coef_kind = self.coef_kind
 
     def __del__(self):
         h = getattr(self, '_h', None)
@@ -537,3 +555,16 @@ class Ga:
             except Exception:
                 pass
             self._h = None
+
+
+def smooth_length(L):
+    """Smallest length >= L whose prime factors are all <= 13 (the FFT radices)."""
+    L = int(L)
+    while True:
+        r = L
+        for p in (2, 3, 5, 7, 11, 13):
+            while r % p == 0:
+                r //= p
+        if r == 1:
+            return L
+        L += 1

@@ -156,6 +156,9 @@ struct fh_plan {
   int generic = 0;               // 1: never use the fixed-length kernels
   bool fused_cg = true;          // mixed CG: fused iteration where available
   hipEvent_t prof_t0 = nullptr, prof_t1 = nullptr;  // matvec profile: events of the next launch
+  int banded = 0;                // 1: projection restricted to |k_a| <= band[a] (view axes)
+  int band[3] = {0, 0, 0};
+  double op_scale = 1.0;         // factor of the operator G A (not of the bare projection)
 };
 
 struct fh_batch {
@@ -225,6 +228,8 @@ GParams make_gparams(const fh_plan* pl, double scale, int ncomp_override = 0) {
     g.iY[2] = 1.0 / pl->Y[1];
   }
   g.scale = scale;
+  g.banded = pl->banded;
+  for (int a = 0; a < 3; ++a) g.band[a] = pl->band[a];
   return g;
 }
 
@@ -400,7 +405,7 @@ int run_operator(fh_plan* pl, const T* x, T* y, int kind, const T* coef, const T
                  int* nblocks_out, const T* rvec = nullptr) {
   const int nc = pl->ncomp;
   cpx<T>* spec = reinterpret_cast<cpx<T>*>(pl->spec);
-  const GParams g = make_gparams(pl, 1.0 / (double)pl->n);
+  const GParams g = make_gparams(pl, (kind != fh::COEF_NONE ? pl->op_scale : 1.0) / (double)pl->n);
   RETCHK(launch_rows_fwd<T>(pl, nc, x, coef, kind, spec, halt, rvec));
   if (pl->n0 > 1) {
     RETCHK(launch_cols<T>(pl, 0, 1, nc, spec, g, halt));
@@ -915,6 +920,26 @@ int fh_plan_set_primaldual(fh_plan* pl, int primaldual) {
   return FH_OK;
 }
 
+int fh_plan_set_band(fh_plan* pl, const int* band, double scale) {
+  RETCHK(check_plan(pl));
+  if (!(scale > 0.0) || !std::isfinite(scale)) return fail(FH_ERR_ARG, "scale must be positive");
+  if (!band) {
+    pl->banded = 0;
+    pl->op_scale = 1.0;
+    return FH_OK;
+  }
+  const int off = 3 - pl->dim;
+  int v[3] = {0, 0, 0};
+  for (int a = 0; a < pl->dim; ++a) {
+    if (band[a] < 0) return fail(FH_ERR_ARG, "band must be >= 0");
+    v[a + off] = band[a];
+  }
+  for (int a = 0; a < 3; ++a) pl->band[a] = v[a];
+  pl->banded = 1;
+  pl->op_scale = scale;
+  return FH_OK;
+}
+
 int fh_set_coeff(fh_plan* pl, int kind, const double* host_vals) {
   RETCHK(check_plan(pl));
   if (!host_vals) return fail(FH_ERR_ARG, "null coefficient array");
@@ -1044,7 +1069,7 @@ int fh_cg_solve(fh_plan* pl, int mode, const void* b_, void* x_, double tol, int
     RETCHK(enqueue_true_residual<float>(pl, b, x, Ax, r));
     HIPCHK(hipMemcpyAsync(p, r, (size_t)nv * sizeof(float), hipMemcpyDeviceToDevice, pl->stream));
     // fused iterations where the fixed-length kernels cover the plan (isotropic coefficient)
-    bool fused = pl->fused_cg;
+    bool fused = pl->fused_cg && pl->op_scale == 1.0;  // the fused step assumes an unscaled G
     const int gb = grid_blocks(nv);
     auto fold = [&]() {
       if (fused)
@@ -1292,7 +1317,7 @@ int fh_matvec_profile(fh_plan* pl, int dtype, const void* x, void* y, int nrep, 
     using T = decltype(tag);
     cpx<T>* spec = reinterpret_cast<cpx<T>*>(pl->spec);
     const T* coef = sizeof(T) == 8 ? (const T*)pl->coef64 : (const T*)pl->coef32;
-    const GParams g = make_gparams(pl, 1.0 / (double)pl->n);
+    const GParams g = make_gparams(pl, pl->op_scale / (double)pl->n);
     if (k == 0) return launch_rows_fwd<T>(pl, nc, (const T*)x, coef, pl->coef_kind, spec, nullptr);
     if (k == nk - 1) return launch_rows_inv<T>(pl, nc, spec, (T*)y, T(1), nullptr, nullptr, nullptr, nullptr);
     if (pl->n0 > 1) {
@@ -1614,13 +1639,18 @@ int fh_batch_stats(const fh_batch* b, int* n_updates, double* norm_res, double* 
 
 // ---- Ga post-processing ------------------------------------------------------------------
 
-int fh_ga_create(fh_plan* coarse, fh_ga** out) {
+int fh_ga_create(fh_plan* coarse, const int* grid, fh_ga** out) {
   RETCHK(check_plan(coarse));
   if (!out) return fail(FH_ERR_ARG, "null pointer argument");
   *out = nullptr;
   const int dim = coarse->dim;
   int M[3] = {1, 1, 1};
   for (int a = 0; a < dim; ++a) {
+    if (grid) {
+      M[a] = grid[a];
+      if (M[a] < 2 * coarse->Nu[a] - 1) return fail(FH_ERR_ARG, "quadrature grid smaller than 2N-1");
+      continue;
+    }
     M[a] = quadrature_length(coarse->Nu[a]);
     if (M[a] < 0)
       return fail(FH_ERR_UNSUPPORTED, "Ga quadrature grid: 2N-1 = " + std::to_string(2 * coarse->Nu[a] - 1) +
@@ -1766,6 +1796,31 @@ int fh_ga_set_grid_coeff(fh_ga* ga, int order, const int* P, int kind, const dou
   return rc;
 }
 
+int fh_ga_set_plan_coeff(const fh_ga* ga, fh_plan* pl) {
+  if (!ga || !pl) return fail(FH_ERR_ARG, "null pointer argument");
+  if (!ga->coef_kind) return fail(FH_ERR_STATE, "Ga coefficient not set");
+  const fh_plan* f = ga->fine;
+  if (pl->dim != f->dim || pl->n0 != f->n0 || pl->n1 != f->n1 || pl->n2 != f->n2)
+    return fail(FH_ERR_ARG, "plan grid differs from the Ga quadrature grid");
+  if (pl->ncomp != f->ncomp || (ga->coef_kind == FH_COEF_LAME && pl->physics != FH_ELASTICITY))
+    return fail(FH_ERR_ARG, "plan physics differs from the Ga coefficient");
+  const size_t cnt = (size_t)ga->ncoef * pl->n;
+  if (pl->coef64) (void)hipFree(pl->coef64);
+  if (pl->coef32) (void)hipFree(pl->coef32);
+  pl->coef64 = nullptr;
+  pl->coef32 = nullptr;
+  pl->coef_kind = 0;
+  HIPCHK(hipMalloc(&pl->coef64, cnt * sizeof(double)));
+  HIPCHK(hipMalloc(&pl->coef32, cnt * sizeof(float)));
+  HIPCHK(hipMemcpy(pl->coef64, ga->coef, cnt * sizeof(double), hipMemcpyDeviceToDevice));
+  hipLaunchKernelGGL((fh::k_convert<double, float>), dim3(grid_blocks((long long)cnt)), dim3(256), 0,
+                     pl->stream, (long long)cnt, pl->coef64, pl->coef32);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(pl->stream));
+  pl->coef_kind = ga->coef_kind;
+  return FH_OK;
+}
+
 int fh_ga_coefficient(const fh_ga* ga, double* host_out) {
   if (!ga || !host_out) return fail(FH_ERR_ARG, "null pointer argument");
   if (!ga->coef_kind) return fail(FH_ERR_STATE, "Ga coefficient not set");
@@ -1776,6 +1831,15 @@ int fh_ga_coefficient(const fh_ga* ga, double* host_out) {
 int fh_ga_interpolate(fh_ga* ga, const void* x, void* out) {
   if (!ga || !x || !out) return fail(FH_ERR_ARG, "null pointer argument");
   return ga_interpolate(ga, (const double*)x, (double*)out);
+}
+
+int fh_ga_energy_on_grid(fh_ga* ga, const void* const* fields, int nf, double* AH) {
+  if (!ga || !fields || !AH) return fail(FH_ERR_ARG, "null pointer argument");
+  if (nf != ga->coarse->ncomp) return fail(FH_ERR_ARG, "need one field per unit load (nf == ncomp)");
+  if (!ga->coef_kind) return fail(FH_ERR_STATE, "Ga coefficient not set");
+  for (int i = 0; i < nf; ++i)
+    if (!fields[i]) return fail(FH_ERR_ARG, "null field pointer");
+  return energy_impl(ga->fine, ga->coef, ga->coef_kind, fields, nf, AH);
 }
 
 int fh_ga_energy(fh_ga* ga, const void* const* fields, int nf, double* AH) {

@@ -265,6 +265,7 @@ __device__ __forceinline__ void apply_G_fx(cpx<T> (&v)[NC], int k0, int k1, int 
   const int s0 = signed_freq(k0, g.n0, nyq);
   const int s1 = signed_freq(k1, g.n1, nyq);
   if ((g.n2 & 1) == 0 && 2 * k2 == g.n2) nyq = true;
+  if (g.banded && out_of_band(s0, s1, k2, g)) nyq = true;
   const T x0 = T(s0 * g.iY[0]), x1 = T(s1 * g.iY[1]), x2 = T(k2 * g.iY[2]);
   const T sc = T(g.scale);
   if constexpr (NC == 2 || NC == 3) {

@@ -34,6 +34,8 @@ struct GParams {
   int n0, n1, n2;
   double iY[3];  // 1/Y per axis of the 3-D view
   double scale;  // folded normalisation (1/n for the matvec, 1 for a bare projection)
+  int banded;    // 1: G is zero outside |k_a| <= band[a] (the enlarged projection of a Ga solve)
+  int band[3];   // per view axis
 };
 
 // Map a linear workgroup id so that consecutive tiles run on the same XCD
@@ -52,6 +54,12 @@ __device__ __forceinline__ int signed_freq(int k, int n, bool& nyq) {
   return (2 * k > n) ? k - n : k;
 }
 
+// Frequency outside the band of an enlarged projection: G built on the solution grid N and
+// zero-padded to the Ga grid (hG.enlarge(Nbar), ffthompy/applications.py:28-31).
+__device__ __forceinline__ bool out_of_band(int s0, int s1, int k2, const GParams& g) {
+  return abs(s0) > g.band[0] || abs(s1) > g.band[1] || k2 > g.band[2];
+}
+
 // Apply G(xi) at frequency (k0, k1, k2) to the ncomp values v, times g.scale.
 //  scalar : G1 = xi xi^T / |xi|^2 (ffthompy/projections.py:77-83; closed form
 //           ffthompy/tensors/projection.py:6-31); G2 = I - G1 (projections.py:84-85).
@@ -65,6 +73,7 @@ __device__ __forceinline__ void apply_G(cpx<T>* v, int k0, int k1, int k2, const
   const int s0 = signed_freq(k0, g.n0, nyq);
   const int s1 = signed_freq(k1, g.n1, nyq);
   if ((g.n2 & 1) == 0 && 2 * k2 == g.n2) nyq = true;
+  if (g.banded && out_of_band(s0, s1, k2, g)) nyq = true;
   T xi3[3] = {T(s0 * g.iY[0]), T(s1 * g.iY[1]), T(k2 * g.iY[2])};
   const T* xi = (g.dim == 3) ? xi3 : xi3 + 1;
   const int d = g.dim;

@@ -32,6 +32,10 @@ def ga_assembly_matrix(A_ga, Afun, solutions):
     device. A_ga: materials.GaCoefficient; Afun: the GpuOperator of the solve (its plan
     holds the solutions' grid)."""
     fields = [s.to_device() for s in solutions]
+    if isinstance(Afun, GpuOperator) and Afun.Nsolve is not None:
+        # solutions of a Ga solve: already on the quadrature grid of the solve grid
+        ga = A_ga.load(nat.Ga(Afun.coarse_plan, grid=Afun.N))
+        return ga.energy_matrix([f.val for f in fields], on_grid=True)
     if isinstance(Afun, GpuOperator):
         plan = Afun.plan
     else:
@@ -60,6 +64,8 @@ def postprocess(pb, A, mat, solutions, results, primaldual):
     matrices = {}
     for pp in pb.postprocess:
         if pp['kind'] in ('GaNi', 'gani'):
+            if isinstance(A, GpuOperator) and A.Nsolve is not None:
+                raise NotImplementedError("'GaNi' post-processing of a Ga solve")
             name = 'AH_%s_%s' % (pp['kind'], primaldual)
             AH = assembly_matrix(A, solutions)
             matrices[name] = AH if primaldual == 'primal' else np.linalg.inv(AH)

@@ -227,7 +227,8 @@ class GpuOperator:
     or directly as (kind, values) with kind 'iso', 'lame' or 'full'.
     """
 
-    def __init__(self, N, Y, physics='scalar', primaldual='primal', A=None, coef=None, name='FiGFA'):
+    def __init__(self, N, Y, physics='scalar', primaldual='primal', A=None, coef=None, name='FiGFA',
+                 _defer=False):
         self.name = name
         self.N = tuple(int(n) for n in N)
         self.Y = np.asarray(Y, dtype=np.float64)
@@ -235,13 +236,33 @@ class GpuOperator:
         self.primaldual = primaldual
         self.plan = nat.Plan(self.N, self.Y, physics, primaldual)
         self.ncomp = self.plan.ncomp
+        self.Nsolve = None  # Ga solve: the grid N whose projection is enlarged onto self.N
         if A is not None:
             kind, vals = classify_coefficient(A.val if isinstance(A, Tensor) else A, physics)
             self.plan.set_coeff(kind, vals)
         elif coef is not None:
             self.plan.set_coeff(*coef)
-        else:
+        elif not _defer:
             raise ValueError('GpuOperator needs a coefficient')
+
+    @classmethod
+    def ga(cls, N, Y, physics, primaldual, A_ga, grid=None, name='FiGFA'):
+        """Operator of the Ga solve (ffthompy/applications.py:28-31, 48-51): the projection of
+        the grid N enlarged onto the Ga grid and the exactly integrated coefficient A_ga
+        (materials.GaCoefficient for Nbar = 2N-1). It runs on `grid` (default: 2N-1 per axis, or
+        the next length with prime factors <= 13): the solutions have frequencies below N/2,
+        so every grid of at least 2N-1 points gives the same operator, inner products and
+        energies."""
+        N = tuple(int(n) for n in N)
+        grid = tuple(nat.smooth_length(2 * n - 1) for n in N) if grid is None else tuple(grid)
+        op = cls(grid, Y, physics, primaldual, name=name, _defer=True)
+        op.Nsolve = N
+        op.coarse_plan = nat.Plan(N, Y, physics, primaldual)
+        # the reference's enlarged hG carries the factor prod(2N-1)/prod(N) (fh_plan_set_band)
+        op.plan.set_band([(n - 1) // 2 for n in N], float(np.prod([2 * n - 1 for n in N])) / float(np.prod(N)))
+        ga = A_ga.load(nat.Ga(op.coarse_plan, grid=grid))
+        ga.set_plan_coeff(op.plan)
+        return op
 
     def __call__(self, x):
         x = x.to_device()

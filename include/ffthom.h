@@ -100,6 +100,16 @@ int fh_plan_set_cg_variant(fh_plan* plan, int fused);
  *  FH_COEF_FULL: ncomp*ncomp*n values, entry (i,j) at [(i*ncomp+j)*n + voxel] */
 int fh_set_coeff(fh_plan* plan, int kind, const double* host_vals);
 
+/* Restrict the projection to |k_a| <= band[a] (dim entries; NULL lifts the restriction): G
+ * built on the solution grid N and zero-padded to a finer grid, hG.enlarge(Nbar) of the Ga
+ * solve (ffthompy/applications.py:28-31, ffthompy/tensors/objects.py:428-467), with
+ * band[a] = (N_a - 1) / 2 (even N: its Nyquist mode is already null). scale multiplies the
+ * operator G A (fh_matvec and the CG), not the bare projection (fh_project): the reference's
+ * enlarge converts the 'r'-form hG through set_fft_form (objects.py:135-167), which leaves it
+ * multiplied by prod(Nbar)/prod(N); passing that factor reproduces its Afun, residual norms and
+ * iteration counts (the solutions do not depend on it). */
+int fh_plan_set_band(fh_plan* plan, const int* band, double scale);
+
 /* y = F^-1 G(xi) F (A(x) x): the matvec Afun(x) (ffthompy/tensors/operators.py:136-144)
  * in fp64 or fp32 arithmetic (dtype of both fields). */
 int fh_matvec(fh_plan* plan, int dtype, const void* x, void* y);
@@ -224,8 +234,11 @@ int fh_batch_stats(const fh_batch* batch, int* n_updates, double* norm_res, doub
  * The quadrature grid is Nbar where its prime factors are <= 13; for odd N with a larger prime
  * in 2N-1 it is the next such length (the integrand is a trigonometric polynomial of degree
  * <= 2N-2 per axis, so every grid of at least 2N-1 points integrates it exactly); for even N
- * without such an Nbar the call fails with FH_ERR_UNSUPPORTED. */
-int fh_ga_create(fh_plan* coarse, fh_ga** out);
+ * without such an Nbar the call fails with FH_ERR_UNSUPPORTED. grid (nullable, dim entries):
+ * an explicit quadrature grid of at least 2N-1 points per axis, exact for fields with
+ * frequencies below N/2 (the solutions of a Ga solve); for GaNi solutions of even N only
+ * 2N-1 reproduces the reference. */
+int fh_ga_create(fh_plan* coarse, const int* grid, fh_ga** out);
 int fh_ga_destroy(fh_ga* ga);
 /* Quadrature grid actually used, dim entries. */
 int fh_ga_grid(const fh_ga* ga, int* M);
@@ -241,6 +254,9 @@ int fh_ga_set_inclusions(fh_ga* ga, int nincl, const int* kinds, const double* p
  * materials.py:74-113): vals host fp64, component-major, each a P-grid array in origin-0
  * voxel order (dual: the pointwise inverse), layout kind as above. */
 int fh_ga_set_grid_coeff(fh_ga* ga, int order, const int* P, int kind, const double* vals);
+/* Copy A_Ga into a plan on the quadrature grid (same physics) as its coefficient: the
+ * operator of a Ga solve (Material.get_A_Ga(Nbar), ffthompy/applications.py:48-51). */
+int fh_ga_set_plan_coeff(const fh_ga* ga, fh_plan* plan);
 /* A_Ga on the quadrature grid to host (ncoef arrays of prod(M) values, origin 0). */
 int fh_ga_coefficient(const fh_ga* ga, double* host_out);
 /* P(x) on the quadrature grid: x a device fp64 field of the coarse plan (ncomp components),
@@ -248,6 +264,9 @@ int fh_ga_coefficient(const fh_ga* ga, double* host_out);
 int fh_ga_interpolate(fh_ga* ga, const void* x, void* out);
 /* AH[i][j] = <A_Ga P(e_i), P(e_j)> for nf = ncomp fp64 device fields of the coarse plan. */
 int fh_ga_energy(fh_ga* ga, const void* const* fields, int nf, double* AH);
+/* The same for fields already on the quadrature grid (solutions of a Ga solve on that grid):
+ * assembly_matrix without projection (ffthompy/postprocess.py:53-70). */
+int fh_ga_energy_on_grid(fh_ga* ga, const void* const* fields, int nf, double* AH);
 
 /* Wait for all GPU work of this process. */
 int fh_synchronize(void);

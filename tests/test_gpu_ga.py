@@ -183,3 +183,57 @@ def test_ga_even_grid_without_smooth_nbar_is_rejected():
     with pytest.raises(nat.FhError) as ei:
         nat.Ga(plan)
     assert ei.value.code == nat.FH_ERR_UNSUPPORTED
+
+
+GA_SOLVES = [c for c in examples.problems() if c[2]['solve']['kind'] == 'Ga']
+
+
+@pytest.mark.parametrize('case', GA_SOLVES, ids=['%s_%s' % (c[0], c[1]) for c in GA_SOLVES])
+def test_ga_solve_goldens(case):
+    """Ga solves (enlarged projection, exactly integrated coefficient) of the example problems:
+    golden bounds at 1e-9, golden CG iteration counts, and the oracle."""
+    f, name, conf, mats = case
+    print('case ga solve golden', f, name, flush=True)
+    pb = Problem(dict(conf), type('conf', (), {'materials': mats()}))
+    pb.calculate()
+    ref = OracleProblem(dict(conf), mats())
+    ref.calculate()
+    gold = GOLD['problems']['%s_%s' % (f, name)]
+    checked = 0
+    for pd in pb.solve['primaldual']:
+        kits = [r['info']['kit'] for r in pb.output['res_' + pd]]
+        assert kits == gold['kit'][pd], (pd, kits)
+        for key, AH in pb.output['mat_' + pd].items():
+            if key in gold.get('diag', {}):
+                assert np.abs(AH - gold['diag'][key] * np.eye(AH.shape[0])).max() < 1e-9, key
+                checked += 1
+            for ij, v in gold.get('entries', {}).get(key, {}).items():
+                i, j = map(int, ij.split(','))
+                assert abs(AH[i, j] - v) < 1e-9, (key, ij)
+                checked += 1
+            assert np.abs(AH - ref.output['mat_' + pd][key]).max() < 1e-10 * max(1., np.abs(AH).max()), key
+    assert checked >= 1
+
+
+@pytest.mark.parametrize('N', [(45, 45), (10, 10), (6, 8), (15, 15, 15)])
+def test_ga_solve_on_quadrature_grid_matches_oracle(N):
+    """2N-1 = 89, 19 and 29 are prime: the device solves on 90, 20 and 30 points per axis, the
+    oracle on 89, 19 and 29; the iterations and bounds agree (the iterates are the same
+    trigonometric polynomials)."""
+    print('case ga solve grid', N, flush=True)
+    dim = len(N)
+    mats = examples.scalar_3d_materials if dim == 3 else examples.scalar_2d_materials
+    conf = {'name': 'ga', 'physics': 'scalar', 'material': 'cube' if dim == 3 else 'square',
+            'solve': {'kind': 'Ga', 'N': np.array(N), 'primaldual': ['primal', 'dual']},
+            'postprocess': [{'kind': 'Ga'}], 'solver': {'kind': 'CG', 'tol': 1e-8, 'maxiter': 1e3}}
+    pb = Problem(dict(conf), type('conf', (), {'materials': mats()}))
+    pb.calculate()
+    ref = OracleProblem(dict(conf), mats())
+    ref.calculate()
+    for pd in ('primal', 'dual'):
+        AH = pb.output['mat_' + pd]['AH_Ga_' + pd]
+        ARef = ref.output['mat_' + pd]['AH_Ga_' + pd]
+        assert np.abs(AH - ARef).max() < 1e-10 * np.abs(ARef).max(), pd
+        kits = [r['info']['kit'] for r in pb.output['res_' + pd]]
+        rkits = [r['info']['kit'] for r in ref.output['res_' + pd]]
+        assert kits == rkits, (pd, kits, rkits)
