@@ -242,6 +242,46 @@ def bench_grad(args, nat, ms, dev):
     print(json.dumps(out), flush=True)
 
 
+def bench_ga(args, nat, ms, dev):
+    """243^3 spheres (contrast 100): GaNi primal+dual solves (mixed precision, tol 1e-8), then Ga
+    post-processing of the voxel coefficient (order 0, P = N: each voxel a constant box,
+    exactly integrated) on the 2N-1 = 485 grid, run on the 486^3 quadrature grid: guaranteed,
+    separated bounds of the voxel microstructure."""
+    N = (args.n,) * 3
+    a, centres, radii = ms.spheres_conductivity(N, count=30, contrast=100., seed=ms.BASE_SEED)
+    H = Homogenizer(nat, a)
+    r = H.step('mixed', delta=args.delta)
+    log('ga243: GaNi solves done, kit %s' % r['kit'])
+    res, times = {}, []
+    for rep in range(args.steps + 1):
+        t0 = time.perf_counter()
+        for pd, coef in (('primal', a), ('dual', 1. / a)):
+            ga = nat.Ga(H.plans[pd])
+            ga.set_grid_coeff(0, N, 'iso', coef)
+            AH = ga.energy_matrix(H.sols[pd])
+            res[pd] = AH if pd == 'primal' else np.linalg.inv(AH)
+            M = ga.M
+            del ga
+        nat.synchronize()
+        if rep:
+            times.append(time.perf_counter() - t0)
+        log('ga243: Ga post-processing %.3f s' % (time.perf_counter() - t0))
+    up, lo = res['primal'], res['dual']
+    out = {'metric': 'time of the Ga post-processing (primal + dual), 243^3 spheres (s)',
+           'value': round(float(np.median(times)), 4), 'unit': 's', 'n_gpus': 1, 'higher_is_better': False,
+           'dtype': 'f64', 'data': 'synthetic (seeded spheres, seed %d)' % ms.BASE_SEED,
+           'config': {'workload': 'Ga post-processing (order 0, P = N) of the GaNi solutions of 3-D scalar '
+                                  '%d^3, 30 spheres contrast 100, loads e1-e3; quadrature grid %s (2N-1 = %d)'
+                                  % (args.n, list(M), 2 * args.n - 1)},
+           'AH_Ga_upper_diag': np.round(np.diag(up), 8).tolist(),
+           'AH_Ga_lower_diag': np.round(np.diag(lo), 8).tolist(),
+           'AH_GaNi_upper_diag': np.round(np.diag(r['AH_primal']), 8).tolist(),
+           'AH_GaNi_lower_diag': np.round(np.diag(r['AH_dual']), 8).tolist(),
+           'min_gap_upper_minus_lower': float(np.linalg.eigvalsh(0.5 * ((up - lo) + (up - lo).T)).min()),
+           'kit_gani_mixed': r['kit']}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument('--gpus', type=int, default=1)
@@ -256,7 +296,7 @@ def main():
     ap.add_argument('--profile-only', action='store_true', help='matvec profile only (for rocprofv3 runs)')
     ap.add_argument('--profile-solve', action='store_true',
                     help='with --profile-only: also one mixed and one fp64 CG solve (load e1, primal)')
-    ap.add_argument('--workload', default='scalar243', choices=['scalar243', 'uq512', 'elastic81', 'grad99'],
+    ap.add_argument('--workload', default='scalar243', choices=['scalar243', 'uq512', 'elastic81', 'grad99', 'ga243'],
                     help='scalar243: the headline (default); the others print their own JSON line')
     args = ap.parse_args()
     world = int(os.environ.get('WORLD_SIZE', '1'))
@@ -269,7 +309,8 @@ def main():
     from ffthompy_amd import microstructure as ms
     dev = nat.init()
     if args.workload != 'scalar243':
-        {'uq512': bench_uq, 'elastic81': bench_elastic, 'grad99': bench_grad}[args.workload](args, nat, ms, dev)
+        {'uq512': bench_uq, 'elastic81': bench_elastic, 'grad99': bench_grad,
+         'ga243': bench_ga}[args.workload](args, nat, ms, dev)
         return
     N = (args.n,) * 3
     n = int(np.prod(N))
