@@ -16,6 +16,7 @@ FH_PRIMAL, FH_DUAL = 0, 1
 FH_F64, FH_F32 = 0, 1
 FH_SOLVE_FP64, FH_SOLVE_MIXED = 0, 1
 FH_COEF_ISO, FH_COEF_LAME, FH_COEF_FULL = 1, 2, 3
+FH_ITER_RICHARDSON, FH_ITER_CHEBYSHEV = 1, 2
 FH_CG_OK, FH_CG_BREAKDOWN = 0, 1
 
 _c_int_p = ctypes.POINTER(ctypes.c_int)
@@ -76,6 +77,8 @@ SIGNATURES = {
     'fh_ga_coefficient': [_vp, _c_double_p],
     'fh_ga_interpolate': [_vp, _vp, _vp],
     'fh_ga_energy': [_vp, ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, _c_double_p],
+    'fh_iter_solve': [_vp, ctypes.c_int, _vp, _vp, _c_double_p, ctypes.c_double, ctypes.c_int, _c_int_p, _c_double_p,
+                      _c_double_p, ctypes.c_int],
     'fh_synchronize': [],
 }
 
@@ -336,6 +339,21 @@ class Plan:
         k = kit.value
         return {'kit': k, 'norm_res': nr.value, 'status': st.value,
                 'res_log': hist[:min(hist_len, k + 1)].copy() if hist_len > 0 else None}
+
+    def iter_solve(self, method, b, x, coefs, tol, maxiter, hist_len=0):
+        """Richardson (coefs = [alpha]) or Chebyshev (coefs = eigenvalue range) on the device."""
+        code = {'richardson': FH_ITER_RICHARDSON, 'chebyshev': FH_ITER_CHEBYSHEV}[method]
+        c = np.ascontiguousarray(np.asarray(coefs, dtype=np.float64).reshape(-1))
+        kit = ctypes.c_int()
+        nres = ctypes.c_double()
+        hist = np.zeros(max(hist_len, 1))
+        call('fh_iter_solve', self._h, ctypes.c_int(code), b.ptr, x.ptr, c.ctypes.data_as(_c_double_p),
+             ctypes.c_double(tol), ctypes.c_int(int(maxiter)), ctypes.byref(kit), ctypes.byref(nres),
+             hist.ctypes.data_as(_c_double_p) if hist_len > 0 else None, ctypes.c_int(hist_len))
+        out = {'kit': kit.value, 'norm_res': nres.value}
+        if hist_len > 0:
+            out['res_log'] = hist[:min(hist_len, kit.value + 1)].copy()
+        return out
 
     def energy_matrix(self, fields):
         nf = len(fields)

@@ -15,6 +15,7 @@
 #include "fh_launch.hpp"
 #include "fh_batch.hpp"
 #include "fh_ga.hpp"
+#include "fh_iter.hpp"
 
 using fh::AxisPlan;
 using fh::CGState;
@@ -1864,6 +1865,137 @@ int fh_ga_energy(fh_ga* ga, const void* const* fields, int nf, double* AH) {
     ptr[i] = ga->fields + i * per;
   }
   return energy_impl(ga->fine, ga->coef, ga->coef_kind, ptr.data(), nf, AH);
+}
+
+
+// ---- Richardson and Chebyshev iterations ---------------------------------------------------
+
+int fh_iter_solve(fh_plan* pl, int method, const void* b_, void* x_, const double* coefs, double tol, int maxiter,
+                  int* kit, double* norm_res, double* res_hist, int hist_len) {
+  RETCHK(check_plan(pl));
+  if (!b_ || !x_ || !coefs) return fail(FH_ERR_ARG, "null pointer argument");
+  if (method != FH_ITER_RICHARDSON && method != FH_ITER_CHEBYSHEV) return fail(FH_ERR_ARG, "bad method");
+  if (!pl->coef_kind) return fail(FH_ERR_STATE, "fh_set_coeff has not been called");
+  if (!(tol >= 0.0) || maxiter < 0) return fail(FH_ERR_ARG, "bad tol/maxiter");
+  const double* b = (const double*)b_;
+  double* x = (double*)x_;
+  const long long nv = (long long)pl->ncomp * pl->n;
+  const long long nvp = (nv + 63) / 64 * 64;
+  RETCHK(ensure_work(pl, 3 * (size_t)nvp * sizeof(double)));
+  double* Ax = (double*)pl->work;
+  double* r = Ax + nvp;
+  double* v = r + nvp;
+  RETCHK(ensure_hist(pl, std::max(hist_len, 1)));
+  fh::it::IterState s;
+  std::memset(&s, 0, sizeof(s));
+  s.tol = tol;
+  s.inv_nvox = 1.0 / (double)pl->n;
+  s.maxiter = maxiter;
+  s.hist_len = hist_len > 0 ? hist_len : 0;
+  s.hist = pl->hist;
+  s.norm = 1e15;  // the reference's initial value (richardson, solver.py:65)
+  fh::it::IterState* st = nullptr;
+  HIPCHK(hipMalloc(&st, sizeof(s)));
+  int rc = FH_OK;
+  double norm_init = s.norm;  // before the first iteration: 1e15 (Richardson) or |r0| / |b| (Chebyshev)
+  auto run = [&]() -> int {
+    const int gb = grid_blocks(nv);
+    int* halt = &st->halt;
+    const int per_check = 8;
+    if (method == FH_ITER_RICHARDSON) {
+      if (!(coefs[0] != 0.0)) return fail(FH_ERR_ARG, "alpha must be nonzero");
+      const double omega = 1.0 / coefs[0];
+      s.done = s.halt = maxiter == 0;
+      HIPCHK(hipMemcpy(st, &s, sizeof(s), hipMemcpyHostToDevice));
+      for (int k = 0; !s.done && k < maxiter;) {
+        for (int j = 0; j < per_check && k < maxiter; ++j, ++k) {
+          RETCHK(run_operator<double>(pl, x, Ax, pl->coef_kind, pl->coef64, nullptr, halt, nullptr));
+          hipLaunchKernelGGL(fh::it::k_it_residual, dim3(gb), dim3(256), 0, pl->stream, nv, b, Ax, r, pl->partial,
+                             halt);
+          hipLaunchKernelGGL(fh::it::k_it_richardson, dim3(gb), dim3(256), 0, pl->stream, nv, x, r, omega, halt);
+          hipLaunchKernelGGL(fh::it::k_it_finish, dim3(1), dim3(256), 0, pl->stream, st, pl->partial, gb);
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(&s, st, sizeof(s), hipMemcpyDeviceToHost, pl->stream));
+        HIPCHK(hipStreamSynchronize(pl->stream));
+      }
+      return FH_OK;
+    }
+    // Chebyshev: initial residual, its norm r0 and the norm of b (solver.py:250-262)
+    const double e0 = coefs[0], e1 = coefs[1];
+    if (!(e1 > e0) || !(e0 > 0.0)) return fail(FH_ERR_ARG, "eigenvalue range must satisfy 0 < min < max");
+    s.halt = 0;
+    HIPCHK(hipMemcpy(st, &s, sizeof(s), hipMemcpyHostToDevice));
+    RETCHK(run_operator<double>(pl, x, Ax, pl->coef_kind, pl->coef64, nullptr, nullptr, nullptr));
+    hipLaunchKernelGGL(fh::it::k_it_residual, dim3(gb), dim3(256), 0, pl->stream, nv, b, Ax, r, pl->partial, halt);
+    HIPCHK(hipGetLastError());
+    std::vector<double> part(gb);
+    HIPCHK(hipMemcpyAsync(part.data(), pl->partial, gb * sizeof(double), hipMemcpyDeviceToHost, pl->stream));
+    HIPCHK(hipStreamSynchronize(pl->stream));
+    double rr = 0.0;
+    for (double q : part) rr += q;
+    const double r0 = std::sqrt(rr * s.inv_nvox);
+    hipLaunchKernelGGL((fh::k_residual<double>), dim3(gb), dim3(256), 0, pl->stream, nv, b, nullptr, v, pl->partial);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(part.data(), pl->partial, gb * sizeof(double), hipMemcpyDeviceToHost, pl->stream));
+    HIPCHK(hipStreamSynchronize(pl->stream));
+    double bb = 0.0;
+    for (double q : part) bb += q;
+    const double bn = std::sqrt(bb * s.inv_nvox);
+    s.norm = r0 / bn;  // the reference's Ib * r0
+    norm_init = s.norm;
+    s.r0 = r0;
+    s.relative = 1;
+    if (s.norm < tol || maxiter == 0) {
+      s.done = 1;
+      if (maxiter == 0 && !(s.norm < tol)) s.norm = 0.0;  // kit == 0 at the end: norm_res = 0 (solver.py:282-283)
+      return FH_OK;
+    }
+    HIPCHK(hipMemsetAsync(v, 0, (size_t)nv * sizeof(double), pl->stream));
+    HIPCHK(hipMemcpyAsync(st, &s, sizeof(s), hipMemcpyHostToDevice, pl->stream));
+    const double d = (e1 + e0) / 2.0, c = (e1 - e0) / 2.0;
+    double w = 0.0;
+    for (int k = 1; !s.done && k <= maxiter;) {
+      for (int j = 0; j < per_check && k <= maxiter; ++j, ++k) {
+        double pk;
+        if (k == 1) {
+          pk = 0.0;
+          w = 1.0 / d;
+        } else if (k == 2) {
+          pk = -(1.0 / 2.0) * (c / d) * (c / d);
+          w = 1.0 / (d - c * c / 2.0 / d);
+        } else {
+          pk = -(c * c / 4.0) * w * w;
+          w = 1.0 / (d - c * c * w / 4.0);
+        }
+        hipLaunchKernelGGL(fh::it::k_it_chebyshev, dim3(gb), dim3(256), 0, pl->stream, nv, x, v, r, pk, w, halt);
+        RETCHK(run_operator<double>(pl, x, Ax, pl->coef_kind, pl->coef64, nullptr, halt, nullptr));
+        hipLaunchKernelGGL(fh::it::k_it_residual, dim3(gb), dim3(256), 0, pl->stream, nv, b, Ax, r, pl->partial,
+                           halt);
+        hipLaunchKernelGGL(fh::it::k_it_finish, dim3(1), dim3(256), 0, pl->stream, st, pl->partial, gb);
+      }
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipMemcpyAsync(&s, st, sizeof(s), hipMemcpyDeviceToHost, pl->stream));
+      HIPCHK(hipStreamSynchronize(pl->stream));
+    }
+    return FH_OK;
+  };
+  rc = run();
+  if (rc == FH_OK) {
+    hipError_t e = hipStreamSynchronize(pl->stream);
+    if (e == hipSuccess && res_hist && hist_len > 0) {
+      const int cnt = std::min(hist_len, s.kit + 1);
+      if (cnt > 1) e = hipMemcpy(res_hist + 1, pl->hist + 1, (size_t)(cnt - 1) * sizeof(double), hipMemcpyDeviceToHost);
+      res_hist[0] = norm_init;
+    }
+    if (e != hipSuccess) rc = fail(FH_ERR_HIP, std::string("iterative solve: ") + hipGetErrorString(e));
+  }
+  (void)hipFree(st);
+  if (rc == FH_OK) {
+    if (kit) *kit = s.kit;
+    if (norm_res) *norm_res = s.norm;
+  }
+  return rc;
 }
 
 }  // extern "C"

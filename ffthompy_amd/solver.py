@@ -1,6 +1,7 @@
-"""Linear solvers on the GPU operator: linear_solver, CG and the residual callback.
+"""Linear solvers on the GPU operator: linear_solver, CG, Richardson, Chebyshev, SciPy's CG and
+the residual callback.
 
-Mirrors ffthompy/general/solver.py:8-139 and ffthompy/general/solver_pp.py:6-34. With a
+Mirrors ffthompy/general/solver.py:8-285 and ffthompy/general/solver_pp.py:6-34. With a
 GpuOperator the whole CG loop runs on the device (fh_cg_solve): same algorithm, same
 absolute stop sqrt(<r,r>) <= tol on the grid-mean inner product, x0 honoured, info with
 'kit', 'norm_res' and 'time'. Extra keys of ``par`` select the arithmetic:
@@ -43,16 +44,87 @@ class CallBack:
 
 
 def linear_solver(Afun, B, ATfun=None, x0=None, par=None, solver=None, callback=None):
-    """Solve Afun x = B (solver.py:8-60). Only 'CG' on a GpuOperator is provided."""
+    """Solve Afun x = B (solver.py:8-60): 'CG', 'richardson'/'iterative' and 'chebyshev'/'cheby' run
+    on the device; 'scipy_cg' ('scipy.sparse.linalg.cg') runs SciPy's CG on the host with the
+    GPU matvec as its LinearOperator, as the reference does. 'BiCG' is not provided (the
+    reference's BiCG needs matrix operations that Tensor lacks, solver.py:174-184)."""
     par = {} if par is None else par
-    if solver is None or solver.lower() != 'cg':
-        raise NotImplementedError("This kind (%s) of linear solver is not implemented" % solver)
     if not isinstance(Afun, GpuOperator):
         raise TypeError('linear_solver runs on a GpuOperator (the HIP matvec)')
+    if solver is None:
+        raise NotImplementedError("This kind (%s) of linear solver is not implemented" % solver)
     t0 = time.perf_counter()
-    x, info = CG(Afun, B, x0=x0, par=par, callback=callback)
+    kind = solver.lower()
+    if kind == 'cg':
+        x, info = CG(Afun, B, x0=x0, par=par, callback=callback)
+    elif kind in ('iterative', 'richardson'):
+        x, info = richardson(Afun, B, x0, par=par, callback=callback)
+    elif kind in ('chebyshev', 'cheby'):
+        x, info = cheby2TERM(Afun, B, x0=x0, par=par, callback=callback)
+    elif solver in ('scipy.sparse.linalg.cg', 'scipy_cg'):
+        x, info = scipy_cg(Afun, B, x0=x0, par=par, callback=callback)
+    else:
+        raise NotImplementedError("This kind (%s) of linear solver is not implemented" % solver)
     info['time'] = time.perf_counter() - t0
     return x, info
+
+
+def _iterate(method, Afun, B, x0, coefs, tol, maxiter, callback):
+    B = B.to_device()
+    x = B.zeros_like(name='x') if x0 is None else x0.to_device().copy(name='x')
+    hist_len = int(maxiter) + 1 if isinstance(callback, CallBack) else 0
+    out = Afun.plan.iter_solve(method, B.val, x.val, coefs, float(tol), int(maxiter), hist_len=hist_len)
+    info = {'kit': out['kit'], 'norm_res': out['norm_res']}
+    if callback is not None:
+        if isinstance(callback, CallBack):
+            callback.record(out['res_log'])
+        else:
+            callback(x)
+    return x, info
+
+
+def richardson(Afun, B, x0=None, par=None, callback=None):
+    """x <- x + (B - A x)/alpha until sqrt(<r,r>) <= tol (solver.py:63-77), on the device.
+    par: 'alpha', 'tol', 'maxiter'. CallBack records the norm of the residual taken before
+    each step (the quantity the reference tests)."""
+    return _iterate('richardson', Afun, B, x0, [par['alpha']], par['tol'], par['maxiter'], callback)
+
+
+def cheby2TERM(Afun, B, x0=None, par=None, callback=None):
+    """Two-term Chebyshev iteration (solver.py:206-285) on the device. par: 'eigrange' (required),
+    'tol' (default 1e-6), 'maxit' (default 1e7); norm_res is relative to the initial residual."""
+    par = {} if par is None else par
+    par.setdefault('tol', 1e-06)
+    par.setdefault('maxit', 1e7)
+    if 'eigrange' not in par:
+        raise NotImplementedError("It is necessary to calculate eigenvalues.")
+    maxit = int(min(par['maxit'], 2 ** 31 - 1))
+    return _iterate('chebyshev', Afun, B, x0, par['eigrange'], par['tol'], maxit, callback)
+
+
+def scipy_cg(Afun, B, x0=None, par=None, callback=None):
+    """scipy.sparse.linalg.cg with the GPU operator (solver.py:28-44): SciPy iterates on the host,
+    every matvec round-trips through the device. Its stop rule is SciPy's (relative residual)."""
+    import scipy.sparse.linalg as spslin
+    shape = B.val.shape
+    n = int(np.prod(shape))
+    dx = nat.DeviceArray(shape)
+    dy = nat.DeviceArray(shape)
+
+    def mv(v):
+        h = np.ascontiguousarray(np.asarray(v, dtype=np.float64).reshape(shape))
+        nat.call('fh_memcpy_htod', dx.ptr, h.ctypes.data, h.nbytes)
+        Afun.plan.matvec(dx, dy)
+        return dy.to_host().reshape(-1)
+
+    op = spslin.LinearOperator((n, n), matvec=mv, dtype=np.float64)
+    b = B.to_host().val.reshape(-1)
+    x0v = np.zeros(n) if x0 is None else x0.to_host().val.reshape(-1)
+    xcol, code = spslin.cg(op, b, x0=x0v, rtol=par['tol'], maxiter=par['maxiter'], M=None,
+                           callback=None if isinstance(callback, CallBack) else callback)
+    x = B.zeros_like(name='x').to_host()
+    x.val = xcol.reshape(shape)
+    return x.to_device(), {'info': code}
 
 
 def CG(Afun, B, x0=None, par=None, callback=None):

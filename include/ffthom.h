@@ -42,6 +42,7 @@ enum { FH_PRIMAL = 0, FH_DUAL = 1 };                 /* ffthompy/applications.py
 enum { FH_F64 = 0, FH_F32 = 1 };                     /* element type of a device field */
 enum { FH_SOLVE_FP64 = 0, FH_SOLVE_MIXED = 1 };      /* CG arithmetic */
 enum { FH_COEF_ISO = 1, FH_COEF_LAME = 2, FH_COEF_FULL = 3 };
+enum { FH_ITER_RICHARDSON = 1, FH_ITER_CHEBYSHEV = 2 };
 enum { FH_CG_OK = 0, FH_CG_BREAKDOWN = 1 };
 
 typedef struct fh_plan fh_plan;
@@ -142,6 +143,18 @@ int fh_apply_projection_hat(fh_plan* plan, int dtype, const void* in_hat, void* 
 int fh_cg_solve(fh_plan* plan, int mode, const void* b, void* x, double tol, int maxiter,
                 double delta, int* kit, double* norm_res, double* res_hist, int hist_len,
                 int* cg_status);
+
+/* Stationary iterations on Afun x = b, fp64, on the device (x holds x0 on entry):
+ *  FH_ITER_RICHARDSON (richardson, ffthompy/general/solver.py:63-77): coefs[0] = alpha,
+ *    x <- x + (b - A x) / alpha; stops once sqrt(<r,r>) <= tol for the residual r = b - A x taken
+ *    before that step, or after maxiter steps; *norm_res is that norm (1e15 if no step ran).
+ *  FH_ITER_CHEBYSHEV (cheby2TERM, solver.py:206-285): coefs[0..1] = eigenvalue range [min, max] of
+ *    Afun on the range of G; two-term recurrence with the reference's coefficients; returns at once
+ *    (kit 0) if |b - A x0| / |b| < tol, else stops once |r| / |r0| <= tol or after maxiter
+ *    iterations (the reference's par['maxit']); *norm_res is that relative norm.
+ * res_hist (nullable): entry 0 the value before the first iteration, entry k after iteration k. */
+int fh_iter_solve(fh_plan* plan, int method, const void* b, void* x, const double* coefs, double tol,
+                  int maxiter, int* kit, double* norm_res, double* res_hist, int hist_len);
 
 /* AH[i][j] = <A e_i, e_j> for nf = ncomp fields e_i (fp64, macroscopic part included),
  * double-double accumulated: assembly_matrix (ffthompy/postprocess.py:53-70). */

@@ -150,6 +150,61 @@ def test_richardson_matches_cg():
     assert np.linalg.norm(X.val - x.val) < 1e-8
 
 
+def _diagonal_operator(rng, N, e0, e1):
+    lam = e0 + (e1 - e0) * rng.random((1,) + N)
+    lam[0, 0, 0], lam[0, 0, 1] = e0, e1
+    return lam, (lambda x: x.copy(val=lam * x.val))
+
+
+def test_chebyshev_residual_is_the_chebyshev_polynomial():
+    """Textbook: after k steps of the two-term Chebyshev iteration on [e0, e1] the residual is
+    T_k((d - A)/c) / T_k(d/c) r0, d, c the centre and half-width (pins the (p, w) sequence)."""
+    from oracle.solver import cheby2TERM
+    rng = np.random.default_rng(0)
+    N, e0, e1 = (7, 7), 1., 10.
+    lam, A = _diagonal_operator(rng, N, e0, e1)
+    B = Field(val=rng.standard_normal((1,) + N), order=1)
+    d, c = (e0 + e1) / 2, (e1 - e0) / 2
+    for k in range(1, 9):
+        x, res = cheby2TERM(A, B, B.zeros_like(), par={'eigrange': [e0, e1], 'tol': 0., 'maxit': k})
+        Tk = np.polynomial.chebyshev.Chebyshev.basis(k)
+        pred = Tk((d - lam) / c) / Tk(d / c) * B.val
+        assert res['kit'] == k
+        assert np.abs((B.val - lam * x.val) - pred).max() < 1e-13
+
+
+def test_richardson_error_is_a_power_of_the_iteration_matrix():
+    """x_k - x = (I - A/alpha)^k (x_0 - x) on a diagonal operator."""
+    from oracle.solver import richardson
+    rng = np.random.default_rng(1)
+    N = (6, 5)
+    lam, A = _diagonal_operator(rng, N, 1., 4.)
+    B = Field(val=rng.standard_normal((1,) + N), order=1)
+    xs = B.val / lam
+    for k in (1, 2, 5):
+        x, res = richardson(A, B, B.zeros_like(), par={'alpha': 3., 'tol': 0., 'maxiter': k})
+        assert res['kit'] == k
+        assert np.abs((x.val - xs) - (1. - lam / 3.) ** k * (0. - xs)).max() < 1e-13
+
+
+def test_chebyshev_matches_cg():
+    """ffthompy/general/unittest_solver.py:35-76 (CG vs chebyshev, eigenvalue range of a)."""
+    rng = np.random.default_rng(4)
+    dim, N = 2, (5, 5)
+    _, G1, _ = projections.scalar(N, np.ones(dim))
+    GN = Operator([[DFT(N, inverse=True), G1, DFT(N)]])
+    A = Field(val=np.einsum('ij,...->ij...', np.eye(dim), 1. + 10. * rng.random(N)), order=2, multype=21)
+    Afun = Operator([[GN, A]])
+    E = Field(shape=(dim,), N=N)
+    E.set_mean(np.array([1., 0.]))
+    B = Afun(-E)
+    par = {'tol': 1e-10, 'maxiter': 10000, 'eigrange': [1., 11.]}
+    X, _ = linear_solver(Afun, B, x0=B.zeros_like(), par=dict(par), solver='CG')
+    x, info = linear_solver(Afun, B, x0=B.zeros_like(), par=dict(par), solver='chebyshev')
+    assert info['norm_res'] <= 1e-10
+    assert np.linalg.norm(X.val - x.val) < 1e-8
+
+
 def _scalar_problem(material, N, pd=('primal', 'dual')):
     return Problem({'physics': 'scalar', 'material': material,
                     'solve': {'kind': 'GaNi', 'N': np.array(N), 'primaldual': list(pd)},
