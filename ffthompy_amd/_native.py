@@ -79,6 +79,10 @@ SIGNATURES = {
     'fh_ga_energy': [_vp, ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, _c_double_p],
     'fh_iter_solve': [_vp, ctypes.c_int, _vp, _vp, _c_double_p, ctypes.c_double, ctypes.c_int, _c_int_p, _c_double_p,
                       _c_double_p, ctypes.c_int],
+    'fh_set_coeff_inclusions': [_vp, ctypes.c_int, _c_int_p, _c_double_p, _c_double_p, ctypes.c_int, _c_double_p,
+                                ctypes.c_int],
+    'fh_coeff_to_host': [_vp, _c_double_p],
+    'fh_grf': [_vp, _vp, ctypes.c_double, ctypes.c_double, ctypes.c_int, _vp],
     'fh_synchronize': [],
 }
 
@@ -305,6 +309,36 @@ class Plan:
         code = {'iso': FH_COEF_ISO, 'lame': FH_COEF_LAME, 'full': FH_COEF_FULL}[kind]
         call('fh_set_coeff', self._h, ctypes.c_int(code), vals.ctypes.data_as(_c_double_p))
         self.coef_kind = kind
+
+    def set_coeff_inclusions(self, kinds, params, positions, kind, vals, invert=False):
+        """Nodal coefficient of an inclusion material evaluated on the device (see
+        Ga.set_inclusions for the arguments)."""
+        n = len(kinds)
+        codes = np.ascontiguousarray([GA_INCLUSION_CODES[k] for k in kinds], dtype=np.int32)
+        P = np.zeros((n, 3))
+        X = np.zeros((n, 3))
+        P[:, :self.dim] = np.asarray(params, dtype=np.float64).reshape(n, self.dim)
+        X[:, :self.dim] = np.asarray(positions, dtype=np.float64).reshape(n, self.dim)
+        ncoef = {'iso': 1, 'lame': 2, 'full': self.ncomp ** 2}[kind]
+        V = np.ascontiguousarray(np.asarray(vals, dtype=np.float64).reshape(n, ncoef))
+        call('fh_set_coeff_inclusions', self._h, ctypes.c_int(n), codes.ctypes.data_as(_c_int_p),
+             np.ascontiguousarray(P).ctypes.data_as(_c_double_p), np.ascontiguousarray(X).ctypes.data_as(_c_double_p),
+             ctypes.c_int(_COEF_CODES[kind]), V.ctypes.data_as(_c_double_p), ctypes.c_int(1 if invert else 0))
+        self.coef_kind = kind
+
+    def coeff_to_host(self):
+        ncoef = {'iso': 1, 'lame': 2, 'full': self.ncomp ** 2}[self.coef_kind]
+        out = np.zeros((ncoef,) + self.N)
+        call('fh_coeff_to_host', self._h, out.ctypes.data_as(_c_double_p))
+        return out
+
+    def grf(self, white, corr_len, std, exponentiate=False, out=None):
+        """Gaussian random field on this plan's grid from device white noise (fh_grf)."""
+        if out is None:
+            out = DeviceArray(self.N)
+        call('fh_grf', self._h, white.ptr, ctypes.c_double(corr_len), ctypes.c_double(std),
+             ctypes.c_int(1 if exponentiate else 0), out.ptr)
+        return out
 
     def matvec(self, x, y):
         call('fh_matvec', self._h, dtype_code(x.dtype), x.ptr, y.ptr)

@@ -21,7 +21,10 @@ def _homogenize(pb, physics):
     nload = pb.dim if physics == 'scalar' else pb.dim * (pb.dim + 1) // 2
     for primaldual in pb.solve['primaldual']:
         mat = Material(pb.material)
-        if kind == 'GaNi':
+        if kind == 'GaNi' and 'inclusions' in mat.conf:
+            Afun = GpuOperator(N, pb.Y, physics, primaldual, _defer=True)
+            mat.set_A_GaNi_device(Afun.plan, primaldual)
+        elif kind == 'GaNi':
             A = mat.get_A_GaNi(N, primaldual)
             Afun = GpuOperator(N, pb.Y, physics, primaldual, A=A)
         else:

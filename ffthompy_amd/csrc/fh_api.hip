@@ -16,6 +16,7 @@
 #include "fh_batch.hpp"
 #include "fh_ga.hpp"
 #include "fh_iter.hpp"
+#include "fh_micro.hpp"
 
 using fh::AxisPlan;
 using fh::CGState;
@@ -1995,6 +1996,152 @@ int fh_iter_solve(fh_plan* pl, int method, const void* b_, void* x_, const doubl
     if (kit) *kit = s.kit;
     if (norm_res) *norm_res = s.norm;
   }
+  return rc;
+}
+
+
+// ---- Coefficients and random fields generated on the device -------------------------------
+
+int fh_set_coeff_inclusions(fh_plan* pl, int nincl, const int* kinds, const double* params, const double* positions,
+                            int kind, const double* vals, int invert) {
+  RETCHK(check_plan(pl));
+  if (!kinds || !params || !positions || !vals) return fail(FH_ERR_ARG, "null pointer argument");
+  if (nincl < 1 || nincl > 4096) return fail(FH_ERR_ARG, "number of inclusions out of range");
+  int ncoef, layout;
+  if (kind == FH_COEF_ISO) {
+    ncoef = 1;
+    layout = 1;
+  } else if (kind == FH_COEF_LAME) {
+    if (pl->physics != FH_ELASTICITY) return fail(FH_ERR_ARG, "Lame coefficients need elasticity");
+    ncoef = 2;
+    layout = 2;
+  } else if (kind == FH_COEF_FULL) {
+    ncoef = pl->ncomp * pl->ncomp;
+    layout = 3;
+  } else {
+    return fail(FH_ERR_ARG, "bad coefficient kind");
+  }
+  const int dim = pl->dim;
+  std::vector<fh::ga::Incl> h(nincl);
+  for (int ii = 0; ii < nincl; ++ii) {
+    fh::ga::Incl& in = h[ii];
+    std::memset(&in, 0, sizeof(in));
+    in.kind = kinds[ii];
+    if (in.kind < fh::ga::INCL_CUBE || in.kind > fh::ga::INCL_OTHERWISE) return fail(FH_ERR_ARG, "unknown inclusion kind");
+    if (in.kind == fh::ga::INCL_OTHERWISE && ii != nincl - 1)
+      return fail(FH_ERR_ARG, "'otherwise' must be the last inclusion");
+    for (int a = 0; a < dim; ++a) {
+      in.p[a] = params[3 * ii + a];
+      in.pos[a] = positions[3 * ii + a];
+      if (!std::isfinite(in.p[a]) || !std::isfinite(in.pos[a])) return fail(FH_ERR_ARG, "non-finite inclusion");
+    }
+    for (int c = 0; c < ncoef; ++c) {
+      in.v[c] = vals[(size_t)ii * ncoef + c];
+      if (!std::isfinite(in.v[c])) return fail(FH_ERR_ARG, "coefficient has a non-finite value");
+    }
+  }
+  fh::micro::NodalGrid g{};
+  g.dim = dim;
+  for (int a = 0; a < dim; ++a) {
+    g.N[a] = pl->Nu[a];
+    g.Y[a] = pl->Y[a];
+  }
+  g.n = pl->n;
+  g.ncoef = ncoef;
+  g.layout = layout;
+  g.k = pl->ncomp;
+  g.invert = invert ? 1 : 0;
+  const size_t cnt = (size_t)ncoef * pl->n;
+  fh::ga::Incl* d = nullptr;
+  int* derr = nullptr;
+  double* c64 = nullptr;
+  float* c32 = nullptr;
+  int rc = FH_OK;
+  hipError_t e = hipMalloc(&d, nincl * sizeof(fh::ga::Incl));
+  if (e == hipSuccess) e = hipMalloc(&derr, 2 * sizeof(int));
+  if (e == hipSuccess) e = hipMalloc(&c64, cnt * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&c32, cnt * sizeof(float));
+  if (e == hipSuccess) e = hipMemcpy(d, h.data(), nincl * sizeof(fh::ga::Incl), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemset(derr, 0, 2 * sizeof(int));
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(fh::micro::k_nodal_coef, dim3(grid_blocks(pl->n, 256, 16384)), dim3(256), 0, pl->stream, d,
+                       nincl, g, c64, derr);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL((fh::k_convert<double, float>), dim3(grid_blocks((long long)cnt)), dim3(256), 0, pl->stream,
+                       (long long)cnt, c64, c32);
+    e = hipGetLastError();
+  }
+  int herr[2] = {0, 0};
+  if (e == hipSuccess) e = hipStreamSynchronize(pl->stream);
+  if (e == hipSuccess) e = hipMemcpy(herr, derr, 2 * sizeof(int), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) rc = fail(FH_ERR_HIP, std::string("nodal coefficient: ") + hipGetErrorString(e));
+  else if (herr[0]) rc = fail(FH_ERR_ARG, "Overlapping inclusions!");
+  else if (herr[1]) rc = fail(FH_ERR_ARG, "singular coefficient matrix at a node");
+  (void)hipFree(d);
+  (void)hipFree(derr);
+  if (rc != FH_OK) {
+    (void)hipFree(c64);
+    (void)hipFree(c32);
+    return rc;
+  }
+  if (pl->coef64) (void)hipFree(pl->coef64);
+  if (pl->coef32) (void)hipFree(pl->coef32);
+  pl->coef64 = c64;
+  pl->coef32 = c32;
+  pl->coef_kind = kind;
+  return FH_OK;
+}
+
+int fh_coeff_to_host(const fh_plan* pl, double* host_out) {
+  RETCHK(check_plan(pl));
+  if (!host_out) return fail(FH_ERR_ARG, "null pointer argument");
+  if (!pl->coef_kind) return fail(FH_ERR_STATE, "fh_set_coeff has not been called");
+  const int per = pl->coef_kind == FH_COEF_ISO ? 1 : (pl->coef_kind == FH_COEF_LAME ? 2 : pl->ncomp * pl->ncomp);
+  HIPCHK(hipMemcpy(host_out, pl->coef64, (size_t)per * pl->n * sizeof(double), hipMemcpyDeviceToHost));
+  return FH_OK;
+}
+
+int fh_grf(fh_plan* pl, const void* white, double corr_len, double stdev, int exponentiate, void* out_) {
+  RETCHK(check_plan(pl));
+  if (!white || !out_) return fail(FH_ERR_ARG, "null pointer argument");
+  if (!(corr_len >= 0.0) || !std::isfinite(stdev)) return fail(FH_ERR_ARG, "bad correlation length / std");
+  // s = std / sqrt(mean g^2), g^2 = prod_a exp(-2 pi^2 l^2 (k_a / Y_a)^2) summed axis by axis
+  const double pi = 3.14159265358979323846;
+  double m2 = 1.0;
+  for (int a = 0; a < pl->dim; ++a) {
+    const int n = pl->Nu[a];
+    double acc = 0.0;
+    for (int i = 0; i < n; ++i) {
+      const int k = (2 * i < n) ? i : i - n;
+      const double q = (double)k / pl->Y[a];
+      acc += std::exp(-2.0 * pi * pi * corr_len * corr_len * q * q);
+    }
+    m2 *= acc / (double)n;
+  }
+  const double sc = stdev / std::sqrt(m2);
+  double* out = (double*)out_;
+  cpx<double>* tmp = nullptr;
+  HIPCHK(hipMalloc(&tmp, (size_t)pl->nh * sizeof(cpx<double>)));
+  int rc = rfftn_impl<double>(pl, 1, (const double*)white, tmp);
+  if (rc == FH_OK) {
+    const double iy0 = pl->dim == 3 ? 1.0 / pl->Y[0] : 0.0;
+    const double iy1 = 1.0 / pl->Y[pl->dim == 3 ? 1 : 0], iy2 = 1.0 / pl->Y[pl->dim - 1];
+    hipLaunchKernelGGL(fh::micro::k_grf_filter, dim3(grid_blocks(pl->nh, 256, 16384)), dim3(256), 0, pl->stream, tmp,
+                       pl->n0, pl->n1, pl->n2, iy0, iy1, iy2, corr_len, sc);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) rc = fail(FH_ERR_HIP, std::string("grf: ") + hipGetErrorString(e));
+  }
+  if (rc == FH_OK) rc = irfftn_impl<double>(pl, 1, tmp, out);
+  if (rc == FH_OK && exponentiate)
+    hipLaunchKernelGGL(fh::micro::k_exp_inplace, dim3(grid_blocks(pl->n, 256, 16384)), dim3(256), 0, pl->stream,
+                       (long long)pl->n, out);
+  if (rc == FH_OK) {
+    hipError_t e = hipStreamSynchronize(pl->stream);
+    if (e != hipSuccess) rc = fail(FH_ERR_HIP, std::string("grf: ") + hipGetErrorString(e));
+  }
+  (void)hipFree(tmp);
   return rc;
 }
 

@@ -66,6 +66,36 @@ class Material:
         return Tensor(name='A_GaNi' if primaldual == 'primal' else 'inv(A_GaNi)', val=A, order=2,
                       Y=self.Y, multype=21)
 
+    def _inclusion_arrays(self):
+        """Inclusion kinds, (nincl, d) parameters (a ball's diameter in column 0) and positions,
+        and the phase coefficient matrices."""
+        d = len(self.Y)
+        kinds = list(self.conf['inclusions'])
+        params = np.zeros((len(kinds), d))
+        positions = np.zeros((len(kinds), d))
+        mats = []
+        for ii, kind in enumerate(kinds):
+            if kind not in GA_KINDS:
+                raise NotImplementedError('The inclusion (%s) is not supported!' % kind)
+            if kind in INCLUSION_KEYS['ball']:
+                params[ii, 0] = float(np.asarray(self.conf['params'][ii]).reshape(-1)[0])
+            elif kind not in ('all', 'otherwise'):
+                params[ii] = np.asarray(self.conf['params'][ii], dtype=np.float64)
+            if kind not in ('all', 'otherwise'):
+                positions[ii] = np.asarray(self.conf['positions'][ii], dtype=np.float64)
+            mats.append(np.asarray(self.conf['vals'][ii], dtype=np.float64))
+        return kinds, params, positions, mats
+
+    def set_A_GaNi_device(self, plan, primaldual='primal'):
+        """get_A_GaNi evaluated on the device straight into `plan` (materials defined by
+        inclusions): same nodes, membership rules and pointwise dual inverse, no host arrays of
+        the grid's size (fh_set_coeff_inclusions)."""
+        kinds, params, positions, mats = self._inclusion_arrays()
+        kind, vals = classify_coefficient(np.stack(mats, axis=-1), _physics_of(mats[0].shape[0], len(self.Y)))
+        plan.set_coeff_inclusions(kinds, params, positions, kind, vals.reshape(-1, len(kinds)).T,
+                                  invert=primaldual == 'dual')
+        return plan
+
     def evaluate(self, coord):
         """Coefficient values at the nodes, (k,k)+N (materials.py:171-210)."""
         Ng = coord.shape[1:]
@@ -145,21 +175,9 @@ class Material:
         if order in (None, 'exact'):
             if 'inclusions' not in self.conf:
                 raise NotImplementedError('exact integration needs a material defined by inclusions')
-            kinds = list(self.conf['inclusions'])
-            params = np.zeros((len(kinds), d))
-            positions = np.zeros((len(kinds), d))
-            mats = []
-            for ii, kind in enumerate(kinds):
-                if kind not in GA_KINDS:
-                    raise NotImplementedError('The inclusion (%s) is not supported!' % kind)
-                if kind in INCLUSION_KEYS['ball']:
-                    params[ii, 0] = float(np.asarray(self.conf['params'][ii]).reshape(-1)[0])
-                elif kind not in ('all', 'otherwise'):
-                    params[ii] = np.asarray(self.conf['params'][ii], dtype=np.float64)
-                if kind not in ('all', 'otherwise'):
-                    positions[ii] = np.asarray(self.conf['positions'][ii], dtype=np.float64)
-                Ai = np.asarray(self.conf['vals'][ii], dtype=np.float64)
-                mats.append(np.linalg.inv(Ai) if primaldual == 'dual' else Ai)
+            kinds, params, positions, mats = self._inclusion_arrays()
+            if primaldual == 'dual':
+                mats = [np.linalg.inv(Ai) for Ai in mats]
             kind, vals = classify_coefficient(np.stack(mats, axis=-1), _physics_of(mats[0].shape[0], d))
             return GaCoefficient(Nbar, self.Y, 'A_Ga', kind, vals, inclusions=(kinds, params, positions))
         if P is None and 'P' in self.conf:

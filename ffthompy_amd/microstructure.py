@@ -157,3 +157,15 @@ def uq_samples(count=512, lengths=(63, 81, 105, 135), corr_len=0.1, std=1.0, see
     dims = rng.choice(np.asarray(lengths), size=(count, 2))
     fields = [np.exp(gaussian_random_field(tuple(d), corr_len, std, seed + 1 + s)) for s, d in enumerate(dims)]
     return dims, fields
+
+
+def gaussian_random_field_device(N, corr_len, std, seed, Y=None, exponentiate=False):
+    """gaussian_random_field with the spectral synthesis on the device (fh_grf): the same white
+    noise (numpy default_rng(seed)), filtered and transformed by the HIP FFT passes; returns a
+    device array of shape N (exp of the field if exponentiate)."""
+    from . import _native as nat
+    N = tuple(int(n) for n in N)
+    Y = np.ones(len(N)) if Y is None else np.asarray(Y, dtype=np.float64)
+    w = np.random.default_rng(seed).standard_normal(N)
+    plan = nat.Plan(N, Y)
+    return plan.grf(nat.DeviceArray.from_host(w), corr_len, std, exponentiate=exponentiate)

@@ -111,6 +111,28 @@ int fh_set_coeff(fh_plan* plan, int kind, const double* host_vals);
  * iteration counts (the solutions do not depend on it). */
 int fh_plan_set_band(fh_plan* plan, const int* band, double scale);
 
+/* Coefficient evaluated on the device at the grid nodes from a material defined by inclusions:
+ * Material.get_A_GaNi (ffthompy/materials.py:115-123) with evaluate / get_topologies
+ * (materials.py:171-301): nodes x = Y k / N on the centred index set, each inclusion summed over
+ * its periodic copies in the neighbouring cells (box: -a/2 < z <= a/2 per axis; ball: |z| < d/2;
+ * pyramid: prod max(1 - |z|/(a/2), 0) with copies shifted by 2Y as in the reference; 'all' = 1;
+ * 'otherwise' = 1 - sum of the others, negative -> "Overlapping inclusions!"), A = sum_ii v_ii t_ii,
+ * for the dual problem (invert = 1) the pointwise inverse (materials.py:121-122); stored in origin-0
+ * voxel order as the plan's coefficient. Inclusions as fh_ga_set_inclusions (kinds, 3 params and 3
+ * position values per inclusion, problem axes first), vals: per inclusion the phase coefficient in
+ * layout kind. The nodal expressions are evaluated without fused multiply-adds, so membership
+ * decisions equal the reference's. */
+int fh_set_coeff_inclusions(fh_plan* plan, int nincl, const int* kinds, const double* params,
+                            const double* positions, int kind, const double* vals, int invert);
+/* The plan's fp64 coefficient to host (layout of fh_set_coeff). No reference counterpart. */
+int fh_coeff_to_host(const fh_plan* plan, double* host_out);
+/* Periodic Gaussian random field by spectral synthesis on the plan grid: out = irfftn(rfftn(white)
+ * s exp(-pi^2 l^2 |xi|^2)), xi_a = k_a / Y_a, s = std / sqrt(mean of the squared filter) (the
+ * field's pointwise variance is std^2 for white noise of unit variance); exponentiate = 1 returns
+ * exp of it (a lognormal conductivity). white, out: device fp64, prod(N) values. The reference
+ * has no random fields; this generates the inputs of batched and differentiable workloads. */
+int fh_grf(fh_plan* plan, const void* white, double corr_len, double std, int exponentiate, void* out);
+
 /* y = F^-1 G(xi) F (A(x) x): the matvec Afun(x) (ffthompy/tensors/operators.py:136-144)
  * in fp64 or fp32 arithmetic (dtype of both fields). */
 int fh_matvec(fh_plan* plan, int dtype, const void* x, void* y);
