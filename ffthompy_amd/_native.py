@@ -82,6 +82,7 @@ SIGNATURES = {
     'fh_set_coeff_inclusions': [_vp, ctypes.c_int, _c_int_p, _c_double_p, _c_double_p, ctypes.c_int, _c_double_p,
                                 ctypes.c_int],
     'fh_coeff_to_host': [_vp, _c_double_p],
+    'fh_set_coeff_device': [_vp, ctypes.c_int, _vp],
     'fh_grf': [_vp, _vp, ctypes.c_double, ctypes.c_double, ctypes.c_int, _vp],
     'fh_synchronize': [],
 }
@@ -305,9 +306,16 @@ class Plan:
         call('fh_plan_use_generic', self._h, ctypes.c_int(1 if flag else 0))
 
     def set_coeff(self, kind, vals):
-        vals = np.ascontiguousarray(vals, dtype=np.float64)
+        """vals: host array, or a DeviceArray (fp64) of the same layout."""
         code = {'iso': FH_COEF_ISO, 'lame': FH_COEF_LAME, 'full': FH_COEF_FULL}[kind]
-        call('fh_set_coeff', self._h, ctypes.c_int(code), vals.ctypes.data_as(_c_double_p))
+        if isinstance(vals, DeviceArray):
+            per = {'iso': 1, 'lame': 2, 'full': self.ncomp ** 2}[kind]
+            if vals.size != per * self.nvox or np.dtype(vals.dtype) != np.float64:
+                raise FhError('device coefficient needs %d fp64 values' % (per * self.nvox), FH_ERR_ARG)
+            call('fh_set_coeff_device', self._h, ctypes.c_int(code), vals.ptr)
+        else:
+            vals = np.ascontiguousarray(vals, dtype=np.float64)
+            call('fh_set_coeff', self._h, ctypes.c_int(code), vals.ctypes.data_as(_c_double_p))
         self.coef_kind = kind
 
     def set_coeff_inclusions(self, kinds, params, positions, kind, vals, invert=False):

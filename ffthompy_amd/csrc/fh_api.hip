@@ -2094,6 +2094,48 @@ int fh_set_coeff_inclusions(fh_plan* pl, int nincl, const int* kinds, const doub
   return FH_OK;
 }
 
+int fh_set_coeff_device(fh_plan* pl, int kind, const void* dev_vals) {
+  RETCHK(check_plan(pl));
+  if (!dev_vals) return fail(FH_ERR_ARG, "null coefficient array");
+  int per;
+  if (kind == FH_COEF_ISO) per = 1;
+  else if (kind == FH_COEF_LAME && pl->physics == FH_ELASTICITY) per = 2;
+  else if (kind == FH_COEF_FULL) per = pl->ncomp * pl->ncomp;
+  else return fail(FH_ERR_ARG, "bad coefficient kind");
+  const size_t cnt = (size_t)per * pl->n;
+  double* c64 = nullptr;
+  float* c32 = nullptr;
+  int* dbad = nullptr;
+  int bad = 0;
+  hipError_t e = hipMalloc(&c64, cnt * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&c32, cnt * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&dbad, sizeof(int));
+  if (e == hipSuccess) e = hipMemsetAsync(dbad, 0, sizeof(int), pl->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(c64, dev_vals, cnt * sizeof(double), hipMemcpyDeviceToDevice, pl->stream);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(fh::micro::k_count_nonfinite, dim3(grid_blocks((long long)cnt)), dim3(256), 0, pl->stream,
+                       (long long)cnt, c64, dbad);
+    hipLaunchKernelGGL((fh::k_convert<double, float>), dim3(grid_blocks((long long)cnt)), dim3(256), 0, pl->stream,
+                       (long long)cnt, c64, c32);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(pl->stream);
+  if (e == hipSuccess) e = hipMemcpy(&bad, dbad, sizeof(int), hipMemcpyDeviceToHost);
+  if (dbad) (void)hipFree(dbad);
+  if (e != hipSuccess || bad) {
+    if (c64) (void)hipFree(c64);
+    if (c32) (void)hipFree(c32);
+    if (e != hipSuccess) return fail(FH_ERR_HIP, std::string("fh_set_coeff_device: ") + hipGetErrorString(e));
+    return fail(FH_ERR_ARG, "coefficient has a non-finite value");
+  }
+  if (pl->coef64) (void)hipFree(pl->coef64);
+  if (pl->coef32) (void)hipFree(pl->coef32);
+  pl->coef64 = c64;
+  pl->coef32 = c32;
+  pl->coef_kind = kind;
+  return FH_OK;
+}
+
 int fh_coeff_to_host(const fh_plan* pl, double* host_out) {
   RETCHK(check_plan(pl));
   if (!host_out) return fail(FH_ERR_ARG, "null pointer argument");

@@ -169,6 +169,11 @@ k_grf_filter(cpx<double>* __restrict__ spec, int n0, int n1, int n2, double iy0,
   }
 }
 
+__global__ void __launch_bounds__(256) k_count_nonfinite(long long n, const double* __restrict__ x, int* __restrict__ bad) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    if (!isfinite(x[i])) *bad = 1;
+}
+
 __global__ void __launch_bounds__(256) k_exp_inplace(long long n, double* __restrict__ x) {
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
     x[i] = exp(x[i]);

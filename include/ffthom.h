@@ -124,6 +124,8 @@ int fh_plan_set_band(fh_plan* plan, const int* band, double scale);
  * decisions equal the reference's. */
 int fh_set_coeff_inclusions(fh_plan* plan, int nincl, const int* kinds, const double* params,
                             const double* positions, int kind, const double* vals, int invert);
+/* fh_set_coeff from a device fp64 array of the same layout (e.g. a field made by fh_grf). */
+int fh_set_coeff_device(fh_plan* plan, int kind, const void* dev_vals);
 /* The plan's fp64 coefficient to host (layout of fh_set_coeff). No reference counterpart. */
 int fh_coeff_to_host(const fh_plan* plan, double* host_out);
 /* Periodic Gaussian random field by spectral synthesis on the plan grid: out = irfftn(rfftn(white)

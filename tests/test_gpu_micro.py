@@ -75,3 +75,22 @@ def test_device_grf_equals_numpy(N, Y, expo):
     if expo:
         ref = np.exp(ref)
     assert np.abs(got - ref).max() < 1e-12 * np.abs(ref).max()
+
+
+def test_device_coefficient_from_device_grf_matches_host_upload():
+    N = (27, 27, 27)
+    a = ms.gaussian_random_field_device(N, 0.15, 0.8, seed=3, exponentiate=True)
+    p1 = nat.Plan(N, (1., 1., 1.))
+    p1.set_coeff('iso', a)
+    p2 = nat.Plan(N, (1., 1., 1.))
+    p2.set_coeff('iso', a.to_host())
+    assert np.array_equal(p1.coeff_to_host(), p2.coeff_to_host())
+    x = nat.DeviceArray.from_host(np.random.default_rng(0).standard_normal((3,) + N))
+    y1, y2 = nat.DeviceArray((3,) + N), nat.DeviceArray((3,) + N)
+    p1.matvec(x, y1)
+    p2.matvec(x, y2)
+    assert np.array_equal(y1.to_host(), y2.to_host())
+    bad = a.to_host()
+    bad[0, 0, 0] = np.nan
+    with pytest.raises(nat.FhError):
+        p1.set_coeff('iso', nat.DeviceArray.from_host(bad))
