@@ -67,6 +67,7 @@ __global__ void __launch_bounds__(256)
 k_it_finish(IterState* st, const double* __restrict__ partial, int np) {
   if (st->halt) return;
   double acc = 0.0;
+#pragma unroll 8  // loads issued together; the additions keep their order
   for (int i = threadIdx.x; i < np; i += blockDim.x) acc += partial[i];
   const double s = block_sum(acc);
   if (threadIdx.x == 0) {

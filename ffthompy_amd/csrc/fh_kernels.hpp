@@ -528,6 +528,7 @@ __global__ void k_dot_partial(long long n, const T* __restrict__ x, const T* __r
 static __global__ void k_sum_partials(const double* __restrict__ partial, int np, double scale,
                                double* __restrict__ out) {
   double acc = 0.0;
+#pragma unroll 8  // loads issued together; the additions keep their order
   for (int i = threadIdx.x; i < np; i += blockDim.x) acc += partial[i];
   const double b = block_sum(acc);
   if (threadIdx.x == 0) *out = b * scale;
@@ -552,6 +553,7 @@ __global__ void k_residual(long long n, const double* __restrict__ b, const doub
 static __global__ void k_cg_restart(CGState* st, const double* __restrict__ partial, int np,
                              double* __restrict__ hist) {
   double acc = 0.0;
+#pragma unroll 8  // loads issued together; the additions keep their order
   for (int i = threadIdx.x; i < np; i += blockDim.x) acc += partial[i];
   const double b = block_sum(acc);
   if (threadIdx.x == 0) {
@@ -572,6 +574,7 @@ static __global__ void k_cg_restart(CGState* st, const double* __restrict__ part
 static __global__ void k_cg_alpha(CGState* st, const double* __restrict__ partial, int np) {
   if (st->halt) return;
   double acc = 0.0;
+#pragma unroll 8  // loads issued together; the additions keep their order
   for (int i = threadIdx.x; i < np; i += blockDim.x) acc += partial[i];
   const double b = block_sum(acc);
   if (threadIdx.x == 0) {
@@ -583,6 +586,7 @@ static __global__ void k_cg_alpha(CGState* st, const double* __restrict__ partia
 static __global__ void k_cg_beta(CGState* st, const double* __restrict__ partial, int np) {
   if (st->halt) return;
   double acc = 0.0;
+#pragma unroll 8  // loads issued together; the additions keep their order
   for (int i = threadIdx.x; i < np; i += blockDim.x) acc += partial[i];
   const double b = block_sum(acc);
   if (threadIdx.x == 0) finalize_beta(st, b);

@@ -308,25 +308,6 @@ k_rows_inv_st(const cpx<T>* __restrict__ in, T* __restrict__ out, long long nrow
   const unsigned len = (unsigned)nr * L;
   const unsigned rbase = (unsigned)(row0 * L);
   for (int e = tid; e < L; e += nthr) twl[e] = twg[e];
-  // the epilogue's operand (w or r) is fetched now; it arrives while the transform runs
-  constexpr int RPB = FxShape<FP, T>::RPB;
-  constexpr int NT = (RPB / 2) * NC * TPP;
-  constexpr int QMAX = (RPB * L + 2 * W - 2) / W + 1;
-  constexpr int CPT = (QMAX + NT - 1) / NT;
-  Chunk<T> V[EPI >= 1 ? NC : 1][CPT];
-  if constexpr (EPI >= 1) {
-    const __amdgpu_buffer_rsrc_t rw = make_rsrc(EPI == 1 ? w : out, (unsigned long long)NC * n * SR);
-#pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      const unsigned s0 = (unsigned)(c * n) + rbase;
-      const unsigned q0 = s0 / W, q1 = (s0 + len + W - 1) / W;
-#pragma unroll
-      for (int i = 0; i < CPT; ++i) {
-        const unsigned q = q0 + tid + i * NT;
-        if (q < q1) V[c][i] = bload_chunk<T>(rw, q * 16u);
-      }
-    }
-  }
   // Hermitian completion: pencil q = rp*NC + m holds real rows 2m (real part) and 2m+1
   // (imaginary part) of row pair rp's 2*NC (row, component) rows. Rows past nr read 0
   // (buffer range); at k = 0 (and k = L/2) both writes hit one element with one value.
@@ -379,8 +360,15 @@ k_rows_inv_st(const cpx<T>* __restrict__ in, T* __restrict__ out, long long nrow
   }
   __syncthreads();
   // epilogue over each component's run of rows, in 16-byte chunks
+  constexpr int RPB = FxShape<FP, T>::RPB;
+  constexpr int NT = (RPB / 2) * NC * TPP;
+  constexpr int QMAX = (RPB * L + 2 * W - 2) / W + 1;  // chunks touched by one run of RPB rows
+  constexpr int CPT = (QMAX + NT - 1) / NT;
   const __amdgpu_buffer_rsrc_t rout = make_rsrc(out, (unsigned long long)NC * n * SR);
   const T alp = EPI == 2 ? T(st->alpha) : T(0);
+  // the epilogue operand (w or r) is read here, not prefetched before the transform: holding
+  // it in registers through the transform costs more occupancy than the early load gains
+  const __amdgpu_buffer_rsrc_t rv = make_rsrc(EPI == 1 ? w : out, (unsigned long long)NC * n * SR);
   double acc = 0.0;
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
@@ -394,6 +382,8 @@ k_rows_inv_st(const cpx<T>* __restrict__ in, T* __restrict__ out, long long nrow
       const unsigned ob = q * 16u;
       const Chunk<T> Y = *reinterpret_cast<const Chunk<T>*>(sr + sb + (int)(q * W - s0));
       Chunk<T> O = Y;
+      Chunk<T> V;
+      if constexpr (EPI >= 1) V = bload_chunk<T>(rv, ob);
       const bool full = q * W >= s0 && q * W + W <= s0 + len;
       if constexpr (EPI >= 1) {
 #pragma unroll
@@ -401,9 +391,9 @@ k_rows_inv_st(const cpx<T>* __restrict__ in, T* __restrict__ out, long long nrow
           const unsigned f = q * W + e;
           const bool inr = full || (f >= s0 && f < s0 + len);
           if constexpr (EPI == 1) {
-            if (inr) acc += (double)Y.e[e] * (double)V[c][i].e[e];
+            if (inr) acc += (double)Y.e[e] * (double)V.e[e];
           } else {
-            O.e[e] = V[c][i].e[e] - alp * Y.e[e];
+            O.e[e] = V.e[e] - alp * Y.e[e];
             if (inr) acc += (double)O.e[e] * (double)O.e[e];
           }
         }
