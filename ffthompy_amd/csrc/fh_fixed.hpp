@@ -198,10 +198,13 @@ __device__ __forceinline__ void fx_lds_stages(int t, Lds&& lds, const cpx<T>* __
 // Stage sizing helpers for launch shapes (host and device).
 template <class FP, typename T>
 struct FxShape {
-  // columns per strided tile and frequency points per projection tile
+  // columns per strided tile and frequency points per projection tile (fp32 with 16 or more
+  // threads per pencil, i.e. lengths 105 and 243: 4 points x 3 components keep a 243-point tile
+  // at 23 KB of LDS, six workgroups per CU, so one workgroup's loads overlap the others'
+  // transforms; 8 points measured 12 % slower at 243^3)
   static constexpr int P = (sizeof(T) == 4) ? (FP::TPP >= 16 ? 16 : (FP::TPP >= 8 ? 32 : 64))
                                             : (FP::TPP >= 16 ? 8 : (FP::TPP >= 8 ? 16 : 32));
-  static constexpr int F = (sizeof(T) == 4) ? (FP::TPP >= 16 ? 8 : (FP::TPP >= 8 ? 16 : 32))
+  static constexpr int F = (sizeof(T) == 4) ? (FP::TPP >= 16 ? 4 : (FP::TPP >= 8 ? 16 : 32))
                                             : (FP::TPP >= 16 ? 4 : (FP::TPP >= 8 ? 8 : 16));
   // rows per workgroup in the contiguous-axis passes (even)
   static constexpr int RPB = (FP::TPP >= 16) ? 8 : (FP::TPP >= 8 ? 16 : 32);
