@@ -353,132 +353,11 @@ __device__ __forceinline__ void apply_G_fx(cpx<T> (&v)[NC], int k0, int k1, int 
   }
 }
 
-// ---- projection pass: forward FFT, G(xi), inverse FFT along the last strided axis ------
-// FP: forward schedule; FPR: the same radices reversed (inverse schedule). A thread owns
-// frequency point f of the tile for all NC components.
-template <typename T, class FP, class FPR, int F, int NC>
-__global__ void __launch_bounds__(FP::TPP * F)
-k_cols_g_fx(cpx<T>* __restrict__ data, long long inner, int ntiles, long long total,
-            const cpx<T>* __restrict__ twg, GParams g, int n2h, const int* __restrict__ halt) {
-  static_assert(FP::TPP == FPR::TPP && FP::L == FPR::L, "schedules must match");
-  static_assert(FP::R[FP::NST - 1] == FPR::R[0], "inverse must start with the last forward radix");
-  if (halt && *halt) return;
-  constexpr int L = FP::L;
-  constexpr int P = F * NC;
-  __shared__ cpx<T> s[(FP::NST > 1) ? L * P : 1];
-  __shared__ cpx<T> twl[L];
-  const int tid = threadIdx.x;
-  const int f = tid % F, t = tid / F;
-  const int tile = xcd_remap(blockIdx.x, gridDim.x);
-  const long long col0 = (long long)tile * P;
-  const int ncol = (int)min((long long)P, inner - col0);
-  constexpr unsigned CS = sizeof(cpx<T>);
-  const __amdgpu_buffer_rsrc_t rs = make_rsrc(data + col0, (unsigned long long)(total - col0) * CS);
-  const unsigned rowb = (unsigned)inner * CS;
-  const unsigned fb = (unsigned)(f * NC) * CS;
-  for (int e = tid; e < L; e += FP::TPP * F) twl[e] = twg[e];
-  const bool act = f * NC < ncol;
-  const long long fp = col0 / NC + f;  // frequency point in the inner (plane) grid
-  int ka, kb;                          // the two fixed frequency indices of this pencil
-  if (g.n0 > 1) {
-    ka = (int)(fp / n2h);
-    kb = (int)(fp - (long long)ka * n2h);
-  } else {
-    ka = 0;
-    kb = (int)fp;
-  }
-  auto lds = [&](int j, int c) -> cpx<T>& { return s[(j * NC + c) * F + f]; };
-  // forward stages
-  constexpr int NF = FP::NST;
-  FxStage<T, FP, NF - 1, -1> last[NC];
-  if constexpr (NF == 1) {
-#pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      last[c].load(t, [=](int j) { return bload_c<T>(rs, (unsigned)j * rowb + fb + c * CS); });
-      last[c].compute(t, twl);
-    }
-  } else {
-    {
-      FxStage<T, FP, 0, -1> s0[NC];
-#pragma unroll
-      for (int c = 0; c < NC; ++c) {
-        s0[c].load(t, [=](int j) { return bload_c<T>(rs, (unsigned)j * rowb + fb + c * CS); });
-        s0[c].compute(t, twl);
-      }
-#pragma unroll
-      for (int c = 0; c < NC; ++c) s0[c].store(t, [&](int j, cpx<T> v) { lds(j, c) = v; });
-    }
-    __syncthreads();
-    fx_lds_stages<T, FP, -1, 1, NF - 1, NC>(t, lds, twl);
-#pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      last[c].load(t, [&](int j) { return lds(j, c); });
-      last[c].compute(t, twl);
-    }
-  }
-  // last forward stage outputs sit at i + j*p (p = L/R_last): apply G per (butterfly, j)
-  using SL = FxStage<T, FP, NF - 1, -1>;
-#pragma unroll
-  for (int m = 0; m < SL::M; ++m) {
-    const int i = t + m * SL::TPP;
-    if (SL::valid(i)) {
-#pragma unroll
-      for (int j = 0; j < SL::R; ++j) {
-        const int kk = i + j * SL::p;
-        cpx<T> v[NC];
-#pragma unroll
-        for (int c = 0; c < NC; ++c) v[c] = last[c].v[m][j];
-        if (g.n0 > 1)
-          apply_G_fx<T, NC>(v, kk, ka, kb, g);
-        else
-          apply_G_fx<T, NC>(v, 0, kk, kb, g);
-#pragma unroll
-        for (int c = 0; c < NC; ++c) last[c].v[m][j] = v[c];
-      }
-    }
-  }
-  // inverse stages: the first inverse stage (radix R_last) reads exactly these registers
-  constexpr int NI = FPR::NST;
-  if constexpr (NI == 1) {
-#pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      FxStage<T, FPR, 0, +1> r0;
-#pragma unroll
-      for (int m = 0; m < SL::M; ++m)
-#pragma unroll
-        for (int j = 0; j < SL::R; ++j) r0.v[m][j] = last[c].v[m][j];
-      r0.compute(t, twl);
-      if (act) r0.store(t, [=](int j, cpx<T> v) { bstore_c<T>(rs, (unsigned)j * rowb + fb + c * CS, v); });
-    }
-  } else {
-    if constexpr (NF > 1) __syncthreads();  // LDS reads of the last forward stage are done
-#pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      FxStage<T, FPR, 0, +1> r0;
-#pragma unroll
-      for (int m = 0; m < SL::M; ++m)
-#pragma unroll
-        for (int j = 0; j < SL::R; ++j) r0.v[m][j] = last[c].v[m][j];
-      r0.compute(t, twl);
-      r0.store(t, [&](int j, cpx<T> v) { lds(j, c) = v; });
-    }
-    __syncthreads();
-    fx_lds_stages<T, FPR, +1, 1, NI - 1, NC>(t, lds, twl);
-#pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      FxStage<T, FPR, NI - 1, +1> rl;
-      rl.load(t, [&](int j) { return lds(j, c); });
-      rl.compute(t, twl);
-      if (act) rl.store(t, [=](int j, cpx<T> v) { bstore_c<T>(rs, (unsigned)j * rowb + fb + c * CS, v); });
-    }
-  }
-}
-
-
-
-// ---- projection pass, one pencil per thread: forward FFT, G(xi) sweep, inverse FFT --------
-// Same contract as k_cols_g_fx. A thread owns one (frequency point, component) pencil, so a
-// tile of F points uses TPP*F*NC threads (three times the waves of k_cols_g_fx for the
+// ---- projection pass: forward FFT, G(xi) sweep, inverse FFT along the last strided axis ----
+// FP: forward schedule; FPR: the same radices reversed (inverse schedule). A tile is F
+// consecutive frequency points of the (k1, k2) plane with all NC components (the AoS
+// spectrum's contiguous run); a thread owns one (frequency point, component) pencil, so a
+// tile uses TPP*F*NC threads (three times the waves of one thread per frequency point at the
 // same LDS); G is applied by a sweep over the tile in LDS between the transforms.
 template <typename T, class FP, class FPR, int F, int NC>
 __global__ void __launch_bounds__(FP::TPP * F * NC)
@@ -681,122 +560,6 @@ k_rows_fwd_fx(const T* __restrict__ in, const T* __restrict__ coef, cpx<T>* __re
     ob[e] = Y;
   }
   if constexpr (UPD == 2) {
-    const double bsum = block_sum(acc);
-    if (tid == 0) partial[blockIdx.x] = bsum;
-  }
-}
-
-// ---- last pass (fixed length): C2R along the contiguous axis ------------------------
-// EPI 0: out = y. EPI 1: out = y and per-workgroup partial sums of <y, w>.
-// EPI 2 (fused iteration): `out` is the CG residual r, replaced by r - alpha y (alpha
-// from the CG state), with per-workgroup partial sums of <r, r>.
-template <typename T, class FP, int EPI, int NC>
-__global__ void __launch_bounds__(1024)
-k_rows_inv_fx(const cpx<T>* __restrict__ in, T* __restrict__ out, long long nrows, long long n,
-              int rpb, const cpx<T>* __restrict__ twg, T scale, const T* __restrict__ w,
-              double* __restrict__ partial, const int* __restrict__ halt, CGState* __restrict__ st) {
-  constexpr int nc = NC;
-  if (halt && *halt) return;
-  constexpr int L = FP::L, n2h = L / 2 + 1, TPP = FP::TPP;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  cpx<T>* s = reinterpret_cast<cpx<T>*>(smem_raw);
-  __shared__ cpx<T> twl[L];
-  const int tid = threadIdx.x, nthr = blockDim.x;
-  const int t = tid % TPP, pen = tid / TPP;
-  const long long row0 = (long long)xcd_remap(blockIdx.x, gridDim.x) * rpb;
-  const int nr = (int)min((long long)rpb, nrows - row0);
-  for (int e = tid; e < L; e += nthr) twl[e] = twg[e];
-  // this thread's pencil: real rows (rowa, ca) and (rowb, cb) of the block
-  const int rp_ = pen / nc, m_ = pen - rp_ * nc;
-  const int rowa = 2 * rp_ + (2 * m_) / nc, ca = (2 * m_) % nc;
-  const int rowb = 2 * rp_ + (2 * m_ + 1) / nc, cb = (2 * m_ + 1) % nc;
-  const bool va = rowa < nr, vb = rowb < nr;
-  constexpr unsigned SR = sizeof(T);
-  const __amdgpu_buffer_rsrc_t rout = make_rsrc(out, (unsigned long long)NC * n * SR);
-  const unsigned oa = (unsigned)(ca * n + (row0 + rowa) * L) * SR;
-  const unsigned ob = (unsigned)(cb * n + (row0 + rowb) * L) * SR;
-  using SL = FxStage<T, FP, FP::NST - 1, +1>;
-  // the last stage writes positions i + j*p (k = i); prefetch the weights there now so
-  // their latency hides behind the FFT
-  T wa[SL::M][SL::R], wbv[SL::M][SL::R];
-  if constexpr (EPI == 1 || EPI == 2) {
-    const __amdgpu_buffer_rsrc_t rw = make_rsrc(EPI == 1 ? w : out, (unsigned long long)NC * n * SR);
-#pragma unroll
-    for (int m = 0; m < SL::M; ++m) {
-      const int i = t + m * SL::TPP;
-#pragma unroll
-      for (int j = 0; j < SL::R; ++j) {
-        const unsigned x = (unsigned)(i + j * SL::p) * SR;
-        wa[m][j] = (va && SL::valid(i)) ? bload_r<T>(rw, oa + x) : T(0);
-        wbv[m][j] = (vb && SL::valid(i)) ? bload_r<T>(rw, ob + x) : T(0);
-      }
-    }
-  }
-  const int nrp = rpb >> 1;
-  const cpx<T>* ib = in + row0 * (long long)n2h * nc;
-  for (int e = tid; e < nrp * n2h; e += nthr) {
-    const int rp = e / n2h, k = e - rp * n2h;
-    const bool self = (k == 0) || (2 * k == L);
-#pragma unroll
-    for (int mm = 0; mm < nc; ++mm) {
-      int rr = 2 * mm;
-      int row = 2 * rp + rr / nc;
-      cpx<T> Ya = mk(T(0), T(0)), Yb = mk(T(0), T(0));
-      if (row < nr) Ya = ib[((long long)row * n2h + k) * nc + rr % nc];
-      rr = 2 * mm + 1;
-      row = 2 * rp + rr / nc;
-      if (row < nr) Yb = ib[((long long)row * n2h + k) * nc + rr % nc];
-      if (self) {
-        Ya.y = T(0);
-        Yb.y = T(0);
-      }
-      cpx<T>* z = s + (rp * nc + mm) * L;
-      z[k] = mk(Ya.x - Yb.y, Ya.y + Yb.x);
-      if (!self) z[L - k] = mk(Ya.x + Yb.y, Yb.x - Ya.y);
-    }
-  }
-  __syncthreads();
-  cpx<T>* sp = s + pen * L;
-  auto at = [&](int j, int) -> cpx<T>& { return sp[j]; };
-  fx_lds_stages<T, FP, +1, 0, FP::NST - 1, 1>(t, at, twl);
-  double acc = 0.0;
-  SL sl;
-  sl.load(t, [&](int j) { return sp[j]; });
-  sl.compute(t, twl);
-#pragma unroll
-  for (int m = 0; m < SL::M; ++m) {
-    const int i = t + m * SL::TPP;
-    if (SL::valid(i)) {
-#pragma unroll
-      for (int j = 0; j < SL::R; ++j) {
-        const unsigned x = (unsigned)(i + j * SL::p) * SR;
-        const T a = scale * sl.v[m][j].x, b = scale * sl.v[m][j].y;
-        if constexpr (EPI == 2) {
-          const T alp = T(st->alpha);
-          if (va) {
-            const T rn = wa[m][j] - alp * a;
-            bstore_r<T>(rout, oa + x, rn);
-            acc += (double)rn * (double)rn;
-          }
-          if (vb) {
-            const T rn = wbv[m][j] - alp * b;
-            bstore_r<T>(rout, ob + x, rn);
-            acc += (double)rn * (double)rn;
-          }
-        } else {
-          if (va) {
-            bstore_r<T>(rout, oa + x, a);
-            if constexpr (EPI == 1) acc += (double)a * (double)wa[m][j];
-          }
-          if (vb) {
-            bstore_r<T>(rout, ob + x, b);
-            if constexpr (EPI == 1) acc += (double)b * (double)wbv[m][j];
-          }
-        }
-      }
-    }
-  }
-  if constexpr (EPI == 1 || EPI == 2) {
     const double bsum = block_sum(acc);
     if (tid == 0) partial[blockIdx.x] = bsum;
   }

@@ -27,7 +27,7 @@ int fixed_rows_fwd(const LaunchCtx& c, int nc, const T* in, const T* coef, int k
                    const int* halt, const T* rvec, const CGState* st, T* xc, double* partial,
                    size_t partial_cap, int* nblocks);
 // epi 0: out = y; epi 1: also partial sums of <out, w>; epi 2: out is r, r -= alpha y and
-// partial sums of <r, r> (k_rows_inv_fx).
+// partial sums of <r, r> (k_rows_inv_st).
 template <typename T>
 int fixed_rows_inv(const LaunchCtx& c, int nc, const cpx<T>* in, T* out, T scale, int epi, const T* w,
                    double* partial, size_t partial_cap, const int* halt, int* nblocks, CGState* st);
