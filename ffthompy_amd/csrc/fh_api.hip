@@ -46,6 +46,9 @@ int fail(int code, const std::string& msg) {
   } while (0)
 
 constexpr size_t kLdsTarget = 48 * 1024;
+// threads of the one-workgroup CG reductions (alpha, beta): each reads ~np/1024 partial sums
+// (a few thousand at 243^3), so the cross-XCD load latency is paid a few times, not ~30
+constexpr int kFinThreads = 1024;
 constexpr size_t kLdsMax = 160 * 1024;
 
 std::mutex g_init_mu;
@@ -601,7 +604,7 @@ static int enqueue_fused_iteration(fh_plan* pl, float* xc, float* r, float* p, c
   if (rr == 0) return FH_OK;
   RETCHK(fixed_result(rr, "rows forward (fused)"));
   *ok = true;
-  hipLaunchKernelGGL(fh::k_cg_alpha, dim3(1), dim3(256), 0, pl->stream, pl->st, pl->partial, nb);
+  hipLaunchKernelGGL(fh::k_cg_alpha, dim3(1), dim3(kFinThreads), 0, pl->stream, pl->st, pl->partial, nb);
   const GParams g = make_gparams(pl, 1.0 / (double)pl->n);
   if (pl->n0 > 1) {
     RETCHK(launch_cols<float>(pl, 0, 1, nc, spec, g, halt));
@@ -615,7 +618,7 @@ static int enqueue_fused_iteration(fh_plan* pl, float* xc, float* r, float* p, c
                                  pl->st);
   if (rr == 0) return fail(FH_ERR_STATE, "fused rows inverse unavailable");
   RETCHK(fixed_result(rr, "rows inverse (fused)"));
-  hipLaunchKernelGGL(fh::k_cg_beta, dim3(1), dim3(256), 0, pl->stream, pl->st, pl->partial, nb2);
+  hipLaunchKernelGGL(fh::k_cg_beta, dim3(1), dim3(kFinThreads), 0, pl->stream, pl->st, pl->partial, nb2);
   HIPCHK(hipGetLastError());
   return FH_OK;
 }
@@ -634,11 +637,11 @@ static int enqueue_iteration(fh_plan* pl, T* x, T* r, T* p, T* Ap, const T* coef
   if (!fused)
     hipLaunchKernelGGL((fh::k_cg_pupdate<T>), dim3(grid_blocks(nv)), dim3(256), 0, pl->stream, nv, p, r, pl->st);
   RETCHK(run_operator<T>(pl, p, Ap, pl->coef_kind, coef, p, halt, &nb, fused ? r : nullptr));
-  hipLaunchKernelGGL(fh::k_cg_alpha, dim3(1), dim3(256), 0, pl->stream, pl->st, pl->partial, nb);
+  hipLaunchKernelGGL(fh::k_cg_alpha, dim3(1), dim3(kFinThreads), 0, pl->stream, pl->st, pl->partial, nb);
   const int gb = grid_blocks(nv);
   hipLaunchKernelGGL((fh::k_cg_update<T>), dim3(gb), dim3(256), 0, pl->stream, nv, x, r, p, Ap, pl->st,
                      pl->partial);
-  hipLaunchKernelGGL(fh::k_cg_beta, dim3(1), dim3(256), 0, pl->stream, pl->st, pl->partial, gb);
+  hipLaunchKernelGGL(fh::k_cg_beta, dim3(1), dim3(kFinThreads), 0, pl->stream, pl->st, pl->partial, gb);
   HIPCHK(hipGetLastError());
   return FH_OK;
 }
