@@ -39,6 +39,8 @@ SIGNATURES = {
     'fh_plan_set_primaldual': [_vp, ctypes.c_int],
     'fh_plan_use_generic': [_vp, ctypes.c_int],
     'fh_plan_set_cg_variant': [_vp, ctypes.c_int],
+    'fh_plan_use_graphs': [_vp, ctypes.c_int],
+    'fh_last_solve_graph_batches': [_vp, _c_int_p],
     'fh_set_coeff': [_vp, ctypes.c_int, _c_double_p],
     'fh_matvec': [_vp, ctypes.c_int, _vp, _vp],
     'fh_project': [_vp, ctypes.c_int, _vp, _vp],
@@ -301,6 +303,15 @@ class Plan:
         the operator G A (see fh_plan_set_band)."""
         B = None if band is None else (ctypes.c_int * self.dim)(*[int(b) for b in band])
         call('fh_plan_set_band', self._h, B, ctypes.c_double(scale))
+
+    def use_graphs(self, flag=True):
+        """Mixed CG: replay batches of fused iterations from a captured HIP graph (default)."""
+        call('fh_plan_use_graphs', self._h, ctypes.c_int(1 if flag else 0))
+
+    def last_graph_batches(self):
+        n = ctypes.c_int()
+        call('fh_last_solve_graph_batches', self._h, ctypes.byref(n))
+        return n.value
 
     def use_generic(self, flag=True):
         call('fh_plan_use_generic', self._h, ctypes.c_int(1 if flag else 0))

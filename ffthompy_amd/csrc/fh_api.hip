@@ -164,6 +164,12 @@ struct fh_plan {
   int banded = 0;                // 1: projection restricted to |k_a| <= band[a] (view axes)
   int band[3] = {0, 0, 0};
   double op_scale = 1.0;         // factor of the operator G A (not of the bare projection)
+  // fused mixed CG: a batch of iterations captured once as a HIP graph, replayed per batch
+  hipGraphExec_t it_exec = nullptr;
+  const void* it_key[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  bool it_graph_off = false;     // capture failed on this plan: direct launches from then on
+  bool use_graphs = true;        // fh_plan_use_graphs
+  int last_graph_batches = 0;    // batches of the last solve replayed from the graph
 };
 
 struct fh_batch {
@@ -805,6 +811,7 @@ int fh_plan_destroy(fh_plan* pl) {
   if (pl->st) (void)hipFree(pl->st);
   if (pl->hist) (void)hipFree(pl->hist);
   if (pl->work) (void)hipFree(pl->work);
+  if (pl->it_exec) (void)hipGraphExecDestroy(pl->it_exec);
   if (pl->ev0) (void)hipEventDestroy(pl->ev0);
   if (pl->ev1) (void)hipEventDestroy(pl->ev1);
   if (pl->stream) (void)hipStreamDestroy(pl->stream);
@@ -922,6 +929,19 @@ int fh_plan_set_primaldual(fh_plan* pl, int primaldual) {
   RETCHK(check_plan(pl));
   if (primaldual != FH_PRIMAL && primaldual != FH_DUAL) return fail(FH_ERR_ARG, "bad primaldual");
   pl->primaldual = primaldual;
+  return FH_OK;
+}
+
+int fh_plan_use_graphs(fh_plan* pl, int enable) {
+  RETCHK(check_plan(pl));
+  pl->use_graphs = enable != 0;
+  return FH_OK;
+}
+
+int fh_last_solve_graph_batches(const fh_plan* pl, int* n) {
+  RETCHK(check_plan(pl));
+  if (!n) return fail(FH_ERR_ARG, "null pointer argument");
+  *n = pl->last_graph_batches;
   return FH_OK;
 }
 
@@ -1082,7 +1102,51 @@ int fh_cg_solve(fh_plan* pl, int mode, const void* b_, void* x_, double tol, int
       else
         hipLaunchKernelGGL(fh::k_fold_correction, dim3(gb), dim3(256), 0, pl->stream, nv, x, xc);
     };
-    for (;;) {
+    // After the first batch (which establishes that the fused kernels cover the plan), a batch of
+    // fused iterations is captured once as a HIP graph (kernel nodes only) and replayed: one
+    // launch per batch instead of seven per iteration. The graph is rebuilt when a buffer moves.
+    int batch = 0;
+    pl->last_graph_batches = 0;
+    for (;; ++batch) {
+      const void* key[5] = {xc, r, p, pl->coef32, pl->spec};
+      if (fused && batch > 0 && pl->use_graphs && !pl->it_graph_off) {
+        if (!pl->it_exec || std::memcmp(key, pl->it_key, sizeof(key)) != 0) {
+          if (pl->it_exec) (void)hipGraphExecDestroy(pl->it_exec);
+          pl->it_exec = nullptr;
+          hipGraph_t graph = nullptr;
+          bool all = true;
+          int rc = FH_OK;
+          if (hipStreamBeginCapture(pl->stream, hipStreamCaptureModeRelaxed) != hipSuccess) all = false;
+          for (int k = 0; all && k < per_check; ++k) {
+            bool ok = false;
+            rc = enqueue_fused_iteration(pl, xc, r, p, pl->coef32, &ok);
+            if (rc != FH_OK || !ok) all = false;
+          }
+          const hipError_t ec = hipStreamEndCapture(pl->stream, &graph);
+          if (all && rc == FH_OK && ec == hipSuccess && graph &&
+              hipGraphInstantiate(&pl->it_exec, graph, nullptr, nullptr, 0) == hipSuccess) {
+            std::memcpy(pl->it_key, key, sizeof(key));
+          } else {
+            pl->it_exec = nullptr;
+            pl->it_graph_off = true;  // capture not available here: enqueue the kernels directly
+          }
+          if (graph) (void)hipGraphDestroy(graph);
+          (void)hipGetLastError();
+        }
+        if (pl->it_exec) {
+          HIPCHK(hipGraphLaunch(pl->it_exec, pl->stream));
+          ++pl->last_graph_batches;
+          HIPCHK(hipMemcpyAsync(&s, pl->st, sizeof(s), hipMemcpyDeviceToHost, pl->stream));
+          HIPCHK(hipStreamSynchronize(pl->stream));
+          if (s.done) break;
+          if (s.pause) {
+            fold();
+            RETCHK(enqueue_true_residual<float>(pl, b, x, Ax, r));
+            HIPCHK(hipGetLastError());
+          }
+          continue;
+        }
+      }
       for (int k = 0; k < per_check; ++k) {
         if (fused) {
           bool ok = false;

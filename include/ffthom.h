@@ -92,6 +92,14 @@ int fh_plan_use_generic(fh_plan* plan, int generic);
  * Plain FP64 solves always use the operator form. */
 int fh_plan_set_cg_variant(fh_plan* plan, int fused);
 
+/* Mixed-precision CG with the fused iteration: enable = 1 (default) replays batches of
+ * iterations from a HIP graph captured on the plan's stream (kernel nodes only, rebuilt when a
+ * work buffer moves); 0 launches every kernel directly. Same arithmetic either way.
+ * fh_last_solve_graph_batches: batches of the last solve that ran from the graph.
+ * No reference counterpart (launch mechanics). */
+int fh_plan_use_graphs(fh_plan* plan, int enable);
+int fh_last_solve_graph_batches(const fh_plan* plan, int* n);
+
 /* Coefficient A(x) of the operator, host array in origin-0 voxel order.
  * Replaces Material.get_A_GaNi (ffthompy/materials.py:115-123) as the right factor of
  * Afun = [[GN, A]] (ffthompy/applications.py:58). For the dual problem pass the inverse
