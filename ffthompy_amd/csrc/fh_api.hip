@@ -654,13 +654,31 @@ static int enqueue_iteration(fh_plan* pl, T* x, T* r, T* p, T* Ap, const T* coef
 
 // r = b - A x (fp64 operator); rr, beta, done via k_cg_restart
 template <typename TR>
-static int enqueue_true_residual(fh_plan* pl, const double* b, const double* x, double* Ax, TR* r) {
+static int enqueue_true_residual(fh_plan* pl, const double* b, const double* x, double* Ax, TR* r,
+                                 bool x_zero = false) {
   const long long nv = (long long)pl->ncomp * pl->n;
-  RETCHK(run_operator<double>(pl, x, Ax, pl->coef_kind, pl->coef64, nullptr, nullptr, nullptr));
+  // x = 0: A x = 0 exactly, so r = b without the operator (the same bits as b - A 0)
+  if (!x_zero) RETCHK(run_operator<double>(pl, x, Ax, pl->coef_kind, pl->coef64, nullptr, nullptr, nullptr));
   const int gb = grid_blocks(nv);
-  hipLaunchKernelGGL((fh::k_residual<TR>), dim3(gb), dim3(256), 0, pl->stream, nv, b, Ax, r, pl->partial);
+  hipLaunchKernelGGL((fh::k_residual<TR>), dim3(gb), dim3(256), 0, pl->stream, nv, b, x_zero ? nullptr : Ax, r,
+                     pl->partial);
   hipLaunchKernelGGL(fh::k_cg_restart, dim3(1), dim3(256), 0, pl->stream, pl->st, pl->partial, gb, pl->hist);
   HIPCHK(hipGetLastError());
+  return FH_OK;
+}
+
+// 1 if every value of x is +0 or -0 (the usual initial guess x0 = 0 of a solve).
+static int field_is_zero(fh_plan* pl, const double* x, bool* zero) {
+  const long long nv = (long long)pl->ncomp * pl->n;
+  int* flag = reinterpret_cast<int*>(pl->partial);  // free at this point of a solve
+  HIPCHK(hipMemsetAsync(flag, 0, sizeof(int), pl->stream));
+  hipLaunchKernelGGL(fh::micro::k_count_nonzero, dim3(grid_blocks(nv, 256, 4096)), dim3(256), 0, pl->stream, nv, x,
+                     flag);
+  HIPCHK(hipGetLastError());
+  int h = 1;
+  HIPCHK(hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, pl->stream));
+  HIPCHK(hipStreamSynchronize(pl->stream));
+  *zero = h == 0;
   return FH_OK;
 }
 
@@ -1068,13 +1086,15 @@ int fh_cg_solve(fh_plan* pl, int mode, const void* b_, void* x_, double tol, int
   HIPCHK(hipMemcpyAsync(pl->st, &s, sizeof(s), hipMemcpyHostToDevice, pl->stream));
   HIPCHK(hipEventRecord(pl->ev0, pl->stream));
   const int per_check = 8;
+  bool x0_zero = false;
+  RETCHK(field_is_zero(pl, x, &x0_zero));
   if (mode == FH_SOLVE_FP64) {
     const long long nvp = (nv + 63) / 64 * 64;  // 512-byte aligned sub-buffers
     RETCHK(ensure_work(pl, 3 * (size_t)nvp * sizeof(double)));
     double* r = (double*)pl->work;
     double* p = r + nvp;
     double* Ap = p + nvp;
-    RETCHK(enqueue_true_residual<double>(pl, b, x, Ap, r));
+    RETCHK(enqueue_true_residual<double>(pl, b, x, Ap, r, x0_zero));
     HIPCHK(hipMemcpyAsync(p, r, (size_t)nv * sizeof(double), hipMemcpyDeviceToDevice, pl->stream));
     for (;;) {
       for (int k = 0; k < per_check; ++k) RETCHK(enqueue_iteration<double>(pl, x, r, p, Ap, pl->coef64));
@@ -1091,7 +1111,7 @@ int fh_cg_solve(fh_plan* pl, int mode, const void* b_, void* x_, double tol, int
     float* Ap = p + nvp;
     float* xc = Ap + nvp;
     HIPCHK(hipMemsetAsync(xc, 0, (size_t)nv * sizeof(float), pl->stream));
-    RETCHK(enqueue_true_residual<float>(pl, b, x, Ax, r));
+    RETCHK(enqueue_true_residual<float>(pl, b, x, Ax, r, x0_zero));
     HIPCHK(hipMemcpyAsync(p, r, (size_t)nv * sizeof(float), hipMemcpyDeviceToDevice, pl->stream));
     // fused iterations where the fixed-length kernels cover the plan (isotropic coefficient)
     bool fused = pl->fused_cg && pl->op_scale == 1.0;  // the fused step assumes an unscaled G

@@ -174,6 +174,13 @@ __global__ void __launch_bounds__(256) k_count_nonfinite(long long n, const doub
     if (!isfinite(x[i])) *bad = 1;
 }
 
+__global__ void __launch_bounds__(256) k_count_nonzero(long long n, const double* __restrict__ x, int* __restrict__ flag) {
+  bool nz = false;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    nz = nz || (x[i] != 0.0);
+  if (nz) *flag = 1;
+}
+
 __global__ void __launch_bounds__(256) k_exp_inplace(long long n, double* __restrict__ x) {
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
     x[i] = exp(x[i]);
