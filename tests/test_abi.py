@@ -64,3 +64,23 @@ def test_argument_validation_without_device():
     assert lib.fh_plan_create(3, N, Yb, 0, 0, ctypes.byref(h)) == nat.FH_ERR_ARG
     assert lib.fh_matvec(None, 0, None, None) == nat.FH_ERR_ARG
     assert lib.fh_cg_solve(None, 0, None, None, 1e-6, 10, 0.1, None, None, None, 0, None) == nat.FH_ERR_ARG
+
+
+def test_argument_validation_of_later_entry_points_without_device():
+    """Null handles and pointers are rejected with FH_ERR_ARG before any device work."""
+    lib = nat.load_library()
+    h = ctypes.c_void_p()
+    assert lib.fh_ga_create(None, None, ctypes.byref(h)) == nat.FH_ERR_ARG
+    assert lib.fh_ga_energy(None, None, 3, None) == nat.FH_ERR_ARG
+    assert lib.fh_ga_energy_on_grid(None, None, 3, None) == nat.FH_ERR_ARG
+    assert lib.fh_ga_set_inclusions(None, 1, None, None, None, 1, None) == nat.FH_ERR_ARG
+    assert lib.fh_ga_set_grid_coeff(None, 0, None, 1, None) == nat.FH_ERR_ARG
+    assert lib.fh_ga_interpolate(None, None, None) == nat.FH_ERR_ARG
+    assert lib.fh_ga_set_plan_coeff(None, None) == nat.FH_ERR_ARG
+    assert lib.fh_iter_solve(None, 1, None, None, None, 1e-6, 10, None, None, None, 0) == nat.FH_ERR_ARG
+    assert lib.fh_set_coeff_inclusions(None, 1, None, None, None, 1, None, 0) == nat.FH_ERR_ARG
+    assert lib.fh_set_coeff_device(None, 1, None) == nat.FH_ERR_ARG
+    assert lib.fh_grf(None, None, 0.1, 1.0, 0, None) == nat.FH_ERR_ARG
+    assert lib.fh_plan_set_band(None, None, 1.0) == nat.FH_ERR_ARG
+    assert lib.fh_plan_use_graphs(None, 1) == nat.FH_ERR_ARG
+    assert len(lib.fh_last_error()) > 0
