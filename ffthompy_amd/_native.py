@@ -78,6 +78,7 @@ SIGNATURES = {
     'fh_ga_set_grid_coeff': [_vp, ctypes.c_int, _c_int_p, ctypes.c_int, _c_double_p],
     'fh_ga_coefficient': [_vp, _c_double_p],
     'fh_ga_interpolate': [_vp, _vp, _vp],
+    'fh_ga_restrict': [_vp, _vp, _vp],
     'fh_ga_energy': [_vp, ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, _c_double_p],
     'fh_iter_solve': [_vp, ctypes.c_int, _vp, _vp, _c_double_p, ctypes.c_double, ctypes.c_int, _c_int_p, _c_double_p,
                       _c_double_p, ctypes.c_int],
@@ -601,6 +602,13 @@ class Ga:
         if out is None:
             out = DeviceArray((self.plan.ncomp,) + self.M)
         call('fh_ga_interpolate', self._h, x.ptr, out.ptr)
+        return out
+
+    def restrict(self, x, out=None):
+        """Field on the quadrature grid -> the coarse plan's grid (spectrum cropped)."""
+        if out is None:
+            out = DeviceArray((self.plan.ncomp,) + self.plan.N)
+        call('fh_ga_restrict', self._h, x.ptr, out.ptr)
         return out
 
     def energy_matrix(self, fields, on_grid=False):

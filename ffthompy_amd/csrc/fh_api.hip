@@ -1922,6 +1922,27 @@ int fh_ga_interpolate(fh_ga* ga, const void* x, void* out) {
   return ga_interpolate(ga, (const double*)x, (double*)out);
 }
 
+int fh_ga_restrict(fh_ga* ga, const void* x_, void* out_) {
+  if (!ga || !x_ || !out_) return fail(FH_ERR_ARG, "null pointer argument");
+  fh_plan* c = ga->coarse;
+  fh_plan* f = ga->fine;
+  const double* x = (const double*)x_;
+  double* out = (double*)out_;
+  cpx<double>* fs = reinterpret_cast<cpx<double>*>(f->spec);
+  cpx<double>* cs = reinterpret_cast<cpx<double>*>(c->spec);
+  const int nb = grid_blocks(c->nh, 256, 8192);
+  for (int k = 0; k < c->ncomp; ++k) {
+    RETCHK(rfftn_impl<double>(f, 1, x + (size_t)k * f->n, fs));
+    HIPCHK(hipStreamSynchronize(f->stream));
+    hipLaunchKernelGGL(fh::ga::k_ga_crop, dim3(nb), dim3(256), 0, c->stream, fs, c->n0, c->n1, c->n2,
+                       1.0 / (double)f->n, ga->g, cs);
+    HIPCHK(hipGetLastError());
+    RETCHK(spec_to_real(c, cs, out + (size_t)k * c->n, 1.0));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  return FH_OK;
+}
+
 int fh_ga_energy_on_grid(fh_ga* ga, const void* const* fields, int nf, double* AH) {
   if (!ga || !fields || !AH) return fail(FH_ERR_ARG, "null pointer argument");
   if (nf != ga->coarse->ncomp) return fail(FH_ERR_ARG, "need one field per unit load (nf == ncomp)");

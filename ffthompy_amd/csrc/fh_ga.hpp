@@ -219,5 +219,34 @@ k_ga_pad(const cpx<double>* __restrict__ in, int n0, int n1, int n2, double scal
   }
 }
 
+// Restriction of a quadrature-grid spectrum to the coarse grid (Tensor.decrease,
+// ffthompy/tensors/objects.py:469-486): coarse frequency k (centred set, the Nyquist index of an
+// even axis standing for -n/2) takes the fine coefficient at k, times scale. On the last axis
+// the half spectrum holds +n2/2 at index n2/2; its value there is the fine coefficient at -n2/2,
+// i.e. the conjugate of the fine one at (-k0, -k1, +n2/2). One thread per coarse frequency.
+__global__ void __launch_bounds__(256)
+k_ga_crop(const cpx<double>* __restrict__ in, int n0, int n1, int n2, double scale, SpecGrid g,
+          cpx<double>* __restrict__ out) {
+  const int n2h = n2 / 2 + 1;
+  const long long nh = (long long)n0 * n1 * n2h;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < nh;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int k2 = (int)(t % n2h);
+    const long long r = t / n2h;
+    const int j1 = (int)(r % n1), j0 = (int)(r / n1);
+    const int k0 = sfreq(j0, n0), k1 = sfreq(j1, n1);
+    cpx<double> v;
+    if ((n2 & 1) == 0 && 2 * k2 == n2) {
+      const int i0 = ((-k0) % g.m0 + g.m0) % g.m0, i1 = ((-k1) % g.m1 + g.m1) % g.m1;
+      const cpx<double> c = in[((long long)i0 * g.m1 + i1) * g.m2h + k2];
+      v = mk(c.x, -c.y);
+    } else {
+      const int i0 = (k0 % g.m0 + g.m0) % g.m0, i1 = (k1 % g.m1 + g.m1) % g.m1;
+      v = in[((long long)i0 * g.m1 + i1) * g.m2h + k2];
+    }
+    out[t] = mk(scale * v.x, scale * v.y);
+  }
+}
+
 }  // namespace ga
 }  // namespace fh

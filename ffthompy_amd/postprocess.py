@@ -46,6 +46,21 @@ def ga_assembly_matrix(A_ga, Afun, solutions):
     return ga.energy_matrix([f.val for f in fields])
 
 
+def gani_of_ga_solve(Afun, mat, solutions, primaldual):
+    """'GaNi' energies of a Ga solve's minimisers: the nodal coefficient on the solve grid N
+    (postprocess.py:11-15) and the solutions restricted from the quadrature grid to N by
+    cropping their spectra (Tensor.project inside assembly_matrix, objects.py:469-511)."""
+    ga = nat.Ga(Afun.coarse_plan, grid=Afun.N)
+    fields = [ga.restrict(s.to_device().val) for s in solutions]
+    plan = nat.Plan(Afun.Nsolve, Afun.Y, Afun.physics, primaldual)
+    if 'inclusions' in mat.conf:
+        mat.set_A_GaNi_device(plan, primaldual)
+    else:
+        kind, vals = classify_coefficient(mat.get_A_GaNi(Afun.Nsolve, primaldual).val, Afun.physics)
+        plan.set_coeff(kind, vals)
+    return plan.energy_matrix(fields)
+
+
 def add_macro2minimizer(X, E):
     """Return the corrector plus the macroscopic value E (postprocess.py:73-86)."""
     m = X.mean()
@@ -60,31 +75,36 @@ def add_macro2minimizer(X, E):
 
 def postprocess(pb, A, mat, solutions, results, primaldual):
     """Store 'mat_<pd>' (named AH matrices), 'sol_<pd>' and 'res_<pd>' in pb.output
-    (postprocess.py:6-51)."""
+    (postprocess.py:6-51). As in the reference, the coefficient carries over from one entry
+    to the next: 'GaNi' switches to the nodal coefficient (unless it is already in use),
+    'Ga' with 'order' to that exactly integrated coefficient, and 'Ga' without 'order' uses
+    whichever coefficient is current (the solve's at first)."""
     matrices = {}
+    ga_solve = isinstance(A, GpuOperator) and A.Nsolve is not None
+    current = ('solve', None)   # the solve's coefficient: nodal (GaNi solve) or A_Ga (Ga solve)
     for pp in pb.postprocess:
+        order_name = Nname = ''
         if pp['kind'] in ('GaNi', 'gani'):
-            if isinstance(A, GpuOperator) and A.Nsolve is not None:
-                raise NotImplementedError("'GaNi' post-processing of a Ga solve")
-            name = 'AH_%s_%s' % (pp['kind'], primaldual)
-            AH = assembly_matrix(A, solutions)
-            matrices[name] = AH if primaldual == 'primal' else np.linalg.inv(AH)
+            if ga_solve or current[0] == 'ga':
+                current = ('gani', None)
         elif pp['kind'] in ('Ga', 'ga'):
-            order_name = Nname = ''
             if 'order' in pp:
                 Nbar = tuple(2 * np.asarray(pb.solve['N']) - 1)
                 if pp['order'] is None:
-                    A_ga = mat.get_A_Ga(Nbar=Nbar, primaldual=primaldual, order=None)
+                    current = ('ga', mat.get_A_Ga(Nbar=Nbar, primaldual=primaldual, order=None))
                 else:
                     order_name = '_o' + str(pp['order'])
                     Nname = '_P%d' % np.mean(pp['P'])
-                    A_ga = mat.get_A_Ga(Nbar=Nbar, primaldual=primaldual, order=pp['order'], P=pp['P'])
-                AH = ga_assembly_matrix(A_ga, A, solutions)
-            else:
-                AH = assembly_matrix(A, solutions)
-            name = 'AH_%s%s%s_%s' % (pp['kind'], order_name, Nname, primaldual)
-            matrices[name] = AH if primaldual == 'primal' else np.linalg.inv(AH)
+                    current = ('ga', mat.get_A_Ga(Nbar=Nbar, primaldual=primaldual, order=pp['order'], P=pp['P']))
         else:
             raise ValueError(pp['kind'])
+        if current[0] == 'ga':
+            AH = ga_assembly_matrix(current[1], A, solutions)
+        elif current[0] == 'gani' and ga_solve:
+            AH = gani_of_ga_solve(A, mat, solutions, primaldual)
+        else:  # the solve's coefficient, or the nodal one of a GaNi solve (the same)
+            AH = assembly_matrix(A, solutions)
+        name = 'AH_%s%s%s_%s' % (pp['kind'], order_name, Nname, primaldual)
+        matrices[name] = AH if primaldual == 'primal' else np.linalg.inv(AH)
     pb.output.update({'sol_' + primaldual: solutions, 'res_' + primaldual: results,
                       'mat_' + primaldual: matrices})

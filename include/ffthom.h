@@ -307,6 +307,11 @@ int fh_ga_coefficient(const fh_ga* ga, double* host_out);
 /* P(x) on the quadrature grid: x a device fp64 field of the coarse plan (ncomp components),
  * out a device fp64 field of ncomp * prod(M) values (Tensor.project, objects.py:488-511). */
 int fh_ga_interpolate(fh_ga* ga, const void* x, void* out);
+/* The reverse: a field on the quadrature grid (ncomp * prod(M) values) restricted to the coarse
+ * grid by cropping its spectrum (Tensor.project to a smaller grid = decrease,
+ * ffthompy/tensors/objects.py:469-511); exact for fields with frequencies below N/2, such as the
+ * solutions of a Ga solve. out: device fp64, ncomp * prod(N) values. */
+int fh_ga_restrict(fh_ga* ga, const void* x, void* out);
 /* AH[i][j] = <A_Ga P(e_i), P(e_j)> for nf = ncomp fp64 device fields of the coarse plan. */
 int fh_ga_energy(fh_ga* ga, const void* const* fields, int nf, double* AH);
 /* The same for fields already on the quadrature grid (solutions of a Ga solve on that grid):

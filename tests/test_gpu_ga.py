@@ -237,3 +237,51 @@ def test_ga_solve_on_quadrature_grid_matches_oracle(N):
         kits = [r['info']['kit'] for r in pb.output['res_' + pd]]
         rkits = [r['info']['kit'] for r in ref.output['res_' + pd]]
         assert kits == rkits, (pd, kits, rkits)
+
+
+@pytest.mark.parametrize('N', [(5, 5), (7, 6), (6, 8, 5), (15, 15, 15)])
+def test_ga_restriction_inverts_interpolation_and_matches_oracle(N):
+    """Interpolate to the quadrature grid and restrict back: identity for every field whose
+    frequencies stay below N/2 (odd N: every field); the restriction of an arbitrary
+    quadrature-grid field equals the oracle's decrease for odd N."""
+    print('case ga restrict', N, flush=True)
+    d = len(N)
+    rng = np.random.default_rng(sum(N) + 1)
+    plan = nat.Plan(N, np.ones(d))
+    ga = nat.Ga(plan, grid=tuple(nat.smooth_length(2 * n - 1) for n in N))
+    x = rng.standard_normal((d,) + N)
+    if any(n % 2 == 0 for n in N):  # remove the Nyquist modes of even axes
+        X = np.fft.fftn(x, axes=tuple(range(1, d + 1)))
+        for a, n in enumerate(N):
+            if n % 2 == 0:
+                idx = [slice(None)] * (d + 1)
+                idx[a + 1] = n // 2
+                X[tuple(idx)] = 0.
+        x = np.real(np.fft.ifftn(X, axes=tuple(range(1, d + 1))))
+    back = ga.restrict(ga.interpolate(nat.DeviceArray.from_host(x))).to_host()
+    assert np.abs(back - x).max() < 1e-13 * np.abs(x).max()
+    if all(n % 2 == 1 for n in N):
+        M = ga.M
+        y = rng.standard_normal((d,) + M)
+        got = ga.restrict(nat.DeviceArray.from_host(y)).to_host()
+        ref = Field(val=y.copy(), order=1, Y=np.ones(d)).project(N).val
+        assert np.abs(got - ref).max() < 1e-13 * np.abs(ref).max()
+
+
+def test_gani_postprocessing_of_a_ga_solve_matches_oracle():
+    """Problem with a Ga solve and 'GaNi' post-processing (the nodal coefficient with the
+    solutions restricted to N), against the oracle."""
+    mats = examples.scalar_2d_materials
+    for N in ((5, 5), (45, 45)):
+        conf = {'name': 'ga-gani', 'physics': 'scalar', 'material': 'square',
+                'solve': {'kind': 'Ga', 'N': np.array(N), 'primaldual': ['primal', 'dual']},
+                'postprocess': [{'kind': 'GaNi'}, {'kind': 'Ga'}],
+                'solver': {'kind': 'CG', 'tol': 1e-8, 'maxiter': 1e3}}
+        pb = Problem(dict(conf), type('conf', (), {'materials': mats()}))
+        pb.calculate()
+        ref = OracleProblem(dict(conf), mats())
+        ref.calculate()
+        for pd in ('primal', 'dual'):
+            for key in ('AH_GaNi_' + pd, 'AH_Ga_' + pd):
+                AH, AR = pb.output['mat_' + pd][key], ref.output['mat_' + pd][key]
+                assert np.abs(AH - AR).max() < 1e-10 * np.abs(AR).max(), (N, key)
