@@ -285,3 +285,21 @@ def test_gani_postprocessing_of_a_ga_solve_matches_oracle():
             for key in ('AH_GaNi_' + pd, 'AH_Ga_' + pd):
                 AH, AR = pb.output['mat_' + pd][key], ref.output['mat_' + pd][key]
                 assert np.abs(AH - AR).max() < 1e-10 * np.abs(AR).max(), (N, key)
+
+
+def test_elastic_ga_solve_on_quadrature_grid_matches_oracle():
+    """3-D elasticity Ga solve at N = 15 (2N-1 = 29 prime: the device solves on 30^3): bounds and
+    iteration counts against the oracle's solve on 29^3."""
+    mats = examples.linelas_3d_materials
+    conf = {'name': 'ga-el', 'physics': 'elasticity', 'material': 'square',
+            'solve': {'kind': 'Ga', 'N': np.array((15, 15, 15)), 'primaldual': ['primal', 'dual']},
+            'postprocess': [{'kind': 'Ga'}], 'solver': {'kind': 'CG', 'tol': 1e-6, 'maxiter': 1e3}}
+    pb = Problem(dict(conf), type('conf', (), {'materials': mats()}))
+    pb.calculate()
+    ref = OracleProblem(dict(conf), mats())
+    ref.calculate()
+    for pd in ('primal', 'dual'):
+        AH, AR = pb.output['mat_' + pd]['AH_Ga_' + pd], ref.output['mat_' + pd]['AH_Ga_' + pd]
+        assert np.abs(AH - AR).max() < 1e-9 * np.abs(AR).max(), pd
+        kits = [r['info']['kit'] for r in pb.output['res_' + pd]]
+        assert kits == [r['info']['kit'] for r in ref.output['res_' + pd]], pd
