@@ -17,6 +17,7 @@
 #include "fh_ga.hpp"
 #include "fh_iter.hpp"
 #include "fh_micro.hpp"
+#include "fh_pot.hpp"
 
 using fh::AxisPlan;
 using fh::CGState;
@@ -750,6 +751,57 @@ int ga_interpolate(fh_ga* ga, const double* x, double* out) {
     RETCHK(spec_to_real(f, fs, out + (size_t)k * f->n, 1.0));
     HIPCHK(hipStreamSynchronize(f->stream));
   }
+  return FH_OK;
+}
+
+// ---- potential formulation -----------------------------------------------------------------
+
+fh::pot::PotParams pot_params(const fh_plan* pl) {
+  fh::pot::PotParams g;
+  g.n0 = pl->n0;
+  g.n1 = pl->n1;
+  g.n2 = pl->n2;
+  g.n2h = pl->n2h;
+  g.dim = pl->dim;
+  g.ax0 = 3 - pl->dim;
+  g.Y[0] = pl->dim == 3 ? pl->Y[0] : 1.0;
+  g.Y[1] = pl->Y[pl->dim == 3 ? 1 : 0];
+  g.Y[2] = pl->Y[pl->dim - 1];
+  g.banded = pl->banded;
+  for (int a = 0; a < 3; ++a) g.band[a] = pl->band[a];
+  return g;
+}
+
+// F A F^-1 on the d-component spectrum in pl->spec (the real field passes through `re`,
+// ncomp * n doubles): inverse passes, coefficient multiply fused into the forward row pass,
+// forward passes.
+int pot_coef_round_trip(fh_plan* pl, double* re, const int* halt) {
+  const int nc = pl->ncomp;
+  cpx<double>* spec = reinterpret_cast<cpx<double>*>(pl->spec);
+  const GParams gp = make_gparams(pl, 1.0);
+  if (pl->n0 > 1) RETCHK(launch_cols<double>(pl, 1, 0, nc, spec, gp, halt));
+  RETCHK(launch_cols<double>(pl, 1, 1, nc, spec, gp, halt));
+  RETCHK(launch_rows_inv<double>(pl, nc, spec, re, 1.0 / (double)pl->n, nullptr, nullptr, halt, nullptr));
+  RETCHK(launch_rows_fwd<double>(pl, nc, re, pl->coef64, pl->coef_kind, spec, halt));
+  RETCHK(launch_cols<double>(pl, 0, 1, nc, spec, gp, halt));
+  if (pl->n0 > 1) RETCHK(launch_cols<double>(pl, 0, 0, nc, spec, gp, halt));
+  return FH_OK;
+}
+
+// out = P (-div) F A F^-1 grad (P in); partial sums of the weighted <out, w> into pl->partial
+// (*nb workgroups).
+int pot_operator(fh_plan* pl, const cpx<double>* in, cpx<double>* out, double* re, const cpx<double>* w,
+                 const int* halt, int* nb) {
+  const fh::pot::PotParams g = pot_params(pl);
+  cpx<double>* spec = reinterpret_cast<cpx<double>*>(pl->spec);
+  const int gb = grid_blocks(pl->nh);
+  hipLaunchKernelGGL(fh::pot::k_pot_grad, dim3(gb), dim3(256), 0, pl->stream, in, spec, g, 1, halt);
+  HIPCHK(hipGetLastError());
+  RETCHK(pot_coef_round_trip(pl, re, halt));
+  hipLaunchKernelGGL(fh::pot::k_pot_div, dim3(gb), dim3(256), 0, pl->stream, spec, out, g, -1.0, w, pl->partial,
+                     halt);
+  HIPCHK(hipGetLastError());
+  *nb = gb;
   return FH_OK;
 }
 
@@ -2293,6 +2345,102 @@ int fh_grf(fh_plan* pl, const void* white, double corr_len, double stdev, int ex
   }
   (void)hipFree(tmp);
   return rc;
+}
+
+
+// ---- potential formulation (ffthompy/tensorsLowRank/homogenisation.py:43-128) ------------------
+
+int fh_potential_solve(fh_plan* pl, const double* E, double tol, int maxiter, void* Fu_, void* X_, int* kit,
+                       double* norm_res, double* res_hist, int hist_len, int* cg_status) {
+  RETCHK(check_plan(pl));
+  if (!E || !Fu_) return fail(FH_ERR_ARG, "null pointer argument");
+  if (pl->physics != FH_SCALAR) return fail(FH_ERR_UNSUPPORTED, "the potential formulation is scalar only");
+  if (!pl->coef_kind) return fail(FH_ERR_STATE, "fh_set_coeff has not been called");
+  if (!(tol >= 0.0) || maxiter < 0) return fail(FH_ERR_ARG, "bad tol/maxiter");
+  if (!pl->banded)
+    for (int a = 0; a < pl->dim; ++a)
+      if ((pl->Nu[a] & 1) == 0)
+        return fail(FH_ERR_UNSUPPORTED, "the potential formulation needs odd grids (even: Nyquist modes)");
+  const fh::pot::PotParams g = pot_params(pl);
+  const long long nh = pl->nh;
+  const long long nhp = (nh + 15) / 16 * 16;
+  const int nc = pl->ncomp;
+  RETCHK(ensure_work(pl, 3 * (size_t)nhp * sizeof(cpx<double>) + (size_t)nc * pl->n * sizeof(double)));
+  cpx<double>* r = reinterpret_cast<cpx<double>*>(pl->work);
+  cpx<double>* p = r + nhp;
+  cpx<double>* Ap = p + nhp;
+  double* re = reinterpret_cast<double*>(Ap + nhp);
+  cpx<double>* x = reinterpret_cast<cpx<double>*>(Fu_);  // the CG iterate; P x at the end
+  const int hl = std::max(hist_len, 1);
+  RETCHK(ensure_hist(pl, hl));
+  CGState s;
+  std::memset(&s, 0, sizeof(s));
+  s.tol = tol;
+  s.maxiter = maxiter;
+  s.inv_nvox = 1.0 / ((double)pl->n * (double)pl->n);  // 'r' spectra: / n^2
+  s.hist_len = hist_len > 0 ? hist_len : 0;
+  s.hist = pl->hist;
+  HIPCHK(hipMemcpyAsync(pl->st, &s, sizeof(s), hipMemcpyHostToDevice, pl->stream));
+  HIPCHK(hipEventRecord(pl->ev0, pl->stream));
+  const int gb = grid_blocks(nh);
+  const dim3 grid(gb), block(256);
+  // right-hand side P div F(A E) (homogenisation.py:106-110), x0 = 0
+  hipLaunchKernelGGL(fh::pot::k_pot_fill, dim3(grid_blocks((long long)nc * pl->n)), block, 0, pl->stream, re,
+                     pl->n, nc, E[0], E[1], nc > 2 ? E[2] : 0.0);
+  HIPCHK(hipGetLastError());
+  cpx<double>* spec = reinterpret_cast<cpx<double>*>(pl->spec);
+  const GParams gp = make_gparams(pl, 1.0);
+  RETCHK(launch_rows_fwd<double>(pl, nc, re, pl->coef64, pl->coef_kind, spec, nullptr));
+  RETCHK(launch_cols<double>(pl, 0, 1, nc, spec, gp, nullptr));
+  if (pl->n0 > 1) RETCHK(launch_cols<double>(pl, 0, 0, nc, spec, gp, nullptr));
+  hipLaunchKernelGGL(fh::pot::k_pot_div, grid, block, 0, pl->stream, spec, r, g, 1.0, nullptr, nullptr, nullptr);
+  hipLaunchKernelGGL(fh::pot::k_pot_norm, grid, block, 0, pl->stream, r, g, pl->partial);
+  hipLaunchKernelGGL(fh::k_cg_restart, dim3(1), dim3(256), 0, pl->stream, pl->st, pl->partial, gb, pl->hist);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemsetAsync(x, 0, (size_t)nh * sizeof(cpx<double>), pl->stream));
+  HIPCHK(hipMemcpyAsync(p, r, (size_t)nh * sizeof(cpx<double>), hipMemcpyDeviceToDevice, pl->stream));
+  const int* halt = &pl->st->halt;
+  const int per_check = 8;
+  for (;;) {
+    for (int k = 0; k < per_check; ++k) {
+      hipLaunchKernelGGL((fh::k_cg_pupdate<double>), grid, block, 0, pl->stream, 2 * nh, (double*)p,
+                         (const double*)r, pl->st);
+      int nb = 0;
+      RETCHK(pot_operator(pl, p, Ap, re, p, halt, &nb));
+      hipLaunchKernelGGL(fh::k_cg_alpha, dim3(1), dim3(kFinThreads), 0, pl->stream, pl->st, pl->partial, nb);
+      hipLaunchKernelGGL(fh::pot::k_pot_update, grid, block, 0, pl->stream, x, r, p, Ap, g, pl->st, pl->partial);
+      hipLaunchKernelGGL(fh::k_cg_beta, dim3(1), dim3(kFinThreads), 0, pl->stream, pl->st, pl->partial, gb);
+      HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipMemcpyAsync(&s, pl->st, sizeof(s), hipMemcpyDeviceToHost, pl->stream));
+    HIPCHK(hipStreamSynchronize(pl->stream));
+    if (s.done) break;
+  }
+  // Fu = P x (homogenisation.py:117); e = F^-1 grad Fu
+  hipLaunchKernelGGL(fh::pot::k_pot_precond, grid, block, 0, pl->stream, x, Ap, g);
+  HIPCHK(hipMemcpyAsync(x, Ap, (size_t)nh * sizeof(cpx<double>), hipMemcpyDeviceToDevice, pl->stream));
+  if (X_) {
+    hipLaunchKernelGGL(fh::pot::k_pot_grad, grid, block, 0, pl->stream, x, spec, g, 0, nullptr);
+    HIPCHK(hipGetLastError());
+    if (pl->n0 > 1) RETCHK(launch_cols<double>(pl, 1, 0, nc, spec, gp, nullptr));
+    RETCHK(launch_cols<double>(pl, 1, 1, nc, spec, gp, nullptr));
+    RETCHK(launch_rows_inv<double>(pl, nc, spec, (double*)X_, 1.0 / (double)pl->n, nullptr, nullptr, nullptr,
+                                   nullptr));
+  }
+  HIPCHK(hipEventRecord(pl->ev1, pl->stream));
+  HIPCHK(hipMemcpyAsync(&s, pl->st, sizeof(s), hipMemcpyDeviceToHost, pl->stream));
+  HIPCHK(hipStreamSynchronize(pl->stream));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, pl->ev0, pl->ev1));
+  pl->last_solve_ms = ms;
+  if (kit) *kit = s.kit;
+  if (norm_res) *norm_res = (s.kit == 0) ? 0.0 : std::sqrt(s.rr);
+  if (cg_status) *cg_status = s.status;
+  if (res_hist && hist_len > 0) {
+    const int cnt = std::min(hist_len, s.kit + 1);
+    HIPCHK(hipMemcpy(res_hist, pl->hist, (size_t)cnt * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  return FH_OK;
 }
 
 }  // extern "C"

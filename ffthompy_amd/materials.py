@@ -215,6 +215,31 @@ class GaCoefficient:
         self.order = order
         self.P = P
 
+    def add_mean(self, mean):
+        """Add a constant matrix (the anisotropic part of the reference's low-rank examples,
+        Tensor.add_mean on A_Ga): an inclusion 'all' with that value, or the value added to every
+        nodal value (box and tent interpolation reproduce constants). The layout becomes 'full'."""
+        mean = np.asarray(mean, dtype=np.float64)
+        k = mean.shape[0]
+        if self.kind == 'iso':
+            full = np.einsum('ij,...->ij...', np.eye(k), self.vals).reshape((k * k,) + self.vals.shape)
+        elif self.kind == 'full':
+            full = self.vals.reshape((k * k,) + self.vals.shape[1:] if self.inclusions is None
+                                     else (k * k, -1))
+        else:
+            raise NotImplementedError('add_mean of a %s coefficient' % self.kind)
+        if self.inclusions is not None:
+            kinds, params, positions = self.inclusions
+            d = len(self.Y)
+            params = np.vstack([np.asarray(params, dtype=np.float64).reshape(-1, d), np.zeros((1, d))])
+            positions = np.vstack([np.asarray(positions, dtype=np.float64).reshape(-1, d), np.zeros((1, d))])
+            self.inclusions = (list(kinds) + ['all'], params, positions)
+            full = np.concatenate([full.reshape(k * k, -1), mean.reshape(k * k, 1)], axis=1)
+        else:
+            full = full + mean.reshape((k * k,) + (1,) * (full.ndim - 1))
+        self.kind, self.vals = 'full', full
+        return self
+
     def load(self, ga):
         """Evaluate this coefficient on the quadrature grid of `ga`."""
         if self.inclusions is not None:

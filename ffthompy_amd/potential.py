@@ -1,0 +1,139 @@
+"""Potential formulation of the scalar homogenisation problem on the GPU: the drop-in for
+homog_GaNi_full_potential / homog_Ga_full_potential / get_preconditioner of
+ffthompy/tensorsLowRank/homogenisation.py:43-128, 296-300.
+
+The unknown is the rfftn spectrum of a scalar potential u (one complex half spectrum instead of
+d real fields), the field is e = grad u, and CG runs on P (-div) F A F^-1 grad P with
+P = 1/|2 pi xi| (fh_potential_solve). As in the reference the macroscopic load is E = e_1, CG
+starts from zero, and AH is the scalar <A (e + E), e + E> (GaNi coefficient, or the Ga
+coefficient after interpolation onto the quadrature grid).
+
+The Ga variant solves on the quadrature grid of the Ga coefficient (2N-1 per axis, or the next
+length with prime factors <= 13) with grad, div and P restricted to the frequencies of the grid N,
+which is the reference's enlarge / project between N and 2N-1 with the same inner products.
+"""
+import time
+
+import numpy as np
+
+from . import _native as nat
+from .tensors import Tensor, classify_coefficient
+
+
+class Struct(dict):
+    """Attribute dictionary for results (the reference's ffthompy.Struct)."""
+
+    def __getattr__(self, key):
+        try:
+            return self[key]
+        except KeyError:
+            raise AttributeError(key)
+
+    def __setattr__(self, key, val):
+        self[key] = val
+
+
+def _par(pars, key, default=None):
+    if isinstance(pars, dict):
+        return pars.get(key, default)
+    return getattr(pars, key, default)
+
+
+def _solver_par(pars):
+    s = _par(pars, 'solver', {}) or {}
+    return float(s.get('tol', 1e-6)), int(s.get('maxiter', 1000))
+
+
+def _e1(d):
+    E = np.zeros(d)
+    E[0] = 1.
+    return E
+
+
+def get_preconditioner(N, pars):
+    """P = 1/|2 pi xi| on the 'r' half spectrum, 1 at the mean (homogenisation.py:296-300);
+    a host Tensor (Fourier, multype 00). The GPU solve evaluates P inside its kernels."""
+    N = tuple(int(n) for n in N)
+    Y = np.asarray(_par(pars, 'Y', np.ones(len(N))), dtype=np.float64)
+    d = len(N)
+    shape = N[:-1] + (N[-1] // 2 + 1,)
+    k2 = np.zeros(shape)
+    for j in range(d):
+        k = np.fft.ifftshift(np.arange(np.fix(-N[j] / 2.), np.fix(N[j] / 2. + 0.5)))
+        if j == d - 1:
+            k = k[:shape[-1]]
+        sh = [1] * d
+        sh[j] = -1
+        k2 = k2 + (2 * np.pi * k / Y[j]).reshape(sh) ** 2
+    k2[(0,) * d] = 1.
+    return Tensor(name='P', val=1. / k2 ** 0.5, order=0, N=N, Y=Y, Fourier=True, multype=0)
+
+
+def _fourier_tensor(name, val, N, Y):
+    return Tensor(name=name, val=val, order=0, N=N, Y=Y, Fourier=True)
+
+
+def homog_GaNi_full_potential(Agani, Aga, pars):
+    """GaNi potential solve (homogenisation.py:84-128). Agani: the nodal coefficient Tensor
+    (d,d)+N (Material.get_A_GaNi); Aga: None (GaNi value) or the Ga coefficient of
+    Material.get_A_Ga(2N-1) for the Ga value of the same solution. Returns
+    Struct(AH, Fu, info, time, pars)."""
+    N = tuple(int(n) for n in Agani.N)
+    Y = np.asarray(getattr(Agani, 'Y', None) if getattr(Agani, 'Y', None) is not None
+                   else _par(pars, 'Y', np.ones(len(N))), dtype=np.float64)
+    d = len(N)
+    tol, maxiter = _solver_par(pars)
+    plan = nat.Plan(N, Y, 'scalar', 'primal')
+    plan.set_coeff(*classify_coefficient(Agani.val, 'scalar'))
+    E = _e1(d)
+    X = nat.DeviceArray((d,) + N)
+    t0 = time.perf_counter()
+    Fu, info = plan.potential_solve(E, tol, maxiter, X=X)
+    elapsed = time.perf_counter() - t0
+    nat.add_constant(X, E)
+    # the Gram matrix of d copies of one field has every entry <A XEN, XEN>
+    if Aga is None:
+        AH = plan.energy_matrix([X] * d)[0, 0]
+    else:
+        ga = Aga.load(nat.Ga(plan))
+        AH = ga.energy_matrix([X] * d)[0, 0]
+    return Struct(AH=AH, Fu=_fourier_tensor('Fu', Fu, N, Y), info=info, time=elapsed, pars=pars)
+
+
+def _crop_spectrum(val, M, N):
+    """'r' spectrum on grid M of a trigonometric polynomial with frequencies of grid N (odd) ->
+    its 'r' spectrum on N (the reference's Tensor.project/decrease, values times prod(N)/prod(M))."""
+    idx = []
+    for a, (m, n) in enumerate(zip(M, N)):
+        k = np.arange(n // 2 + 1) if a == len(N) - 1 else np.fft.ifftshift(np.arange(-(n // 2), n // 2 + 1))
+        idx.append(k % m)
+    return val[np.ix_(*idx)] * (float(np.prod(N)) / float(np.prod(M)))
+
+
+def homog_Ga_full_potential(Aga, pars):
+    """Ga potential solve (homogenisation.py:43-82): unknown on N = (Nbar+1)/2, coefficient the
+    exactly integrated Aga of Material.get_A_Ga(Nbar). Returns Struct(AH, e, Fu, info, time):
+    e = F^-1 grad Fu on the quadrature grid (the reference's Nbar grid when 2N-1 has no prime
+    factor above 13), Fu on the grid N."""
+    Nbar = tuple(int(n) for n in Aga.N)
+    N = tuple((n + 1) // 2 for n in Nbar)
+    Y = np.asarray(Aga.Y, dtype=np.float64)
+    d = len(N)
+    tol, maxiter = _solver_par(pars)
+    coarse = nat.Plan(N, Y, 'scalar', 'primal')
+    ga = Aga.load(nat.Ga(coarse))
+    M = ga.M
+    plan = nat.Plan(M, Y, 'scalar', 'primal')
+    plan.set_band([(n - 1) // 2 for n in N], 1.0)
+    ga.set_plan_coeff(plan)
+    E = _e1(d)
+    X = nat.DeviceArray((d,) + M)
+    t0 = time.perf_counter()
+    FuM, info = plan.potential_solve(E, tol, maxiter, X=X)
+    elapsed = time.perf_counter() - t0
+    XEN = X.copy()
+    nat.add_constant(XEN, E)
+    AH = ga.energy_matrix([XEN] * d, on_grid=True)[0, 0]
+    Fu = _crop_spectrum(FuM.to_host(), M, N)
+    return Struct(AH=AH, e=Tensor(name='e', val=X, order=1, Y=Y), Fu=_fourier_tensor('Fu', Fu, N, Y), info=info,
+                  time=elapsed, pars=pars)

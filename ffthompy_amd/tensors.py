@@ -124,8 +124,14 @@ class Tensor:
         return scalar_product(self, self) ** 0.5
 
     def mean(self):
-        """Mean per component (objects.py:259-272)."""
+        """Mean per component (objects.py:259-272); of an 'r' spectrum: Re(val[0]) / prod(N)."""
         m = np.zeros(self.shape)
+        if self.Fourier or not self.on_device:
+            val = self.val.to_host() if self.on_device else np.asarray(self.val)
+            for di in (np.ndindex(*self.shape) if self.order else [()]):
+                v = val[di]
+                m[di] = np.real(v[(0,) * v.ndim]) / float(np.prod(self.N)) if self.Fourier else np.mean(v)
+            return m
         n = self.pN()
         ones = _ones(n, self.val.dtype)
         flat = self.val.reshape((int(np.prod(self.shape)) if self.order else 1, n))
@@ -135,7 +141,11 @@ class Tensor:
         return m
 
     def add_mean(self, mean):
-        nat.add_constant(self.val, np.asarray(mean, dtype=np.float64).ravel())
+        mean = np.asarray(mean, dtype=np.float64)
+        if not self.on_device:  # a coefficient held on the host: add to every voxel
+            self.val = self.val + mean.reshape(mean.shape + (1,) * len(self.N))
+            return self
+        nat.add_constant(self.val, mean.ravel())
         return self
 
     def set_mean(self, mean):

@@ -17,6 +17,9 @@ Each function cites the reference file:line it restates. What pins each part:
 * ``solver``       -- textbook CG, dense brute-force solve on tiny grids, golden
                       iteration counts.
 * ``homogenize``   -- golden AH matrices of every reference example problem (1e-9).
+* ``potential``    -- golden GaNi / Ga values of the gradient formulation (same Galerkin
+                      problems), the constant-coefficient multiplier closed form, symmetry and
+                      positivity of the operator.
 
 Parity unpinned (no fixture or invariant fixes the value, documented in the tests):
 * ``materials.weights_ball`` in 3-D: the reference uses the 2-D disk formula

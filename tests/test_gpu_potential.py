@@ -1,0 +1,164 @@
+"""Potential formulation on the GPU (fh_potential_solve through ffthompy_amd.potential) against
+the oracle (oracle/potential.py), fp64 both:
+
+* GaNi: CG iteration counts equal, AH within 1e-12 (relative), Fu within 1e-10 of its largest
+  entry; Aga = None and Ga post-processing (exact and order-0 coefficients); 2-D and 3-D; an
+  anisotropic (full) coefficient; a rough random 27^3 coefficient (residual histories, count at a
+  tolerance with margin, AH).
+* Ga: on quadrature grids 2N-1 = 9 (as the reference) and 29 (the device solves on 30): counts
+  equal, AH within 1e-10, Fu on the grid N.
+* At 45^3 (random spheres, contrast 10) the potential value equals the gradient formulation's
+  AH_11 from the device CG (both to tol 1e-10): size-independent check of the same Galerkin
+  problem.
+* Even grids without a band are rejected.
+"""
+import numpy as np
+import pytest
+
+from oracle import examples
+from oracle import potential as opot
+from oracle.fields import Field
+from oracle.materials import Material as OMaterial
+
+pytestmark = pytest.mark.gpu
+nat = pytest.importorskip('ffthompy_amd._native')
+from ffthompy_amd import potential as gpot  # noqa: E402
+from ffthompy_amd.materials import Material  # noqa: E402
+from ffthompy_amd.microstructure import spheres_conductivity  # noqa: E402
+
+ANISO = {2: np.array([[5.5, -4.5], [-4.5, 5.5]]),
+         3: np.array([[4.25, -3.25, -1.76776695], [-3.25, 4.25, 1.76776695], [-1.76776695, 1.76776695, 7.5]])}
+PARS = {'solver': {'tol': 1e-10, 'maxiter': 300}}
+
+
+def _check_fu(got, ref, tol):
+    g = got.val.to_host() if hasattr(got.val, 'to_host') else np.asarray(got.val)
+    assert g.shape == ref.val.shape
+    assert np.abs(g - ref.val).max() <= tol * np.abs(ref.val).max()
+
+
+def _materials(maker, name):
+    conf = maker()[name]
+    return Material(dict(conf)), OMaterial(dict(conf))
+
+
+@pytest.mark.parametrize('maker,name,N,ga', [
+    (examples.scalar_2d_materials, 'square', (5, 5), None),
+    (examples.scalar_2d_materials, 'square', (45, 45), None),
+    (examples.scalar_2d_materials, 'pyramid', (15, 15), 'exact'),
+    (examples.scalar_2d_materials, 'square_Ga', (5, 5), 0),
+    (examples.scalar_3d_materials, 'cube', (5, 5, 5), 'exact'),
+    (examples.scalar_3d_materials, 'cube', (15, 15, 15), None),
+])
+def test_gani_potential_matches_oracle(maker, name, N, ga):
+    print('case gani potential', name, N, ga, flush=True)
+    mat, omat = _materials(maker, name)
+    Agani, oAgani = mat.get_A_GaNi(N, 'primal'), omat.get_A_GaNi(N, 'primal')
+    Nbar = tuple(2 * n - 1 for n in N)
+    order = None if ga == 'exact' else ga
+    Aga = None if ga is None else mat.get_A_Ga(Nbar, 'primal', order=order, P=N if order == 0 else None)
+    oAga = None if ga is None else omat.get_A_Ga(Nbar, 'primal', order=order, P=N if order == 0 else None)
+    res = gpot.homog_GaNi_full_potential(Agani, Aga, PARS)
+    ref = opot.homog_GaNi_full_potential(oAgani, oAga, tol=1e-10, maxiter=300)
+    assert res.info['kit'] == ref['info']['kit']
+    assert abs(res.AH - ref['AH']) < 1e-12 * abs(ref['AH'])
+    _check_fu(res.Fu, ref['Fu'], 1e-10)
+    assert abs(res.Fu.mean()) < 1e-14
+
+
+@pytest.mark.parametrize('maker,name,N', [
+    (examples.scalar_2d_materials, 'square', (5, 5)),
+    (examples.scalar_2d_materials, 'ball', (5, 5)),
+    (examples.scalar_2d_materials, 'pyramid', (15, 15)),
+    (examples.scalar_3d_materials, 'cube_Ga', (5, 5, 5)),
+    (examples.from_file_materials, 'file', (5, 5)),
+])
+def test_ga_potential_matches_oracle(maker, name, N):
+    print('case ga potential', name, N, flush=True)
+    mat, omat = _materials(maker, name)
+    Nbar = tuple(2 * n - 1 for n in N)
+    res = gpot.homog_Ga_full_potential(mat.get_A_Ga(Nbar, 'primal'), PARS)
+    ref = opot.homog_Ga_full_potential(omat.get_A_Ga(Nbar, 'primal'), tol=1e-10, maxiter=300)
+    assert res.info['kit'] == ref['info']['kit']
+    assert abs(res.AH - ref['AH']) < 1e-10 * abs(ref['AH'])
+    _check_fu(res.Fu, ref['Fu'], 1e-9)
+
+
+@pytest.mark.parametrize('d', [2, 3])
+def test_anisotropic_coefficient_matches_oracle(d):
+    """The reference's materials with an anisotropic constant added to both coefficients
+    (square inclusion, order-0 Ga coefficient on P = N)."""
+    print('case aniso potential', d, flush=True)
+    N = (5,) * d
+    conf = {'inclusions': ['square', 'otherwise'], 'positions': [np.zeros(d), ''], 'params': [0.6 * np.ones(d), ''],
+            'vals': [10 * np.eye(d), 1. * np.eye(d)], 'Y': np.ones(d), 'P': N, 'order': 0}
+    mat, omat = Material(dict(conf)), OMaterial(dict(conf))
+    Nbar = tuple(2 * n - 1 for n in N)
+    Agani = mat.get_A_GaNi(N, 'primal').add_mean(ANISO[d])
+    oAgani = omat.get_A_GaNi(N, 'primal').add_mean(ANISO[d])
+    Aga = mat.get_A_Ga(Nbar, 'primal').add_mean(ANISO[d])
+    oAga = omat.get_A_Ga(Nbar, 'primal').add_mean(ANISO[d])
+    res = gpot.homog_GaNi_full_potential(Agani, Aga, PARS)
+    ref = opot.homog_GaNi_full_potential(oAgani, oAga, tol=1e-10, maxiter=300)
+    assert res.info['kit'] == ref['info']['kit']
+    assert abs(res.AH - ref['AH']) < 1e-12 * abs(ref['AH'])
+    res = gpot.homog_Ga_full_potential(Aga, PARS)
+    ref = opot.homog_Ga_full_potential(oAga, tol=1e-10, maxiter=300)
+    assert res.info['kit'] == ref['info']['kit']
+    assert abs(res.AH - ref['AH']) < 1e-10 * abs(ref['AH'])
+
+
+def test_random_coefficient_27_matches_oracle():
+    """A rough coefficient (exp of white noise): the residual histories agree while the residual
+    is above 1e-7, the iteration count at a tolerance placed midway (geometrically) between the
+    oracle's residuals of iterations 40 and 41 is equal, and AH at tol 1e-10 agrees to 1e-12 (the
+    late iterations of the two solves round differently, so their counts at 1e-10 may differ)."""
+    print('case potential random 27^3', flush=True)
+    N = (27, 27, 27)
+    rng = np.random.default_rng(11)
+    a = np.exp(rng.standard_normal(N))
+    A = np.einsum('ij,...->ij...', np.eye(3), a)
+    ref = opot.homog_GaNi_full_potential(Field(val=A, order=2, multype=21), None, tol=1e-10, maxiter=300)
+    log = np.asarray(ref['info']['norm_res_log'])
+    plan = nat.Plan(N, (1., 1., 1.), 'scalar', 'primal')
+    plan.set_coeff('iso', a)
+    _, info = plan.potential_solve(np.eye(3)[0], 1e-10, 300, hist_len=400)
+    got = info['res_log']
+    m = int(np.sum(log > 1e-7))
+    assert np.abs(got[:m] - log[:m]).max() <= 1e-4 * log[:m].min() + 1e-9 * log[0]
+    tol = float(np.sqrt(log[40] * log[41]))
+    _, info = plan.potential_solve(np.eye(3)[0], tol, 300)
+    assert info['kit'] == 41
+    Agani = gpot.Tensor(name='A', val=A, order=2, multype=21)
+    res = gpot.homog_GaNi_full_potential(Agani, None, PARS)
+    assert abs(res.AH - ref['AH']) < 1e-12 * abs(ref['AH'])
+    _check_fu(res.Fu, ref['Fu'], 1e-8)
+
+
+def test_potential_equals_gradient_formulation_45():
+    print('case potential vs gradient 45^3', flush=True)
+    N = (45, 45, 45)
+    a = spheres_conductivity(N, count=30, contrast=10.)[0]
+    plan = nat.Plan(N, (1., 1., 1.), 'scalar', 'primal')
+    plan.set_coeff('iso', a)
+    E = np.eye(3)[0]
+    EN = nat.DeviceArray.zeros((3,) + N)
+    nat.add_constant(EN, -E)
+    B = nat.DeviceArray((3,) + N)
+    plan.matvec(EN, B)
+    x = nat.DeviceArray.zeros((3,) + N)
+    info = plan.cg_solve(B, x, tol=1e-10, maxiter=2000, mode='fp64')
+    assert info['status'] == nat.FH_CG_OK
+    nat.add_constant(x, E)
+    ah_grad = plan.energy_matrix([x, x, x])[0, 0]
+    Agani = gpot.Tensor(name='A', val=np.einsum('ij,...->ij...', np.eye(3), a), order=2, multype=21)
+    res = gpot.homog_GaNi_full_potential(Agani, None, {'solver': {'tol': 1e-10, 'maxiter': 2000}})
+    assert res.info['status'] == 0
+    assert abs(res.AH - ah_grad) < 1e-9 * ah_grad, (res.AH, ah_grad)
+
+
+def test_even_grid_without_band_rejected():
+    plan = nat.Plan((6, 6), (1., 1.), 'scalar', 'primal')
+    plan.set_coeff('iso', np.ones((6, 6)))
+    with pytest.raises(nat.FhError):
+        plan.potential_solve(np.eye(2)[0], 1e-8, 10)
