@@ -772,17 +772,25 @@ fh::pot::PotParams pot_params(const fh_plan* pl) {
   return g;
 }
 
-// F A F^-1 on the d-component spectrum in pl->spec (the real field passes through `re`,
-// ncomp * n doubles): inverse passes, coefficient multiply fused into the forward row pass,
-// forward passes.
+// F A F^-1 on the d-component spectrum in pl->spec: inverse strided passes, then along the
+// contiguous axis one in-place kernel (C2R, isotropic a(x), R2C; the real field stays in LDS)
+// where the fixed lengths cover the grid, else the inverse row pass into `re` (ncomp * n
+// doubles) and the forward row pass with the coefficient; forward strided passes.
 int pot_coef_round_trip(fh_plan* pl, double* re, const int* halt) {
   const int nc = pl->ncomp;
   cpx<double>* spec = reinterpret_cast<cpx<double>*>(pl->spec);
   const GParams gp = make_gparams(pl, 1.0);
   if (pl->n0 > 1) RETCHK(launch_cols<double>(pl, 1, 0, nc, spec, gp, halt));
   RETCHK(launch_cols<double>(pl, 1, 1, nc, spec, gp, halt));
-  RETCHK(launch_rows_inv<double>(pl, nc, spec, re, 1.0 / (double)pl->n, nullptr, nullptr, halt, nullptr));
-  RETCHK(launch_rows_fwd<double>(pl, nc, re, pl->coef64, pl->coef_kind, spec, halt));
+  int fused = 0;
+  if (!pl->generic && pl->coef_kind == fh::COEF_ISO) {
+    fused = fh::fixed_rows_rt<double>(make_ctx<double>(pl), nc, spec, pl->coef64, 1.0 / (double)pl->n, halt);
+    if (fused < 0) return fixed_result(fused, "rows round trip (fixed)");
+  }
+  if (!fused) {
+    RETCHK(launch_rows_inv<double>(pl, nc, spec, re, 1.0 / (double)pl->n, nullptr, nullptr, halt, nullptr));
+    RETCHK(launch_rows_fwd<double>(pl, nc, re, pl->coef64, pl->coef_kind, spec, halt));
+  }
   RETCHK(launch_cols<double>(pl, 0, 1, nc, spec, gp, halt));
   if (pl->n0 > 1) RETCHK(launch_cols<double>(pl, 0, 0, nc, spec, gp, halt));
   return FH_OK;

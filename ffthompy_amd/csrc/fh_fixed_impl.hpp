@@ -286,6 +286,48 @@ int cols_g_fx_launch(const LaunchCtx& c, int axis, int nc, cpx<T>* data, const G
   }
 }
 
+template <typename T, class FP>
+int rows_rt_fx_launch(const LaunchCtx& c, int nc, cpx<T>* spec, const T* coef, T scale, const int* halt) {
+  // half the rows of the one-way passes: the round trip holds its pencils twice as long, and
+  // the smaller LDS tile lets more workgroups share a CU
+  constexpr int rpb = FxShape<FP, T>::RPB >= 4 ? FxShape<FP, T>::RPB / 2 : 2;
+  const int npen = (rpb / 2) * nc;
+  const int nthr = npen * FP::TPP;
+  if (nthr > 1024) return 0;
+  if (!offsets_fit(c.nh * nc, sizeof(cpx<T>))) return 0;
+  const long long nrows = (long long)c.n0 * c.n1;
+  const long long nb = (nrows + rpb - 1) / rpb;
+  const dim3 grid((unsigned)nb), block(nthr);
+  const cpx<T>* tw = reinterpret_cast<const cpx<T>*>(c.tw[2]);
+#define FH_LAUNCH_RT(NCC)                                                                                     \
+  do {                                                                                                        \
+    const size_t slds = (size_t)RowStage<FP, T, NCC>::pen_stride(rpb) * FP::L * sizeof(cpx<T>);              \
+    if (slds + FP::L * sizeof(cpx<T>) > 160 * 1024) return 0;                                                 \
+    auto kf = k_rows_rt_st<T, FP, NCC>;                                                                       \
+    allow_lds(kf, slds);                                                                                      \
+    hipExtLaunchKernelGGL(kf, grid, block, slds, c.stream, c.t0, c.t1, 0, spec, coef, nrows, c.n, rpb, tw, scale, halt); \
+  } while (0)
+  switch (nc) {
+    case 2: FH_LAUNCH_RT(2); break;
+    case 3: FH_LAUNCH_RT(3); break;
+    default: return 0;
+  }
+#undef FH_LAUNCH_RT
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 1 : -(int)e;
+}
+
+template <typename T>
+int fixed_rows_rt(const LaunchCtx& c, int nc, cpx<T>* spec, const T* coef, T scale, const int* halt) {
+  switch (c.n2) {
+#define FH_CASE(L, FP, FPR) \
+  case L: return rows_rt_fx_launch<T, FP>(c, nc, spec, coef, scale, halt);
+    FH_FIXED_LENGTHS(FH_CASE)
+#undef FH_CASE
+    default: return 0;
+  }
+}
+
 template <typename T>
 int fixed_rows_fwd(const LaunchCtx& c, int nc, const T* in, const T* coef, int kind, cpx<T>* out,
                    const int* halt, const T* rvec, const CGState* st, T* xc, double* partial,

@@ -31,6 +31,10 @@ int fixed_rows_fwd(const LaunchCtx& c, int nc, const T* in, const T* coef, int k
 template <typename T>
 int fixed_rows_inv(const LaunchCtx& c, int nc, const cpx<T>* in, T* out, T scale, int epi, const T* w,
                    double* partial, size_t partial_cap, const int* halt, int* nblocks, CGState* st);
+// In place on the AoS half spectrum: C2R (times scale), isotropic a(x), R2C along the
+// contiguous axis (k_rows_rt_st); nc 2 or 3.
+template <typename T>
+int fixed_rows_rt(const LaunchCtx& c, int nc, cpx<T>* spec, const T* coef, T scale, const int* halt);
 template <typename T>
 int fixed_cols(const LaunchCtx& c, int dir, int axis, int nc, cpx<T>* data, const int* halt);
 template <typename T>
@@ -48,6 +52,7 @@ extern template int fixed_rows_inv<float>(const LaunchCtx&, int, const cpx<float
                                           const float*, double*, size_t, const int*, int*, CGState*);
 extern template int fixed_rows_inv<double>(const LaunchCtx&, int, const cpx<double>*, double*, double, int,
                                            const double*, double*, size_t, const int*, int*, CGState*);
+extern template int fixed_rows_rt<double>(const LaunchCtx&, int, cpx<double>*, const double*, double, const int*);
 extern template int fixed_cols<float>(const LaunchCtx&, int, int, int, cpx<float>*, const int*);
 extern template int fixed_cols<double>(const LaunchCtx&, int, int, int, cpx<double>*, const int*);
 extern template int fixed_cols_g<float>(const LaunchCtx&, int, int, cpx<float>*, const GParams&, const int*);

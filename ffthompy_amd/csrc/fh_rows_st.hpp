@@ -415,4 +415,110 @@ k_rows_inv_st(const cpx<T>* __restrict__ in, T* __restrict__ out, long long nrow
   }
 }
 
+// ---- round trip: C2R, a(x) v, R2C along the contiguous axis, in place ---------------------
+// The middle of F A F^-1 for a field held as a spectrum (the potential formulation): the AoS
+// half spectrum of rpb rows x NC components is completed into pencils as in k_rows_inv_st,
+// transformed back (times scale), multiplied by the isotropic coefficient and transformed
+// forward as in k_rows_fwd_st; the result overwrites the block's own rows. A pencil holds the
+// same two real rows (row, component) in both directions, so the product needs no exchange
+// between pencils and the real field never leaves LDS. a(x) is read straight from HBM: the
+// block's rows are one contiguous run, shared by the NC pencils of a row pair.
+template <typename T, class FP, int NC>
+__global__ void __launch_bounds__(1024)
+k_rows_rt_st(cpx<T>* __restrict__ spec, const T* __restrict__ coef, long long nrows, long long n, int rpb,
+             const cpx<T>* __restrict__ twg, T scale, const int* __restrict__ halt) {
+  if (halt && *halt) return;
+  constexpr int L = FP::L, n2h = L / 2 + 1;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<T>* s = reinterpret_cast<cpx<T>*>(smem_raw);
+  __shared__ cpx<T> twl[L];
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int npen = (rpb / 2) * NC, S = RowStage<FP, T, NC>::pen_stride(rpb);
+  const int pen = tid % npen, t = tid / npen;
+  const long long row0 = (long long)xcd_remap(blockIdx.x, gridDim.x) * rpb;
+  const int nr = (int)min((long long)rpb, nrows - row0);
+  for (int e = tid; e < L; e += nthr) twl[e] = twg[e];
+  // Hermitian completion (rows past nr read 0 through the buffer range)
+  const int nrp = rpb >> 1;
+  constexpr unsigned CS = sizeof(cpx<T>);
+  cpx<T>* blk = spec + row0 * (long long)n2h * NC;
+  const __amdgpu_buffer_rsrc_t rsp = make_rsrc(blk, (unsigned long long)nr * n2h * NC * CS);
+  for (int e = tid; e < nrp * n2h; e += nthr) {
+    const int rp = e / n2h, k = e - rp * n2h;
+    const bool self = (k == 0) || (2 * k == L);
+    const unsigned o0 = (unsigned)(((2 * rp) * n2h + k) * NC) * CS;
+    const unsigned o1 = o0 + (unsigned)(n2h * NC) * CS;
+    cpx<T> Y[2 * NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      Y[c] = bload_c<T>(rsp, o0 + c * CS);
+      Y[NC + c] = bload_c<T>(rsp, o1 + c * CS);
+    }
+    const int kc = k ? L - k : 0;
+#pragma unroll
+    for (int m = 0; m < NC; ++m) {
+      cpx<T> Ya = Y[2 * m], Yb = Y[2 * m + 1];
+      if (self) {
+        Ya.y = T(0);
+        Yb.y = T(0);
+      }
+      const int q = rp * NC + m;
+      s[k * S + q] = mk(Ya.x - Yb.y, Ya.y + Yb.x);
+      s[kc * S + q] = mk(Ya.x + Yb.y, Yb.x - Ya.y);
+    }
+  }
+  __syncthreads();
+  auto at = [&](int j, int) -> cpx<T>& { return s[j * S + pen]; };
+  fx_lds_stages<T, FP, +1, 0, FP::NST - 1, 1>(t, at, twl);
+  {
+    // last inverse stage, then a(x) times the two real rows, back into the pencil
+    const int rp_ = pen / NC, m_ = pen - rp_ * NC;
+    const int rowa = 2 * rp_ + (2 * m_) / NC, rowb = 2 * rp_ + (2 * m_ + 1) / NC;
+    const bool va = rowa < nr, vb = rowb < nr;
+    const T* aa = coef + (row0 + (va ? rowa : 0)) * L;
+    const T* ab = coef + (row0 + (vb ? rowb : 0)) * L;
+    FxStage<T, FP, FP::NST - 1, +1> sl;
+    sl.load(t, [&](int j) { return s[j * S + pen]; });
+    __syncthreads();
+    sl.compute(t, twl);
+    sl.store(t, [&](int j, cpx<T> v) {
+      s[j * S + pen] = mk(va ? scale * v.x * aa[j] : T(0), vb ? scale * v.y * ab[j] : T(0));
+    });
+  }
+  __syncthreads();
+  {
+    FxStage<T, FP, 0, -1> s0;
+    s0.load(t, [&](int x) { return s[x * S + pen]; });
+    __syncthreads();
+    s0.compute(t, twl);
+    s0.store(t, [&](int j, cpx<T> v) { s[j * S + pen] = v; });
+  }
+  __syncthreads();
+  fx_lds_stages<T, FP, -1, 1, FP::NST, 1>(t, at, twl);
+  // untangle the paired rows into the AoS half spectrum (as k_rows_fwd_st)
+  const int nrp_v = (nr + 1) >> 1;
+  for (int e = tid; e < nrp_v * n2h; e += nthr) {
+    const int rp = e / n2h, k = e - rp * n2h;
+    const int kc = k ? L - k : 0;
+    cpx<T> Y[2 * NC];
+#pragma unroll
+    for (int m = 0; m < NC; ++m) {
+      const int q = rp * NC + m;
+      const cpx<T> Z = s[k * S + q];
+      const cpx<T> Zc = conj(s[kc * S + q]);
+      const cpx<T> d = Z - Zc;
+      Y[2 * m] = T(0.5) * (Z + Zc);
+      Y[2 * m + 1] = mk(T(0.5) * d.y, T(-0.5) * d.x);
+    }
+    cpx<T>* o0 = blk + ((long long)(2 * rp) * n2h + k) * NC;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) o0[c] = Y[c];
+    if (2 * rp + 1 < nr) {
+      cpx<T>* o1 = o0 + (long long)n2h * NC;
+#pragma unroll
+      for (int c = 0; c < NC; ++c) o1[c] = Y[NC + c];
+    }
+  }
+}
+
 }  // namespace fh
