@@ -772,44 +772,60 @@ fh::pot::PotParams pot_params(const fh_plan* pl) {
   return g;
 }
 
-// F A F^-1 on the d-component spectrum in pl->spec: inverse strided passes, then along the
-// contiguous axis one in-place kernel (C2R, isotropic a(x), R2C; the real field stays in LDS)
-// where the fixed lengths cover the grid, else the inverse row pass into `re` (ncomp * n
-// doubles) and the forward row pass with the coefficient; forward strided passes.
-int pot_coef_round_trip(fh_plan* pl, double* re, const int* halt) {
+// The potential operator out = P (-div) F A F^-1 grad (P p) on the plan's spectra, p first
+// replaced by r + beta p when r != null; partial sums of the weighted <out, w> into pl->partial
+// (*nb workgroups). Along the outermost axis the expansion into the gradient and the contraction
+// of the divergence run inside that axis' strided passes, and along the contiguous axis C2R,
+// a(x) and R2C run as one in-place kernel (the real field stays in LDS), where the fixed lengths
+// cover the grid; otherwise separate elementwise kernels and the one-way passes (with the real
+// field in `re`, ncomp * n doubles).
+int pot_operator(fh_plan* pl, cpx<double>* p, const cpx<double>* r, cpx<double>* out, double* re,
+                 const cpx<double>* w, const int* halt, int* nb) {
   const int nc = pl->ncomp;
+  const fh::pot::PotParams g = pot_params(pl);
   cpx<double>* spec = reinterpret_cast<cpx<double>*>(pl->spec);
   const GParams gp = make_gparams(pl, 1.0);
-  if (pl->n0 > 1) RETCHK(launch_cols<double>(pl, 1, 0, nc, spec, gp, halt));
-  RETCHK(launch_cols<double>(pl, 1, 1, nc, spec, gp, halt));
-  int fused = 0;
-  if (!pl->generic && pl->coef_kind == fh::COEF_ISO) {
-    fused = fh::fixed_rows_rt<double>(make_ctx<double>(pl), nc, spec, pl->coef64, 1.0 / (double)pl->n, halt);
-    if (fused < 0) return fixed_result(fused, "rows round trip (fixed)");
+  const int gb = grid_blocks(pl->nh);
+  const int ax = pl->n0 > 1 ? 0 : 1;  // outermost axis of the grid
+  const fh::LaunchCtx ctx = make_ctx<double>(pl);
+  int in_fused = 0;
+  if (!pl->generic) {
+    in_fused = fh::fixed_pot_cols<double>(ctx, ax, +1, nc, spec, p, r, pl->st, 0.0, nullptr, nullptr, nullptr, 0, g,
+                                          halt, nullptr);
+    if (in_fused < 0) return fixed_result(in_fused, "potential strided pass (fixed)");
   }
-  if (!fused) {
+  if (!in_fused) {
+    if (r)
+      hipLaunchKernelGGL((fh::k_cg_pupdate<double>), dim3(gb), dim3(256), 0, pl->stream, 2 * pl->nh, (double*)p,
+                         (const double*)r, pl->st);
+    hipLaunchKernelGGL(fh::pot::k_pot_grad, dim3(gb), dim3(256), 0, pl->stream, p, spec, g, 1, halt);
+    HIPCHK(hipGetLastError());
+    RETCHK(launch_cols<double>(pl, 1, ax, nc, spec, gp, halt));
+  }
+  if (ax == 0) RETCHK(launch_cols<double>(pl, 1, 1, nc, spec, gp, halt));
+  int rt_fused = 0;
+  if (!pl->generic && pl->coef_kind == fh::COEF_ISO) {
+    rt_fused = fh::fixed_rows_rt<double>(ctx, nc, spec, pl->coef64, 1.0 / (double)pl->n, halt);
+    if (rt_fused < 0) return fixed_result(rt_fused, "rows round trip (fixed)");
+  }
+  if (!rt_fused) {
     RETCHK(launch_rows_inv<double>(pl, nc, spec, re, 1.0 / (double)pl->n, nullptr, nullptr, halt, nullptr));
     RETCHK(launch_rows_fwd<double>(pl, nc, re, pl->coef64, pl->coef_kind, spec, halt));
   }
-  RETCHK(launch_cols<double>(pl, 0, 1, nc, spec, gp, halt));
-  if (pl->n0 > 1) RETCHK(launch_cols<double>(pl, 0, 0, nc, spec, gp, halt));
-  return FH_OK;
-}
-
-// out = P (-div) F A F^-1 grad (P in); partial sums of the weighted <out, w> into pl->partial
-// (*nb workgroups).
-int pot_operator(fh_plan* pl, const cpx<double>* in, cpx<double>* out, double* re, const cpx<double>* w,
-                 const int* halt, int* nb) {
-  const fh::pot::PotParams g = pot_params(pl);
-  cpx<double>* spec = reinterpret_cast<cpx<double>*>(pl->spec);
-  const int gb = grid_blocks(pl->nh);
-  hipLaunchKernelGGL(fh::pot::k_pot_grad, dim3(gb), dim3(256), 0, pl->stream, in, spec, g, 1, halt);
-  HIPCHK(hipGetLastError());
-  RETCHK(pot_coef_round_trip(pl, re, halt));
-  hipLaunchKernelGGL(fh::pot::k_pot_div, dim3(gb), dim3(256), 0, pl->stream, spec, out, g, -1.0, w, pl->partial,
-                     halt);
-  HIPCHK(hipGetLastError());
-  *nb = gb;
+  if (ax == 0) RETCHK(launch_cols<double>(pl, 0, 1, nc, spec, gp, halt));
+  int out_fused = 0;
+  if (!pl->generic) {
+    out_fused = fh::fixed_pot_cols<double>(ctx, ax, -1, nc, spec, nullptr, nullptr, nullptr, -1.0, out, w,
+                                           pl->partial, pl->partial_cap, g, halt, nb);
+    if (out_fused < 0) return fixed_result(out_fused, "potential strided pass (fixed)");
+  }
+  if (!out_fused) {
+    RETCHK(launch_cols<double>(pl, 0, ax, nc, spec, gp, halt));
+    hipLaunchKernelGGL(fh::pot::k_pot_div, dim3(gb), dim3(256), 0, pl->stream, spec, out, g, -1.0, w, pl->partial,
+                       halt);
+    HIPCHK(hipGetLastError());
+    *nb = gb;
+  }
   return FH_OK;
 }
 
@@ -2411,10 +2427,8 @@ int fh_potential_solve(fh_plan* pl, const double* E, double tol, int maxiter, vo
   const int per_check = 8;
   for (;;) {
     for (int k = 0; k < per_check; ++k) {
-      hipLaunchKernelGGL((fh::k_cg_pupdate<double>), grid, block, 0, pl->stream, 2 * nh, (double*)p,
-                         (const double*)r, pl->st);
       int nb = 0;
-      RETCHK(pot_operator(pl, p, Ap, re, p, halt, &nb));
+      RETCHK(pot_operator(pl, p, r, Ap, re, p, halt, &nb));
       hipLaunchKernelGGL(fh::k_cg_alpha, dim3(1), dim3(kFinThreads), 0, pl->stream, pl->st, pl->partial, nb);
       hipLaunchKernelGGL(fh::pot::k_pot_update, grid, block, 0, pl->stream, x, r, p, Ap, g, pl->st, pl->partial);
       hipLaunchKernelGGL(fh::k_cg_beta, dim3(1), dim3(kFinThreads), 0, pl->stream, pl->st, pl->partial, gb);

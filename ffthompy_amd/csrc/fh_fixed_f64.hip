@@ -7,6 +7,9 @@ template int fixed_rows_fwd<double>(const LaunchCtx&, int, const double*, const 
 template int fixed_rows_inv<double>(const LaunchCtx&, int, const cpx<double>*, double*, double, int, const double*, double*,
                                    size_t, const int*, int*, CGState*);
 template int fixed_rows_rt<double>(const LaunchCtx&, int, cpx<double>*, const double*, double, const int*);
+template int fixed_pot_cols<double>(const LaunchCtx&, int, int, int, cpx<double>*, cpx<double>*, const cpx<double>*,
+                                    const CGState*, double, cpx<double>*, const cpx<double>*, double*, size_t,
+                                    const pot::PotParams&, const int*, int*);
 template int fixed_cols<double>(const LaunchCtx&, int, int, int, cpx<double>*, const int*);
 template int fixed_cols_g<double>(const LaunchCtx&, int, int, cpx<double>*, const GParams&, const int*);
 }  // namespace fh

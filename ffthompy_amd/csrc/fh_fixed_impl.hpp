@@ -4,6 +4,7 @@
 #include "fh_fixed.hpp"
 #include "fh_launch.hpp"
 #include "fh_rows_st.hpp"
+#include "fh_pot_fx.hpp"
 
 namespace fh {
 
@@ -322,6 +323,67 @@ int fixed_rows_rt(const LaunchCtx& c, int nc, cpx<T>* spec, const T* coef, T sca
   switch (c.n2) {
 #define FH_CASE(L, FP, FPR) \
   case L: return rows_rt_fx_launch<T, FP>(c, nc, spec, coef, scale, halt);
+    FH_FIXED_LENGTHS(FH_CASE)
+#undef FH_CASE
+    default: return 0;
+  }
+}
+
+// frequency points per tile of the potential's strided passes (as the projection tile)
+template <typename T, class FP, int NC>
+constexpr int pot_points() {
+  int f = FxShape<FP, T>::F;
+  while (f > 1 && FP::TPP * f * NC > 1024) f /= 2;
+  return f;
+}
+
+template <typename T, class FP, int NC>
+int pot_cols_launch_nc(const LaunchCtx& c, int axis, int dir, cpx<T>* data, cpx<T>* p, const cpx<T>* r,
+                       const CGState* st, T sgn, cpx<T>* out, const cpx<T>* w, double* partial, size_t partial_cap,
+                       const pot::PotParams& g, const int* halt, int* nblocks) {
+  constexpr int F = pot_points<T, FP, NC>();
+  if (FP::TPP * F * NC > 1024) return 0;
+  long long outer, ostride, inner;
+  strided_geometry(c, axis, NC, &outer, &ostride, &inner);
+  if (outer != 1) return 0;
+  const long long total = c.nh * NC;
+  if (!offsets_fit(total, sizeof(cpx<T>))) return 0;
+  const long long ntiles = (inner + F * NC - 1) / (F * NC);
+  if (ntiles > 0x7fffffffLL) return 0;
+  if (dir < 0 && w && (size_t)ntiles > partial_cap) return 0;
+  const dim3 grid((unsigned)ntiles), block(FP::TPP * F * NC);
+  const cpx<T>* tw = reinterpret_cast<const cpx<T>*>(c.tw[axis]);
+  if (dir > 0) {
+    if (r)
+      hipExtLaunchKernelGGL((pot::k_pot_cols_in<T, FP, F, NC, 1>), grid, block, 0, c.stream, c.t0, c.t1, 0, p, r, st,
+                            data, inner, total, tw, g, halt);
+    else
+      hipExtLaunchKernelGGL((pot::k_pot_cols_in<T, FP, F, NC, 0>), grid, block, 0, c.stream, c.t0, c.t1, 0, p, r, st,
+                            data, inner, total, tw, g, halt);
+  } else {
+    hipExtLaunchKernelGGL((pot::k_pot_cols_out<T, FP, F, NC>), grid, block, 0, c.stream, c.t0, c.t1, 0, data, inner,
+                          total, tw, g, sgn, out, w, partial, halt);
+  }
+  if (nblocks) *nblocks = (int)ntiles;
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 1 : -(int)e;
+}
+
+template <typename T>
+int fixed_pot_cols(const LaunchCtx& c, int axis, int dir, int nc, cpx<T>* data, cpx<T>* p, const cpx<T>* r,
+                   const CGState* st, T sgn, cpx<T>* out, const cpx<T>* w, double* partial, size_t partial_cap,
+                   const pot::PotParams& g, const int* halt, int* nblocks) {
+  const int L = axis == 0 ? c.n0 : c.n1;
+  switch (L) {
+#define FH_CASE(LL, FP, FPR)                                                                                      \
+  case LL:                                                                                                        \
+    if (nc == 2)                                                                                                  \
+      return pot_cols_launch_nc<T, FP, 2>(c, axis, dir, data, p, r, st, sgn, out, w, partial, partial_cap, g, halt, \
+                                          nblocks);                                                               \
+    if (nc == 3)                                                                                                  \
+      return pot_cols_launch_nc<T, FP, 3>(c, axis, dir, data, p, r, st, sgn, out, w, partial, partial_cap, g, halt, \
+                                          nblocks);                                                               \
+    return 0;
     FH_FIXED_LENGTHS(FH_CASE)
 #undef FH_CASE
     default: return 0;

@@ -5,6 +5,7 @@
 #include <hip/hip_ext.h>
 
 #include "fh_kernels.hpp"
+#include "fh_pot.hpp"
 
 namespace fh {
 
@@ -35,6 +36,14 @@ int fixed_rows_inv(const LaunchCtx& c, int nc, const cpx<T>* in, T* out, T scale
 // contiguous axis (k_rows_rt_st); nc 2 or 3.
 template <typename T>
 int fixed_rows_rt(const LaunchCtx& c, int nc, cpx<T>* spec, const T* coef, T scale, const int* halt);
+// Potential formulation, strided pass along the outermost axis (0 in 3-D, 1 in 2-D) fused with
+// the scalar <-> gradient spectra: dir +1 expands p (after p = r + beta p when r != null) and
+// transforms back into data; dir -1 transforms data forward and writes sgn P div into out, with
+// partial sums of the weighted <out, w> (w != null; *nblocks workgroups).
+template <typename T>
+int fixed_pot_cols(const LaunchCtx& c, int axis, int dir, int nc, cpx<T>* data, cpx<T>* p, const cpx<T>* r,
+                   const CGState* st, T sgn, cpx<T>* out, const cpx<T>* w, double* partial, size_t partial_cap,
+                   const pot::PotParams& g, const int* halt, int* nblocks);
 template <typename T>
 int fixed_cols(const LaunchCtx& c, int dir, int axis, int nc, cpx<T>* data, const int* halt);
 template <typename T>
@@ -53,6 +62,10 @@ extern template int fixed_rows_inv<float>(const LaunchCtx&, int, const cpx<float
 extern template int fixed_rows_inv<double>(const LaunchCtx&, int, const cpx<double>*, double*, double, int,
                                            const double*, double*, size_t, const int*, int*, CGState*);
 extern template int fixed_rows_rt<double>(const LaunchCtx&, int, cpx<double>*, const double*, double, const int*);
+extern template int fixed_pot_cols<double>(const LaunchCtx&, int, int, int, cpx<double>*, cpx<double>*,
+                                           const cpx<double>*, const CGState*, double, cpx<double>*,
+                                           const cpx<double>*, double*, size_t, const pot::PotParams&, const int*,
+                                           int*);
 extern template int fixed_cols<float>(const LaunchCtx&, int, int, int, cpx<float>*, const int*);
 extern template int fixed_cols<double>(const LaunchCtx&, int, int, int, cpx<double>*, const int*);
 extern template int fixed_cols_g<float>(const LaunchCtx&, int, int, cpx<float>*, const GParams&, const int*);

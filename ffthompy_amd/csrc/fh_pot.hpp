@@ -62,7 +62,7 @@ __device__ __forceinline__ double plane_weight(long long t, const PotParams& g) 
 
 // spec[t][j] = 2 pi i xi_j (P) u[t]: the gradient of the potential in the operator's spectral
 // layout (frequency-major, components adjacent); with_P multiplies by P first.
-__global__ void __launch_bounds__(256)
+static __global__ void __launch_bounds__(256)
 k_pot_grad(const cpx<double>* __restrict__ u, cpx<double>* __restrict__ spec, PotParams g, int with_P,
            const int* __restrict__ halt) {
   if (halt && *halt) return;
@@ -82,7 +82,7 @@ k_pot_grad(const cpx<double>* __restrict__ u, cpx<double>* __restrict__ spec, Po
 
 // out[t] = sgn P sum_j 2 pi i xi_j spec[t][j] (sgn = -1: the operator, +1: the right-hand side
 // div F(A E)); with w != null also per-workgroup partial sums of the weighted <out, w>.
-__global__ void __launch_bounds__(256)
+static __global__ void __launch_bounds__(256)
 k_pot_div(const cpx<double>* __restrict__ spec, cpx<double>* __restrict__ out, PotParams g, double sgn,
           const cpx<double>* __restrict__ w, double* __restrict__ partial, const int* __restrict__ halt) {
   if (halt && *halt) return;
@@ -117,7 +117,7 @@ k_pot_div(const cpx<double>* __restrict__ spec, cpx<double>* __restrict__ out, P
 }
 
 // out[t] = P u[t] inside the band, 0 outside (Fu = P x after the solve).
-__global__ void __launch_bounds__(256)
+static __global__ void __launch_bounds__(256)
 k_pot_precond(const cpx<double>* __restrict__ u, cpx<double>* __restrict__ out, PotParams g) {
   const long long nh = (long long)g.n0 * g.n1 * g.n2h;
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < nh;
@@ -131,7 +131,7 @@ k_pot_precond(const cpx<double>* __restrict__ u, cpx<double>* __restrict__ out, 
 }
 
 // Partial sums of the weighted <r, r> (the initial residual r = b of a start from x = 0).
-__global__ void __launch_bounds__(256)
+static __global__ void __launch_bounds__(256)
 k_pot_norm(const cpx<double>* __restrict__ r, PotParams g, double* __restrict__ partial) {
   const long long nh = (long long)g.n0 * g.n1 * g.n2h;
   double acc = 0.0;
@@ -145,7 +145,7 @@ k_pot_norm(const cpx<double>* __restrict__ r, PotParams g, double* __restrict__ 
 }
 
 // x += alpha p, r -= alpha Ap, partial sums of the weighted <r, r>.
-__global__ void __launch_bounds__(256)
+static __global__ void __launch_bounds__(256)
 k_pot_update(cpx<double>* __restrict__ x, cpx<double>* __restrict__ r, const cpx<double>* __restrict__ p,
              const cpx<double>* __restrict__ Ap, PotParams g, CGState* __restrict__ st,
              double* __restrict__ partial) {
@@ -168,7 +168,7 @@ k_pot_update(cpx<double>* __restrict__ x, cpx<double>* __restrict__ r, const cpx
 }
 
 // re[c][i] = E[c]: the constant macroscopic field (d <= 3 components).
-__global__ void __launch_bounds__(256)
+static __global__ void __launch_bounds__(256)
 k_pot_fill(double* __restrict__ re, long long n, int nc, double e0, double e1, double e2) {
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n * nc;
        i += (long long)gridDim.x * blockDim.x) {
