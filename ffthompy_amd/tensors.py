@@ -190,7 +190,14 @@ def scalar_product(y, x):
     """Grid inner product sum(y*x)/prod(N) (objects.py:618-636, real space)."""
     assert isinstance(x, Tensor) and y.val.shape == x.val.shape
     if y.Fourier:
-        raise NotImplementedError('Fourier-space inner product of device spectra')
+        # 'r' spectra (results such as a potential's Fu, read on the host): plane k2 = 0, and the
+        # Nyquist plane of an even last axis, once, the others twice, / prod(N)^2
+        Yv, Xv = np.asarray(_host(y)), np.asarray(_host(x))
+        w = np.full(Yv.shape[-1], 2.)
+        w[0] = 1.
+        if y.N[-1] % 2 == 0:
+            w[-1] = 1.
+        return float(np.sum(w * (Yv * np.conj(Xv)).real) / float(np.prod(y.N)) ** 2)
     if not (y.on_device and x.on_device):
         return float(np.sum(np.asarray(_host(y)) * np.asarray(_host(x))) / np.prod(y.N))
     return nat.dot(y.val, x.val) / float(np.prod(y.N))

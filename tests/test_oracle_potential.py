@@ -7,8 +7,8 @@
 * For a constant coefficient the operator is the multiplier xi^T A xi / |xi|^2 (0 at xi = 0).
 * The operator is symmetric and positive in the inner product of 'r' spectra for a random
   coefficient; the potential has zero mean.
-* The host helpers of ffthompy_amd.potential (the preconditioner, the spectrum crop) against the
-  oracle's.
+* The host helpers of ffthompy_amd (the preconditioner, the spectrum crop, the inner product and
+  mean of 'r' spectra) against the oracle's.
 """
 import json
 import os
@@ -114,3 +114,14 @@ def test_host_preconditioner_and_crop_match_oracle():
     big = X.project(M)
     got = _crop_spectrum(big.val, M, N)
     assert np.abs(got - X.val).max() < 1e-12 * np.abs(X.val).max()
+
+
+def test_host_inner_product_of_r_spectra_matches_oracle():
+    from ffthompy_amd.tensors import Tensor
+    rng = np.random.default_rng(21)
+    for N in ((5, 7), (4, 6), (3, 5, 8)):
+        X, Z = _random_spectrum(N, rng), _random_spectrum(N, rng)
+        tx = Tensor(val=X.val, order=0, N=N, Fourier=True)
+        tz = Tensor(val=Z.val, order=0, N=N, Fourier=True)
+        assert abs(tx * tz - X * Z) < 1e-13 * abs(X * X)
+        assert abs(tx.mean() - X.mean()) < 1e-15
