@@ -184,3 +184,26 @@ def test_fused_passes_equal_generic_kernels(N):
     assert out[0][0] == out[1][0]
     for i in (1, 2):
         assert np.abs(out[0][i] - out[1][i]).max() <= 1e-12 * np.abs(out[1][i]).max()
+
+
+@pytest.mark.parametrize('N,Y', [((15, 9, 21), (1., 2., 0.5)), ((27, 15), (0.5, 3.))])
+def test_rectangular_cell_matches_oracle(N, Y):
+    """Unequal axes and cell sizes (the frequencies xi = k / Y per axis of the view); the tolerance
+    sits midway (geometrically) between two of the oracle's residuals, five iterations before its
+    1e-10 stop, so the count is compared with a margin."""
+    print('case potential cell', N, Y, flush=True)
+    d = len(N)
+    rng = np.random.default_rng(23)
+    a = np.exp(0.7 * rng.standard_normal(N))
+    A = np.einsum('ij,...->ij...', np.eye(d), a)
+    oA = Field(val=A, order=2, Y=np.asarray(Y), multype=21)
+    log = opot.homog_GaNi_full_potential(oA, None, tol=1e-10, maxiter=300)['info']['norm_res_log']
+    k = len(log) - 6
+    tol = float(np.sqrt(log[k] * log[k + 1]))
+    res = gpot.homog_GaNi_full_potential(gpot.Tensor(name='A', val=A, order=2, Y=np.asarray(Y), multype=21), None,
+                                         {'solver': {'tol': tol, 'maxiter': 300}})
+    ref = opot.homog_GaNi_full_potential(oA, None, tol=tol, maxiter=300)
+    assert ref['info']['kit'] == k + 1
+    assert res.info['kit'] == k + 1
+    assert abs(res.AH - ref['AH']) < 1e-12 * abs(ref['AH'])
+    _check_fu(res.Fu, ref['Fu'], 1e-9)
