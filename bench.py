@@ -296,7 +296,8 @@ def bench_ga(args, nat, ms, dev):
 def bench_potential(args, nat, ms, dev):
     """243^3 spheres (contrast 100), load e1, primal, tol 1e-8, fp64: the potential formulation
     (one scalar spectrum per CG vector, fh_potential_solve) beside the gradient formulation's
-    fp64 CG of the same load; both end in the same AH_11."""
+    fp64 CG of the same load (both end in the same AH_11) and the potential solve in mixed
+    precision (fp32 iterations, fp64 reliable updates)."""
     N = (args.n,) * 3
     a = ms.spheres_conductivity(N, count=30, contrast=100., seed=ms.BASE_SEED)[0]
     plan = nat.Plan(N, (1., 1., 1.), 'scalar', 'primal')
@@ -309,12 +310,16 @@ def bench_potential(args, nat, ms, dev):
     del EN
     X = nat.DeviceArray((3,) + N)
     Fu = nat.DeviceArray((args.n, args.n, args.n // 2 + 1), np.complex128)
-    tp, tg = [], []
+    tp, tg, tm = [], [], []
+    Xm = nat.DeviceArray((3,) + N)
+    Fum = nat.DeviceArray((args.n, args.n, args.n // 2 + 1), np.complex128)
     for rep in range(args.warmup + args.steps):
         nat.synchronize()
         t0 = time.perf_counter()
         _, pinfo = plan.potential_solve(E, 1e-8, 3000, Fu=Fu, X=X)
         t1 = time.perf_counter()
+        _, minfo = plan.potential_solve(E, 1e-8, 3000, Fu=Fum, X=Xm, mode='mixed', delta=args.delta)
+        tm.append(time.perf_counter() - t1)
         x = nat.DeviceArray.zeros((3,) + N)
         nat.synchronize()
         t2 = time.perf_counter()
@@ -323,10 +328,12 @@ def bench_potential(args, nat, ms, dev):
         if rep >= args.warmup:
             tp.append(t1 - t0)
             tg.append(t3 - t2)
-        log('potential243: potential %.4f s (kit %d), gradient fp64 %.4f s (kit %d)'
-            % (t1 - t0, pinfo['kit'], t3 - t2, ginfo['kit']))
+        log('potential243: potential %.4f s (kit %d), mixed %.4f s (kit %d), gradient fp64 %.4f s (kit %d)'
+            % (t1 - t0, pinfo['kit'], tm[-1], minfo['kit'], t3 - t2, ginfo['kit']))
     nat.add_constant(X, E)
     nat.add_constant(x, E)
+    nat.add_constant(Xm, E)
+    ahm = plan.energy_matrix([Xm, Xm, Xm])[0, 0]
     ahp = plan.energy_matrix([X, X, X])[0, 0]
     ahg = plan.energy_matrix([x, x, x])[0, 0]
     vp, vg = float(np.median(tp)), float(np.median(tg))
@@ -338,7 +345,9 @@ def bench_potential(args, nat, ms, dev):
            'kit_potential': pinfo['kit'], 'ms_per_iteration_potential': round(1e3 * vp / pinfo['kit'], 4),
            'gradient_fp64_s': round(vg, 4), 'kit_gradient_fp64': ginfo['kit'],
            'ms_per_iteration_gradient_fp64': round(1e3 * vg / ginfo['kit'], 4),
-           'AH11_potential': ahp, 'AH11_gradient': ahg, 'AH11_rel_diff': abs(ahp - ahg) / ahg}
+           'AH11_potential': ahp, 'AH11_gradient': ahg, 'AH11_rel_diff': abs(ahp - ahg) / ahg,
+           'mixed_s': round(float(np.median(tm[args.warmup:])), 4), 'kit_mixed': minfo['kit'],
+           'AH11_mixed_rel_diff': abs(ahm - ahp) / ahp}
     print(json.dumps(out), flush=True)
 
 

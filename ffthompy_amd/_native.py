@@ -82,8 +82,8 @@ SIGNATURES = {
     'fh_ga_energy': [_vp, ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, _c_double_p],
     'fh_iter_solve': [_vp, ctypes.c_int, _vp, _vp, _c_double_p, ctypes.c_double, ctypes.c_int, _c_int_p, _c_double_p,
                       _c_double_p, ctypes.c_int],
-    'fh_potential_solve': [_vp, _c_double_p, ctypes.c_double, ctypes.c_int, _vp, _vp, _c_int_p, _c_double_p,
-                           _c_double_p, ctypes.c_int, _c_int_p],
+    'fh_potential_solve': [_vp, ctypes.c_int, _c_double_p, ctypes.c_double, ctypes.c_int, ctypes.c_double, _vp, _vp,
+                           _c_int_p, _c_double_p, _c_double_p, ctypes.c_int, _c_int_p],
     'fh_set_coeff_inclusions': [_vp, ctypes.c_int, _c_int_p, _c_double_p, _c_double_p, ctypes.c_int, _c_double_p,
                                 ctypes.c_int],
     'fh_coeff_to_host': [_vp, _c_double_p],
@@ -411,9 +411,10 @@ class Plan:
             out['res_log'] = hist[:min(hist_len, kit.value + 1)].copy()
         return out
 
-    def potential_solve(self, E, tol, maxiter, Fu=None, X=None, hist_len=0):
-        """CG of the potential formulation (fh_potential_solve): Fu, the potential's rfftn
-        spectrum (complex, N[:-1] + (N[-1]//2+1,)), and X = F^-1 grad Fu ((ncomp,) + N, optional)."""
+    def potential_solve(self, E, tol, maxiter, Fu=None, X=None, hist_len=0, mode='fp64', delta=1e-4):
+        """CG of the potential formulation (fh_potential_solve), mode 'fp64' or 'mixed': Fu, the
+        potential's rfftn spectrum (complex, N[:-1] + (N[-1]//2+1,)), and X = F^-1 grad Fu
+        ((ncomp,) + N, optional)."""
         E = np.ascontiguousarray(np.asarray(E, dtype=np.float64).reshape(-1))
         if E.size != self.ncomp:
             raise FhError('E needs %d values' % self.ncomp, FH_ERR_ARG)
@@ -421,8 +422,9 @@ class Plan:
             Fu = DeviceArray(self.N[:-1] + (self.N[-1] // 2 + 1,), np.complex128)
         kit, nres, status = ctypes.c_int(), ctypes.c_double(), ctypes.c_int()
         hist = np.zeros(max(hist_len, 1))
-        call('fh_potential_solve', self._h, E.ctypes.data_as(_c_double_p), ctypes.c_double(tol),
-             ctypes.c_int(int(maxiter)), Fu.ptr, X.ptr if X is not None else None, ctypes.byref(kit),
+        call('fh_potential_solve', self._h, ctypes.c_int(FH_SOLVE_FP64 if mode == 'fp64' else FH_SOLVE_MIXED),
+             E.ctypes.data_as(_c_double_p), ctypes.c_double(tol), ctypes.c_int(int(maxiter)), ctypes.c_double(delta),
+             Fu.ptr, X.ptr if X is not None else None, ctypes.byref(kit),
              ctypes.byref(nres), hist.ctypes.data_as(_c_double_p) if hist_len > 0 else None,
              ctypes.c_int(hist_len), ctypes.byref(status))
         out = {'kit': kit.value, 'norm_res': nres.value, 'status': status.value}

@@ -779,52 +779,54 @@ fh::pot::PotParams pot_params(const fh_plan* pl) {
 // a(x) and R2C run as one in-place kernel (the real field stays in LDS), where the fixed lengths
 // cover the grid; otherwise separate elementwise kernels and the one-way passes (with the real
 // field in `re`, ncomp * n doubles).
-int pot_operator(fh_plan* pl, cpx<double>* p, const cpx<double>* r, cpx<double>* out, double* re,
-                 const cpx<double>* w, const int* halt, int* nb) {
+template <typename T>
+int pot_operator(fh_plan* pl, cpx<T>* p, const cpx<T>* r, cpx<T>* out, T* re, const cpx<T>* w, const int* halt,
+                 int* nb) {
   const int nc = pl->ncomp;
   const fh::pot::PotParams g = pot_params(pl);
-  cpx<double>* spec = reinterpret_cast<cpx<double>*>(pl->spec);
+  cpx<T>* spec = reinterpret_cast<cpx<T>*>(pl->spec);
   const GParams gp = make_gparams(pl, 1.0);
   const int gb = grid_blocks(pl->nh);
   const int ax = pl->n0 > 1 ? 0 : 1;  // outermost axis of the grid
-  const fh::LaunchCtx ctx = make_ctx<double>(pl);
+  const fh::LaunchCtx ctx = make_ctx<T>(pl);
+  const T* coef = plan_coef<T>(pl);
   int in_fused = 0;
   if (!pl->generic) {
-    in_fused = fh::fixed_pot_cols<double>(ctx, ax, +1, nc, spec, p, r, pl->st, 0.0, nullptr, nullptr, nullptr, 0, g,
-                                          halt, nullptr);
+    in_fused = fh::fixed_pot_cols<T>(ctx, ax, +1, nc, spec, p, r, pl->st, T(0), nullptr, nullptr, nullptr, 0, g, halt,
+                                     nullptr);
     if (in_fused < 0) return fixed_result(in_fused, "potential strided pass (fixed)");
   }
   if (!in_fused) {
     if (r)
-      hipLaunchKernelGGL((fh::k_cg_pupdate<double>), dim3(gb), dim3(256), 0, pl->stream, 2 * pl->nh, (double*)p,
-                         (const double*)r, pl->st);
-    hipLaunchKernelGGL(fh::pot::k_pot_grad, dim3(gb), dim3(256), 0, pl->stream, p, spec, g, 1, halt);
+      hipLaunchKernelGGL((fh::k_cg_pupdate<T>), dim3(gb), dim3(256), 0, pl->stream, 2 * pl->nh, (T*)p, (const T*)r,
+                         pl->st);
+    hipLaunchKernelGGL((fh::pot::k_pot_grad<T>), dim3(gb), dim3(256), 0, pl->stream, p, spec, g, 1, halt);
     HIPCHK(hipGetLastError());
-    RETCHK(launch_cols<double>(pl, 1, ax, nc, spec, gp, halt));
+    RETCHK(launch_cols<T>(pl, 1, ax, nc, spec, gp, halt));
   }
-  if (ax == 0) RETCHK(launch_cols<double>(pl, 1, 1, nc, spec, gp, halt));
+  if (ax == 0) RETCHK(launch_cols<T>(pl, 1, 1, nc, spec, gp, halt));
   int rt_fused = 0;
   if (!pl->generic && pl->coef_kind == fh::COEF_ISO) {
-    rt_fused = fh::fixed_rows_rt<double>(ctx, nc, spec, pl->coef64, 1.0 / (double)pl->n, halt);
+    rt_fused = fh::fixed_rows_rt<T>(ctx, nc, spec, coef, T(1.0 / (double)pl->n), halt);
     if (rt_fused < 0) return fixed_result(rt_fused, "rows round trip (fixed)");
   }
   if (!rt_fused) {
-    RETCHK(launch_rows_inv<double>(pl, nc, spec, re, 1.0 / (double)pl->n, nullptr, nullptr, halt, nullptr));
-    RETCHK(launch_rows_fwd<double>(pl, nc, re, pl->coef64, pl->coef_kind, spec, halt));
+    RETCHK(launch_rows_inv<T>(pl, nc, spec, re, T(1.0 / (double)pl->n), nullptr, nullptr, halt, nullptr));
+    RETCHK(launch_rows_fwd<T>(pl, nc, re, coef, pl->coef_kind, spec, halt));
   }
-  if (ax == 0) RETCHK(launch_cols<double>(pl, 0, 1, nc, spec, gp, halt));
+  if (ax == 0) RETCHK(launch_cols<T>(pl, 0, 1, nc, spec, gp, halt));
   int out_fused = 0;
   if (!pl->generic) {
-    out_fused = fh::fixed_pot_cols<double>(ctx, ax, -1, nc, spec, nullptr, nullptr, nullptr, -1.0, out, w,
-                                           pl->partial, pl->partial_cap, g, halt, nb);
+    out_fused = fh::fixed_pot_cols<T>(ctx, ax, -1, nc, spec, nullptr, nullptr, nullptr, T(-1), out, w, pl->partial,
+                                      pl->partial_cap, g, halt, nb);
     if (out_fused < 0) return fixed_result(out_fused, "potential strided pass (fixed)");
   }
   if (!out_fused) {
-    RETCHK(launch_cols<double>(pl, 0, ax, nc, spec, gp, halt));
-    hipLaunchKernelGGL(fh::pot::k_pot_div, dim3(gb), dim3(256), 0, pl->stream, spec, out, g, -1.0, w, pl->partial,
-                       halt);
+    RETCHK(launch_cols<T>(pl, 0, ax, nc, spec, gp, halt));
+    hipLaunchKernelGGL((fh::pot::k_pot_div<T>), dim3(gb), dim3(256), 0, pl->stream, spec, out, g, T(-1), w,
+                       pl->partial, halt);
     HIPCHK(hipGetLastError());
-    *nb = gb;
+    if (nb) *nb = gb;
   }
   return FH_OK;
 }
@@ -2374,26 +2376,32 @@ int fh_grf(fh_plan* pl, const void* white, double corr_len, double stdev, int ex
 
 // ---- potential formulation (ffthompy/tensorsLowRank/homogenisation.py:43-128) ------------------
 
-int fh_potential_solve(fh_plan* pl, const double* E, double tol, int maxiter, void* Fu_, void* X_, int* kit,
-                       double* norm_res, double* res_hist, int hist_len, int* cg_status) {
+int fh_potential_solve(fh_plan* pl, int mode, const double* E, double tol, int maxiter, double delta, void* Fu_,
+                       void* X_, int* kit, double* norm_res, double* res_hist, int hist_len, int* cg_status) {
   RETCHK(check_plan(pl));
   if (!E || !Fu_) return fail(FH_ERR_ARG, "null pointer argument");
+  if (mode != FH_SOLVE_FP64 && mode != FH_SOLVE_MIXED) return fail(FH_ERR_ARG, "bad solve mode");
   if (pl->physics != FH_SCALAR) return fail(FH_ERR_UNSUPPORTED, "the potential formulation is scalar only");
   if (!pl->coef_kind) return fail(FH_ERR_STATE, "fh_set_coeff has not been called");
   if (!(tol >= 0.0) || maxiter < 0) return fail(FH_ERR_ARG, "bad tol/maxiter");
+  if (mode == FH_SOLVE_MIXED && !(delta > 0.0 && delta < 1.0)) return fail(FH_ERR_ARG, "delta must be in (0,1)");
   if (!pl->banded)
     for (int a = 0; a < pl->dim; ++a)
       if ((pl->Nu[a] & 1) == 0)
         return fail(FH_ERR_UNSUPPORTED, "the potential formulation needs odd grids (even: Nyquist modes)");
+  const bool mixed = mode == FH_SOLVE_MIXED;
   const fh::pot::PotParams g = pot_params(pl);
   const long long nh = pl->nh;
-  const long long nhp = (nh + 15) / 16 * 16;
+  const long long nhp = (nh + 15) / 16 * 16;  // 256-byte aligned sub-buffers
   const int nc = pl->ncomp;
-  RETCHK(ensure_work(pl, 3 * (size_t)nhp * sizeof(cpx<double>) + (size_t)nc * pl->n * sizeof(double)));
-  cpx<double>* r = reinterpret_cast<cpx<double>*>(pl->work);
-  cpx<double>* p = r + nhp;
-  cpx<double>* Ap = p + nhp;
-  double* re = reinterpret_cast<double*>(Ap + nhp);
+  // fp64: b, r, p, Ap; mixed: b and A x in fp64, r, p, Ap and the correction xc in fp32; both: the
+  // real field of the right-hand side (and of the one-way passes where the fused kernels do not apply)
+  const size_t vec_bytes = mixed ? 2 * (size_t)nhp * sizeof(cpx<double>) + 4 * (size_t)nhp * sizeof(cpx<float>)
+                                 : 4 * (size_t)nhp * sizeof(cpx<double>);
+  RETCHK(ensure_work(pl, vec_bytes + (size_t)nc * pl->n * sizeof(double)));
+  cpx<double>* b = reinterpret_cast<cpx<double>*>(pl->work);
+  cpx<double>* v1 = b + nhp;  // fp64: r; mixed: A x
+  double* re = reinterpret_cast<double*>(reinterpret_cast<char*>(pl->work) + vec_bytes);
   cpx<double>* x = reinterpret_cast<cpx<double>*>(Fu_);  // the CG iterate; P x at the end
   const int hl = std::max(hist_len, 1);
   RETCHK(ensure_hist(pl, hl));
@@ -2401,6 +2409,8 @@ int fh_potential_solve(fh_plan* pl, const double* E, double tol, int maxiter, vo
   std::memset(&s, 0, sizeof(s));
   s.tol = tol;
   s.maxiter = maxiter;
+  s.mixed = mixed;
+  s.delta = delta;
   s.inv_nvox = 1.0 / ((double)pl->n * (double)pl->n);  // 'r' spectra: / n^2
   s.hist_len = hist_len > 0 ? hist_len : 0;
   s.hist = pl->hist;
@@ -2408,7 +2418,7 @@ int fh_potential_solve(fh_plan* pl, const double* E, double tol, int maxiter, vo
   HIPCHK(hipEventRecord(pl->ev0, pl->stream));
   const int gb = grid_blocks(nh);
   const dim3 grid(gb), block(256);
-  // right-hand side P div F(A E) (homogenisation.py:106-110), x0 = 0
+  // right-hand side b = P div F(A E) (homogenisation.py:106-110), x0 = 0
   hipLaunchKernelGGL(fh::pot::k_pot_fill, dim3(grid_blocks((long long)nc * pl->n)), block, 0, pl->stream, re,
                      pl->n, nc, E[0], E[1], nc > 2 ? E[2] : 0.0);
   HIPCHK(hipGetLastError());
@@ -2417,32 +2427,78 @@ int fh_potential_solve(fh_plan* pl, const double* E, double tol, int maxiter, vo
   RETCHK(launch_rows_fwd<double>(pl, nc, re, pl->coef64, pl->coef_kind, spec, nullptr));
   RETCHK(launch_cols<double>(pl, 0, 1, nc, spec, gp, nullptr));
   if (pl->n0 > 1) RETCHK(launch_cols<double>(pl, 0, 0, nc, spec, gp, nullptr));
-  hipLaunchKernelGGL(fh::pot::k_pot_div, grid, block, 0, pl->stream, spec, r, g, 1.0, nullptr, nullptr, nullptr);
-  hipLaunchKernelGGL(fh::pot::k_pot_norm, grid, block, 0, pl->stream, r, g, pl->partial);
-  hipLaunchKernelGGL(fh::k_cg_restart, dim3(1), dim3(256), 0, pl->stream, pl->st, pl->partial, gb, pl->hist);
-  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL((fh::pot::k_pot_div<double>), grid, block, 0, pl->stream, spec, b, g, 1.0, nullptr, nullptr,
+                     nullptr);
   HIPCHK(hipMemsetAsync(x, 0, (size_t)nh * sizeof(cpx<double>), pl->stream));
-  HIPCHK(hipMemcpyAsync(p, r, (size_t)nh * sizeof(cpx<double>), hipMemcpyDeviceToDevice, pl->stream));
   const int* halt = &pl->st->halt;
   const int per_check = 8;
-  for (;;) {
-    for (int k = 0; k < per_check; ++k) {
-      int nb = 0;
-      RETCHK(pot_operator(pl, p, r, Ap, re, p, halt, &nb));
-      hipLaunchKernelGGL(fh::k_cg_alpha, dim3(1), dim3(kFinThreads), 0, pl->stream, pl->st, pl->partial, nb);
-      hipLaunchKernelGGL(fh::pot::k_pot_update, grid, block, 0, pl->stream, x, r, p, Ap, g, pl->st, pl->partial);
-      hipLaunchKernelGGL(fh::k_cg_beta, dim3(1), dim3(kFinThreads), 0, pl->stream, pl->st, pl->partial, gb);
-      HIPCHK(hipGetLastError());
+  if (!mixed) {
+    cpx<double>* r = v1;
+    cpx<double>* p = r + nhp;
+    cpx<double>* Ap = p + nhp;
+    hipLaunchKernelGGL((fh::pot::k_pot_residual<double>), grid, block, 0, pl->stream, b, nullptr, r, g, pl->partial);
+    hipLaunchKernelGGL(fh::k_cg_restart, dim3(1), dim3(256), 0, pl->stream, pl->st, pl->partial, gb, pl->hist);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(p, r, (size_t)nh * sizeof(cpx<double>), hipMemcpyDeviceToDevice, pl->stream));
+    for (;;) {
+      for (int k = 0; k < per_check; ++k) {
+        int nb = 0;
+        RETCHK(pot_operator<double>(pl, p, r, Ap, re, p, halt, &nb));
+        hipLaunchKernelGGL(fh::k_cg_alpha, dim3(1), dim3(kFinThreads), 0, pl->stream, pl->st, pl->partial, nb);
+        hipLaunchKernelGGL((fh::pot::k_pot_update<double>), grid, block, 0, pl->stream, x, r, p, Ap, g, pl->st,
+                           pl->partial);
+        hipLaunchKernelGGL(fh::k_cg_beta, dim3(1), dim3(kFinThreads), 0, pl->stream, pl->st, pl->partial, gb);
+        HIPCHK(hipGetLastError());
+      }
+      HIPCHK(hipMemcpyAsync(&s, pl->st, sizeof(s), hipMemcpyDeviceToHost, pl->stream));
+      HIPCHK(hipStreamSynchronize(pl->stream));
+      if (s.done) break;
     }
-    HIPCHK(hipMemcpyAsync(&s, pl->st, sizeof(s), hipMemcpyDeviceToHost, pl->stream));
-    HIPCHK(hipStreamSynchronize(pl->stream));
-    if (s.done) break;
+  } else {
+    // fp32 iterations on the correction xc; a reliable update (x += xc, r = b - A x in fp64)
+    // whenever the recursive residual has dropped by delta or meets the tolerance
+    cpx<double>* Ax = v1;
+    cpx<float>* r = reinterpret_cast<cpx<float>*>(Ax + nhp);
+    cpx<float>* p = r + nhp;
+    cpx<float>* Ap = p + nhp;
+    cpx<float>* xc = Ap + nhp;
+    float* re32 = reinterpret_cast<float*>(re);
+    HIPCHK(hipMemsetAsync(xc, 0, (size_t)nh * sizeof(cpx<float>), pl->stream));
+    hipLaunchKernelGGL((fh::pot::k_pot_residual<float>), grid, block, 0, pl->stream, b, nullptr, r, g, pl->partial);
+    hipLaunchKernelGGL(fh::k_cg_restart, dim3(1), dim3(256), 0, pl->stream, pl->st, pl->partial, gb, pl->hist);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(p, r, (size_t)nh * sizeof(cpx<float>), hipMemcpyDeviceToDevice, pl->stream));
+    const int fb = grid_blocks(2 * nh);
+    for (;;) {
+      for (int k = 0; k < per_check; ++k) {
+        int nb = 0;
+        RETCHK(pot_operator<float>(pl, p, r, Ap, re32, p, halt, &nb));
+        hipLaunchKernelGGL(fh::k_cg_alpha, dim3(1), dim3(kFinThreads), 0, pl->stream, pl->st, pl->partial, nb);
+        hipLaunchKernelGGL((fh::pot::k_pot_update<float>), grid, block, 0, pl->stream, xc, r, p, Ap, g, pl->st,
+                           pl->partial);
+        hipLaunchKernelGGL(fh::k_cg_beta, dim3(1), dim3(kFinThreads), 0, pl->stream, pl->st, pl->partial, gb);
+        HIPCHK(hipGetLastError());
+      }
+      HIPCHK(hipMemcpyAsync(&s, pl->st, sizeof(s), hipMemcpyDeviceToHost, pl->stream));
+      HIPCHK(hipStreamSynchronize(pl->stream));
+      if (s.done) break;
+      if (s.pause) {
+        hipLaunchKernelGGL(fh::k_fold_correction, dim3(fb), block, 0, pl->stream, 2 * nh, (double*)x, (float*)xc);
+        RETCHK(pot_operator<double>(pl, x, nullptr, Ax, re, nullptr, nullptr, nullptr));
+        hipLaunchKernelGGL((fh::pot::k_pot_residual<float>), grid, block, 0, pl->stream, b, Ax, r, g, pl->partial);
+        hipLaunchKernelGGL(fh::k_cg_restart, dim3(1), dim3(256), 0, pl->stream, pl->st, pl->partial, gb, pl->hist);
+        HIPCHK(hipGetLastError());
+      }
+    }
+    hipLaunchKernelGGL(fh::k_fold_correction, dim3(fb), block, 0, pl->stream, 2 * nh, (double*)x, (float*)xc);
+    HIPCHK(hipGetLastError());
   }
   // Fu = P x (homogenisation.py:117); e = F^-1 grad Fu
-  hipLaunchKernelGGL(fh::pot::k_pot_precond, grid, block, 0, pl->stream, x, Ap, g);
-  HIPCHK(hipMemcpyAsync(x, Ap, (size_t)nh * sizeof(cpx<double>), hipMemcpyDeviceToDevice, pl->stream));
+  cpx<double>* tmp = v1;
+  hipLaunchKernelGGL(fh::pot::k_pot_precond, grid, block, 0, pl->stream, x, tmp, g);
+  HIPCHK(hipMemcpyAsync(x, tmp, (size_t)nh * sizeof(cpx<double>), hipMemcpyDeviceToDevice, pl->stream));
   if (X_) {
-    hipLaunchKernelGGL(fh::pot::k_pot_grad, grid, block, 0, pl->stream, x, spec, g, 0, nullptr);
+    hipLaunchKernelGGL((fh::pot::k_pot_grad<double>), grid, block, 0, pl->stream, x, spec, g, 0, nullptr);
     HIPCHK(hipGetLastError());
     if (pl->n0 > 1) RETCHK(launch_cols<double>(pl, 1, 0, nc, spec, gp, nullptr));
     RETCHK(launch_cols<double>(pl, 1, 1, nc, spec, gp, nullptr));
@@ -2455,6 +2511,7 @@ int fh_potential_solve(fh_plan* pl, const double* E, double tol, int maxiter, vo
   float ms = 0.f;
   HIPCHK(hipEventElapsedTime(&ms, pl->ev0, pl->ev1));
   pl->last_solve_ms = ms;
+  pl->last_updates = s.nupdates;
   if (kit) *kit = s.kit;
   if (norm_res) *norm_res = (s.kit == 0) ? 0.0 : std::sqrt(s.rr);
   if (cg_status) *cg_status = s.status;

@@ -6,6 +6,10 @@ template int fixed_rows_fwd<float>(const LaunchCtx&, int, const float*, const fl
                                    const float*, const CGState*, float*, double*, size_t, int*);
 template int fixed_rows_inv<float>(const LaunchCtx&, int, const cpx<float>*, float*, float, int, const float*, double*,
                                    size_t, const int*, int*, CGState*);
+template int fixed_rows_rt<float>(const LaunchCtx&, int, cpx<float>*, const float*, float, const int*);
+template int fixed_pot_cols<float>(const LaunchCtx&, int, int, int, cpx<float>*, cpx<float>*, const cpx<float>*,
+                                   const CGState*, float, cpx<float>*, const cpx<float>*, double*, size_t,
+                                   const pot::PotParams&, const int*, int*);
 template int fixed_cols<float>(const LaunchCtx&, int, int, int, cpx<float>*, const int*);
 template int fixed_cols_g<float>(const LaunchCtx&, int, int, cpx<float>*, const GParams&, const int*);
 

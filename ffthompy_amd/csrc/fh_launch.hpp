@@ -61,7 +61,11 @@ extern template int fixed_rows_inv<float>(const LaunchCtx&, int, const cpx<float
                                           const float*, double*, size_t, const int*, int*, CGState*);
 extern template int fixed_rows_inv<double>(const LaunchCtx&, int, const cpx<double>*, double*, double, int,
                                            const double*, double*, size_t, const int*, int*, CGState*);
+extern template int fixed_rows_rt<float>(const LaunchCtx&, int, cpx<float>*, const float*, float, const int*);
 extern template int fixed_rows_rt<double>(const LaunchCtx&, int, cpx<double>*, const double*, double, const int*);
+extern template int fixed_pot_cols<float>(const LaunchCtx&, int, int, int, cpx<float>*, cpx<float>*,
+                                          const cpx<float>*, const CGState*, float, cpx<float>*, const cpx<float>*,
+                                          double*, size_t, const pot::PotParams&, const int*, int*);
 extern template int fixed_pot_cols<double>(const LaunchCtx&, int, int, int, cpx<double>*, cpx<double>*,
                                            const cpx<double>*, const CGState*, double, cpx<double>*,
                                            const cpx<double>*, double*, size_t, const pot::PotParams&, const int*,

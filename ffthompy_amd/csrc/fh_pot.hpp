@@ -62,8 +62,9 @@ __device__ __forceinline__ double plane_weight(long long t, const PotParams& g) 
 
 // spec[t][j] = 2 pi i xi_j (P) u[t]: the gradient of the potential in the operator's spectral
 // layout (frequency-major, components adjacent); with_P multiplies by P first.
-static __global__ void __launch_bounds__(256)
-k_pot_grad(const cpx<double>* __restrict__ u, cpx<double>* __restrict__ spec, PotParams g, int with_P,
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_pot_grad(const cpx<T>* __restrict__ u, cpx<T>* __restrict__ spec, PotParams g, int with_P,
            const int* __restrict__ halt) {
   if (halt && *halt) return;
   const long long nh = (long long)g.n0 * g.n1 * g.n2h;
@@ -73,18 +74,19 @@ k_pot_grad(const cpx<double>* __restrict__ u, cpx<double>* __restrict__ spec, Po
     freq_of(t, g, k);
     double q[3] = {0.0, 0.0, 0.0}, P = 1.0;
     const bool in = grad_symbol(k, g, q, P);
-    cpx<double> v = u[t];
-    if (with_P) v = mk(P * v.x, P * v.y);
+    cpx<T> v = u[t];
+    if (with_P) v = mk(T(P) * v.x, T(P) * v.y);
     for (int j = 0; j < g.dim; ++j)
-      spec[t * g.dim + j] = in ? mk(-q[j] * v.y, q[j] * v.x) : mk(0.0, 0.0);
+      spec[t * g.dim + j] = in ? mk(-T(q[j]) * v.y, T(q[j]) * v.x) : mk(T(0), T(0));
   }
 }
 
 // out[t] = sgn P sum_j 2 pi i xi_j spec[t][j] (sgn = -1: the operator, +1: the right-hand side
 // div F(A E)); with w != null also per-workgroup partial sums of the weighted <out, w>.
-static __global__ void __launch_bounds__(256)
-k_pot_div(const cpx<double>* __restrict__ spec, cpx<double>* __restrict__ out, PotParams g, double sgn,
-          const cpx<double>* __restrict__ w, double* __restrict__ partial, const int* __restrict__ halt) {
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_pot_div(const cpx<T>* __restrict__ spec, cpx<T>* __restrict__ out, PotParams g, T sgn,
+          const cpx<T>* __restrict__ w, double* __restrict__ partial, const int* __restrict__ halt) {
   if (halt && *halt) return;
   const long long nh = (long long)g.n0 * g.n1 * g.n2h;
   double acc = 0.0;
@@ -93,21 +95,21 @@ k_pot_div(const cpx<double>* __restrict__ spec, cpx<double>* __restrict__ out, P
     int k[3];
     freq_of(t, g, k);
     double q[3] = {0.0, 0.0, 0.0}, P = 1.0;
-    cpx<double> o = mk(0.0, 0.0);
+    cpx<T> o = mk(T(0), T(0));
     if (grad_symbol(k, g, q, P)) {
-      double re = 0.0, im = 0.0;
+      T re = T(0), im = T(0);
       for (int j = 0; j < g.dim; ++j) {
-        const cpx<double> s = spec[t * g.dim + j];
-        re += -q[j] * s.y;
-        im += q[j] * s.x;
+        const cpx<T> s = spec[t * g.dim + j];
+        re += -T(q[j]) * s.y;
+        im += T(q[j]) * s.x;
       }
-      const double f = sgn * P;
+      const T f = sgn * T(P);
       o = mk(f * re, f * im);
     }
     out[t] = o;
     if (w) {
-      const cpx<double> wv = w[t];
-      acc += plane_weight(t, g) * (o.x * wv.x + o.y * wv.y);
+      const cpx<T> wv = w[t];
+      acc += plane_weight(t, g) * ((double)o.x * (double)wv.x + (double)o.y * (double)wv.y);
     }
   }
   if (w) {
@@ -130,38 +132,46 @@ k_pot_precond(const cpx<double>* __restrict__ u, cpx<double>* __restrict__ out, 
   }
 }
 
-// Partial sums of the weighted <r, r> (the initial residual r = b of a start from x = 0).
-static __global__ void __launch_bounds__(256)
-k_pot_norm(const cpx<double>* __restrict__ r, PotParams g, double* __restrict__ partial) {
+// x += alpha p, r -= alpha Ap, partial sums of the weighted <r, r>.
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_pot_update(cpx<T>* __restrict__ x, cpx<T>* __restrict__ r, const cpx<T>* __restrict__ p,
+             const cpx<T>* __restrict__ Ap, PotParams g, CGState* __restrict__ st, double* __restrict__ partial) {
+  if (st->halt) return;
+  const T a = T(st->alpha);
   const long long nh = (long long)g.n0 * g.n1 * g.n2h;
   double acc = 0.0;
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < nh;
        t += (long long)gridDim.x * blockDim.x) {
-    const cpx<double> v = r[t];
-    acc += plane_weight(t, g) * (v.x * v.x + v.y * v.y);
+    const cpx<T> pv = p[t], av = Ap[t];
+    cpx<T> xv = x[t], rv = r[t];
+    xv = mk(xv.x + a * pv.x, xv.y + a * pv.y);
+    rv = mk(rv.x - a * av.x, rv.y - a * av.y);
+    x[t] = xv;
+    r[t] = rv;
+    acc += plane_weight(t, g) * ((double)rv.x * (double)rv.x + (double)rv.y * (double)rv.y);
   }
   const double s = block_sum(acc);
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-// x += alpha p, r -= alpha Ap, partial sums of the weighted <r, r>.
-static __global__ void __launch_bounds__(256)
-k_pot_update(cpx<double>* __restrict__ x, cpx<double>* __restrict__ r, const cpx<double>* __restrict__ p,
-             const cpx<double>* __restrict__ Ap, PotParams g, CGState* __restrict__ st,
-             double* __restrict__ partial) {
-  if (st->halt) return;
-  const double a = st->alpha;
+// r = b - Ax (Ax null: r = b) in the residual precision T, partial sums of the weighted <r, r>
+// of the fp64 values (the true residual of a reliable update).
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_pot_residual(const cpx<double>* __restrict__ b, const cpx<double>* __restrict__ Ax, cpx<T>* __restrict__ r,
+               PotParams g, double* __restrict__ partial) {
   const long long nh = (long long)g.n0 * g.n1 * g.n2h;
   double acc = 0.0;
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < nh;
        t += (long long)gridDim.x * blockDim.x) {
-    const cpx<double> pv = p[t], av = Ap[t];
-    cpx<double> xv = x[t], rv = r[t];
-    xv = mk(xv.x + a * pv.x, xv.y + a * pv.y);
-    rv = mk(rv.x - a * av.x, rv.y - a * av.y);
-    x[t] = xv;
-    r[t] = rv;
-    acc += plane_weight(t, g) * (rv.x * rv.x + rv.y * rv.y);
+    cpx<double> v = b[t];
+    if (Ax) {
+      const cpx<double> a = Ax[t];
+      v = mk(v.x - a.x, v.y - a.y);
+    }
+    r[t] = mk(T(v.x), T(v.y));
+    acc += plane_weight(t, g) * (v.x * v.x + v.y * v.y);
   }
   const double s = block_sum(acc);
   if (threadIdx.x == 0) partial[blockIdx.x] = s;

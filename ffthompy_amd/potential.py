@@ -40,8 +40,11 @@ def _par(pars, key, default=None):
 
 
 def _solver_par(pars):
+    """tol, maxiter and the arithmetic: solver['precision'] 'fp64' (default, the reference's) or
+    'mixed' (fp32 iterations with fp64 reliable updates, solver['delta'] their trigger)."""
     s = _par(pars, 'solver', {}) or {}
-    return float(s.get('tol', 1e-6)), int(s.get('maxiter', 1000))
+    return (float(s.get('tol', 1e-6)), int(s.get('maxiter', 1000)),
+            {'mode': s.get('precision', 'fp64'), 'delta': float(s.get('delta', 1e-4))})
 
 
 def _e1(d):
@@ -82,13 +85,13 @@ def homog_GaNi_full_potential(Agani, Aga, pars):
     Y = np.asarray(getattr(Agani, 'Y', None) if getattr(Agani, 'Y', None) is not None
                    else _par(pars, 'Y', np.ones(len(N))), dtype=np.float64)
     d = len(N)
-    tol, maxiter = _solver_par(pars)
+    tol, maxiter, prec = _solver_par(pars)
     plan = nat.Plan(N, Y, 'scalar', 'primal')
     plan.set_coeff(*classify_coefficient(Agani.val, 'scalar'))
     E = _e1(d)
     X = nat.DeviceArray((d,) + N)
     t0 = time.perf_counter()
-    Fu, info = plan.potential_solve(E, tol, maxiter, X=X)
+    Fu, info = plan.potential_solve(E, tol, maxiter, X=X, **prec)
     elapsed = time.perf_counter() - t0
     nat.add_constant(X, E)
     # the Gram matrix of d copies of one field has every entry <A XEN, XEN>
@@ -119,7 +122,7 @@ def homog_Ga_full_potential(Aga, pars):
     N = tuple((n + 1) // 2 for n in Nbar)
     Y = np.asarray(Aga.Y, dtype=np.float64)
     d = len(N)
-    tol, maxiter = _solver_par(pars)
+    tol, maxiter, prec = _solver_par(pars)
     coarse = nat.Plan(N, Y, 'scalar', 'primal')
     ga = Aga.load(nat.Ga(coarse))
     M = ga.M
@@ -129,7 +132,7 @@ def homog_Ga_full_potential(Aga, pars):
     E = _e1(d)
     X = nat.DeviceArray((d,) + M)
     t0 = time.perf_counter()
-    FuM, info = plan.potential_solve(E, tol, maxiter, X=X)
+    FuM, info = plan.potential_solve(E, tol, maxiter, X=X, **prec)
     elapsed = time.perf_counter() - t0
     XEN = X.copy()
     nat.add_constant(XEN, E)

@@ -192,15 +192,17 @@ int fh_iter_solve(fh_plan* plan, int method, const void* b, void* x, const doubl
  * banded plan (fh_plan_set_band) carrying the Ga coefficient (fh_ga_set_plan_coeff), of
  * homog_Ga_full_potential (ffthompy/tensorsLowRank/homogenisation.py:43-128; preconditioner
  * get_preconditioner, homogenisation.py:296-300). Unknown: the rfftn spectrum of a scalar
- * potential; CG (fp64, x0 = 0, the reference's stop rule on the inner product of 'r' spectra,
+ * potential; CG (x0 = 0, the reference's stop rule on the inner product of 'r' spectra,
  * ffthompy/tensors/objects.py:618-631) on  x -> P (-div) F A F^-1 grad (P x)  with right-hand
  * side P div F(A E), P = 1/|2 pi xi| (1 at xi = 0). On a banded plan grad, div and P act on
  * |k_a| <= band[a] only (the reference's enlarge / project between N and 2N-1).
+ * mode FH_SOLVE_FP64 (the reference's precision) or FH_SOLVE_MIXED (fp32 iterations, fp64 right-hand
+ * side and iterate, reliable updates when the residual drops by delta, as fh_cg_solve).
  * E: ncomp macroscopic gradient values (host). Outputs (device, fp64): Fu = P x, the potential's
  * spectrum ((n0, n1, n2/2+1) complex, the plan's grid); X (nullable) = F^-1 grad Fu, ncomp real
  * components. Plan of scalar physics; odd grid unless banded. res_hist as fh_cg_solve. */
-int fh_potential_solve(fh_plan* plan, const double* E, double tol, int maxiter, void* Fu, void* X, int* kit,
-                       double* norm_res, double* res_hist, int hist_len, int* cg_status);
+int fh_potential_solve(fh_plan* plan, int mode, const double* E, double tol, int maxiter, double delta, void* Fu,
+                       void* X, int* kit, double* norm_res, double* res_hist, int hist_len, int* cg_status);
 
 /* AH[i][j] = <A e_i, e_j> for nf = ncomp fields e_i (fp64, macroscopic part included),
  * double-double accumulated: assembly_matrix (ffthompy/postprocess.py:53-70). */

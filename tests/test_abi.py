@@ -78,7 +78,7 @@ def test_argument_validation_of_later_entry_points_without_device():
     assert lib.fh_ga_interpolate(None, None, None) == nat.FH_ERR_ARG
     assert lib.fh_ga_set_plan_coeff(None, None) == nat.FH_ERR_ARG
     assert lib.fh_iter_solve(None, 1, None, None, None, 1e-6, 10, None, None, None, 0) == nat.FH_ERR_ARG
-    assert lib.fh_potential_solve(None, None, 1e-6, 10, None, None, None, None, None, 0, None) == nat.FH_ERR_ARG
+    assert lib.fh_potential_solve(None, 0, None, 1e-6, 10, 1e-4, None, None, None, None, None, 0, None) == nat.FH_ERR_ARG
     assert lib.fh_set_coeff_inclusions(None, 1, None, None, None, 1, None, 0) == nat.FH_ERR_ARG
     assert lib.fh_set_coeff_device(None, 1, None) == nat.FH_ERR_ARG
     assert lib.fh_grf(None, None, 0.1, 1.0, 0, None) == nat.FH_ERR_ARG

@@ -207,3 +207,46 @@ def test_rectangular_cell_matches_oracle(N, Y):
     assert res.info['kit'] == k + 1
     assert abs(res.AH - ref['AH']) < 1e-12 * abs(ref['AH'])
     _check_fu(res.Fu, ref['Fu'], 1e-9)
+
+
+@pytest.mark.parametrize('case', ['gani45x2', 'gani27_rough', 'gani45_spheres', 'aniso2', 'ga15x2', 'ga5x3'])
+def test_mixed_precision_matches_fp64(case):
+    """fp32 iterations with fp64 reliable updates (delta 1e-4) against the fp64 solve, both to
+    tol 1e-10: AH within 1e-8, Fu within 2e-6 of its largest entry, final true residual <= tol.
+    Covers the fixed-length kernels (45, 27, 9), the runtime-radix passes (30) and the full
+    coefficient (one-way row passes)."""
+    print('case potential mixed', case, flush=True)
+    rng = np.random.default_rng(29)
+    mixed = {'solver': {'tol': 1e-10, 'maxiter': 2000, 'precision': 'mixed', 'delta': 1e-4}}
+    fp64 = {'solver': {'tol': 1e-10, 'maxiter': 2000}}
+    if case.startswith('ga'):
+        N = (15, 15) if case == 'ga15x2' else (5, 5, 5)
+        maker = examples.scalar_2d_materials if len(N) == 2 else examples.scalar_3d_materials
+        mat = Material(dict(maker()['pyramid' if len(N) == 2 else 'cube_Ga']))
+        Aga = mat.get_A_Ga(tuple(2 * n - 1 for n in N), 'primal')
+        r64 = gpot.homog_Ga_full_potential(Aga, fp64)
+        r32 = gpot.homog_Ga_full_potential(Aga, mixed)
+    else:
+        if case == 'aniso2':
+            N = (45, 45)
+            conf = examples.scalar_2d_materials()['square']
+            A = Material(dict(conf)).get_A_GaNi(N, 'primal').add_mean(ANISO[2]).val
+        else:
+            N = {'gani45x2': (45, 45), 'gani27_rough': (27, 27, 27), 'gani45_spheres': (45, 45, 45)}[case]
+            if case == 'gani45_spheres':
+                a = spheres_conductivity(N, count=30, contrast=100.)[0]
+            elif case == 'gani27_rough':
+                a = np.exp(rng.standard_normal(N))
+            else:
+                a = np.exp(0.5 * rng.standard_normal(N))
+            A = np.einsum('ij,...->ij...', np.eye(len(N)), a)
+        Agani = gpot.Tensor(name='A', val=A, order=2, multype=21)
+        r64 = gpot.homog_GaNi_full_potential(Agani, None, fp64)
+        r32 = gpot.homog_GaNi_full_potential(Agani, None, mixed)
+    assert r32.info['status'] == 0 and r32.info['norm_res'] <= 1e-10
+    assert abs(r32.AH - r64.AH) < 1e-8 * abs(r64.AH), (r32.AH, r64.AH)
+    f64 = r64.Fu.val.to_host() if hasattr(r64.Fu.val, 'to_host') else r64.Fu.val
+    f32 = r32.Fu.val.to_host() if hasattr(r32.Fu.val, 'to_host') else r32.Fu.val
+    # both residuals are <= 1e-10 in the grid-mean norm, i.e. 1e-10 * prod(N) in the unnormalised
+    # spectrum; times the operator's condition number this allows ~1e-6 of the largest entry
+    assert np.abs(f32 - f64).max() <= 2e-6 * np.abs(f64).max()
