@@ -459,6 +459,17 @@ def main():
                           % (args.cpu_iters, s_it, kit64, os.cpu_count(), len(os.sched_getaffinity(0)))),
                'sec_per_iter': s_it}
         log('cpu baseline: %.2f s/iter -> %.0f s per step (extrapolated)' % (s_it, cpu['value']))
+        # the same iterations without the CallBack matvec, and a tuned CPU variant (scipy.fft on every
+        # core of this process, no CallBack): reported beside the baseline, not used as it
+        s_nc = time_cg_iterations(a, np.ones(3), iters=args.cpu_iters, callback=False)
+        nw = len(os.sched_getaffinity(0))
+        s_tuned = time_cg_iterations(a, np.ones(3), iters=args.cpu_iters, callback=False, workers=nw)
+        cpu['sec_per_iter_no_callback'] = s_nc
+        cpu['tuned_cpu'] = {'value': s_tuned * kit64, 'unit': 's', 'cores': nw, 'sec_per_iter': s_tuned,
+                            'sample': 'same iterations with scipy.fft (workers=%d) and no CallBack, '
+                                      'extrapolated to %d iterations' % (nw, kit64)}
+        log('cpu baseline: %.2f s/iter without CallBack; tuned (scipy.fft, %d workers) %.2f s/iter'
+            % (s_nc, nw, s_tuned))
 
     out = {
         'metric': METRIC,

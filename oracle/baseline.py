@@ -32,26 +32,35 @@ def materialised_G1_half(N, Y):
 
 
 class RefStyleOperator:
-    """Afun(x) = irfftn(hG1 . rfftn(A . x)) with fresh arrays at every step."""
+    """Afun(x) = irfftn(hG1 . rfftn(A . x)) with fresh arrays at every step. workers=None: numpy.fft
+    (the reference's, one core); workers=k: scipy.fft with k threads (the tuned CPU variant)."""
 
-    def __init__(self, a, Y):
+    def __init__(self, a, Y, workers=None):
         self.N = a.shape
         d = len(self.N)
         self.A = np.einsum('ij,...->ij...', np.eye(d), a)
         self.G = materialised_G1_half(self.N, Y)
         self.axes = tuple(range(1, d + 1))
+        self.workers = workers
 
     def __call__(self, x):
         y = np.einsum('ij...,j...->i...', self.A, x)
-        Y = np.fft.rfftn(y, s=self.N, axes=self.axes)
+        if self.workers is None:
+            Y = np.fft.rfftn(y, s=self.N, axes=self.axes)
+        else:
+            import scipy.fft
+            Y = scipy.fft.rfftn(y, s=self.N, axes=self.axes, workers=self.workers)
         Y = np.einsum('ij...,j...->i...', self.G, Y)
-        return np.fft.irfftn(Y, s=self.N, axes=self.axes) + 0.
+        if self.workers is None:
+            return np.fft.irfftn(Y, s=self.N, axes=self.axes) + 0.
+        import scipy.fft
+        return scipy.fft.irfftn(Y, s=self.N, axes=self.axes, workers=self.workers) + 0.
 
 
-def time_cg_iterations(a, Y, iters=2, callback=True):
+def time_cg_iterations(a, Y, iters=2, callback=True, workers=None):
     """Run `iters` reference-style CG iterations on the load E = e_1 and return
-    seconds per iteration (with the CallBack matvec when callback=True)."""
-    Afun = RefStyleOperator(a, Y)
+    seconds per iteration (with the CallBack matvec when callback=True; FFTs as RefStyleOperator)."""
+    Afun = RefStyleOperator(a, Y, workers=workers)
     d = a.ndim
     n = float(np.prod(a.shape))
     EN = np.zeros((d,) + a.shape)
