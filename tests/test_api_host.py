@@ -71,3 +71,22 @@ def test_problem_parsing():
     bad['inclusions'] = ['otherwise', 'square']
     with pytest.raises(ValueError):
         Problem.parse_material(bad)
+
+
+def test_ga_coefficient_add_mean_layout():
+    """GaCoefficient.add_mean: an inclusion coefficient gains an 'all' inclusion carrying the
+    constant, a nodal one the constant on every node; both become the 'full' layout."""
+    from ffthompy_amd.materials import Material
+    d = 2
+    K = np.array([[5.5, -4.5], [-4.5, 5.5]])
+    conf = {'inclusions': ['square', 'otherwise'], 'positions': [np.zeros(d), ''], 'params': [0.6 * np.ones(d), ''],
+            'vals': [10 * np.eye(d), 1. * np.eye(d)], 'Y': np.ones(d), 'P': (5, 5), 'order': None}
+    A = Material(dict(conf)).get_A_Ga((9, 9), 'primal', order=None).add_mean(K)
+    assert A.kind == 'full' and list(A.inclusions[0]) == ['square', 'otherwise', 'all']
+    v = A.vals.reshape(d * d, 3)
+    assert np.allclose(v[:, 0], (10 * np.eye(d)).ravel()) and np.allclose(v[:, 2], K.ravel())
+    G = Material(dict(conf)).get_A_Ga((9, 9), 'primal', order=0, P=(5, 5))
+    base = G.vals.copy()
+    G.add_mean(K)
+    assert G.kind == 'full' and G.vals.shape == (d * d, 5, 5)
+    assert np.allclose(G.vals[1], K[0, 1]) and np.allclose(G.vals[0], base + K[0, 0])
