@@ -250,3 +250,41 @@ def test_mixed_precision_matches_fp64(case):
     # both residuals are <= 1e-10 in the grid-mean norm, i.e. 1e-10 * prod(N) in the unnormalised
     # spectrum; times the operator's condition number this allows ~1e-6 of the largest entry
     assert np.abs(f32 - f64).max() <= 2e-6 * np.abs(f64).max()
+
+
+def _lowrank_example_material(material, dim, N):
+    """The material dictionaries of the reference's low-rank examples (square with order 0 on P = 5,
+    pyramid with order 1 on P = N; material 3 adds an anisotropic constant)."""
+    if material in (0, 3):
+        return {'inclusions': ['square', 'otherwise'], 'positions': [np.zeros(dim), ''],
+                'params': [0.6 * np.ones(dim), ''], 'vals': [10 * np.eye(dim), 1. * np.eye(dim)],
+                'Y': np.ones(dim), 'P': (5,) * dim, 'order': 0}
+    return {'inclusions': ['pyramid', 'all'], 'positions': [np.zeros(dim), ''], 'params': [0.8 * np.ones(dim), ''],
+            'vals': [10 * np.eye(dim), 1. * np.eye(dim)], 'Y': np.ones(dim), 'P': N, 'order': 1}
+
+
+@pytest.mark.parametrize('dim,material', [(2, 0), (2, 3), (3, 1), (3, 0)])
+def test_reference_unittest_cases_match_oracle(dim, material):
+    """The cases of the reference's potential test (N = 5, tol 1e-8, maxiter 50): materials 0 and 3
+    through homog_Ga_full_potential, material 1 through homog_GaNi_full_potential with A_Ga;
+    mean of Fu zero, AH and iteration counts as the oracle's."""
+    print('case potential unittest', dim, material, flush=True)
+    N = (5,) * dim
+    conf = _lowrank_example_material(material, dim, N)
+    mat, omat = Material(dict(conf)), OMaterial(dict(conf))
+    Nbar = tuple(2 * n - 1 for n in N)
+    pars = {'solver': {'tol': 1e-8, 'maxiter': 50}}
+    Aga, oAga = mat.get_A_Ga(Nbar, 'primal'), omat.get_A_Ga(Nbar, 'primal')
+    if material == 3:
+        Aga.add_mean(ANISO[dim])
+        oAga.add_mean(ANISO[dim])
+    if material in (0, 3):
+        res = gpot.homog_Ga_full_potential(Aga, pars)
+        ref = opot.homog_Ga_full_potential(oAga, tol=1e-8, maxiter=50)
+    else:
+        Agani, oAgani = mat.get_A_GaNi(N, 'primal'), omat.get_A_GaNi(N, 'primal')
+        res = gpot.homog_GaNi_full_potential(Agani, Aga, pars)
+        ref = opot.homog_GaNi_full_potential(oAgani, oAga, tol=1e-8, maxiter=50)
+    assert abs(res.Fu.mean()) < 1e-12
+    assert res.info['kit'] == ref['info']['kit']
+    assert abs(res.AH - ref['AH']) < 1e-10 * abs(ref['AH'])
