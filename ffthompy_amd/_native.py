@@ -82,6 +82,8 @@ SIGNATURES = {
     'fh_ga_energy': [_vp, ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, _c_double_p],
     'fh_iter_solve': [_vp, ctypes.c_int, _vp, _vp, _c_double_p, ctypes.c_double, ctypes.c_int, _c_int_p, _c_double_p,
                       _c_double_p, ctypes.c_int],
+    'fh_plan_set_residual_log': [_vp, ctypes.c_int],
+    'fh_last_residual_log': [_vp, _c_int_p, _c_double_p, ctypes.c_int, _c_int_p],
     'fh_potential_solve': [_vp, ctypes.c_int, _c_double_p, ctypes.c_double, ctypes.c_int, ctypes.c_double, _vp, _vp,
                            _c_int_p, _c_double_p, _c_double_p, ctypes.c_int, _c_int_p],
     'fh_set_coeff_inclusions': [_vp, ctypes.c_int, _c_int_p, _c_double_p, _c_double_p, ctypes.c_int, _c_double_p,
@@ -315,6 +317,20 @@ class Plan:
         n = ctypes.c_int()
         call('fh_last_solve_graph_batches', self._h, ctypes.byref(n))
         return n.value
+
+    def set_residual_log(self, every):
+        """fp64 CG: log ||b - A x|| every `every` iterations (0: off), fh_plan_set_residual_log."""
+        call('fh_plan_set_residual_log', self._h, ctypes.c_int(int(every)))
+
+    def residual_log(self):
+        """(iterations, norms) of the last solve's residual log."""
+        cnt = ctypes.c_int()
+        call('fh_last_residual_log', self._h, None, None, ctypes.c_int(0), ctypes.byref(cnt))
+        k = np.zeros(max(cnt.value, 1), dtype=np.int32)
+        v = np.zeros(max(cnt.value, 1))
+        call('fh_last_residual_log', self._h, k.ctypes.data_as(_c_int_p), v.ctypes.data_as(_c_double_p),
+             ctypes.c_int(cnt.value), ctypes.byref(cnt))
+        return k[:cnt.value].copy(), v[:cnt.value].copy()
 
     def use_generic(self, flag=True):
         call('fh_plan_use_generic', self._h, ctypes.c_int(1 if flag else 0))

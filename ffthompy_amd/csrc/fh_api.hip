@@ -171,6 +171,9 @@ struct fh_plan {
   bool it_graph_off = false;     // capture failed on this plan: direct launches from then on
   bool use_graphs = true;        // fh_plan_use_graphs
   int last_graph_batches = 0;    // batches of the last solve replayed from the graph
+  int res_every = 0;             // fp64 CG: log ||b - A x|| every res_every iterations (0: off)
+  std::vector<int> res_kit;      // the log of the last solve: iteration counts and norms
+  std::vector<double> res_norm;
 };
 
 struct fh_batch {
@@ -1034,6 +1037,25 @@ int fh_plan_use_graphs(fh_plan* pl, int enable) {
   return FH_OK;
 }
 
+int fh_plan_set_residual_log(fh_plan* pl, int every) {
+  RETCHK(check_plan(pl));
+  if (every < 0) return fail(FH_ERR_ARG, "every must be >= 0");
+  pl->res_every = every;
+  return FH_OK;
+}
+
+int fh_last_residual_log(const fh_plan* pl, int* kits, double* norms, int cap, int* count) {
+  RETCHK(check_plan(pl));
+  if (!count || cap < 0 || (cap > 0 && (!kits || !norms))) return fail(FH_ERR_ARG, "bad arguments");
+  const int n = (int)pl->res_norm.size();
+  *count = n;
+  for (int i = 0; i < std::min(n, cap); ++i) {
+    kits[i] = pl->res_kit[i];
+    norms[i] = pl->res_norm[i];
+  }
+  return FH_OK;
+}
+
 int fh_last_solve_graph_batches(const fh_plan* pl, int* n) {
   RETCHK(check_plan(pl));
   if (!n) return fail(FH_ERR_ARG, "null pointer argument");
@@ -1147,6 +1169,10 @@ int fh_cg_solve(fh_plan* pl, int mode, const void* b_, void* x_, double tol, int
   if (!pl->coef_kind) return fail(FH_ERR_STATE, "fh_set_coeff has not been called");
   if (!(tol >= 0.0) || maxiter < 0) return fail(FH_ERR_ARG, "bad tol/maxiter");
   if (mode == FH_SOLVE_MIXED && !(delta > 0.0 && delta < 1.0)) return fail(FH_ERR_ARG, "delta must be in (0,1)");
+  if (mode == FH_SOLVE_MIXED && pl->res_every > 0)
+    return fail(FH_ERR_UNSUPPORTED, "the residual log (fh_plan_set_residual_log) is kept by fp64 solves only");
+  pl->res_kit.clear();
+  pl->res_norm.clear();
   const double* b = (const double*)b_;
   double* x = (double*)x_;
   const long long nv = (long long)pl->ncomp * pl->n;
@@ -1174,10 +1200,33 @@ int fh_cg_solve(fh_plan* pl, int mode, const void* b_, void* x_, double tol, int
     double* Ap = p + nvp;
     RETCHK(enqueue_true_residual<double>(pl, b, x, Ap, r, x0_zero));
     HIPCHK(hipMemcpyAsync(p, r, (size_t)nv * sizeof(double), hipMemcpyDeviceToDevice, pl->stream));
+    // residual log: ||b - A x|| in fp64 (the reference's CallBack, solver_pp.py:13-24) at x0 and
+    // after every res_every-th iteration and the last; Ap is free between iterations
+    const int every = pl->res_every;
+    int logged = -1;
+    auto log_true = [&](int kit_now) -> int {
+      RETCHK(run_operator<double>(pl, x, Ap, pl->coef_kind, pl->coef64, nullptr, nullptr, nullptr));
+      const int nb = grid_blocks(nv);
+      hipLaunchKernelGGL(fh::k_resid_norm_partial, dim3(nb), dim3(256), 0, pl->stream, nv, b, Ap, pl->partial);
+      double* dsum = pl->partial + nb;
+      hipLaunchKernelGGL(fh::k_sum_partials, dim3(1), dim3(256), 0, pl->stream, pl->partial, nb, 1.0 / (double)pl->n,
+                         dsum);
+      HIPCHK(hipGetLastError());
+      double h = 0.0;
+      HIPCHK(hipMemcpyAsync(&h, dsum, sizeof(double), hipMemcpyDeviceToHost, pl->stream));
+      HIPCHK(hipStreamSynchronize(pl->stream));
+      pl->res_kit.push_back(kit_now);
+      pl->res_norm.push_back(std::sqrt(h));
+      logged = kit_now;
+      return FH_OK;
+    };
+    if (every > 0) RETCHK(log_true(0));
+    const int batch = every > 0 ? every : per_check;
     for (;;) {
-      for (int k = 0; k < per_check; ++k) RETCHK(enqueue_iteration<double>(pl, x, r, p, Ap, pl->coef64));
+      for (int k = 0; k < batch; ++k) RETCHK(enqueue_iteration<double>(pl, x, r, p, Ap, pl->coef64));
       HIPCHK(hipMemcpyAsync(&s, pl->st, sizeof(s), hipMemcpyDeviceToHost, pl->stream));
       HIPCHK(hipStreamSynchronize(pl->stream));
+      if (every > 0 && s.kit > logged) RETCHK(log_true(s.kit));
       if (s.done) break;
     }
   } else {

@@ -525,6 +525,20 @@ __global__ void k_dot_partial(long long n, const T* __restrict__ x, const T* __r
   if (threadIdx.x == 0) partial[blockIdx.x] = b;
 }
 
+// partial sums of |b - Ax|^2 (nothing written: the true residual of a residual log)
+static __global__ void __launch_bounds__(256)
+k_resid_norm_partial(long long n, const double* __restrict__ b, const double* __restrict__ Ax,
+                     double* __restrict__ partial) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  double acc = 0.0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const double v = b[i] - Ax[i];
+    acc += v * v;
+  }
+  const double s = block_sum(acc);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
 static __global__ void k_sum_partials(const double* __restrict__ partial, int np, double scale,
                                double* __restrict__ out) {
   double acc = 0.0;

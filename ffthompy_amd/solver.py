@@ -18,9 +18,12 @@ from .tensors import GpuOperator, Tensor
 
 class CallBack:
     """Residual log of a solve (solver_pp.py:6-24). The reference evaluates B - A x after
-    every iteration with an extra matvec; the device solve instead records
+    every iteration with an extra matvec; the device solve by default records
     sqrt(<r,r>) of the CG residual after every iteration (the same quantity up to
-    rounding), with no extra work. res_norm[0] is the initial residual."""
+    rounding), with no extra work. res_norm[0] is the initial residual. With
+    par['true_residual_every'] = k (fp64 CG) the device evaluates ||B - A x|| itself at x0, every
+    k-th iteration and the last (one extra fp64 matvec each): res_norm holds those values,
+    res_kit their iterations, res_norm_recursive the recursive log."""
 
     def __init__(self, A=None, B=None, **kwargs):
         self.__dict__.update(kwargs)
@@ -136,8 +139,14 @@ def CG(Afun, B, x0=None, par=None, callback=None):
     B = B.to_device()
     x = B.zeros_like(name='x') if x0 is None else x0.to_device().copy(name='x')
     hist_len = int(par['maxiter']) + 1 if callback is not None else 0
-    out = Afun.plan.cg_solve(B.val, x.val, tol=float(par['tol']), maxiter=int(par['maxiter']), mode=mode,
-                             delta=float(par.get('delta', 0.01)), hist_len=hist_len)
+    every = int(par.get('true_residual_every', 0)) if isinstance(callback, CallBack) else 0
+    Afun.plan.set_residual_log(every)
+    try:
+        out = Afun.plan.cg_solve(B.val, x.val, tol=float(par['tol']), maxiter=int(par['maxiter']), mode=mode,
+                                 delta=float(par.get('delta', 0.01)), hist_len=hist_len)
+    finally:
+        if every:
+            Afun.plan.set_residual_log(0)
     if out['status'] == nat.FH_CG_BREAKDOWN:
         raise FloatingPointError('CG breakdown: <p, A p> <= 0 (coefficient not positive definite?)')
     info = {'kit': out['kit'], 'norm_res': out['norm_res']}
@@ -146,6 +155,12 @@ def CG(Afun, B, x0=None, par=None, callback=None):
     if callback is not None:
         if isinstance(callback, CallBack):
             callback.record(out['res_log'])
+            if every:
+                # the reference's CallBack quantity ||B - A x|| (fp64 operator), iterations in res_kit
+                callback.res_norm_recursive = callback.res_norm
+                kits, norms = Afun.plan.residual_log()
+                callback.res_kit = [int(k) for k in kits]
+                callback.res_norm = [float(v) for v in norms]
         else:
             callback(x)
     return x, info

@@ -98,6 +98,15 @@ int fh_plan_set_cg_variant(fh_plan* plan, int fused);
  * fh_last_solve_graph_batches: batches of the last solve that ran from the graph.
  * No reference counterpart (launch mechanics). */
 int fh_plan_use_graphs(fh_plan* plan, int enable);
+
+/* Residual log of the fp64 CG: every >= 1 makes fh_cg_solve evaluate ||b - A x|| (grid-mean norm,
+ * one extra fp64 operator application) at x0 and after every every-th iteration and the last one:
+ * the reference's CallBack, which evaluates B - A(x) after each iteration (every = 1),
+ * ffthompy/general/solver_pp.py:13-24, installed at ffthompy/applications.py:71-72. 0 (default): off.
+ * Mixed-precision solves reject a plan with the log on. fh_last_residual_log: the (iteration, norm)
+ * pairs of the last solve, at most cap of them, *count = how many there are. */
+int fh_plan_set_residual_log(fh_plan* plan, int every);
+int fh_last_residual_log(const fh_plan* plan, int* kits, double* norms, int cap, int* count);
 int fh_last_solve_graph_batches(const fh_plan* plan, int* n);
 
 /* Coefficient A(x) of the operator, host array in origin-0 voxel order.

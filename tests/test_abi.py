@@ -84,4 +84,6 @@ def test_argument_validation_of_later_entry_points_without_device():
     assert lib.fh_grf(None, None, 0.1, 1.0, 0, None) == nat.FH_ERR_ARG
     assert lib.fh_plan_set_band(None, None, 1.0) == nat.FH_ERR_ARG
     assert lib.fh_plan_use_graphs(None, 1) == nat.FH_ERR_ARG
+    assert lib.fh_plan_set_residual_log(None, 1) == nat.FH_ERR_ARG
+    assert lib.fh_last_residual_log(None, None, None, 0, None) == nat.FH_ERR_ARG
     assert len(lib.fh_last_error()) > 0

@@ -1,0 +1,92 @@
+"""The reference's CallBack on the device: ||B - A x|| evaluated in fp64 at x0 and after the
+iterations of the CG (fh_plan_set_residual_log), against the oracle's CallBack, which evaluates
+B - A(x) on the host after every iteration (ffthompy/general/solver_pp.py:13-24).
+
+* every = 1: one entry per iteration (and x0), equal to the oracle's within 1e-12 of the initial
+  residual plus 1e-6 relative (both are fp64 evaluations of nearly the same iterates).
+* every = 3: the entries at iterations 0, 3, 6, ... and the last, bitwise those of every = 1 (the
+  log does not touch the CG's arithmetic).
+* The mixed-precision solve rejects a plan with the log on.
+"""
+import numpy as np
+import pytest
+
+from oracle import projections
+from oracle.fields import DFT, Field, Operator
+from oracle.solver import CallBack as OCallBack
+from oracle.solver import linear_solver as o_linear_solver
+
+pytestmark = pytest.mark.gpu
+nat = pytest.importorskip('ffthompy_amd._native')
+from ffthompy_amd import GpuOperator, Tensor, linear_solver  # noqa: E402
+from ffthompy_amd.solver import CallBack  # noqa: E402
+
+
+def _case(N, seed, spread=1.0):
+    rng = np.random.default_rng(seed)
+    d = len(N)
+    a = np.exp(spread * rng.standard_normal(N))
+    return a, np.einsum('ij,...->ij...', np.eye(d), a)
+
+
+def _oracle_log(N, A, tol):
+    """The oracle's CallBack log at a tolerance placed midway (geometrically) between the two
+    recursive residuals around `tol`, so that both solves stop at the same iteration with margin."""
+    d = len(N)
+    _, hG1, _ = projections.scalar(N, np.ones(d), NyqNul=True)
+    Afun = Operator([[Operator([[DFT(N, inverse=True), hG1, DFT(N)]]), Field(val=A, order=2, multype=21)]])
+    EN = Field(shape=(d,), N=N)
+    EN.set_mean(np.eye(d)[0])
+    B = Afun(-EN)
+    _, info = o_linear_solver(Afun, B, x0=B.zeros_like(), par={'tol': tol, 'maxiter': 500}, solver='CG')
+    log = info['norm_res_log']
+    k = info['kit']
+    tol_mid = float(np.sqrt(log[k - 1] * log[k]))
+    cb = OCallBack(Afun, B)
+    _, info = o_linear_solver(Afun, B, x0=B.zeros_like(), par={'tol': tol_mid, 'maxiter': 500}, solver='CG',
+                              callback=cb)
+    assert info['kit'] == k
+    return np.asarray(cb.res_norm), k, tol_mid
+
+
+@pytest.mark.parametrize('N', [(27, 27, 27), (45, 45)])
+def test_true_residual_log_matches_oracle_callback(N):
+    print('case residual log', N, flush=True)
+    d = len(N)
+    # a moderate contrast keeps the two fp64 CGs' iterates within rounding of each other to the
+    # end (with a rough coefficient they drift apart by more than the late true residuals)
+    a, A = _case(N, 31, spread=0.3)
+    ref, kit_ref, tol = _oracle_log(N, A, 1e-8)
+    Afun = GpuOperator(N, np.ones(d), 'scalar', 'primal', A=Tensor(name='A', val=A, order=2, multype=21))
+    EN = Tensor(name='EN', shape=(d,), N=N)
+    EN.set_mean(np.eye(d)[0])
+    B = Afun(-EN)
+    logs = {}
+    for every in (1, 3):
+        cb = CallBack(A=Afun, B=B)
+        x, info = linear_solver(Afun, B, x0=B.zeros_like(), par={'tol': tol, 'maxiter': 500,
+                                                                'true_residual_every': every},
+                                solver='CG', callback=cb)
+        assert info['kit'] == kit_ref
+        logs[every] = (np.asarray(cb.res_kit), np.asarray(cb.res_norm))
+        assert len(cb.res_norm_recursive) == kit_ref + 1
+    k1, v1 = logs[1]
+    assert list(k1) == list(range(kit_ref + 1))
+    assert np.all(np.abs(v1 - ref) <= 1e-12 * ref[0] + 1e-6 * ref)
+    k3, v3 = logs[3]
+    want = sorted(set(list(range(0, kit_ref + 1, 3)) + [kit_ref]))
+    assert list(k3) == want
+    assert np.array_equal(v3, v1[k3])
+
+
+def test_mixed_solve_rejects_residual_log():
+    N = (15, 15, 15)
+    a, A = _case(N, 33)
+    plan = nat.Plan(N, (1., 1., 1.), 'scalar', 'primal')
+    plan.set_coeff('iso', a)
+    b = nat.DeviceArray.zeros((3,) + N)
+    x = nat.DeviceArray.zeros((3,) + N)
+    plan.set_residual_log(1)
+    with pytest.raises(nat.FhError):
+        plan.cg_solve(b, x, tol=1e-8, maxiter=10, mode='mixed', delta=1e-4)
+    plan.set_residual_log(0)
