@@ -1221,12 +1221,15 @@ int fh_cg_solve(fh_plan* pl, int mode, const void* b_, void* x_, double tol, int
       return FH_OK;
     };
     if (every > 0) RETCHK(log_true(0));
-    const int batch = every > 0 ? every : per_check;
+    int kit_host = 0;
     for (;;) {
+      // batches end on the logged iterations (multiples of every), at most per_check long
+      const int batch = every > 0 ? std::min(per_check, every - kit_host % every) : per_check;
       for (int k = 0; k < batch; ++k) RETCHK(enqueue_iteration<double>(pl, x, r, p, Ap, pl->coef64));
       HIPCHK(hipMemcpyAsync(&s, pl->st, sizeof(s), hipMemcpyDeviceToHost, pl->stream));
       HIPCHK(hipStreamSynchronize(pl->stream));
-      if (every > 0 && s.kit > logged) RETCHK(log_true(s.kit));
+      kit_host = s.kit;
+      if (every > 0 && s.kit > logged && (s.kit % every == 0 || s.done)) RETCHK(log_true(s.kit));
       if (s.done) break;
     }
   } else {

@@ -5,7 +5,7 @@ B - A(x) on the host after every iteration (ffthompy/general/solver_pp.py:13-24)
 * every = 1: one entry per iteration (and x0), equal to the oracle's within 1e-12 of the initial
   residual plus 1e-6 relative (both are fp64 evaluations of nearly the same iterates).
 * every = 3: the entries at iterations 0, 3, 6, ... and the last, bitwise those of every = 1 (the
-  log does not touch the CG's arithmetic).
+  log does not touch the CG's arithmetic); every = 100 > the iteration count: x0 and the last.
 * The mixed-precision solve rejects a plan with the log on.
 """
 import numpy as np
@@ -62,7 +62,7 @@ def test_true_residual_log_matches_oracle_callback(N):
     EN.set_mean(np.eye(d)[0])
     B = Afun(-EN)
     logs = {}
-    for every in (1, 3):
+    for every in (1, 3, 100):
         cb = CallBack(A=Afun, B=B)
         x, info = linear_solver(Afun, B, x0=B.zeros_like(), par={'tol': tol, 'maxiter': 500,
                                                                 'true_residual_every': every},
@@ -77,6 +77,8 @@ def test_true_residual_log_matches_oracle_callback(N):
     want = sorted(set(list(range(0, kit_ref + 1, 3)) + [kit_ref]))
     assert list(k3) == want
     assert np.array_equal(v3, v1[k3])
+    k100, v100 = logs[100]
+    assert list(k100) == [0, kit_ref] and np.array_equal(v100, v1[k100])
 
 
 def test_mixed_solve_rejects_residual_log():
