@@ -120,6 +120,9 @@ def homog_Ga_full_potential(Aga, pars):
     factor above 13), Fu on the grid N."""
     Nbar = tuple(int(n) for n in Aga.N)
     N = tuple((n + 1) // 2 for n in Nbar)
+    if any(n % 2 == 0 for n in N):
+        # an even N has a Nyquist mode, which the band |k| <= (N-1)/2 of the device solve leaves out
+        raise NotImplementedError('homog_Ga_full_potential needs an odd grid N = (Nbar+1)/2 (got %s)' % (N,))
     Y = np.asarray(Aga.Y, dtype=np.float64)
     d = len(N)
     tol, maxiter, prec = _solver_par(pars)

@@ -125,3 +125,12 @@ def test_host_inner_product_of_r_spectra_matches_oracle():
         tz = Tensor(val=Z.val, order=0, N=N, Fourier=True)
         assert abs(tx * tz - X * Z) < 1e-13 * abs(X * X)
         assert abs(tx.mean() - X.mean()) < 1e-15
+
+
+def test_ga_potential_rejects_even_solution_grid():
+    from ffthompy_amd.materials import GaCoefficient
+    from ffthompy_amd.potential import homog_Ga_full_potential
+    A = GaCoefficient((7, 7), np.ones(2), 'A', 'iso', np.ones(1), inclusions=(['all'], np.zeros((1, 2)),
+                                                                              np.zeros((1, 2))))
+    with pytest.raises(NotImplementedError):
+        homog_Ga_full_potential(A, {'solver': {'tol': 1e-8, 'maxiter': 10}})
