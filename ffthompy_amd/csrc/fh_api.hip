@@ -171,6 +171,10 @@ struct fh_plan {
   bool it_graph_off = false;     // capture failed on this plan: direct launches from then on
   bool use_graphs = true;        // fh_plan_use_graphs
   int last_graph_batches = 0;    // batches of the last solve replayed from the graph
+  // potential solve: a batch of iterations captured once as a HIP graph, replayed per batch
+  hipGraphExec_t pot_exec = nullptr;
+  const void* pot_key[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  bool pot_graph_off = false;
   int res_every = 0;             // fp64 CG: log ||b - A x|| every res_every iterations (0: off)
   std::vector<int> res_kit;      // the log of the last solve: iteration counts and norms
   std::vector<double> res_norm;
@@ -834,6 +838,49 @@ int pot_operator(fh_plan* pl, cpx<T>* p, const cpx<T>* r, cpx<T>* out, T* re, co
   return FH_OK;
 }
 
+// Forget the captured iteration graphs of a plan: their kernels and arguments were chosen for the
+// plan's projection, band, kernel family and coefficient at capture time. Called by every entry
+// point that changes one of those (all entry points have finished their GPU work on return).
+void drop_graphs(fh_plan* pl) {
+  if (pl->it_exec) (void)hipGraphExecDestroy(pl->it_exec);
+  if (pl->pot_exec) (void)hipGraphExecDestroy(pl->pot_exec);
+  pl->it_exec = nullptr;
+  pl->pot_exec = nullptr;
+}
+
+// Run one batch of potential-solve iterations from a HIP graph captured on the plan's stream
+// (kernel nodes only; rebuilt when a buffer in key moves). *ran = false when capture is not
+// available: the caller then enqueues the batch itself.
+template <class Enq>
+int pot_graph_batch(fh_plan* pl, const void* const (&key)[8], Enq&& enq, bool* ran) {
+  *ran = false;
+  if (!pl->use_graphs || pl->pot_graph_off) return FH_OK;
+  if (!pl->pot_exec || std::memcmp(key, pl->pot_key, sizeof(key)) != 0) {
+    if (pl->pot_exec) (void)hipGraphExecDestroy(pl->pot_exec);
+    pl->pot_exec = nullptr;
+    hipGraph_t graph = nullptr;
+    int rc = FH_OK;
+    const bool ok = hipStreamBeginCapture(pl->stream, hipStreamCaptureModeRelaxed) == hipSuccess;
+    if (ok) rc = enq();
+    const hipError_t ec = ok ? hipStreamEndCapture(pl->stream, &graph) : hipErrorUnknown;
+    if (ok && rc == FH_OK && ec == hipSuccess && graph &&
+        hipGraphInstantiate(&pl->pot_exec, graph, nullptr, nullptr, 0) == hipSuccess) {
+      std::memcpy(pl->pot_key, key, sizeof(key));
+    } else {
+      pl->pot_exec = nullptr;
+      pl->pot_graph_off = true;  // capture not available here: direct launches from then on
+    }
+    if (graph) (void)hipGraphDestroy(graph);
+    (void)hipGetLastError();
+  }
+  if (pl->pot_exec) {
+    HIPCHK(hipGraphLaunch(pl->pot_exec, pl->stream));
+    *ran = true;
+    ++pl->last_graph_batches;
+  }
+  return FH_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -911,6 +958,7 @@ int fh_plan_destroy(fh_plan* pl) {
   if (pl->hist) (void)hipFree(pl->hist);
   if (pl->work) (void)hipFree(pl->work);
   if (pl->it_exec) (void)hipGraphExecDestroy(pl->it_exec);
+  if (pl->pot_exec) (void)hipGraphExecDestroy(pl->pot_exec);
   if (pl->ev0) (void)hipEventDestroy(pl->ev0);
   if (pl->ev1) (void)hipEventDestroy(pl->ev1);
   if (pl->stream) (void)hipStreamDestroy(pl->stream);
@@ -1014,18 +1062,21 @@ int fh_plan_ncomp(const fh_plan* pl) { return pl ? pl->ncomp : -1; }
 
 int fh_plan_use_generic(fh_plan* pl, int generic) {
   RETCHK(check_plan(pl));
+  drop_graphs(pl);
   pl->generic = generic ? 1 : 0;
   return FH_OK;
 }
 
 int fh_plan_set_cg_variant(fh_plan* pl, int fused) {
   RETCHK(check_plan(pl));
+  drop_graphs(pl);
   pl->fused_cg = fused != 0;
   return FH_OK;
 }
 
 int fh_plan_set_primaldual(fh_plan* pl, int primaldual) {
   RETCHK(check_plan(pl));
+  drop_graphs(pl);
   if (primaldual != FH_PRIMAL && primaldual != FH_DUAL) return fail(FH_ERR_ARG, "bad primaldual");
   pl->primaldual = primaldual;
   return FH_OK;
@@ -1065,6 +1116,7 @@ int fh_last_solve_graph_batches(const fh_plan* pl, int* n) {
 
 int fh_plan_set_band(fh_plan* pl, const int* band, double scale) {
   RETCHK(check_plan(pl));
+  drop_graphs(pl);
   if (!(scale > 0.0) || !std::isfinite(scale)) return fail(FH_ERR_ARG, "scale must be positive");
   if (!band) {
     pl->banded = 0;
@@ -1085,6 +1137,7 @@ int fh_plan_set_band(fh_plan* pl, const int* band, double scale) {
 
 int fh_set_coeff(fh_plan* pl, int kind, const double* host_vals) {
   RETCHK(check_plan(pl));
+  drop_graphs(pl);
   if (!host_vals) return fail(FH_ERR_ARG, "null coefficient array");
   int per;
   if (kind == FH_COEF_ISO) {
@@ -2017,6 +2070,7 @@ int fh_ga_set_grid_coeff(fh_ga* ga, int order, const int* P, int kind, const dou
 
 int fh_ga_set_plan_coeff(const fh_ga* ga, fh_plan* pl) {
   if (!ga || !pl) return fail(FH_ERR_ARG, "null pointer argument");
+  drop_graphs(pl);
   if (!ga->coef_kind) return fail(FH_ERR_STATE, "Ga coefficient not set");
   const fh_plan* f = ga->fine;
   if (pl->dim != f->dim || pl->n0 != f->n0 || pl->n1 != f->n1 || pl->n2 != f->n2)
@@ -2243,6 +2297,7 @@ int fh_iter_solve(fh_plan* pl, int method, const void* b_, void* x_, const doubl
 int fh_set_coeff_inclusions(fh_plan* pl, int nincl, const int* kinds, const double* params, const double* positions,
                             int kind, const double* vals, int invert) {
   RETCHK(check_plan(pl));
+  drop_graphs(pl);
   if (!kinds || !params || !positions || !vals) return fail(FH_ERR_ARG, "null pointer argument");
   if (nincl < 1 || nincl > 4096) return fail(FH_ERR_ARG, "number of inclusions out of range");
   int ncoef, layout;
@@ -2334,6 +2389,7 @@ int fh_set_coeff_inclusions(fh_plan* pl, int nincl, const int* kinds, const doub
 
 int fh_set_coeff_device(fh_plan* pl, int kind, const void* dev_vals) {
   RETCHK(check_plan(pl));
+  drop_graphs(pl);
   if (!dev_vals) return fail(FH_ERR_ARG, "null coefficient array");
   int per;
   if (kind == FH_COEF_ISO) per = 1;
@@ -2442,6 +2498,7 @@ int fh_potential_solve(fh_plan* pl, int mode, const double* E, double tol, int m
       if ((pl->Nu[a] & 1) == 0)
         return fail(FH_ERR_UNSUPPORTED, "the potential formulation needs odd grids (even: Nyquist modes)");
   const bool mixed = mode == FH_SOLVE_MIXED;
+  pl->last_graph_batches = 0;
   const fh::pot::PotParams g = pot_params(pl);
   const long long nh = pl->nh;
   const long long nhp = (nh + 15) / 16 * 16;  // 256-byte aligned sub-buffers
@@ -2492,7 +2549,7 @@ int fh_potential_solve(fh_plan* pl, int mode, const double* E, double tol, int m
     hipLaunchKernelGGL(fh::k_cg_restart, dim3(1), dim3(256), 0, pl->stream, pl->st, pl->partial, gb, pl->hist);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(p, r, (size_t)nh * sizeof(cpx<double>), hipMemcpyDeviceToDevice, pl->stream));
-    for (;;) {
+    auto batch64 = [&]() -> int {
       for (int k = 0; k < per_check; ++k) {
         int nb = 0;
         RETCHK(pot_operator<double>(pl, p, r, Ap, re, p, halt, &nb));
@@ -2502,6 +2559,13 @@ int fh_potential_solve(fh_plan* pl, int mode, const double* E, double tol, int m
         hipLaunchKernelGGL(fh::k_cg_beta, dim3(1), dim3(kFinThreads), 0, pl->stream, pl->st, pl->partial, gb);
         HIPCHK(hipGetLastError());
       }
+      return FH_OK;
+    };
+    for (int batch = 0;; ++batch) {
+      bool ran = false;
+      const void* key[8] = {x, r, p, Ap, re, pl->spec, pl->coef64, nullptr};
+      if (batch > 0) RETCHK(pot_graph_batch(pl, key, batch64, &ran));
+      if (!ran) RETCHK(batch64());
       HIPCHK(hipMemcpyAsync(&s, pl->st, sizeof(s), hipMemcpyDeviceToHost, pl->stream));
       HIPCHK(hipStreamSynchronize(pl->stream));
       if (s.done) break;
@@ -2521,7 +2585,7 @@ int fh_potential_solve(fh_plan* pl, int mode, const double* E, double tol, int m
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(p, r, (size_t)nh * sizeof(cpx<float>), hipMemcpyDeviceToDevice, pl->stream));
     const int fb = grid_blocks(2 * nh);
-    for (;;) {
+    auto batch32 = [&]() -> int {
       for (int k = 0; k < per_check; ++k) {
         int nb = 0;
         RETCHK(pot_operator<float>(pl, p, r, Ap, re32, p, halt, &nb));
@@ -2531,6 +2595,13 @@ int fh_potential_solve(fh_plan* pl, int mode, const double* E, double tol, int m
         hipLaunchKernelGGL(fh::k_cg_beta, dim3(1), dim3(kFinThreads), 0, pl->stream, pl->st, pl->partial, gb);
         HIPCHK(hipGetLastError());
       }
+      return FH_OK;
+    };
+    for (int batch = 0;; ++batch) {
+      bool ran = false;
+      const void* key[8] = {xc, r, p, Ap, re32, pl->spec, pl->coef32, x};
+      if (batch > 0) RETCHK(pot_graph_batch(pl, key, batch32, &ran));
+      if (!ran) RETCHK(batch32());
       HIPCHK(hipMemcpyAsync(&s, pl->st, sizeof(s), hipMemcpyDeviceToHost, pl->stream));
       HIPCHK(hipStreamSynchronize(pl->stream));
       if (s.done) break;
