@@ -4,6 +4,7 @@ import ctypes
 import os
 import re
 
+import numpy as np
 import pytest
 
 from ffthompy_amd import _native as nat
@@ -87,3 +88,18 @@ def test_argument_validation_of_later_entry_points_without_device():
     assert lib.fh_plan_set_residual_log(None, 1) == nat.FH_ERR_ARG
     assert lib.fh_last_residual_log(None, None, None, 0, None) == nat.FH_ERR_ARG
     assert len(lib.fh_last_error()) > 0
+
+
+def test_missing_library_fails_loudly(monkeypatch):
+    """Without the built library every product entry raises FhError: there is no CPU fallback."""
+    monkeypatch.setattr(nat, '_LIB', None)
+    monkeypatch.setattr(nat, 'LIB_PATH', '/nonexistent/libffthom.so')
+    with pytest.raises(nat.FhError):
+        nat.load_library()
+    from ffthompy_amd import potential
+    with pytest.raises(nat.FhError):
+        nat.Plan((5, 5), (1., 1.), 'scalar', 'primal')
+    with pytest.raises(nat.FhError):
+        potential.homog_GaNi_full_potential(
+            potential.Tensor(name='A', val=np.einsum('ij,...->ij...', np.eye(2), np.ones((5, 5))), order=2),
+            None, {'solver': {'tol': 1e-8, 'maxiter': 10}})
