@@ -288,3 +288,22 @@ def test_reference_unittest_cases_match_oracle(dim, material):
     assert abs(res.Fu.mean()) < 1e-12
     assert res.info['kit'] == ref['info']['kit']
     assert abs(res.AH - ref['AH']) < 1e-10 * abs(ref['AH'])
+
+
+def test_ga_potential_rectangular_cell_matches_oracle():
+    """Ga variant on a cell of sizes (1, 2) and grids 7 x 5 (quadrature grids 13 x 9), centred inclusion.
+    (An off-centre inclusion gets the reference's phase convention in its Ga coefficient, which can make
+    the coefficient negative somewhere; CG then meets <p, A p> <= 0, which the device reports as a
+    breakdown where the reference carries on.)"""
+    print('case ga potential cell', flush=True)
+    d = 2
+    conf = {'inclusions': ['square', 'otherwise'], 'positions': [np.zeros(d), ''],
+            'params': [np.array([0.5, 1.1]), ''], 'vals': [7 * np.eye(d), 1. * np.eye(d)],
+            'Y': np.array([1., 2.]), 'order': None}
+    mat, omat = Material(dict(conf)), OMaterial(dict(conf))
+    Nbar = (13, 9)
+    res = gpot.homog_Ga_full_potential(mat.get_A_Ga(Nbar, 'primal'), PARS)
+    ref = opot.homog_Ga_full_potential(omat.get_A_Ga(Nbar, 'primal'), tol=1e-10, maxiter=300)
+    assert res.info['kit'] == ref['info']['kit']
+    assert abs(res.AH - ref['AH']) < 1e-10 * abs(ref['AH'])
+    _check_fu(res.Fu, ref['Fu'], 1e-9)
