@@ -570,19 +570,25 @@ using FxP5 = FxPlan<5, 1, 5>;
 using FxP9 = FxPlan<9, 1, 9>;
 using FxP15 = FxPlan<15, 5, 5, 3>;
 using FxP15r = FxPlan<15, 5, 3, 5>;
+using FxP25 = FxPlan<25, 5, 5, 5>;
 using FxP27 = FxPlan<27, 3, 9, 3>;
 using FxP27r = FxPlan<27, 3, 3, 9>;
 using FxP45 = FxPlan<45, 9, 9, 5>;
 using FxP45r = FxPlan<45, 9, 5, 9>;
 using FxP63 = FxPlan<63, 9, 9, 7>;
 using FxP63r = FxPlan<63, 9, 7, 9>;
+using FxP75 = FxPlan<75, 15, 5, 5, 3>;
+using FxP75r = FxPlan<75, 15, 3, 5, 5>;
 using FxP81 = FxPlan<81, 9, 9, 9>;
 using FxP99 = FxPlan<99, 11, 9, 11>;
 using FxP99r = FxPlan<99, 11, 11, 9>;
 using FxP105 = FxPlan<105, 21, 7, 5, 3>;
 using FxP105r = FxPlan<105, 21, 3, 5, 7>;
+using FxP125 = FxPlan<125, 25, 5, 5, 5>;
 using FxP135 = FxPlan<135, 15, 9, 3, 5>;
 using FxP135r = FxPlan<135, 15, 5, 3, 9>;
+using FxP225 = FxPlan<225, 25, 9, 5, 5>;
+using FxP225r = FxPlan<225, 25, 5, 5, 9>;
 using FxP243 = FxPlan<243, 27, 9, 9, 3>;
 using FxP243r = FxPlan<243, 27, 3, 9, 9>;
 

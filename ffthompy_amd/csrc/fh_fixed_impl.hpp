@@ -13,13 +13,17 @@ namespace fh {
   X(5, FxP5, FxP5)            \
   X(9, FxP9, FxP9)            \
   X(15, FxP15, FxP15r)        \
+  X(25, FxP25, FxP25)         \
   X(27, FxP27, FxP27r)        \
   X(45, FxP45, FxP45r)        \
   X(63, FxP63, FxP63r)        \
+  X(75, FxP75, FxP75r)        \
   X(81, FxP81, FxP81)         \
   X(99, FxP99, FxP99r)        \
   X(105, FxP105, FxP105r)     \
+  X(125, FxP125, FxP125)      \
   X(135, FxP135, FxP135r)     \
+  X(225, FxP225, FxP225r)     \
   X(243, FxP243, FxP243r)
 
 // the buffer addressing of the fixed kernels uses 32-bit byte offsets

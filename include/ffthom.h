@@ -79,7 +79,7 @@ int fh_plan_ncomp(const fh_plan* plan);
 /* Switch the projection between G1 (primal) and G2 (dual): applications.py:53-56. */
 int fh_plan_set_primaldual(fh_plan* plan, int primaldual);
 /* generic = 1 forces the runtime-radix kernels even where a compile-time specialised
- * schedule exists (lengths 5, 9, 15, 27, 45, 63, 81, 99, 105, 135, 243); used to test
+ * schedule exists (lengths 5, 9, 15, 25, 27, 45, 63, 75, 81, 99, 105, 125, 135, 225, 243); used to test
  * one path against the other. No reference counterpart. */
 int fh_plan_use_generic(fh_plan* plan, int generic);
 

@@ -180,10 +180,11 @@ def test_cg_fp64_matches_oracle(physics, kind, N):
     assert rel(dx.to_host(), xref) < 1e-8
 
 
-# Shapes that put every specialised length (5, 9, 15, 27, 45, 63, 81, 99, 105, 135, 243)
-# on the contiguous axis, the middle axis and the outer (projection) axis.
+# Shapes that put every specialised length (5, 9, 15, 25, 27, 45, 63, 75, 81, 99, 105, 125, 135, 225,
+# 243) on the contiguous axis, the middle axis and the outer (projection) axis.
 FIXED_SHAPES = [(243, 9, 243), (5, 243, 27), (27, 45, 63), (81, 99, 105), (135, 15, 9), (9, 5, 15),
-                (45, 63, 81), (99, 105, 135), (63, 81), (105, 135), (243, 45), (15, 99)]
+                (45, 63, 81), (99, 105, 135), (63, 81), (105, 135), (243, 45), (15, 99),
+                (25, 75, 125), (125, 225, 25), (75, 25, 225), (225, 75), (125, 25)]
 
 
 @pytest.mark.parametrize('N', FIXED_SHAPES, ids=[str(s) for s in FIXED_SHAPES])
@@ -206,7 +207,8 @@ def test_fixed_length_fft_matches_numpy(N):
 
 
 FIXED_MV = [('scalar', (243, 9, 45)), ('scalar', (63, 81)), ('scalar', (105, 135)), ('elasticity', (27, 45, 15)),
-            ('elasticity', (81, 9, 5)), ('scalar', (99, 27, 63)), ('elasticity', (15, 45))]
+            ('elasticity', (81, 9, 5)), ('scalar', (99, 27, 63)), ('elasticity', (15, 45)),
+            ('scalar', (25, 75, 125)), ('scalar', (225, 125)), ('elasticity', (75, 25, 25))]
 
 
 @pytest.mark.parametrize('physics,N', FIXED_MV, ids=['%s-%s' % c for c in FIXED_MV])
@@ -397,3 +399,28 @@ def test_graph_replayed_iterations_equal_direct_launches(case):
     for kit, xh in res[1:]:
         assert kit == res[0][0]
         assert np.array_equal(xh, res[0][1])
+
+
+@pytest.mark.parametrize('N', [(75, 75, 75), (125, 225), (25, 125, 75)])
+def test_fused_mixed_cg_on_added_lengths(N):
+    """Fused fp32 iteration with fp64 reliable updates on the 25/75/125/225 schedules: the energy
+    of the solution equals the fp64 solve's (1e-8), final true residual <= tol."""
+    print('case fused mixed lengths', N, flush=True)
+    d = len(N)
+    rng = np.random.default_rng(73)
+    a = np.where(rng.random(N) < 0.3, 30., 1.)
+    plan = nat.Plan(N, np.ones(d), 'scalar', 'primal')
+    plan.set_coeff('iso', a)
+    E = np.eye(d)[0]
+    EN = nat.DeviceArray.zeros((d,) + N)
+    nat.add_constant(EN, -E)
+    B = nat.DeviceArray((d,) + N)
+    plan.matvec(EN, B)
+    ah = {}
+    for mode in ('fp64', 'mixed'):
+        x = nat.DeviceArray.zeros((d,) + N)
+        info = plan.cg_solve(B, x, tol=1e-9, maxiter=3000, mode=mode, delta=1e-4)
+        assert info['status'] == nat.FH_CG_OK and info['norm_res'] <= 1e-9
+        nat.add_constant(x, E)
+        ah[mode] = plan.energy_matrix([x] * d)[0, 0]
+    assert abs(ah['mixed'] - ah['fp64']) < 1e-8 * ah['fp64']

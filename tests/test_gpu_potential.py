@@ -307,3 +307,19 @@ def test_ga_potential_rectangular_cell_matches_oracle():
     assert res.info['kit'] == ref['info']['kit']
     assert abs(res.AH - ref['AH']) < 1e-10 * abs(ref['AH'])
     _check_fu(res.Fu, ref['Fu'], 1e-9)
+
+
+def test_potential_on_new_fixed_lengths_equals_generic():
+    """The potential kernels on the 25/75/125 schedules against the runtime-radix passes."""
+    print('case potential new lengths', flush=True)
+    N = (25, 75, 125)
+    a = np.exp(0.5 * np.random.default_rng(71).standard_normal(N))
+    out = []
+    for generic in (False, True):
+        plan = nat.Plan(N, (1., 1., 1.), 'scalar', 'primal')
+        plan.set_coeff('iso', a)
+        plan.use_generic(generic)
+        Fu, info = plan.potential_solve(np.eye(3)[0], 1e-9, 500)
+        out.append((info['kit'], Fu.to_host()))
+    assert out[0][0] == out[1][0]
+    assert np.abs(out[0][1] - out[1][1]).max() <= 1e-12 * np.abs(out[1][1]).max()
