@@ -171,10 +171,6 @@ struct fh_plan {
   bool it_graph_off = false;     // capture failed on this plan: direct launches from then on
   bool use_graphs = true;        // fh_plan_use_graphs
   int last_graph_batches = 0;    // batches of the last solve replayed from the graph
-  // potential solve: a batch of iterations captured once as a HIP graph, replayed per batch
-  hipGraphExec_t pot_exec = nullptr;
-  const void* pot_key[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-  bool pot_graph_off = false;
   int res_every = 0;             // fp64 CG: log ||b - A x|| every res_every iterations (0: off)
   std::vector<int> res_kit;      // the log of the last solve: iteration counts and norms
   std::vector<double> res_norm;
@@ -838,47 +834,12 @@ int pot_operator(fh_plan* pl, cpx<T>* p, const cpx<T>* r, cpx<T>* out, T* re, co
   return FH_OK;
 }
 
-// Forget the captured iteration graphs of a plan: their kernels and arguments were chosen for the
+// Forget the captured iteration graph of a plan: its kernels and arguments were chosen for the
 // plan's projection, band, kernel family and coefficient at capture time. Called by every entry
 // point that changes one of those (all entry points have finished their GPU work on return).
 void drop_graphs(fh_plan* pl) {
   if (pl->it_exec) (void)hipGraphExecDestroy(pl->it_exec);
-  if (pl->pot_exec) (void)hipGraphExecDestroy(pl->pot_exec);
   pl->it_exec = nullptr;
-  pl->pot_exec = nullptr;
-}
-
-// Run one batch of potential-solve iterations from a HIP graph captured on the plan's stream
-// (kernel nodes only; rebuilt when a buffer in key moves). *ran = false when capture is not
-// available: the caller then enqueues the batch itself.
-template <class Enq>
-int pot_graph_batch(fh_plan* pl, const void* const (&key)[8], Enq&& enq, bool* ran) {
-  *ran = false;
-  if (!pl->use_graphs || pl->pot_graph_off) return FH_OK;
-  if (!pl->pot_exec || std::memcmp(key, pl->pot_key, sizeof(key)) != 0) {
-    if (pl->pot_exec) (void)hipGraphExecDestroy(pl->pot_exec);
-    pl->pot_exec = nullptr;
-    hipGraph_t graph = nullptr;
-    int rc = FH_OK;
-    const bool ok = hipStreamBeginCapture(pl->stream, hipStreamCaptureModeRelaxed) == hipSuccess;
-    if (ok) rc = enq();
-    const hipError_t ec = ok ? hipStreamEndCapture(pl->stream, &graph) : hipErrorUnknown;
-    if (ok && rc == FH_OK && ec == hipSuccess && graph &&
-        hipGraphInstantiate(&pl->pot_exec, graph, nullptr, nullptr, 0) == hipSuccess) {
-      std::memcpy(pl->pot_key, key, sizeof(key));
-    } else {
-      pl->pot_exec = nullptr;
-      pl->pot_graph_off = true;  // capture not available here: direct launches from then on
-    }
-    if (graph) (void)hipGraphDestroy(graph);
-    (void)hipGetLastError();
-  }
-  if (pl->pot_exec) {
-    HIPCHK(hipGraphLaunch(pl->pot_exec, pl->stream));
-    *ran = true;
-    ++pl->last_graph_batches;
-  }
-  return FH_OK;
 }
 
 }  // namespace
@@ -958,7 +919,6 @@ int fh_plan_destroy(fh_plan* pl) {
   if (pl->hist) (void)hipFree(pl->hist);
   if (pl->work) (void)hipFree(pl->work);
   if (pl->it_exec) (void)hipGraphExecDestroy(pl->it_exec);
-  if (pl->pot_exec) (void)hipGraphExecDestroy(pl->pot_exec);
   if (pl->ev0) (void)hipEventDestroy(pl->ev0);
   if (pl->ev1) (void)hipEventDestroy(pl->ev1);
   if (pl->stream) (void)hipStreamDestroy(pl->stream);
@@ -2561,11 +2521,8 @@ int fh_potential_solve(fh_plan* pl, int mode, const double* E, double tol, int m
       }
       return FH_OK;
     };
-    for (int batch = 0;; ++batch) {
-      bool ran = false;
-      const void* key[8] = {x, r, p, Ap, re, pl->spec, pl->coef64, nullptr};
-      if (batch > 0) RETCHK(pot_graph_batch(pl, key, batch64, &ran));
-      if (!ran) RETCHK(batch64());
+    for (;;) {
+      RETCHK(batch64());
       HIPCHK(hipMemcpyAsync(&s, pl->st, sizeof(s), hipMemcpyDeviceToHost, pl->stream));
       HIPCHK(hipStreamSynchronize(pl->stream));
       if (s.done) break;
@@ -2597,11 +2554,8 @@ int fh_potential_solve(fh_plan* pl, int mode, const double* E, double tol, int m
       }
       return FH_OK;
     };
-    for (int batch = 0;; ++batch) {
-      bool ran = false;
-      const void* key[8] = {xc, r, p, Ap, re32, pl->spec, pl->coef32, x};
-      if (batch > 0) RETCHK(pot_graph_batch(pl, key, batch32, &ran));
-      if (!ran) RETCHK(batch32());
+    for (;;) {
+      RETCHK(batch32());
       HIPCHK(hipMemcpyAsync(&s, pl->st, sizeof(s), hipMemcpyDeviceToHost, pl->stream));
       HIPCHK(hipStreamSynchronize(pl->stream));
       if (s.done) break;

@@ -1,10 +1,7 @@
-"""Captured iteration graphs and plan changes.
-
-* Potential solve: batches replayed from a captured HIP graph equal direct launches bitwise (fp64 and
-  mixed, iteration counts and Fu), with eager work between the solves and graphs before and after.
-* A plan reused after set_primaldual / set_coeff / use_generic captures anew: a mixed gradient solve
-  and a potential solve after each change equal the same solve on a fresh plan with those settings
-  (the captured kernels and their arguments belong to the settings at capture time).
+"""Captured iteration graphs and plan changes: a plan reused after set_primaldual / set_coeff /
+use_generic captures its mixed-CG graph anew, so a mixed gradient solve (and a potential solve) after
+each change equals the same solve on a fresh plan with those settings (the captured kernels and their
+arguments belong to the settings at capture time).
 """
 import numpy as np
 import pytest
@@ -16,26 +13,6 @@ nat = pytest.importorskip('ffthompy_amd._native')
 def _coef(N, seed):
     rng = np.random.default_rng(seed)
     return np.where(rng.random(N) < 0.3, 50., 1.)
-
-
-@pytest.mark.parametrize('mode', ['fp64', 'mixed'])
-def test_potential_graph_replay_equals_direct_launches(mode):
-    print('case potential graphs', mode, flush=True)
-    N = (45, 45, 45)
-    plan = nat.Plan(N, (1., 1., 1.), 'scalar', 'primal')
-    plan.set_coeff('iso', _coef(N, 61))
-    res = []
-    for graphs in (True, False, True):
-        plan.use_graphs(graphs)
-        Fu, info = plan.potential_solve(np.eye(3)[0], 1e-9, 1000, mode=mode)
-        nb = plan.last_graph_batches()
-        assert (nb >= 1) if graphs else (nb == 0), (graphs, nb)
-        res.append((info['kit'], Fu.to_host()))
-        y = nat.DeviceArray((3,) + N)
-        plan.matvec(nat.DeviceArray.zeros((3,) + N), y)  # eager work between the solves
-    for kit, f in res[1:]:
-        assert kit == res[0][0]
-        assert np.array_equal(f, res[0][1])
 
 
 def _gradient_mixed(plan, N, E):
@@ -56,7 +33,6 @@ def test_reused_plan_captures_anew_after_changes():
     plan = nat.Plan(N, (1., 1., 1.), 'scalar', 'primal')
     plan.set_coeff('iso', a)
     _gradient_mixed(plan, N, E)
-    plan.potential_solve(E, 1e-9, 1000, mode='mixed')
     assert plan.last_graph_batches() >= 1
     # dual projection and inverse coefficient on the same plan (same work buffers)
     plan.set_primaldual('dual')
