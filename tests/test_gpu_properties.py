@@ -21,7 +21,7 @@ odd_lengths = st.sampled_from([3, 5, 7, 9, 11, 13, 15, 21, 25, 27])
 grids = st.one_of(st.tuples(lengths, lengths), st.tuples(lengths, lengths, lengths))
 odd_grids = st.one_of(st.tuples(odd_lengths, odd_lengths), st.tuples(odd_lengths, odd_lengths, odd_lengths))
 seeds = st.integers(min_value=0, max_value=2 ** 31 - 1)
-SETTINGS = dict(max_examples=20, deadline=None, suppress_health_check=[HealthCheck.too_slow])
+SETTINGS = dict(max_examples=20, deadline=None, derandomize=True, suppress_health_check=[HealthCheck.too_slow])
 
 
 def _rel(a, b):

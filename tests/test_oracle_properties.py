@@ -25,7 +25,7 @@ def _field(N, shape, rng):
     return Field(val=rng.standard_normal(tuple(shape) + tuple(N)), order=len(shape))
 
 
-@settings(max_examples=25, deadline=None)
+@settings(max_examples=25, deadline=None, derandomize=True)
 @given(grids, seeds)
 def test_scalar_projections_are_complementary_orthogonal_projections(N, seed):
     rng = np.random.default_rng(seed)
@@ -41,7 +41,7 @@ def test_scalar_projections_are_complementary_orthogonal_projections(N, seed):
     assert np.abs(g1 + g2 - eye).max() < 1e-13
 
 
-@settings(max_examples=25, deadline=None)
+@settings(max_examples=25, deadline=None, derandomize=True)
 @given(grids, seeds)
 def test_gradient_fields_are_fixed_by_g1_and_removed_by_g2(N, seed):
     rng = np.random.default_rng(seed)
@@ -55,7 +55,7 @@ def test_gradient_fields_are_fixed_by_g1_and_removed_by_g2(N, seed):
     assert np.abs(v2).max() < 1e-11 * np.abs(e.val).max()
 
 
-@settings(max_examples=25, deadline=None)
+@settings(max_examples=25, deadline=None, derandomize=True)
 @given(grids, seeds)
 def test_fft_round_trip_and_enlarge_decrease(N, seed):
     rng = np.random.default_rng(seed)
@@ -73,7 +73,7 @@ def test_fft_round_trip_and_enlarge_decrease(N, seed):
     assert np.abs(back.val - X.val).max() < 1e-12 * np.abs(X.val).max()
 
 
-@settings(max_examples=25, deadline=None)
+@settings(max_examples=25, deadline=None, derandomize=True)
 @given(grids, seeds)
 def test_preconditioned_laplacian_is_identity_off_the_mean(N, seed):
     rng = np.random.default_rng(seed)
