@@ -363,6 +363,7 @@ def main():
     ap.add_argument('--delta', type=float, default=1e-4,
                     help='mixed precision: residual reduction that triggers an fp64 reliable update')
     ap.add_argument('--profile-only', action='store_true', help='matvec profile only (for rocprofv3 runs)')
+    ap.add_argument('--no-autotune', action='store_true', help='keep the default kernel schedules')
     ap.add_argument('--profile-solve', action='store_true',
                     help='with --profile-only: also one mixed and one fp64 CG solve (load e1, primal)')
     ap.add_argument('--workload', default='scalar243', choices=['scalar243', 'uq512', 'elastic81', 'grad99', 'ga243', 'potential243'],
@@ -391,8 +392,19 @@ def main():
                                                                         time.perf_counter() - t))
     H = Homogenizer(nat, a)
 
-    # --- matvec profile (per kernel, HIP events on the plan stream) ---
+    # --- schedule autotuner: every variant of the projection and strided passes, per arithmetic ---
     pl = H.plans['primal']
+    sched = {}
+    if not args.no_autotune:
+        for dt, name in ((np.float32, 'fp32'), (np.float64, 'fp64')):
+            res = pl.autotune(dt, nrep=5)
+            H.plans['dual'].set_schedule(dt, res['proj_variant'], res['cols_variant'])
+            sched[name] = res
+            log('autotune %s: projection tile variants %s ms -> %d; strided %s ms -> %d' % (
+                name, np.round(res['proj_ms'], 4).tolist(), res['proj_variant'], np.round(res['cols_ms'], 4).tolist(),
+                res['cols_variant']))
+
+    # --- matvec profile (per kernel, HIP events on the plan stream) ---
     prof = {}
     for dt, name in ((np.float32, 'fp32'), (np.float64, 'fp64')):
         x = nat.DeviceArray.from_host(np.random.default_rng(1).standard_normal((3,) + N).astype(dt))
@@ -500,6 +512,7 @@ def main():
         'matvec_GBps_fp64': round(prof['fp64']['matvec_GBps'], 1),
         'matvec_ms_fp32': round(prof['fp32']['matvec_ms'], 4),
         'matvec_ms_fp64': round(prof['fp64']['matvec_ms'], 4),
+        'schedule': sched,
         'matvec_kernels_fp32': {'names': names, 'ms': [round(v, 4) for v in k32['kernel_ms']],
                                 'bytes': k32['kernel_bytes']},
         'roofline': {'bound': 'hbm', 'kernel': names[dom], 'achieved': round(ach, 1), 'peak': HBM_PEAK_GBS,

@@ -60,6 +60,9 @@ SIGNATURES = {
     'fh_last_solve_updates': [_vp, _c_int_p],
     'fh_matvec_profile': [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int, _c_double_p, _c_double_p,
                           _c_int_p],
+    'fh_plan_set_schedule': [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int],
+    'fh_plan_get_schedule': [_vp, ctypes.c_int, _c_int_p, _c_int_p],
+    'fh_plan_autotune': [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int, _c_double_p, _c_double_p, _c_int_p, _c_int_p],
     'fh_batch_create': [ctypes.c_int, _c_int_p, _c_double_p, ctypes.POINTER(ctypes.c_void_p)],
     'fh_batch_destroy': [_vp],
     'fh_batch_set_coeff': [_vp, _c_double_p],
@@ -267,6 +270,37 @@ def add_constant(y, E):
     return y
 
 
+_MODES = {'fp64': FH_SOLVE_FP64, 'mixed': FH_SOLVE_MIXED}
+
+
+def solve_mode(mode):
+    """CG arithmetic code of a precision string: 'fp64' or 'mixed' (anything else raises)."""
+    try:
+        return _MODES[mode]
+    except (KeyError, TypeError):
+        raise FhError("precision must be 'fp64' or 'mixed', got %r" % (mode,), FH_ERR_ARG)
+
+
+def check_array(arr, size, dtypes, what):
+    """Raise FhError(FH_ERR_ARG) on the host unless `arr` is a DeviceArray of `size` elements
+    whose dtype is one of `dtypes`: every pointer handed to the library covers what the kernels
+    touch."""
+    if not isinstance(arr, DeviceArray):
+        raise FhError('%s must be a DeviceArray, got %s' % (what, type(arr).__name__), FH_ERR_ARG)
+    dts = tuple(np.dtype(d) for d in dtypes)
+    if arr.dtype not in dts:
+        raise FhError('%s has dtype %s, expected %s' % (what, arr.dtype, ' or '.join(str(d) for d in dts)),
+                      FH_ERR_ARG)
+    if arr.size != int(size):
+        raise FhError('%s has %d elements (shape %s), expected %d' % (what, arr.size, arr.shape, int(size)),
+                      FH_ERR_ARG)
+    return arr
+
+
+_REAL = (np.float64, np.float32)
+_COMPLEX_OF = {np.dtype(np.float64): np.complex128, np.dtype(np.float32): np.complex64}
+
+
 class Plan:
     """Owner of an fh_plan: grid, cell, physics, projection and coefficient."""
 
@@ -294,6 +328,15 @@ class Plan:
     @property
     def handle(self):
         return self._h
+
+    def _field(self, arr, what, dtypes=_REAL, ncomp=None):
+        """Check a real field of ncomp (default: the plan's) components on the plan grid."""
+        return check_array(arr, (self.ncomp if ncomp is None else ncomp) * self.nvox, dtypes, what)
+
+    def _spectrum(self, arr, what, dtypes, ncomp=None):
+        """Check an 'r'-form spectrum of ncomp (default: the plan's) components."""
+        nh = int(np.prod(self.Nhalf))
+        return check_array(arr, (self.ncomp if ncomp is None else ncomp) * nh, dtypes, what)
 
     def set_primaldual(self, primaldual):
         call('fh_plan_set_primaldual', self._h, ctypes.c_int(FH_PRIMAL if primaldual == 'primal' else FH_DUAL))
@@ -372,39 +415,56 @@ class Plan:
 
     def grf(self, white, corr_len, std, exponentiate=False, out=None):
         """Gaussian random field on this plan's grid from device white noise (fh_grf)."""
+        self._field(white, 'white noise', (np.float64,), ncomp=1)
         if out is None:
             out = DeviceArray(self.N)
+        self._field(out, 'out', (np.float64,), ncomp=1)
         call('fh_grf', self._h, white.ptr, ctypes.c_double(corr_len), ctypes.c_double(std),
              ctypes.c_int(1 if exponentiate else 0), out.ptr)
         return out
 
+    def _pair(self, x, y):
+        self._field(x, 'x')
+        self._field(y, 'y', (x.dtype,))
+
     def matvec(self, x, y):
+        self._pair(x, y)
         call('fh_matvec', self._h, dtype_code(x.dtype), x.ptr, y.ptr)
         return y
 
     def project(self, x, y):
+        self._pair(x, y)
         call('fh_project', self._h, dtype_code(x.dtype), x.ptr, y.ptr)
         return y
 
     def rfftn(self, x, xhat, ncomp):
+        self._field(x, 'x', ncomp=ncomp)
+        self._spectrum(xhat, 'xhat', (_COMPLEX_OF[x.dtype],), ncomp=ncomp)
         call('fh_rfftn', self._h, dtype_code(x.dtype), ctypes.c_int(ncomp), x.ptr, xhat.ptr)
         return xhat
 
     def irfftn(self, xhat, x, ncomp):
+        self._field(x, 'x', ncomp=ncomp)
+        self._spectrum(xhat, 'xhat', (_COMPLEX_OF[x.dtype],), ncomp=ncomp)
         call('fh_irfftn', self._h, dtype_code(x.dtype), ctypes.c_int(ncomp), xhat.ptr, x.ptr)
         return x
 
     def apply_projection_hat(self, xin, xout):
+        self._spectrum(xin, 'in_hat', (np.complex128, np.complex64))
+        self._spectrum(xout, 'out_hat', (xin.dtype,))
         real = np.float64 if xin.dtype == np.complex128 else np.float32
         call('fh_apply_projection_hat', self._h, dtype_code(real), xin.ptr, xout.ptr)
         return xout
 
     def cg_solve(self, b, x, tol, maxiter, mode='fp64', delta=0.01, hist_len=0):
+        code = solve_mode(mode)
+        self._field(b, 'b', (np.float64,))
+        self._field(x, 'x', (np.float64,))
         kit = ctypes.c_int()
         nr = ctypes.c_double()
         st = ctypes.c_int()
         hist = np.zeros(max(hist_len, 1))
-        call('fh_cg_solve', self._h, ctypes.c_int(FH_SOLVE_FP64 if mode == 'fp64' else FH_SOLVE_MIXED),
+        call('fh_cg_solve', self._h, ctypes.c_int(code),
              b.ptr, x.ptr, ctypes.c_double(tol), ctypes.c_int(int(maxiter)), ctypes.c_double(delta),
              ctypes.byref(kit), ctypes.byref(nr), hist.ctypes.data_as(_c_double_p), ctypes.c_int(hist_len),
              ctypes.byref(st))
@@ -415,6 +475,8 @@ class Plan:
     def iter_solve(self, method, b, x, coefs, tol, maxiter, hist_len=0):
         """Richardson (coefs = [alpha]) or Chebyshev (coefs = eigenvalue range) on the device."""
         code = {'richardson': FH_ITER_RICHARDSON, 'chebyshev': FH_ITER_CHEBYSHEV}[method]
+        self._field(b, 'b', (np.float64,))
+        self._field(x, 'x', (np.float64,))
         c = np.ascontiguousarray(np.asarray(coefs, dtype=np.float64).reshape(-1))
         kit = ctypes.c_int()
         nres = ctypes.c_double()
@@ -431,14 +493,18 @@ class Plan:
         """CG of the potential formulation (fh_potential_solve), mode 'fp64' or 'mixed': Fu, the
         potential's rfftn spectrum (complex, N[:-1] + (N[-1]//2+1,)), and X = F^-1 grad Fu
         ((ncomp,) + N, optional)."""
+        code = solve_mode(mode)
         E = np.ascontiguousarray(np.asarray(E, dtype=np.float64).reshape(-1))
         if E.size != self.ncomp:
             raise FhError('E needs %d values' % self.ncomp, FH_ERR_ARG)
         if Fu is None:
             Fu = DeviceArray(self.N[:-1] + (self.N[-1] // 2 + 1,), np.complex128)
+        self._spectrum(Fu, 'Fu', (np.complex128,), ncomp=1)
+        if X is not None:
+            self._field(X, 'X', (np.float64,))
         kit, nres, status = ctypes.c_int(), ctypes.c_double(), ctypes.c_int()
         hist = np.zeros(max(hist_len, 1))
-        call('fh_potential_solve', self._h, ctypes.c_int(FH_SOLVE_FP64 if mode == 'fp64' else FH_SOLVE_MIXED),
+        call('fh_potential_solve', self._h, ctypes.c_int(code),
              E.ctypes.data_as(_c_double_p), ctypes.c_double(tol), ctypes.c_int(int(maxiter)), ctypes.c_double(delta),
              Fu.ptr, X.ptr if X is not None else None, ctypes.byref(kit),
              ctypes.byref(nres), hist.ctypes.data_as(_c_double_p) if hist_len > 0 else None,
@@ -448,7 +514,14 @@ class Plan:
             out['res_log'] = hist[:min(hist_len, kit.value + 1)].copy()
         return Fu, out
 
+    def _fields(self, fields, what='fields'):
+        if len(fields) != self.ncomp:
+            raise FhError('need %d %s (one per unit load), got %d' % (self.ncomp, what, len(fields)), FH_ERR_ARG)
+        for i, f in enumerate(fields):
+            self._field(f, '%s[%d]' % (what, i), (np.float64,))
+
     def energy_matrix(self, fields):
+        self._fields(fields)
         nf = len(fields)
         arr = (ctypes.c_void_p * nf)(*[f.ptr.value for f in fields])
         AH = np.zeros((nf, nf))
@@ -457,23 +530,37 @@ class Plan:
 
     def energy_gradient(self, fields, W, chain=None, out=None):
         """Device gradient of sum_ij W_ij AH_ij w.r.t. the coefficient (see fh_energy_gradient)."""
+        self._fields(fields)
         nf = len(fields)
         arr = (ctypes.c_void_p * nf)(*[f.ptr.value for f in fields])
         W = np.ascontiguousarray(W, dtype=np.float64)
+        if W.size != nf * nf:
+            raise FhError('W needs %d values' % (nf * nf), FH_ERR_ARG)
         nvals = 1 if self.coef_kind == 'iso' else 2
+        if chain is not None:
+            self._field(chain, 'chain', (np.float64,), ncomp=1)
         if out is None:
             out = DeviceArray((nvals,) + self.N if nvals > 1 else self.N, np.float64)
+        self._field(out, 'out', (np.float64,), ncomp=nvals)
         call('fh_energy_gradient', self._h, arr, ctypes.c_int(nf), W.ctypes.data_as(_c_double_p),
              chain.ptr if chain is not None else None, out.ptr)
         return out
 
     def field_gradient(self, fields, adjoints, chain=None, out=None):
         """Device gradient -chain * sum_i adjoints_i . fields_i (see fh_field_gradient)."""
+        if len(adjoints) != len(fields):
+            raise FhError('one adjoint per field', FH_ERR_ARG)
+        for i, (f, a) in enumerate(zip(fields, adjoints)):
+            self._field(f, 'fields[%d]' % i, (np.float64,))
+            self._field(a, 'adjoints[%d]' % i, (np.float64,))
+        if chain is not None:
+            self._field(chain, 'chain', (np.float64,), ncomp=1)
         nf = len(fields)
         fa = (ctypes.c_void_p * nf)(*[f.ptr.value for f in fields])
         aa = (ctypes.c_void_p * nf)(*[f.ptr.value for f in adjoints])
         if out is None:
             out = DeviceArray(self.N, np.float64)
+        self._field(out, 'out', (np.float64,), ncomp=1)
         call('fh_field_gradient', self._h, fa, aa, ctypes.c_int(nf), chain.ptr if chain is not None else None,
              out.ptr)
         return out
@@ -491,12 +578,36 @@ class Plan:
 
     def matvec_profile(self, x, y, nrep=10):
         """Average ms and algorithmic bytes per kernel of one matvec (HIP events)."""
+        self._pair(x, y)
         ms = np.zeros(8)
         by = np.zeros(8)
         nk = ctypes.c_int()
         call('fh_matvec_profile', self._h, dtype_code(x.dtype), x.ptr, y.ptr, ctypes.c_int(nrep), ctypes.c_int(8),
              ms.ctypes.data_as(_c_double_p), by.ctypes.data_as(_c_double_p), ctypes.byref(nk))
         return ms[:nk.value].copy(), by[:nk.value].copy()
+
+    def set_schedule(self, dtype, proj_variant, cols_variant):
+        """Schedule variants of the fixed-length passes for one arithmetic (fh_plan_set_schedule)."""
+        call('fh_plan_set_schedule', self._h, ctypes.c_int(dtype_code(dtype)), ctypes.c_int(int(proj_variant)),
+             ctypes.c_int(int(cols_variant)))
+
+    def schedule(self, dtype):
+        p, c = ctypes.c_int(), ctypes.c_int()
+        call('fh_plan_get_schedule', self._h, ctypes.c_int(dtype_code(dtype)), ctypes.byref(p), ctypes.byref(c))
+        return p.value, c.value
+
+    def autotune(self, dtype=np.float32, nrep=5, seed=0):
+        """Time every schedule variant of the projection and strided passes on this plan (random
+        field, nrep matvecs per variant) and keep the fastest: fh_plan_autotune. Returns
+        {'proj_ms': [...], 'cols_ms': [...], 'proj_variant': p, 'cols_variant': c}."""
+        dt = np.dtype(dtype)
+        x = DeviceArray.from_host(np.random.default_rng(seed).standard_normal((self.ncomp,) + self.N).astype(dt))
+        y = DeviceArray((self.ncomp,) + self.N, dt)
+        tp, tc = np.zeros(4), np.zeros(2)
+        p, c = ctypes.c_int(), ctypes.c_int()
+        call('fh_plan_autotune', self._h, ctypes.c_int(dtype_code(dt)), x.ptr, y.ptr, ctypes.c_int(int(nrep)),
+             tp.ctypes.data_as(_c_double_p), tc.ctypes.data_as(_c_double_p), ctypes.byref(p), ctypes.byref(c))
+        return {'proj_ms': tp.tolist(), 'cols_ms': tc.tolist(), 'proj_variant': p.value, 'cols_variant': c.value}
 
     def __del__(self):
         h = getattr(self, '_h', None)

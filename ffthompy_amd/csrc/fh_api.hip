@@ -51,6 +51,8 @@ constexpr size_t kLdsTarget = 48 * 1024;
 // (a few thousand at 243^3), so the cross-XCD load latency is paid a few times, not ~30
 constexpr int kFinThreads = 1024;
 constexpr size_t kLdsMax = 160 * 1024;
+// doubles of a plan's small-result block (after its partial sums)
+constexpr size_t kAuxDoubles = 64;
 
 std::mutex g_init_mu;
 int g_init_state = -1;  // -1 unknown, FH_OK, or an error code
@@ -152,6 +154,7 @@ struct fh_plan {
   void* spec = nullptr;          // ncomp*nh complex doubles (fp32 passes reuse it)
   double* partial = nullptr;     // per-workgroup partial sums
   size_t partial_cap = 0;
+  double* aux = nullptr;         // kAuxDoubles after the partial sums: small results (energies, sums)
   CGState* st = nullptr;
   double* hist = nullptr;
   int hist_cap = 0;
@@ -171,6 +174,8 @@ struct fh_plan {
   bool it_graph_off = false;     // capture failed on this plan: direct launches from then on
   bool use_graphs = true;        // fh_plan_use_graphs
   int last_graph_batches = 0;    // batches of the last solve replayed from the graph
+  int proj_var[2] = {0, 0};      // schedule variants of the fixed-length passes: [fp64, fp32]
+  int cols_var[2] = {0, 0};
   int res_every = 0;             // fp64 CG: log ||b - A x|| every res_every iterations (0: off)
   std::vector<int> res_kit;      // the log of the last solve: iteration counts and norms
   std::vector<double> res_norm;
@@ -266,6 +271,8 @@ fh::LaunchCtx make_ctx(const fh_plan* pl) {
   for (int a = 0; a < 3; ++a) c.tw[a] = sizeof(T) == 8 ? pl->tw64[a] : pl->tw32[a];
   c.t0 = pl->prof_t0;
   c.t1 = pl->prof_t1;
+  c.proj_var = pl->proj_var[sizeof(T) == 8 ? 0 : 1];
+  c.cols_var = pl->cols_var[sizeof(T) == 8 ? 0 : 1];
   return c;
 }
 
@@ -458,54 +465,52 @@ int check_dtype(int dtype) {
   return FH_OK;
 }
 
-// AH[i][j] = <A e_i, e_j> on the grid of pl with coefficient `coef` of layout `kind`.
-int energy_impl(fh_plan* pl, const double* coef, int kind, const void* const* fields, int nf, double* AH) {
+// Partial-sum workgroups of the energy kernels for n voxels (at most 1024).
+int energy_blocks(long long n) { return (int)std::min<long long>(1024, std::max<long long>(1, (n + 256 * 8 - 1) / (256 * 8))); }
+
+// Enqueue AH[i][j] = <A e_i, e_j> (nf fields of nf components, n voxels, coefficient `coef` of
+// layout `kind`) on `stream`: per-workgroup double-double partial sums into partial[0, 2 nf^2 nb)
+// and the fixed-order final sum into AHdev[0, nf^2). Nothing is allocated.
+int enqueue_energy(hipStream_t stream, const double* coef, int kind, const void* const* fields, int nf, long long n,
+                   double* partial, double* AHdev) {
+  if (nf < 1 || nf > 6) return fail(FH_ERR_ARG, "unsupported number of components");
   const int nn = nf * nf;
-  const int nb = std::min<long long>(1024, std::max<long long>(1, (pl->n + 256 * 8 - 1) / (256 * 8)));
-  if ((size_t)2 * nn * nb > pl->partial_cap) return fail(FH_ERR_STATE, "partial buffer too small");
-  void* dptrs = nullptr;
-  double* dAH = nullptr;
-  HIPCHK(hipMalloc(&dptrs, nf * sizeof(void*)));
-  HIPCHK(hipMalloc(&dAH, nn * sizeof(double)));
-  int rc = FH_OK;
-  hipError_t e = hipMemcpy(dptrs, fields, nf * sizeof(void*), hipMemcpyHostToDevice);
-  if (e != hipSuccess) rc = fail(FH_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
-  if (rc == FH_OK) {
-    const double* const* F = (const double* const*)dptrs;
-    const dim3 grid(nb), block(256);
+  const int nb = energy_blocks(n);
+  fh::FieldPtrs F{};
+  for (int i = 0; i < nf; ++i) F.p[i] = (const double*)fields[i];
+  const dim3 grid(nb), block(256);
 #define FH_ENERGY(NC)                                                                                \
   do {                                                                                               \
     if (kind == FH_COEF_ISO)                                                                         \
-      hipLaunchKernelGGL((fh::k_energy<NC, fh::COEF_ISO>), grid, block, 0, pl->stream, F, coef, \
-                         pl->n, pl->partial);                                                        \
+      hipLaunchKernelGGL((fh::k_energy<NC, fh::COEF_ISO>), grid, block, 0, stream, F, coef, n, partial);  \
     else if (kind == FH_COEF_LAME)                                                                   \
-      hipLaunchKernelGGL((fh::k_energy<NC, fh::COEF_LAME>), grid, block, 0, pl->stream, F,            \
-                         coef, pl->n, pl->partial);                                            \
+      hipLaunchKernelGGL((fh::k_energy<NC, fh::COEF_LAME>), grid, block, 0, stream, F, coef, n, partial); \
     else                                                                                             \
-      hipLaunchKernelGGL((fh::k_energy<NC, fh::COEF_FULL>), grid, block, 0, pl->stream, F,            \
-                         coef, pl->n, pl->partial);                                            \
+      hipLaunchKernelGGL((fh::k_energy<NC, fh::COEF_FULL>), grid, block, 0, stream, F, coef, n, partial); \
   } while (0)
-    if (nf == 2)
-      FH_ENERGY(2);
-    else if (nf == 3)
-      FH_ENERGY(3);
-    else if (nf == 6)
-      FH_ENERGY(6);
-    else
-      rc = fail(FH_ERR_ARG, "unsupported number of components");
+  if (nf == 2)
+    FH_ENERGY(2);
+  else if (nf == 3)
+    FH_ENERGY(3);
+  else if (nf == 6)
+    FH_ENERGY(6);
+  else
+    return fail(FH_ERR_ARG, "unsupported number of components");
 #undef FH_ENERGY
-    if (rc == FH_OK) {
-      hipLaunchKernelGGL(fh::k_energy_final, dim3(1), dim3(256), 0, pl->stream, pl->partial, nb, nn,
-                         1.0 / (double)pl->n, dAH);
-      e = hipGetLastError();
-      if (e == hipSuccess) e = hipStreamSynchronize(pl->stream);
-      if (e == hipSuccess) e = hipMemcpy(AH, dAH, nn * sizeof(double), hipMemcpyDeviceToHost);
-      if (e != hipSuccess) rc = fail(FH_ERR_HIP, std::string("energy: ") + hipGetErrorString(e));
-    }
-  }
-  (void)hipFree(dptrs);
-  (void)hipFree(dAH);
-  return rc;
+  hipLaunchKernelGGL(fh::k_energy_final, dim3(1), dim3(256), 0, stream, partial, nb, nn, 1.0 / (double)n, AHdev);
+  HIPCHK(hipGetLastError());
+  return FH_OK;
+}
+
+// AH[i][j] = <A e_i, e_j> on the grid of pl with coefficient `coef` of layout `kind`; the plan's
+// partial-sum buffer and its small result block hold the device side (no allocation).
+int energy_impl(fh_plan* pl, const double* coef, int kind, const void* const* fields, int nf, double* AH) {
+  const int nn = nf * nf;
+  if ((size_t)2 * nn * energy_blocks(pl->n) > pl->partial_cap) return fail(FH_ERR_STATE, "partial buffer too small");
+  RETCHK(enqueue_energy(pl->stream, coef, kind, fields, nf, pl->n, pl->partial, pl->aux));
+  HIPCHK(hipMemcpyAsync(AH, pl->aux, nn * sizeof(double), hipMemcpyDeviceToHost, pl->stream));
+  HIPCHK(hipStreamSynchronize(pl->stream));
+  return FH_OK;
 }
 
 std::mutex g_scratch_mu;
@@ -1011,9 +1016,10 @@ int fh_plan_create(int dim, const int* N, const double* Y, int physics, int prim
     return bail(fail(FH_ERR_HIP, "stream/event creation failed"));
   const size_t spec_bytes = (size_t)pl->ncomp * pl->nh * sizeof(cpx<double>);
   if (hipMalloc(&pl->spec, spec_bytes) != hipSuccess ||
-      hipMalloc(&pl->partial, pl->partial_cap * sizeof(double)) != hipSuccess ||
+      hipMalloc(&pl->partial, (pl->partial_cap + kAuxDoubles) * sizeof(double)) != hipSuccess ||
       hipMalloc(&pl->st, sizeof(CGState)) != hipSuccess)
     return bail(fail(FH_ERR_HIP, "hipMalloc plan buffers failed (out of device memory?)"));
+  pl->aux = pl->partial + pl->partial_cap;
   *out = pl;
   return FH_OK;
 }
@@ -1370,49 +1376,36 @@ int fh_energy_gradient(fh_plan* pl, const void* const* fields, int nf, const dou
     return fail(FH_ERR_STATE, "the gradient needs an iso or Lame coefficient");
   for (int i = 0; i < nf; ++i)
     if (!fields[i]) return fail(FH_ERR_ARG, "null field pointer");
-  const int nn = nf * nf;
-  void* dptrs = nullptr;
-  double* dW = nullptr;
-  HIPCHK(hipMalloc(&dptrs, nf * sizeof(void*)));
-  hipError_t e = hipMalloc(&dW, nn * sizeof(double));
-  if (e == hipSuccess) e = hipMemcpy(dptrs, fields, nf * sizeof(void*), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(dW, W, nn * sizeof(double), hipMemcpyHostToDevice);
-  int rc = FH_OK;
-  if (e != hipSuccess) rc = fail(FH_ERR_HIP, std::string("fh_energy_gradient: ") + hipGetErrorString(e));
-  if (rc == FH_OK) {
-    const double* const* F = (const double* const*)dptrs;
-    const dim3 grid(grid_blocks(pl->n, 256, 4096)), block(256);
-    const double inv_n = 1.0 / (double)pl->n;
-    const double* ch = (const double*)chain;
-    double* g = (double*)grad;
-    const bool iso = pl->coef_kind == FH_COEF_ISO;
-#define FH_EGRAD(NC)                                                                                          \
-  do {                                                                                                        \
-    if (iso)                                                                                                  \
-      hipLaunchKernelGGL((fh::k_energy_grad<NC, fh::COEF_ISO>), grid, block, 0, pl->stream, F, pl->n, dW, inv_n, \
-                         ch, g);                                                                              \
-    else                                                                                                      \
-      hipLaunchKernelGGL((fh::k_energy_grad<NC, fh::COEF_LAME>), grid, block, 0, pl->stream, F, pl->n, dW,     \
-                         inv_n, ch, g);                                                                       \
+  fh::FieldPtrs F{};
+  for (int i = 0; i < nf; ++i) F.p[i] = (const double*)fields[i];
+  fh::Mat6 Wm{};
+  for (int k = 0; k < nf * nf; ++k) Wm.w[k] = W[k];
+  const dim3 grid(grid_blocks(pl->n, 256, 4096)), block(256);
+  const double inv_n = 1.0 / (double)pl->n;
+  const double* ch = (const double*)chain;
+  double* g = (double*)grad;
+  const bool iso = pl->coef_kind == FH_COEF_ISO;
+#define FH_EGRAD(NC)                                                                                            \
+  do {                                                                                                          \
+    if (iso)                                                                                                    \
+      hipLaunchKernelGGL((fh::k_energy_grad<NC, fh::COEF_ISO>), grid, block, 0, pl->stream, F, pl->n, Wm, inv_n, \
+                         ch, g);                                                                                \
+    else                                                                                                        \
+      hipLaunchKernelGGL((fh::k_energy_grad<NC, fh::COEF_LAME>), grid, block, 0, pl->stream, F, pl->n, Wm,      \
+                         inv_n, ch, g);                                                                         \
   } while (0)
-    if (nf == 2)
-      FH_EGRAD(2);
-    else if (nf == 3)
-      FH_EGRAD(3);
-    else if (nf == 6)
-      FH_EGRAD(6);
-    else
-      rc = fail(FH_ERR_ARG, "unsupported number of components");
+  if (nf == 2)
+    FH_EGRAD(2);
+  else if (nf == 3)
+    FH_EGRAD(3);
+  else if (nf == 6)
+    FH_EGRAD(6);
+  else
+    return fail(FH_ERR_ARG, "unsupported number of components");
 #undef FH_EGRAD
-    if (rc == FH_OK) {
-      e = hipGetLastError();
-      if (e == hipSuccess) e = hipStreamSynchronize(pl->stream);
-      if (e != hipSuccess) rc = fail(FH_ERR_HIP, std::string("fh_energy_gradient: ") + hipGetErrorString(e));
-    }
-  }
-  (void)hipFree(dptrs);
-  (void)hipFree(dW);
-  return rc;
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(pl->stream));
+  return FH_OK;
 }
 
 int fh_field_gradient(fh_plan* pl, const void* const* fields, const void* const* adjoints, int nf,
@@ -1422,34 +1415,22 @@ int fh_field_gradient(fh_plan* pl, const void* const* fields, const void* const*
   if (pl->coef_kind != FH_COEF_ISO) return fail(FH_ERR_STATE, "the field gradient needs an iso coefficient");
   for (int i = 0; i < nf; ++i)
     if (!fields[i] || !adjoints[i]) return fail(FH_ERR_ARG, "null field pointer");
-  void* dptrs = nullptr;
-  HIPCHK(hipMalloc(&dptrs, 2 * nf * sizeof(void*)));
-  std::vector<const void*> hp(2 * nf);
+  fh::FieldPtrs F{}, Lm{};
   for (int i = 0; i < nf; ++i) {
-    hp[i] = fields[i];
-    hp[nf + i] = adjoints[i];
+    F.p[i] = (const double*)fields[i];
+    Lm.p[i] = (const double*)adjoints[i];
   }
-  hipError_t e = hipMemcpy(dptrs, hp.data(), 2 * nf * sizeof(void*), hipMemcpyHostToDevice);
-  int rc = FH_OK;
-  if (e == hipSuccess) {
-    const double* const* F = (const double* const*)dptrs;
-    const double* const* Lm = F + nf;
-    const dim3 grid(grid_blocks(pl->n, 256, 4096)), block(256);
-    const double* ch = (const double*)chain;
-    double* g = (double*)grad;
-    switch (pl->ncomp) {
-      case 2: hipLaunchKernelGGL((fh::k_field_grad<2>), grid, block, 0, pl->stream, F, Lm, nf, pl->n, ch, g); break;
-      case 3: hipLaunchKernelGGL((fh::k_field_grad<3>), grid, block, 0, pl->stream, F, Lm, nf, pl->n, ch, g); break;
-      default: rc = fail(FH_ERR_ARG, "unsupported number of components");
-    }
-    if (rc == FH_OK) {
-      e = hipGetLastError();
-      if (e == hipSuccess) e = hipStreamSynchronize(pl->stream);
-    }
+  const dim3 grid(grid_blocks(pl->n, 256, 4096)), block(256);
+  const double* ch = (const double*)chain;
+  double* g = (double*)grad;
+  switch (pl->ncomp) {
+    case 2: hipLaunchKernelGGL((fh::k_field_grad<2>), grid, block, 0, pl->stream, F, Lm, nf, pl->n, ch, g); break;
+    case 3: hipLaunchKernelGGL((fh::k_field_grad<3>), grid, block, 0, pl->stream, F, Lm, nf, pl->n, ch, g); break;
+    default: return fail(FH_ERR_ARG, "unsupported number of components");
   }
-  if (rc == FH_OK && e != hipSuccess) rc = fail(FH_ERR_HIP, std::string("fh_field_gradient: ") + hipGetErrorString(e));
-  (void)hipFree(dptrs);
-  return rc;
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(pl->stream));
+  return FH_OK;
 }
 
 int fh_axpby(int dtype, size_t n, double a, const void* x, double b, void* y) {
@@ -1495,21 +1476,16 @@ int fh_dot(int dtype, size_t n, const void* x, const void* y, double* out) {
 int fh_add_constant(int dtype, size_t nvox, int ncomp, const double* E, void* y) {
   RETCHK(check_dtype(dtype));
   RETCHK(ensure_device());
-  if (!E || !y || ncomp < 1 || ncomp > 64) return fail(FH_ERR_ARG, "bad arguments");
-  double* dE = nullptr;
-  HIPCHK(hipMalloc(&dE, ncomp * sizeof(double)));
-  hipError_t e = hipMemcpy(dE, E, ncomp * sizeof(double), hipMemcpyHostToDevice);
-  if (e == hipSuccess) {
-    const dim3 grid(grid_blocks((long long)nvox)), block(256);
-    if (dtype == FH_F64)
-      hipLaunchKernelGGL(fh::k_add_const<double>, grid, block, 0, 0, (long long)nvox, ncomp, dE, (double*)y);
-    else
-      hipLaunchKernelGGL(fh::k_add_const<float>, grid, block, 0, 0, (long long)nvox, ncomp, dE, (float*)y);
-    e = hipGetLastError();
-    if (e == hipSuccess) e = hipDeviceSynchronize();
-  }
-  (void)hipFree(dE);
-  if (e != hipSuccess) return fail(FH_ERR_HIP, std::string("fh_add_constant: ") + hipGetErrorString(e));
+  if (!E || !y || ncomp < 1 || ncomp > 36) return fail(FH_ERR_ARG, "bad arguments (1 <= ncomp <= 36)");
+  fh::Mat6 Ev{};
+  for (int c = 0; c < ncomp; ++c) Ev.w[c] = E[c];
+  const dim3 grid(grid_blocks((long long)nvox)), block(256);
+  if (dtype == FH_F64)
+    hipLaunchKernelGGL(fh::k_add_const<double>, grid, block, 0, 0, (long long)nvox, ncomp, Ev, (double*)y);
+  else
+    hipLaunchKernelGGL(fh::k_add_const<float>, grid, block, 0, 0, (long long)nvox, ncomp, Ev, (float*)y);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipDeviceSynchronize());
   return FH_OK;
 }
 
@@ -1587,6 +1563,79 @@ int fh_matvec_profile(fh_plan* pl, int dtype, const void* x, void* y, int nrep, 
       kernel_bytes[k] = 2.0 * nb_spec;
   }
   *nkernels = nk;
+  return FH_OK;
+}
+
+int fh_plan_set_schedule(fh_plan* pl, int dtype, int proj_variant, int cols_variant) {
+  RETCHK(check_plan(pl));
+  RETCHK(check_dtype(dtype));
+  if (proj_variant < 0 || proj_variant >= fh::kProjVariants || cols_variant < 0 || cols_variant >= fh::kColsVariants)
+    return fail(FH_ERR_ARG, "schedule variant out of range");
+  const int i = dtype == FH_F64 ? 0 : 1;
+  if (pl->proj_var[i] != proj_variant || pl->cols_var[i] != cols_variant) drop_graphs(pl);
+  pl->proj_var[i] = proj_variant;
+  pl->cols_var[i] = cols_variant;
+  return FH_OK;
+}
+
+int fh_plan_get_schedule(const fh_plan* pl, int dtype, int* proj_variant, int* cols_variant) {
+  RETCHK(check_plan(pl));
+  RETCHK(check_dtype(dtype));
+  if (!proj_variant || !cols_variant) return fail(FH_ERR_ARG, "null pointer argument");
+  const int i = dtype == FH_F64 ? 0 : 1;
+  *proj_variant = pl->proj_var[i];
+  *cols_variant = pl->cols_var[i];
+  return FH_OK;
+}
+
+int fh_plan_autotune(fh_plan* pl, int dtype, const void* x, void* y, int nrep, double* proj_ms, double* cols_ms,
+                     int* proj_variant, int* cols_variant) {
+  RETCHK(check_plan(pl));
+  RETCHK(check_dtype(dtype));
+  if (!x || !y || nrep < 1) return fail(FH_ERR_ARG, "bad arguments");
+  const int i = dtype == FH_F64 ? 0 : 1;
+  const int nk = pl->n0 > 1 ? 5 : 3;
+  const int kproj = pl->n0 > 1 ? 2 : 1;
+  double kms[8], kby[8];
+  int nkr = 0;
+  const int keep_p = pl->proj_var[i], keep_c = pl->cols_var[i];
+  double tp[fh::kProjVariants], tc[fh::kColsVariants];
+  // each variant of one pass with the other pass at its current choice; the faster wins
+  int rc = FH_OK;
+  for (int v = 0; v < fh::kProjVariants && rc == FH_OK; ++v) {
+    pl->proj_var[i] = v;
+    rc = fh_matvec_profile(pl, dtype, x, y, nrep, 8, kms, kby, &nkr);
+    tp[v] = rc == FH_OK ? kms[kproj] : 0.0;
+  }
+  int bp = 0;
+  for (int v = 1; v < fh::kProjVariants; ++v)
+    if (tp[v] < tp[bp]) bp = v;
+  pl->proj_var[i] = bp;
+  int bc = keep_c;
+  for (int v = 0; v < fh::kColsVariants && rc == FH_OK; ++v) {
+    if (nk == 3) {
+      tc[v] = 0.0;
+      continue;
+    }
+    pl->cols_var[i] = v;
+    rc = fh_matvec_profile(pl, dtype, x, y, nrep, 8, kms, kby, &nkr);
+    tc[v] = rc == FH_OK ? kms[1] + kms[3] : 0.0;
+  }
+  if (nk == 5) {
+    bc = 0;
+    for (int v = 1; v < fh::kColsVariants; ++v)
+      if (tc[v] < tc[bc]) bc = v;
+  }
+  pl->proj_var[i] = keep_p;
+  pl->cols_var[i] = keep_c;
+  if (rc != FH_OK) return rc;
+  RETCHK(fh_plan_set_schedule(pl, dtype, bp, bc));
+  if (proj_ms)
+    for (int v = 0; v < fh::kProjVariants; ++v) proj_ms[v] = tp[v];
+  if (cols_ms)
+    for (int v = 0; v < fh::kColsVariants; ++v) cols_ms[v] = tc[v];
+  if (proj_variant) *proj_variant = bp;
+  if (cols_variant) *cols_variant = bc;
   return FH_OK;
 }
 
@@ -2453,6 +2502,11 @@ int fh_potential_solve(fh_plan* pl, int mode, const double* E, double tol, int m
   if (!pl->coef_kind) return fail(FH_ERR_STATE, "fh_set_coeff has not been called");
   if (!(tol >= 0.0) || maxiter < 0) return fail(FH_ERR_ARG, "bad tol/maxiter");
   if (mode == FH_SOLVE_MIXED && !(delta > 0.0 && delta < 1.0)) return fail(FH_ERR_ARG, "delta must be in (0,1)");
+  // the logs of a previous fh_cg_solve never describe this solve
+  pl->res_kit.clear();
+  pl->res_norm.clear();
+  if (pl->res_every > 0)
+    return fail(FH_ERR_UNSUPPORTED, "the residual log (fh_plan_set_residual_log) is kept by fh_cg_solve only");
   if (!pl->banded)
     for (int a = 0; a < pl->dim; ++a)
       if ((pl->Nu[a] & 1) == 0)

@@ -208,6 +208,17 @@ struct FxShape {
                                             : (FP::TPP >= 16 ? 4 : (FP::TPP >= 8 ? 8 : 16));
   // rows per workgroup in the contiguous-axis passes (even)
   static constexpr int RPB = (FP::TPP >= 16) ? 8 : (FP::TPP >= 8 ? 16 : 32);
+  // Schedule variants a plan can select (fh_plan_set_schedule / fh_plan_autotune); variant 0 is
+  // the default. Projection pass: frequency points per tile (TPP*F*NC threads, so F also sets how
+  // full the last wavefront of a workgroup is); strided passes: columns per tile.
+  static constexpr int NPROJ = 4, NCOLS = 2;
+  static constexpr int proj_f(int v) {
+    constexpr int c32a[NPROJ] = {4, 6, 5, 7}, c32b[NPROJ] = {16, 8, 12, 24}, c32c[NPROJ] = {32, 16, 24, 48};
+    constexpr int c64a[NPROJ] = {4, 2, 3, 6}, c64b[NPROJ] = {8, 4, 6, 12}, c64c[NPROJ] = {16, 8, 12, 24};
+    if (sizeof(T) == 4) return FP::TPP >= 16 ? c32a[v] : (FP::TPP >= 8 ? c32b[v] : c32c[v]);
+    return FP::TPP >= 16 ? c64a[v] : (FP::TPP >= 8 ? c64b[v] : c64c[v]);
+  }
+  static constexpr int cols_p(int v) { return v == 0 ? P : (P >= 2 ? P / 2 : P); }
 };
 
 // ---- strided pass: forward (DIR -1) or inverse (DIR +1) along a non-contiguous axis ----

@@ -11,7 +11,6 @@ template int fixed_pot_cols<float>(const LaunchCtx&, int, int, int, cpx<float>*,
                                    const CGState*, float, cpx<float>*, const cpx<float>*, double*, size_t,
                                    const pot::PotParams&, const int*, int*);
 template int fixed_cols<float>(const LaunchCtx&, int, int, int, cpx<float>*, const int*);
-template int fixed_cols_g<float>(const LaunchCtx&, int, int, cpx<float>*, const GParams&, const int*);
 
 bool fixed_length_supported(int L) {
   switch (L) {

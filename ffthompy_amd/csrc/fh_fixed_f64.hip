@@ -11,5 +11,4 @@ template int fixed_pot_cols<double>(const LaunchCtx&, int, int, int, cpx<double>
                                     const CGState*, double, cpx<double>*, const cpx<double>*, double*, size_t,
                                     const pot::PotParams&, const int*, int*);
 template int fixed_cols<double>(const LaunchCtx&, int, int, int, cpx<double>*, const int*);
-template int fixed_cols_g<double>(const LaunchCtx&, int, int, cpx<double>*, const GParams&, const int*);
 }  // namespace fh

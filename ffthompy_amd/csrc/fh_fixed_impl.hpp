@@ -235,9 +235,8 @@ inline void strided_geometry(const LaunchCtx& c, int axis, int nc, long long* ou
   }
 }
 
-template <typename T, class FP>
-int cols_fx_launch(const LaunchCtx& c, int dir, int axis, int nc, cpx<T>* data, const int* halt) {
-  constexpr int P = FxShape<FP, T>::P;
+template <typename T, class FP, int P>
+int cols_fx_launch_p(const LaunchCtx& c, int dir, int axis, int nc, cpx<T>* data, const int* halt) {
   long long outer, ostride, inner;
   strided_geometry(c, axis, nc, &outer, &ostride, &inner);
   const long long total = c.nh * nc;
@@ -257,14 +256,24 @@ int cols_fx_launch(const LaunchCtx& c, int dir, int axis, int nc, cpx<T>* data, 
   return e == hipSuccess ? 1 : -(int)e;
 }
 
-template <typename T, class FP, class FPR, int NC>
-int cols_g_fx_launch_nc(const LaunchCtx& c, int axis, cpx<T>* data, const GParams& g, const int* halt) {
-  // frequency points per tile: the shape default, halved until TPP*F*NC threads fit a workgroup
-  constexpr int F = [] {
-    int f = FxShape<FP, T>::F;
-    while (f > 1 && FP::TPP * f * NC > 1024) f /= 2;
-    return f;
-  }();
+template <typename T, class FP>
+int cols_fx_launch(const LaunchCtx& c, int dir, int axis, int nc, cpx<T>* data, const int* halt) {
+  if (c.cols_var == 1) return cols_fx_launch_p<T, FP, FxShape<FP, T>::cols_p(1)>(c, dir, axis, nc, data, halt);
+  return cols_fx_launch_p<T, FP, FxShape<FP, T>::cols_p(0)>(c, dir, axis, nc, data, halt);
+}
+
+// frequency points per projection tile of schedule variant V, halved until TPP*F*NC threads fit
+// a workgroup
+template <typename T, class FP, int NC, int V>
+constexpr int proj_points() {
+  int f = FxShape<FP, T>::proj_f(V);
+  while (f > 1 && FP::TPP * f * NC > 1024) f /= 2;
+  return f;
+}
+
+template <typename T, class FP, class FPR, int NC, int V>
+int cols_g_fx_launch_v(const LaunchCtx& c, int axis, cpx<T>* data, const GParams& g, const int* halt) {
+  constexpr int F = proj_points<T, FP, NC, V>();
   if (FP::TPP * F * NC > 1024) return 0;
   long long outer, ostride, inner;
   strided_geometry(c, axis, NC, &outer, &ostride, &inner);
@@ -279,6 +288,20 @@ int cols_g_fx_launch_nc(const LaunchCtx& c, int axis, cpx<T>* data, const GParam
                      tw, g, c.n2h, halt);
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? 1 : -(int)e;
+}
+
+template <typename T, class FP, class FPR, int NC>
+int cols_g_fx_launch_nc(const LaunchCtx& c, int axis, cpx<T>* data, const GParams& g, const int* halt) {
+  // the scalar problems (2 or 3 components) get every schedule variant, elasticity the default
+  if constexpr (NC == 2 || NC == 3) {
+    switch (c.proj_var) {
+      case 1: return cols_g_fx_launch_v<T, FP, FPR, NC, 1>(c, axis, data, g, halt);
+      case 2: return cols_g_fx_launch_v<T, FP, FPR, NC, 2>(c, axis, data, g, halt);
+      case 3: return cols_g_fx_launch_v<T, FP, FPR, NC, 3>(c, axis, data, g, halt);
+      default: break;
+    }
+  }
+  return cols_g_fx_launch_v<T, FP, FPR, NC, 0>(c, axis, data, g, halt);
 }
 
 template <typename T, class FP, class FPR>
@@ -333,12 +356,10 @@ int fixed_rows_rt(const LaunchCtx& c, int nc, cpx<T>* spec, const T* coef, T sca
   }
 }
 
-// frequency points per tile of the potential's strided passes (as the projection tile)
+// frequency points per tile of the potential's strided passes (the default projection tile)
 template <typename T, class FP, int NC>
 constexpr int pot_points() {
-  int f = FxShape<FP, T>::F;
-  while (f > 1 && FP::TPP * f * NC > 1024) f /= 2;
-  return f;
+  return proj_points<T, FP, NC, 0>();
 }
 
 template <typename T, class FP, int NC>

@@ -12,6 +12,11 @@
 
 namespace fh {
 
+// A small matrix (up to 6 x 6 doubles, row-major) passed by value as a kernel argument.
+struct Mat6 {
+  double w[36];
+};
+
 constexpr int kMaxComp = 6;
 
 enum CoefKind { COEF_NONE = 0, COEF_ISO = 1, COEF_LAME = 2, COEF_FULL = 3 };
@@ -501,10 +506,10 @@ __global__ void k_axpby(long long n, double a, const T* __restrict__ x, double b
 }
 
 template <typename T>
-__global__ void k_add_const(long long nvox, int nc, const double* __restrict__ E, T* __restrict__ y) {
+__global__ void k_add_const(long long nvox, int nc, Mat6 E, T* __restrict__ y) {
   const long long stride = (long long)gridDim.x * blockDim.x;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nvox; i += stride)
-    for (int c = 0; c < nc; ++c) y[c * nvox + i] += T(E[c]);
+    for (int c = 0; c < nc; ++c) y[c * nvox + i] += T(E.w[c]);
 }
 
 template <typename TS, typename TD>
@@ -693,10 +698,14 @@ static __global__ void k_fold_correction(long long n, double* __restrict__ x64, 
 
 // ---- energy forms: AH[i][j] = <A e_i, e_j> (ffthompy/postprocess.py:53-70) ------------
 // All NC fields are read once; per-workgroup double-double partial sums.
+// Up to six device field pointers, passed by value (kernel arguments need no device copy).
+struct FieldPtrs {
+  const double* p[6];
+};
+
 template <int NC, int COEF>
 __global__ void __launch_bounds__(256)
-k_energy(const double* const* __restrict__ F, const double* __restrict__ coef, long long n,
-         double* __restrict__ partial) {
+k_energy(FieldPtrs F, const double* __restrict__ coef, long long n, double* __restrict__ partial) {
   double acc[NC * NC];
 #pragma unroll
   for (int k = 0; k < NC * NC; ++k) acc[k] = 0.0;
@@ -706,7 +715,7 @@ k_energy(const double* const* __restrict__ F, const double* __restrict__ coef, l
 #pragma unroll
     for (int i = 0; i < NC; ++i)
 #pragma unroll
-      for (int c = 0; c < NC; ++c) e[i][c] = F[i][c * n + v];
+      for (int c = 0; c < NC; ++c) e[i][c] = F.p[i][c * n + v];
 #pragma unroll
     for (int i = 0; i < NC; ++i) {
       double Ae[NC];
@@ -777,18 +786,18 @@ static __global__ void k_energy_final(const double* __restrict__ partial, int np
 // chain (nullable) multiplies the ISO gradient per voxel (e.g. a = exp(theta) -> da/dtheta = a).
 template <int NC, int COEF>
 __global__ void __launch_bounds__(256)
-k_energy_grad(const double* const* __restrict__ F, long long n, const double* __restrict__ W, double inv_n,
-              const double* __restrict__ chain, double* __restrict__ grad) {
+k_energy_grad(FieldPtrs F, long long n, Mat6 W, double inv_n, const double* __restrict__ chain,
+              double* __restrict__ grad) {
   double w[NC * NC];
 #pragma unroll
-  for (int k = 0; k < NC * NC; ++k) w[k] = W[k];
+  for (int k = 0; k < NC * NC; ++k) w[k] = W.w[k];
   const long long stride = (long long)gridDim.x * blockDim.x;
   for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
     double e[NC][NC];
 #pragma unroll
     for (int i = 0; i < NC; ++i)
 #pragma unroll
-      for (int c = 0; c < NC; ++c) e[i][c] = F[i][c * n + v];
+      for (int c = 0; c < NC; ++c) e[i][c] = F.p[i][c * n + v];
     double gdot = 0.0, gtr = 0.0;
 #pragma unroll
     for (int i = 0; i < NC; ++i) {
@@ -824,14 +833,14 @@ k_energy_grad(const double* const* __restrict__ F, long long n, const double* __
 // lam_i the adjoint fields (solutions of G(a lam_i) = G dL/de_i). All pointers fp64.
 template <int NC>
 __global__ void __launch_bounds__(256)
-k_field_grad(const double* const* __restrict__ F, const double* const* __restrict__ Lm, int nf, long long n,
-             const double* __restrict__ chain, double* __restrict__ grad) {
+k_field_grad(FieldPtrs F, FieldPtrs Lm, int nf, long long n, const double* __restrict__ chain,
+             double* __restrict__ grad) {
   const long long stride = (long long)gridDim.x * blockDim.x;
   for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
     double acc = 0.0;
     for (int i = 0; i < nf; ++i) {
 #pragma unroll
-      for (int c = 0; c < NC; ++c) acc += Lm[i][c * n + v] * F[i][c * n + v];
+      for (int c = 0; c < NC; ++c) acc += Lm.p[i][c * n + v] * F.p[i][c * n + v];
     }
     grad[v] = -acc * (chain ? chain[v] : 1.0);
   }

@@ -16,7 +16,12 @@ struct LaunchCtx {
   long long n, nh;
   const void* tw[3];  // per-axis twiddle table W_L^m, m < L, in the launch precision
   hipEvent_t t0 = nullptr, t1 = nullptr;  // if set: start/stop time of the launched kernel
+  int proj_var = 0;  // projection pass schedule variant (FxShape::proj_f)
+  int cols_var = 0;  // strided pass schedule variant (FxShape::cols_p)
 };
+
+// Number of schedule variants of the projection and strided passes.
+constexpr int kProjVariants = 4, kColsVariants = 2;
 
 // Return 1 if a specialised kernel was launched, 0 if the length has none (caller falls
 // back to the generic kernels), or a negative hipError_t on launch failure.

@@ -43,8 +43,10 @@ def _solver_par(pars):
     """tol, maxiter and the arithmetic: solver['precision'] 'fp64' (default, the reference's) or
     'mixed' (fp32 iterations with fp64 reliable updates, solver['delta'] their trigger)."""
     s = _par(pars, 'solver', {}) or {}
+    mode = s.get('precision', 'fp64')
+    nat.solve_mode(mode)
     return (float(s.get('tol', 1e-6)), int(s.get('maxiter', 1000)),
-            {'mode': s.get('precision', 'fp64'), 'delta': float(s.get('delta', 1e-4))})
+            {'mode': mode, 'delta': float(s.get('delta', 1e-4))})
 
 
 def _e1(d):

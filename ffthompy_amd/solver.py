@@ -73,8 +73,7 @@ def linear_solver(Afun, B, ATfun=None, x0=None, par=None, solver=None, callback=
 
 
 def _iterate(method, Afun, B, x0, coefs, tol, maxiter, callback):
-    B = B.to_device()
-    x = B.zeros_like(name='x') if x0 is None else x0.to_device().copy(name='x')
+    B, x = _start(B, x0)
     hist_len = int(maxiter) + 1 if isinstance(callback, CallBack) else 0
     out = Afun.plan.iter_solve(method, B.val, x.val, coefs, float(tol), int(maxiter), hist_len=hist_len)
     info = {'kit': out['kit'], 'norm_res': out['norm_res']}
@@ -130,14 +129,25 @@ def scipy_cg(Afun, B, x0=None, par=None, callback=None):
     return x.to_device(), {'info': code}
 
 
+def _start(B, x0):
+    """Device copies of the right-hand side and the initial guess; x0 must match B."""
+    B = B.to_device()
+    if x0 is None:
+        return B, B.zeros_like(name='x')
+    if tuple(x0.val.shape) != tuple(B.val.shape) or np.dtype(x0.val.dtype) != np.dtype(B.val.dtype):
+        raise ValueError('x0 (shape %s, %s) does not match B (shape %s, %s)'
+                         % (tuple(x0.val.shape), x0.val.dtype, tuple(B.val.shape), B.val.dtype))
+    return B, x0.to_device().copy(name='x')
+
+
 def CG(Afun, B, x0=None, par=None, callback=None):
     """Conjugate gradients on the device (solver.py:80-139)."""
     par = {} if par is None else par
     par.setdefault('tol', 1e-6)
     par.setdefault('maxiter', int(1e3))
     mode = par.get('precision', 'fp64')
-    B = B.to_device()
-    x = B.zeros_like(name='x') if x0 is None else x0.to_device().copy(name='x')
+    nat.solve_mode(mode)
+    B, x = _start(B, x0)
     hist_len = int(par['maxiter']) + 1 if callback is not None else 0
     every = int(par.get('true_residual_every', 0)) if isinstance(callback, CallBack) else 0
     Afun.plan.set_residual_log(every)

@@ -217,6 +217,7 @@ int fh_potential_solve(fh_plan* plan, int mode, const double* E, double tol, int
  * double-double accumulated: assembly_matrix (ffthompy/postprocess.py:53-70). */
 int fh_energy_matrix(fh_plan* plan, const void* const* fields, int nf, double* AH);
 
+
 /* Gradient of a loss L(AH) w.r.t. the coefficient, for the minimisers e_i (fp64 fields,
  * macroscopic part included) of the solve: with W = dL/dAH (host, nf x nf) and the fields
  * held fixed (envelope theorem, the e_i are stationary points of the energy),
@@ -261,6 +262,21 @@ int fh_last_solve_updates(const fh_plan* plan, int* n_updates);
  * entries; *nkernels receives the count. No reference counterpart (measurement only). */
 int fh_matvec_profile(fh_plan* plan, int dtype, const void* x, void* y, int nrep, int max_kernels,
                       double* kernel_ms, double* kernel_bytes, int* nkernels);
+
+/* Schedule of the fixed-length kernels, per arithmetic (dtype FH_F64 / FH_F32). proj_variant
+ * (0..3) sets the frequency points per tile of the projection pass (forward FFT along the
+ * outermost axis, G(xi), inverse FFT), which also sets the threads per workgroup (scalar plans;
+ * elasticity keeps variant 0's tile); cols_variant (0..1) the columns per tile of the other strided
+ * passes (3-D). Every variant computes the same values (the same transforms per pencil); only
+ * speed differs. Variant 0 is the default. fh_plan_autotune times every variant on this plan
+ * (nrep matvecs of x into y per variant, each pass timed by its own HIP events, as
+ * fh_matvec_profile), keeps the fastest of each pass and returns the times in ms
+ * (proj_ms[4], cols_ms[2]; either may be NULL) and the choice. FH_ERR_UNSUPPORTED where the plan
+ * runs the runtime-radix kernels. No reference counterpart (the reference calls NumPy's FFT). */
+int fh_plan_set_schedule(fh_plan* plan, int dtype, int proj_variant, int cols_variant);
+int fh_plan_get_schedule(const fh_plan* plan, int dtype, int* proj_variant, int* cols_variant);
+int fh_plan_autotune(fh_plan* plan, int dtype, const void* x, void* y, int nrep, double* proj_ms, double* cols_ms,
+                     int* proj_variant, int* cols_variant);
 
 /* ---- Batched 2-D scalar homogenization (many samples, uneven grids, one launch) ----------
  * Each sample s is the reference's scalar GaNi problem on its own grid

@@ -15,13 +15,24 @@ def _grid_axes(ndim, N):
     return tuple(range(ndim - len(N), ndim))
 
 
+# None: numpy.fft (pocketfft, one thread), the reference's library. An integer k: scipy.fft with k
+# threads (pocketfft as well), for large offline fixtures; the results agree to rounding.
+FFT_WORKERS = None
+
+
 def rfftn(x, N):
     """Unnormalised real FFT over the last len(N) axes (fft.py:39-40)."""
+    if FFT_WORKERS:
+        import scipy.fft
+        return scipy.fft.rfftn(x, s=tuple(N), axes=_grid_axes(x.ndim, N), workers=FFT_WORKERS)
     return np.fft.rfftn(x, s=tuple(N), axes=_grid_axes(x.ndim, N))
 
 
 def irfftn(x, N):
     """Normalised inverse real FFT (fft.py:42-43)."""
+    if FFT_WORKERS:
+        import scipy.fft
+        return scipy.fft.irfftn(x, s=tuple(N), axes=_grid_axes(x.ndim, N), workers=FFT_WORKERS)
     return np.fft.irfftn(x, s=tuple(N), axes=_grid_axes(x.ndim, N))
 
 

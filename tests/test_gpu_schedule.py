@@ -1,0 +1,71 @@
+"""Schedule variants of the fixed-length passes and the plan autotuner (fh_plan_set_schedule,
+fh_plan_autotune): every variant computes the same transforms per pencil, so the matvec and the
+mixed CG must be equal to the last bit across variants; the autotuner must pick one of them and
+leave the results unchanged. The matvec itself is pinned to the oracle at variant 0 by
+test_gpu_kernels.py."""
+import numpy as np
+import pytest
+
+from ffthompy_amd import _native as nat
+
+pytestmark = pytest.mark.gpu
+
+CASES = [((45, 45, 45), np.float32), ((45, 45, 45), np.float64), ((81, 81, 81), np.float32),
+         ((243, 27, 243), np.float32), ((63, 135), np.float32), ((99, 99), np.float64)]
+
+
+@pytest.mark.parametrize('N,dt', CASES, ids=['%s-%s' % (c[0], np.dtype(c[1]).name) for c in CASES])
+def test_schedule_variants_are_bitwise_equal(N, dt):
+    rng = np.random.default_rng(11)
+    pl = nat.Plan(N, np.ones(len(N)), 'scalar', 'primal')
+    pl.set_coeff('iso', 1. + 9. * rng.random(N))
+    x = nat.DeviceArray.from_host(rng.standard_normal((len(N),) + N).astype(dt))
+    y = nat.DeviceArray((len(N),) + N, dt)
+    ref = None
+    for pv in range(4):
+        for cv in range(2):
+            pl.set_schedule(dt, pv, cv)
+            assert pl.schedule(dt) == (pv, cv)
+            pl.matvec(x, y)
+            got = y.to_host()
+            if ref is None:
+                ref = got
+            else:
+                assert np.array_equal(got, ref), (pv, cv)
+    print('%s %s: 8 schedules equal' % (N, np.dtype(dt).name), flush=True)
+
+
+def test_schedule_rejects_bad_variants():
+    pl = nat.Plan((15, 15, 15), np.ones(3), 'scalar', 'primal')
+    with pytest.raises(nat.FhError):
+        pl.set_schedule(np.float32, 4, 0)
+    with pytest.raises(nat.FhError):
+        pl.set_schedule(np.float64, 0, 2)
+
+
+@pytest.mark.parametrize('N', [(81, 81, 81), (105, 135)])
+def test_autotune_picks_a_variant_and_keeps_results(N):
+    rng = np.random.default_rng(12)
+    a = 1. + 99. * (rng.random(N) > 0.7)
+    pl = nat.Plan(N, np.ones(len(N)), 'scalar', 'primal')
+    pl.set_coeff('iso', a)
+    d = len(N)
+    E = np.zeros(d)
+    E[0] = 1.
+    EN = nat.DeviceArray.zeros((d,) + N)
+    nat.add_constant(EN, -E)
+    B = nat.DeviceArray((d,) + N)
+    pl.matvec(EN, B)
+    x0 = nat.DeviceArray.zeros((d,) + N)
+    info0 = pl.cg_solve(B, x0, tol=1e-8, maxiter=500, mode='mixed', delta=1e-4)
+    for dt in (np.float32, np.float64):
+        res = pl.autotune(dt, nrep=3)
+        print(N, np.dtype(dt).name, res, flush=True)
+        assert 0 <= res['proj_variant'] < 4 and 0 <= res['cols_variant'] < 2
+        assert all(t > 0 for t in res['proj_ms'])
+        assert pl.schedule(dt) == (res['proj_variant'], res['cols_variant'])
+        assert res['proj_ms'][res['proj_variant']] == min(res['proj_ms'])
+    x1 = nat.DeviceArray.zeros((d,) + N)
+    info1 = pl.cg_solve(B, x1, tol=1e-8, maxiter=500, mode='mixed', delta=1e-4)
+    assert info1['kit'] == info0['kit']
+    assert np.array_equal(x1.to_host(), x0.to_host())
