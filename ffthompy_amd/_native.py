@@ -50,6 +50,8 @@ SIGNATURES = {
     'fh_cg_solve': [_vp, ctypes.c_int, _vp, _vp, ctypes.c_double, ctypes.c_int, ctypes.c_double,
                     _c_int_p, _c_double_p, _c_double_p, ctypes.c_int, _c_int_p],
     'fh_energy_matrix': [_vp, ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, _c_double_p],
+    'fh_energy_matrix_noalloc': [ctypes.c_longlong, ctypes.c_int, _vp, ctypes.POINTER(ctypes.c_void_p), ctypes.c_int,
+                                 _c_double_p],
     'fh_energy_gradient': [_vp, ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, _c_double_p, _vp, _vp],
     'fh_field_gradient': [_vp, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, _vp,
                           _vp],
@@ -299,6 +301,25 @@ def check_array(arr, size, dtypes, what):
 
 _REAL = (np.float64, np.float32)
 _COMPLEX_OF = {np.dtype(np.float64): np.complex128, np.dtype(np.float32): np.complex64}
+
+
+def energy_matrix_noalloc(kind, coef, fields):
+    """AH[i, j] = <A e_i, e_j> without a plan or any device allocation (fh_energy_matrix_noalloc):
+    kind 'iso' / 'lame' / 'full', coef a fp64 DeviceArray of that layout, fields nf fp64 DeviceArrays
+    of nf components each (nf = 2, 3 or 6)."""
+    nf = len(fields)
+    if nf not in (2, 3, 6):
+        raise FhError('nf must be 2, 3 or 6', FH_ERR_ARG)
+    nvox = fields[0].size // nf
+    per = {'iso': 1, 'lame': 2, 'full': nf * nf}[kind]
+    check_array(coef, per * nvox, (np.float64,), 'coef')
+    for i, f in enumerate(fields):
+        check_array(f, nf * nvox, (np.float64,), 'fields[%d]' % i)
+    arr = (ctypes.c_void_p * nf)(*[f.ptr.value for f in fields])
+    AH = np.zeros((nf, nf))
+    call('fh_energy_matrix_noalloc', ctypes.c_longlong(nvox), ctypes.c_int(_COEF_CODES[kind]), coef.ptr, arr,
+         ctypes.c_int(nf), AH.ctypes.data_as(_c_double_p))
+    return AH
 
 
 class Plan:

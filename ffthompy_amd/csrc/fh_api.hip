@@ -54,6 +54,16 @@ constexpr size_t kLdsMax = 160 * 1024;
 // doubles of a plan's small-result block (after its partial sums)
 constexpr size_t kAuxDoubles = 64;
 
+// Scratch of fh_energy_matrix_noalloc: the per-workgroup double-double partial sums of up to
+// six fields (2 * 36 * 1024 doubles) and the 36 results. One copy per GPU (576.5 KiB), held until
+// the process ends. Host code never uses this name as an address: ensure_device() takes the
+// device address of the current GPU's copy with hipGetSymbolAddress.
+constexpr size_t kNoAllocDoubles = 2 * 36 * 1024 + 64;
+__device__ __align__(256) double scratch[kNoAllocDoubles];
+constexpr int kMaxDevices = 64;
+double* g_scratch_dev[kMaxDevices] = {};  // device address of `scratch`, per device
+std::mutex g_noalloc_mu;                  // one fh_energy_matrix_noalloc at a time in this process
+
 std::mutex g_init_mu;
 int g_init_state = -1;  // -1 unknown, FH_OK, or an error code
 std::string g_init_msg;
@@ -89,6 +99,15 @@ int ensure_device() {
     return fail(g_init_state, g_init_msg);
   }
   g_dev_name = std::string(prop.name[0] ? prop.name : "AMD GPU") + " (" + prop.gcnArchName + ")";
+  // the device's own copy of the no-alloc scratch (device 0 is current: set just above)
+  void* sp = nullptr;
+  e = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !sp) {
+    g_init_state = FH_ERR_HIP;
+    g_init_msg = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+    return fail(g_init_state, g_init_msg);
+  }
+  g_scratch_dev[0] = static_cast<double*>(sp);
   g_init_state = FH_OK;
   return FH_OK;
 }
@@ -1354,6 +1373,32 @@ int fh_cg_solve(fh_plan* pl, int mode, const void* b_, void* x_, double tol, int
     const int cnt = std::min(hist_len, s.kit + 1);
     HIPCHK(hipMemcpy(res_hist, pl->hist, (size_t)cnt * sizeof(double), hipMemcpyDeviceToHost));
   }
+  return FH_OK;
+}
+
+int fh_energy_matrix_noalloc(long long nvox, int kind, const void* coef, const void* const* fields, int nf,
+                             double* AH) {
+  RETCHK(ensure_device());
+  if (!coef || !fields || !AH) return fail(FH_ERR_ARG, "null pointer argument");
+  if (nvox < 1) return fail(FH_ERR_ARG, "nvox must be positive");
+  if (kind != FH_COEF_ISO && kind != FH_COEF_LAME && kind != FH_COEF_FULL) return fail(FH_ERR_ARG, "bad coefficient kind");
+  if (nf != 2 && nf != 3 && nf != 6)
+    return fail(FH_ERR_ARG, "fh_energy_matrix_noalloc: nf must be 2, 3 or 6 (the scratch holds at most 6 fields)");
+  if (kind == FH_COEF_LAME && nf == 2) return fail(FH_ERR_ARG, "a Lame coefficient needs 3 or 6 Mandel components");
+  for (int i = 0; i < nf; ++i)
+    if (!fields[i]) return fail(FH_ERR_ARG, "null field pointer");
+  const size_t need = (size_t)2 * nf * nf * energy_blocks(nvox) + (size_t)nf * nf;
+  if (need > kNoAllocDoubles) return fail(FH_ERR_ARG, "fh_energy_matrix_noalloc: scratch too small");
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices || !g_scratch_dev[dev])
+    return fail(FH_ERR_STATE, "no scratch address for the current device (fh_init selects device 0)");
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  double* part = g_scratch_dev[dev];
+  double* res = part + (kNoAllocDoubles - 64);
+  // the kernels write every partial sum they read: nothing has to start at zero
+  RETCHK(enqueue_energy(nullptr, (const double*)coef, kind, fields, nf, nvox, part, res));
+  HIPCHK(hipMemcpy(AH, res, (size_t)nf * nf * sizeof(double), hipMemcpyDeviceToHost));
   return FH_OK;
 }
 

@@ -217,6 +217,17 @@ int fh_potential_solve(fh_plan* plan, int mode, const double* E, double tol, int
  * double-double accumulated: assembly_matrix (ffthompy/postprocess.py:53-70). */
 int fh_energy_matrix(fh_plan* plan, const void* const* fields, int nf, double* AH);
 
+/* fh_energy_matrix without a plan and without any device allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code inside a stream capture,
+ * tight serving loops, hosts that forbid hipMalloc). nvox voxels; kind FH_COEF_ISO / LAME / FULL
+ * with coef a device fp64 array of that layout (as fh_set_coeff); nf = 2, 3 or 6 fp64 device fields of
+ * nf components each (nf > 6 is rejected: the scratch holds the partial sums of at most six fields);
+ * AH host, nf x nf. Same kernels and result as fh_energy_matrix. Its partial sums live in one
+ * static device array of 576.5 KiB per GPU (held until the process ends, located at fh_init);
+ * every call on a GPU shares it, so overlapping calls would race: the library serialises its own
+ * calls with a lock, and the call returns after its GPU work (null stream) is done. */
+int fh_energy_matrix_noalloc(long long nvox, int kind, const void* coef, const void* const* fields, int nf,
+                             double* AH);
 
 /* Gradient of a loss L(AH) w.r.t. the coefficient, for the minimisers e_i (fp64 fields,
  * macroscopic part included) of the solve: with W = dL/dAH (host, nf x nf) and the fields
