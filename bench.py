@@ -5,7 +5,10 @@ Workload: 3-D scalar conductivity on a 243^3 grid, 30 non-overlapping spheres
 (seeded centres, radii U[0.08, 0.15]) of conductivity 100 in a matrix of 1, unit
 loads e1, e2, e3. One step = the complete primal + dual homogenization: 6 CG solves
 to the reference's absolute tolerance 1e-8 (grid-mean residual norm) plus the
-energy forms that give the upper (primal) and lower (dual) bound matrices.
+energy forms that give the GaNi primal and dual matrices (equal analytically; reported
+with their ordering slack and against the float64 oracle fixture). The guaranteed,
+separated bounds (Ga post-processing of the same solutions) are computed after the
+timed steps.
 The headline runs the fp32 CG with fp64 reliable updates; the plain fp64 solve is
 timed beside it. Inputs are resident in HBM before the timed region.
 
@@ -72,6 +75,35 @@ class Homogenizer:
             out['solve_ms'][pd] = times
             out['updates'][pd] = ups
         return out
+
+
+def bounds_report(r_mixed, r_fp64, n):
+    """GaNi bound matrices of both arithmetics, the ordering check and the comparison with the float64
+    oracle fixture of this workload (tests/golden/oracle_scalar243.json, written on the CPU by
+    tests/golden/make_oracle_scalar243.py). GaNi primal and dual agree analytically; the CG iterates
+    only overestimate both energies, so lower <= upper holds up to the rounding of the energies: the
+    slack stated is 64 eps max|AH|."""
+    eps = np.finfo(np.float64).eps
+    out = {'note': 'GaNi upper = primal, lower = inverted dual; equal analytically, ordered within the slack'}
+    for name, r in (('fp64', r_fp64), ('mixed', r_mixed)):
+        up, lo = r['AH_primal'], r['AH_dual']
+        slack = 64 * eps * float(np.abs(up).max())
+        gap = float(np.linalg.eigvalsh(0.5 * ((up - lo) + (up - lo).T)).min())
+        out[name] = {'AH_GaNi_primal': up.tolist(), 'AH_GaNi_dual': lo.tolist(), 'slack': slack,
+                     'min_eig_upper_minus_lower': gap, 'lower_le_upper_within_slack': bool(gap >= -slack)}
+    path = os.path.join(ROOT, 'tests', 'golden', 'oracle_scalar243.json')
+    if n == 243 and os.path.exists(path):
+        fx = json.load(open(path))
+        ref = {pd: np.asarray(fx['AH_' + pd]) for pd in ('primal', 'dual')}
+        rel = {}
+        for name, r in (('fp64', r_fp64), ('mixed', r_mixed)):
+            rel[name] = [float(np.abs(r['AH_' + pd] - ref[pd]).max() / np.abs(ref[pd]).max())
+                         for pd in ('primal', 'dual')]
+        out['oracle'] = {'source': 'tests/golden/oracle_scalar243.json (float64 CPU oracle, same microstructure)',
+                         'rel_diff': rel,
+                         'kit_equal_fp64': r_fp64['kit'] == {pd: fx['kit_' + pd] for pd in ('primal', 'dual')},
+                         'kit_equal_mixed': r_mixed['kit'] == {pd: fx['kit_' + pd] for pd in ('primal', 'dual')}}
+    return out
 
 
 def measured_traffic(key):
@@ -154,6 +186,7 @@ def bench_elastic(args, nat, ms, dev):
     """81^3 small-strain elasticity: seeded random ellipsoids (~25 %), matrix (E 1, nu 0.3) vs inclusion
     (10, 0.2), 6 Mandel unit strains, primal + dual, fp32 CG + fp64 refinement vs plain fp64; plans and
     coefficients resident, timed: right-hand sides, 12 solves, energies."""
+    from ffthompy_amd.materials import lame_compliance, mandel_to_voigt
     N = (81, 81, 81)
     ind, n_ell = ms.ellipsoid_indicator(N)
     ind = ind.astype(bool)
@@ -171,8 +204,8 @@ def bench_elastic(args, nat, ms, dev):
         if pd == 'primal':
             pl.set_coeff('lame', np.stack([lam, mu]))
         else:
-            # compliance of an isotropic C (Mandel): lambda' = -lambda/(2mu(3lambda+2mu)), mu' = 1/(4mu)
-            pl.set_coeff('lame', np.stack([-lam / (2 * mu * (3 * lam + 2 * mu)), 1. / (4 * mu)]))
+            # the dual coefficient: the pointwise compliance C^-1, again of Lame form
+            pl.set_coeff('lame', np.stack(lame_compliance(lam, mu)))
         plans[pd] = pl
 
     def run(mode):
@@ -217,6 +250,9 @@ def bench_elastic(args, nat, ms, dev):
            'kit_mixed': {k: v[1] for k, v in rm.items()}, 'kit_fp64': {k: v[1] for k, v in r64.items()},
            'AH_upper_mandel_diag': np.round(np.diag(up), 6).tolist(),
            'AH_lower_mandel_diag': np.round(np.diag(lo), 6).tolist(),
+           'AH_upper_voigt_diag': np.round(np.diag(mandel_to_voigt(up)), 6).tolist(),
+           'AH_lower_voigt_diag': np.round(np.diag(mandel_to_voigt(lo)), 6).tolist(),
+           'voigt_from_mandel': 'AH_voigt = T AH_mandel T, T = diag(1, 1, 1, 1/sqrt2, 1/sqrt2, 1/sqrt2)',
            'bounds_rel_diff_mixed_vs_fp64': float(max(
                np.abs(up - r64['primal'][0]).max() / np.abs(r64['primal'][0]).max(),
                np.abs(lo - r64['dual'][0]).max() / np.abs(r64['dual'][0]).max())),
@@ -364,6 +400,7 @@ def main():
                     help='mixed precision: residual reduction that triggers an fp64 reliable update')
     ap.add_argument('--profile-only', action='store_true', help='matvec profile only (for rocprofv3 runs)')
     ap.add_argument('--no-autotune', action='store_true', help='keep the default kernel schedules')
+    ap.add_argument('--no-ga', action='store_true', help='skip the Ga post-processing bounds after the timed steps')
     ap.add_argument('--profile-solve', action='store_true',
                     help='with --profile-only: also one mixed and one fp64 CG solve (load e1, primal)')
     ap.add_argument('--workload', default='scalar243', choices=['scalar243', 'uq512', 'elastic81', 'grad99', 'ga243', 'potential243'],
@@ -456,6 +493,30 @@ def main():
     relp = float(np.abs(r_mixed['AH_primal'] - r_fp64['AH_primal']).max() / np.abs(r_fp64['AH_primal']).max())
     reld = float(np.abs(r_mixed['AH_dual'] - r_fp64['AH_dual']).max() / np.abs(r_fp64['AH_dual']).max())
     kit64 = sum(sum(v) for v in r_fp64['kit'].values())
+    bounds = bounds_report(r_mixed, r_fp64, args.n)
+
+    # --- guaranteed bounds: Ga post-processing of the last step's (fp64) solutions, untimed ---
+    ga = None
+    if not args.no_ga:
+        t = time.perf_counter()
+        gres = {}
+        for pd, coef in (('primal', a), ('dual', 1. / a)):
+            gpp = nat.Ga(H.plans[pd])
+            gpp.set_grid_coeff(0, N, 'iso', coef)  # order 0, P = N: each voxel an exactly integrated box
+            AHg = gpp.energy_matrix(H.sols[pd])
+            gres[pd] = AHg if pd == 'primal' else np.linalg.inv(AHg)
+            M = gpp.M
+            del gpp
+        nat.synchronize()
+        up, lo = gres['primal'], gres['dual']
+        ga = {'kind': 'Ga post-processing, order 0, P = N, of the fp64 solutions (guaranteed bounds of the '
+                      'voxel microstructure, ffthompy/postprocess.py:20-46)',
+              'quadrature_grid': list(M), 'AH_Ga_upper_diag': np.diag(up).tolist(),
+              'AH_Ga_lower_diag': np.diag(lo).tolist(),
+              'min_eig_upper_minus_lower': float(np.linalg.eigvalsh(0.5 * ((up - lo) + (up - lo).T)).min()),
+              'seconds': round(time.perf_counter() - t, 3)}
+        log('Ga bounds: upper %s lower %s (%.2f s)' % (np.round(np.diag(up), 5), np.round(np.diag(lo), 5),
+                                                       ga['seconds']))
 
     # --- CPU baseline: reference-style NumPy path on this host, bounded sample ---
     cpu = None
@@ -505,8 +566,10 @@ def main():
         'kit_mixed': r_mixed['kit'], 'kit_fp64': r_fp64['kit'],
         'reliable_updates': r_mixed['updates'],
         'solve_ms_mixed': r_mixed['solve_ms'], 'solve_ms_fp64': r_fp64['solve_ms'],
-        'AH_upper_fp64': float(r_fp64['AH_primal'][0, 0]), 'AH_lower_fp64': float(r_fp64['AH_dual'][0, 0]),
+        'bounds': bounds,
         'bounds_rel_diff_mixed_vs_fp64': [relp, reld],
+        'bounds_rel_diff_vs_oracle': bounds.get('oracle', {}).get('rel_diff'),
+        'ga_bounds': ga,
         'matvec_GBps_fp32': round(prof['fp32']['matvec_GBps'], 1),
         'matvec_frac_of_hbm_peak_fp32': round(prof['fp32']['matvec_GBps'] / HBM_PEAK_GBS, 4),
         'matvec_GBps_fp64': round(prof['fp64']['matvec_GBps'], 1),

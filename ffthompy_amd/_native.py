@@ -89,6 +89,7 @@ SIGNATURES = {
                       _c_double_p, ctypes.c_int],
     'fh_plan_set_residual_log': [_vp, ctypes.c_int],
     'fh_last_residual_log': [_vp, _c_int_p, _c_double_p, ctypes.c_int, _c_int_p],
+    'fh_plan_set_iterate_callback': [_vp, _vp, _vp, ctypes.c_int],
     'fh_potential_solve': [_vp, ctypes.c_int, _c_double_p, ctypes.c_double, ctypes.c_int, ctypes.c_double, _vp, _vp,
                            _c_int_p, _c_double_p, _c_double_p, ctypes.c_int, _c_int_p],
     'fh_set_coeff_inclusions': [_vp, ctypes.c_int, _c_int_p, _c_double_p, _c_double_p, ctypes.c_int, _c_double_p,
@@ -98,6 +99,10 @@ SIGNATURES = {
     'fh_grf': [_vp, _vp, ctypes.c_double, ctypes.c_double, ctypes.c_int, _vp],
     'fh_synchronize': [],
 }
+
+
+# int fn(void* user, int kit, const void* x): the library's host callback on the iterate
+ITERATE_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p)
 
 
 class FhError(RuntimeError):
@@ -214,6 +219,18 @@ class DeviceArray:
         assert other.nbytes == self.nbytes
         call('fh_memcpy_dtod', self.ptr, other.ptr, ctypes.c_size_t(self.nbytes))
         return self
+
+    @classmethod
+    def borrow(cls, ptr, shape, dtype=np.float64):
+        """A non-owning view of device memory the library owns (e.g. the iterate handed to an
+        iterate callback): valid only while the library keeps it."""
+        v = object.__new__(cls)
+        v.shape = tuple(int(s) for s in shape)
+        v.dtype = np.dtype(dtype)
+        v.nbytes = int(np.prod(v.shape, dtype=np.int64)) * v.dtype.itemsize
+        v._ptr = ctypes.c_void_p(ptr)
+        v._owner = True  # not ours to free
+        return v
 
     def reshape(self, shape):
         """A view with another shape (same memory, keeps the owner alive)."""
@@ -385,6 +402,34 @@ class Plan:
     def set_residual_log(self, every):
         """fp64 CG: log ||b - A x|| every `every` iterations (0: off), fh_plan_set_residual_log."""
         call('fh_plan_set_residual_log', self._h, ctypes.c_int(int(every)))
+
+    def set_iterate_callback(self, fn, every=1):
+        """fn(kit, x) after every `every`-th iteration of fh_cg_solve (fp64) / fh_iter_solve and after
+        the last one, x a borrowed DeviceArray view of the current iterate ((ncomp,)+N, fp64); fn=None
+        clears it. An exception in fn stops the solve and is re-raised by the solve's caller through
+        pop_callback_error()."""
+        if fn is None or every <= 0:
+            call('fh_plan_set_iterate_callback', self._h, None, None, ctypes.c_int(0))
+            self._cfn = None
+            return
+        shape = (self.ncomp,) + self.N
+        self._cb_error = None
+
+        def tramp(user, kit, xptr):
+            try:
+                fn(int(kit), DeviceArray.borrow(xptr, shape))
+                return 0
+            except BaseException as e:  # handed back to Python after the solve returns
+                self._cb_error = e
+                return 1
+
+        self._cfn = ITERATE_FN(tramp)
+        call('fh_plan_set_iterate_callback', self._h, ctypes.cast(self._cfn, ctypes.c_void_p), None,
+             ctypes.c_int(int(every)))
+
+    def pop_callback_error(self):
+        e, self._cb_error = getattr(self, '_cb_error', None), None
+        return e
 
     def residual_log(self):
         """(iterations, norms) of the last solve's residual log."""

@@ -112,10 +112,12 @@ int ensure_device() {
   return FH_OK;
 }
 
-// Radix schedule: 4s first, then 2, 3, 5, 7, 11, 13.
+// Radix schedule: 4s first, then 2, 3, 5, 7, 11, 13. False if L has a larger prime factor.
 bool factorize(int L, AxisPlan* ap) {
   std::memset(ap, 0, sizeof(*ap));
   ap->L = L;
+  ap->M = L;
+  ap->blue = 0;
   int rem = L, st = 0, pre = 1;
   const int rads[] = {4, 2, 3, 5, 7, 11, 13};
   for (int R : rads) {
@@ -132,11 +134,86 @@ bool factorize(int L, AxisPlan* ap) {
   return rem == 1;
 }
 
+bool smooth7(int n) {
+  for (int p : {2, 3, 5, 7})
+    while (n % p == 0) n /= p;
+  return n == 1;
+}
+
+// Schedule of an axis of any length: the radix stages of L, or for a length with a prime factor
+// above 13 Bluestein's convolution on the smallest 7-smooth M >= 2L - 1 (fh_fft.hpp, fft_lds).
+// Bluestein lengths are capped at kMaxBluestein: a pencil of a longer convolution no longer fits
+// one workgroup's LDS in double precision (2 or more pencils of 16-byte values in 160 KiB).
+constexpr int kMaxBluestein = 4096;
+
+bool plan_axis(int L, AxisPlan* ap) {
+  if (factorize(L, ap)) return true;
+  int M = 2 * L - 1;
+  while (!smooth7(M)) ++M;
+  if (M > kMaxBluestein) return false;
+  if (!factorize(M, ap)) return false;
+  ap->L = L;
+  ap->M = M;
+  ap->blue = 1;
+  return true;
+}
+
+// Twiddle table of an axis (see fft_lds): W_M^m (m < M), and with Bluestein the chirp
+// w_m = exp(-i pi m^2 / L) (m < L) and the spectra FFT_M(conj w)/M, FFT_M(w)/M of the periodically
+// extended convolution kernel; complex values as (re, im) pairs, computed in long double.
+std::vector<long double> axis_table(const AxisPlan& ap) {
+  const long double pi = 3.14159265358979323846264338327950288L;
+  const int L = ap.L, M = ap.M;
+  std::vector<long double> t(2 * (size_t)(ap.blue ? 3 * M + L : M));
+  for (int m = 0; m < M; ++m) {
+    const long double ang = -2.0L * pi * (long double)m / (long double)M;
+    t[2 * m] = cosl(ang);
+    t[2 * m + 1] = sinl(ang);
+  }
+  if (!ap.blue) return t;
+  std::vector<long double> wr(L), wi(L);
+  for (int m = 0; m < L; ++m) {
+    const long long r = ((long long)m * m) % (2LL * L);  // w_m depends on m^2 mod 2L only
+    const long double ang = -pi * (long double)r / (long double)L;
+    wr[m] = cosl(ang);
+    wi[m] = sinl(ang);
+    t[2 * (M + m)] = wr[m];
+    t[2 * (M + m) + 1] = wi[m];
+  }
+  // kernel b_j = conj(w_|j|) for |j| < L, placed circularly on M points
+  std::vector<long double> br(M, 0.0L), bi(M, 0.0L);
+  for (int m = 0; m < L; ++m) {
+    br[m] = wr[m];
+    bi[m] = -wi[m];
+    if (m > 0) {
+      br[M - m] = wr[m];
+      bi[M - m] = -wi[m];
+    }
+  }
+  const size_t o1 = 2 * (size_t)(M + L), o2 = o1 + 2 * (size_t)M;
+  for (int k = 0; k < M; ++k) {
+    long double fr = 0.0L, fi = 0.0L, gr = 0.0L, gi = 0.0L;  // FFT_M(b), FFT_M(conj b)
+    for (int j = 0; j < M; ++j) {
+      const long long r = ((long long)j * k) % M;
+      const long double c = t[2 * r], sn = t[2 * r + 1];  // W_M^{jk}
+      fr += br[j] * c - bi[j] * sn;
+      fi += br[j] * sn + bi[j] * c;
+      gr += br[j] * c + bi[j] * sn;
+      gi += br[j] * sn - bi[j] * c;
+    }
+    t[o1 + 2 * k] = fr / M;
+    t[o1 + 2 * k + 1] = fi / M;
+    t[o2 + 2 * k] = gr / M;
+    t[o2 + 2 * k + 1] = gi / M;
+  }
+  return t;
+}
+
 int fft_threads(const AxisPlan& ap, int npen) {
   long long need = 64;
   for (int s = 0; s < ap.nst; ++s) {
     const int R = ap.rad[s];
-    const long long bf = (long long)npen * (ap.L / R);
+    const long long bf = (long long)npen * (ap.M / R);
     const long long mb = fh::max_butterflies(R);
     need = std::max(need, (bf + mb - 1) / mb);
   }
@@ -196,6 +273,9 @@ struct fh_plan {
   int proj_var[2] = {0, 0};      // schedule variants of the fixed-length passes: [fp64, fp32]
   int cols_var[2] = {0, 0};
   int res_every = 0;             // fp64 CG: log ||b - A x|| every res_every iterations (0: off)
+  fh_iterate_fn it_fn = nullptr; // host callback on the iterate every it_every iterations (0: off)
+  void* it_user = nullptr;
+  int it_every = 0;
   std::vector<int> res_kit;      // the log of the last solve: iteration counts and norms
   std::vector<double> res_norm;
 };
@@ -300,18 +380,27 @@ int fixed_result(int r, const char* what) {
   return FH_OK;
 }
 
+// Launch of a runtime-radix kernel: LDS tiles above 64 KiB (long Bluestein pencils) need the
+// kernel's dynamic-LDS limit raised first.
+template <typename... KArgs, typename... Args>
+void launch_gen(void (*kernel)(KArgs...), dim3 grid, dim3 block, size_t lds, hipStream_t stream, Args... args) {
+  if (lds > 64 * 1024)
+    (void)hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL(kernel, grid, block, lds, stream, args...);
+}
+
 int row_config(const AxisPlan& ap, int nc, size_t csz, long long nrows, RowCfg* c) {
-  const int L = ap.L;
+  const int L = ap.L, W = ap.M;  // W: pencil length in LDS (Bluestein: the convolution length)
   int best = 2;
   for (int rp = 2; rp <= 32; rp += 2) {
-    const size_t lds = (size_t)(rp / 2) * nc * L * csz;
+    const size_t lds = (size_t)(rp / 2) * nc * W * csz;
     const int thr = fft_threads(ap, (rp / 2) * nc);
     if (rp > 2 && (lds > kLdsTarget || thr > 1024)) break;
     best = rp;
     if (rp >= nrows) break;
   }
   c->rows_pb = best;
-  c->lds = (size_t)(best / 2) * nc * L * csz;
+  c->lds = (size_t)(best / 2) * nc * W * csz;
   c->nthr = std::max(fft_threads(ap, (best / 2) * nc), 128);
   c->nthr = std::min(c->nthr, 1024);
   if (fft_threads(ap, (best / 2) * nc) > 1024 || c->lds > kLdsMax)
@@ -321,14 +410,14 @@ int row_config(const AxisPlan& ap, int nc, size_t csz, long long nrows, RowCfg* 
 }
 
 int col_config(const AxisPlan& ap, int unit, size_t csz, long long inner, ColCfg* c) {
-  const int L = ap.L;
-  long long P = (long long)(kLdsTarget / ((size_t)L * csz)) / unit * unit;
+  const int L = ap.L, W = ap.M;  // W: pencil length in LDS (Bluestein: the convolution length)
+  long long P = (long long)(kLdsTarget / ((size_t)W * csz)) / unit * unit;
   if (P < unit) P = unit;
   const long long cap = (inner + unit - 1) / unit * unit;
   if (P > cap) P = cap;
   while (P > unit && fft_threads(ap, (int)P) > 1024) P -= unit;
   c->P = (int)P;
-  c->lds = (size_t)L * P * csz;
+  c->lds = (size_t)W * P * csz;
   const int need = fft_threads(ap, (int)P);
   if (need > 1024 || c->lds > kLdsMax)
     return fail(FH_ERR_UNSUPPORTED, "grid axis of length " + std::to_string(L) + " does not fit one workgroup");
@@ -355,10 +444,10 @@ int launch_rows_fwd(fh_plan* pl, int nc, const T* in, const T* coef, int kind, c
 #define FH_GEN_RF(KIND)                                                                                        \
   do {                                                                                                         \
     if (rvec)                                                                                                  \
-      hipLaunchKernelGGL((fh::k_rows_fwd<T, KIND, 1>), grid, block, c.lds, pl->stream, io, coef, out, nrows, nc, \
+      launch_gen(fh::k_rows_fwd<T, KIND, 1>, grid, block, c.lds, pl->stream, io, coef, out, nrows, nc, \
                          pl->n, c.rows_pb, pl->ax[2], tw, halt, rvec, pl->st);                                 \
     else                                                                                                       \
-      hipLaunchKernelGGL((fh::k_rows_fwd<T, KIND, 0>), grid, block, c.lds, pl->stream, io, coef, out, nrows, nc, \
+      launch_gen(fh::k_rows_fwd<T, KIND, 0>, grid, block, c.lds, pl->stream, io, coef, out, nrows, nc, \
                          pl->n, c.rows_pb, pl->ax[2], tw, halt, rvec, pl->st);                                 \
   } while (0)
   switch (kind) {
@@ -390,10 +479,10 @@ int launch_rows_inv(fh_plan* pl, int nc, const cpx<T>* in, T* out, T scale, cons
   dim3 grid(c.nblocks), block(c.nthr);
   const cpx<T>* tw = twiddles<T>(pl, 2);
   if (w)
-    hipLaunchKernelGGL((fh::k_rows_inv<T, 1>), grid, block, c.lds, pl->stream, in, out, nrows, nc, pl->n,
+    launch_gen(fh::k_rows_inv<T, 1>, grid, block, c.lds, pl->stream, in, out, nrows, nc, pl->n,
                        c.rows_pb, pl->ax[2], tw, scale, w, partial, halt, pl->st);
   else
-    hipLaunchKernelGGL((fh::k_rows_inv<T, 0>), grid, block, c.lds, pl->stream, in, out, nrows, nc, pl->n,
+    launch_gen(fh::k_rows_inv<T, 0>, grid, block, c.lds, pl->stream, in, out, nrows, nc, pl->n,
                        c.rows_pb, pl->ax[2], tw, scale, w, partial, halt, pl->st);
   HIPCHK(hipGetLastError());
   if (nblocks) *nblocks = c.nblocks;
@@ -426,13 +515,13 @@ int launch_cols(fh_plan* pl, int mode, int axis, int nc, cpx<T>* data, const GPa
   dim3 grid((unsigned)nb), block(c.nthr);
   const cpx<T>* tw = twiddles<T>(pl, axis);
   if (mode == 0)
-    hipLaunchKernelGGL((fh::k_cols<T, 0>), grid, block, c.lds, pl->stream, data, ostride, inner, c.P, c.ntiles,
+    launch_gen(fh::k_cols<T, 0>, grid, block, c.lds, pl->stream, data, ostride, inner, c.P, c.ntiles,
                        ap, tw, g, pl->n2h, halt);
   else if (mode == 1)
-    hipLaunchKernelGGL((fh::k_cols<T, 1>), grid, block, c.lds, pl->stream, data, ostride, inner, c.P, c.ntiles,
+    launch_gen(fh::k_cols<T, 1>, grid, block, c.lds, pl->stream, data, ostride, inner, c.P, c.ntiles,
                        ap, tw, g, pl->n2h, halt);
   else
-    hipLaunchKernelGGL((fh::k_cols<T, 2>), grid, block, c.lds, pl->stream, data, ostride, inner, c.P, c.ntiles,
+    launch_gen(fh::k_cols<T, 2>, grid, block, c.lds, pl->stream, data, ostride, inner, c.P, c.ntiles,
                        ap, tw, g, pl->n2h, halt);
   HIPCHK(hipGetLastError());
   return FH_OK;
@@ -720,15 +809,17 @@ static int scratch_parts(double** out) {
 
 // Quadrature grid length for the Ga energy on an axis of N points: 2N-1 (the reference's
 // Nbar, ffthompy/postprocess.py:24) when its prime factors are <= 13, else for odd N the next
-// such length (the energy is exact on any length >= 2N-1 there); -1 when none applies.
+// such length (the energy is exact on any length >= 2N-1 there, and the radix kernels are faster
+// than Bluestein's), and for even N 2N-1 itself (only that grid reproduces the reference's aliased
+// quadrature; its transforms use Bluestein's algorithm).
 int quadrature_length(int N) {
   const int nb = 2 * N - 1;
   AxisPlan ap;
   if (factorize(nb, &ap)) return nb;
-  if ((N & 1) == 0) return -1;
+  if ((N & 1) == 0) return nb;
   for (int L = nb + 1; L < 2 * nb + 16; ++L)
     if (factorize(L, &ap)) return L;
-  return -1;
+  return nb;
 }
 
 // Inverse real FFT of one spectrum component held in `spec` (the plan's half-spectrum layout,
@@ -988,20 +1079,17 @@ int fh_plan_create(int dim, const int* N, const double* Y, int physics, int prim
     return code;
   };
   for (int a = 0; a < 3; ++a) {
-    if (!factorize(lens[a], &pl->ax[a])) {
+    if (!plan_axis(lens[a], &pl->ax[a])) {
       fh_plan_destroy(pl);
-      return fail(FH_ERR_UNSUPPORTED,
-                  "grid length " + std::to_string(lens[a]) + " has a prime factor above 13");
+      return fail(FH_ERR_UNSUPPORTED, "grid length " + std::to_string(lens[a]) +
+                                          ": its Bluestein convolution does not fit one workgroup");
     }
-    const int L = lens[a];
-    std::vector<double> t64(2 * (size_t)L);
-    std::vector<float> t32(2 * (size_t)L);
-    for (int m = 0; m < L; ++m) {
-      const long double ang = -2.0L * 3.14159265358979323846264338327950288L * (long double)m / (long double)L;
-      t64[2 * m] = (double)cosl(ang);
-      t64[2 * m + 1] = (double)sinl(ang);
-      t32[2 * m] = (float)t64[2 * m];
-      t32[2 * m + 1] = (float)t64[2 * m + 1];
+    const std::vector<long double> tl = axis_table(pl->ax[a]);
+    std::vector<double> t64(tl.size());
+    std::vector<float> t32(tl.size());
+    for (size_t m = 0; m < tl.size(); ++m) {
+      t64[m] = (double)tl[m];
+      t32[m] = (float)tl[m];
     }
     if (hipMalloc(&pl->tw64[a], t64.size() * sizeof(double)) != hipSuccess ||
         hipMalloc(&pl->tw32[a], t32.size() * sizeof(float)) != hipSuccess)
@@ -1077,6 +1165,15 @@ int fh_plan_set_residual_log(fh_plan* pl, int every) {
   RETCHK(check_plan(pl));
   if (every < 0) return fail(FH_ERR_ARG, "every must be >= 0");
   pl->res_every = every;
+  return FH_OK;
+}
+
+int fh_plan_set_iterate_callback(fh_plan* pl, fh_iterate_fn fn, void* user, int every) {
+  RETCHK(check_plan(pl));
+  if (every < 0) return fail(FH_ERR_ARG, "every must be >= 0");
+  pl->it_fn = every > 0 ? fn : nullptr;
+  pl->it_user = user;
+  pl->it_every = fn ? every : 0;
   return FH_OK;
 }
 
@@ -1209,6 +1306,8 @@ int fh_cg_solve(fh_plan* pl, int mode, const void* b_, void* x_, double tol, int
   if (mode == FH_SOLVE_MIXED && !(delta > 0.0 && delta < 1.0)) return fail(FH_ERR_ARG, "delta must be in (0,1)");
   if (mode == FH_SOLVE_MIXED && pl->res_every > 0)
     return fail(FH_ERR_UNSUPPORTED, "the residual log (fh_plan_set_residual_log) is kept by fp64 solves only");
+  if (mode == FH_SOLVE_MIXED && pl->it_every > 0)
+    return fail(FH_ERR_UNSUPPORTED, "the iterate callback (fh_plan_set_iterate_callback) runs in fp64 solves only");
   pl->res_kit.clear();
   pl->res_norm.clear();
   const double* b = (const double*)b_;
@@ -1259,15 +1358,35 @@ int fh_cg_solve(fh_plan* pl, int mode, const void* b_, void* x_, double tol, int
       return FH_OK;
     };
     if (every > 0) RETCHK(log_true(0));
+    // host callback on the iterate (the reference's callback(xCG) after every iteration,
+    // ffthompy/general/solver.py:134-136): x is current after each synchronisation
+    const int cb_every = pl->it_fn ? pl->it_every : 0;
+    int cb_last = 0;
     int kit_host = 0;
     for (;;) {
-      // batches end on the logged iterations (multiples of every), at most per_check long
-      const int batch = every > 0 ? std::min(per_check, every - kit_host % every) : per_check;
+      // batches end on the logged / called-back iterations (multiples of every, cb_every), at
+      // most per_check long
+      int batch = per_check;
+      if (every > 0) batch = std::min(batch, every - kit_host % every);
+      if (cb_every > 0) batch = std::min(batch, cb_every - kit_host % cb_every);
       for (int k = 0; k < batch; ++k) RETCHK(enqueue_iteration<double>(pl, x, r, p, Ap, pl->coef64));
       HIPCHK(hipMemcpyAsync(&s, pl->st, sizeof(s), hipMemcpyDeviceToHost, pl->stream));
       HIPCHK(hipStreamSynchronize(pl->stream));
       kit_host = s.kit;
+      if (s.done && x0_zero) {
+        // x <- G x once at the end: the recurrence leaves x outside the range of G by its accumulated
+        // rounding, which enters the energy (and the bound ordering) at first order; the projection
+        // removes it (x changes by that rounding only). With a nonzero x0 its component outside the
+        // range of G belongs to the reference's result and is kept.
+        RETCHK(run_operator<double>(pl, x, Ap, fh::COEF_NONE, nullptr, nullptr, nullptr, nullptr));
+        HIPCHK(hipMemcpyAsync(x, Ap, (size_t)nv * sizeof(double), hipMemcpyDeviceToDevice, pl->stream));
+        HIPCHK(hipStreamSynchronize(pl->stream));
+      }
       if (every > 0 && s.kit > logged && (s.kit % every == 0 || s.done)) RETCHK(log_true(s.kit));
+      if (cb_every > 0 && s.kit > cb_last && (s.kit % cb_every == 0 || s.done)) {
+        cb_last = s.kit;
+        if (pl->it_fn(pl->it_user, s.kit, x) != 0) return fail(FH_ERR_STATE, "solve stopped by the iterate callback");
+      }
       if (s.done) break;
     }
   } else {
@@ -2245,16 +2364,30 @@ int fh_iter_solve(fh_plan* pl, int method, const void* b_, void* x_, const doubl
   HIPCHK(hipMalloc(&st, sizeof(s)));
   int rc = FH_OK;
   double norm_init = s.norm;  // before the first iteration: 1e15 (Richardson) or |r0| / |b| (Chebyshev)
+  // host callback on the iterate (the reference's callback(x) after every step, solver.py:75-76,
+  // 277-278): the batches end on its iterations
+  const int cb_every = pl->it_fn ? pl->it_every : 0;
+  int cb_last = 0;
+  auto batch_len = [&](int done_iters) {
+    return cb_every > 0 ? std::min(8, cb_every - done_iters % cb_every) : 8;
+  };
+  auto call_back = [&](int limit) -> int {
+    if (cb_every > 0 && s.kit > cb_last && (s.kit % cb_every == 0 || s.done || s.kit >= limit)) {
+      cb_last = s.kit;
+      if (pl->it_fn(pl->it_user, s.kit, x) != 0) return fail(FH_ERR_STATE, "solve stopped by the iterate callback");
+    }
+    return FH_OK;
+  };
   auto run = [&]() -> int {
     const int gb = grid_blocks(nv);
     int* halt = &st->halt;
-    const int per_check = 8;
     if (method == FH_ITER_RICHARDSON) {
       if (!(coefs[0] != 0.0)) return fail(FH_ERR_ARG, "alpha must be nonzero");
       const double omega = 1.0 / coefs[0];
       s.done = s.halt = maxiter == 0;
       HIPCHK(hipMemcpy(st, &s, sizeof(s), hipMemcpyHostToDevice));
       for (int k = 0; !s.done && k < maxiter;) {
+        const int per_check = batch_len(k);
         for (int j = 0; j < per_check && k < maxiter; ++j, ++k) {
           RETCHK(run_operator<double>(pl, x, Ax, pl->coef_kind, pl->coef64, nullptr, halt, nullptr));
           hipLaunchKernelGGL(fh::it::k_it_residual, dim3(gb), dim3(256), 0, pl->stream, nv, b, Ax, r, pl->partial,
@@ -2265,6 +2398,7 @@ int fh_iter_solve(fh_plan* pl, int method, const void* b_, void* x_, const doubl
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(&s, st, sizeof(s), hipMemcpyDeviceToHost, pl->stream));
         HIPCHK(hipStreamSynchronize(pl->stream));
+        RETCHK(call_back(maxiter));
       }
       return FH_OK;
     }
@@ -2303,6 +2437,7 @@ int fh_iter_solve(fh_plan* pl, int method, const void* b_, void* x_, const doubl
     const double d = (e1 + e0) / 2.0, c = (e1 - e0) / 2.0;
     double w = 0.0;
     for (int k = 1; !s.done && k <= maxiter;) {
+      const int per_check = batch_len(k - 1);
       for (int j = 0; j < per_check && k <= maxiter; ++j, ++k) {
         double pk;
         if (k == 1) {
@@ -2324,6 +2459,7 @@ int fh_iter_solve(fh_plan* pl, int method, const void* b_, void* x_, const doubl
       HIPCHK(hipGetLastError());
       HIPCHK(hipMemcpyAsync(&s, st, sizeof(s), hipMemcpyDeviceToHost, pl->stream));
       HIPCHK(hipStreamSynchronize(pl->stream));
+      RETCHK(call_back(maxiter));
     }
     return FH_OK;
   };

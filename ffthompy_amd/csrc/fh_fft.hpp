@@ -6,7 +6,8 @@
 // applies twiddles and the radix-R DFT, then, after a barrier, stores the R
 // outputs in place. Radices 2, 3, 4, 5, 7, 11, 13 are supported, which covers
 // every grid length whose prime factors are at most 13 (the odd grids of the
-// Fourier-Galerkin method: 5, 9, 15, 45, 63, 81, 99, 105, 135, 243, ...).
+// Fourier-Galerkin method: 5, 9, 15, 45, 63, 81, 99, 105, 135, 243, ...); any other
+// length goes through Bluestein's chirp convolution on a 7-smooth length (fft_lds).
 //
 // The transform is unnormalised, forward sign exp(-2 pi i jk/L), like
 // numpy.fft.fft, which the reference calls through ffthompy/tensors/fft.py:33-43.
@@ -167,9 +168,16 @@ __device__ __forceinline__ void dft(cpx<T> (&v)[R]) {
 constexpr int kMaxStages = 24;
 
 // Radix schedule of one axis length; pre[s] is the product of the radices
-// before stage s (the Stockham sub-transform length).
+// before stage s (the Stockham sub-transform length). A length L with a prime
+// factor above 13 is transformed by Bluestein's algorithm (blue = 1): the DFT is the
+// chirp-weighted circular convolution X_k = w_k sum_j (x_j w_j) conj(w_{k-j}),
+// w_m = exp(-i pi m^2 / L), evaluated with forward and inverse transforms of a
+// length M >= 2L-1 whose prime factors are at most 7; the stages then describe M.
+// Otherwise M = L and blue = 0. Pencils in LDS are M elements long.
 struct AxisPlan {
   int L;
+  int M;
+  int blue;
   int nst;
   int rad[kMaxStages];
   int pre[kMaxStages];
@@ -235,22 +243,70 @@ __device__ __forceinline__ void fft_stage(cpx<T>* s, int sp, int sj, int npen, i
   __syncthreads();
 }
 
+// The Stockham stages of ap (length ap.M) in direction DIR.
 template <typename T, int DIR>
-__device__ void fft_lds(cpx<T>* s, int sp, int sj, int npen, const AxisPlan& ap,
-                        const cpx<T>* __restrict__ tw, int tid, int nthr, bool rowmap) {
+__device__ void fft_stages(cpx<T>* s, int sp, int sj, int npen, const AxisPlan& ap,
+                           const cpx<T>* __restrict__ tw, int tid, int nthr, bool rowmap) {
   for (int st = 0; st < ap.nst; ++st) {
     const int R = ap.rad[st], p = ap.pre[st];
     switch (R) {
-      case 2: fft_stage<T, 2, DIR>(s, sp, sj, npen, ap.L, p, tw, tid, nthr, rowmap); break;
-      case 3: fft_stage<T, 3, DIR>(s, sp, sj, npen, ap.L, p, tw, tid, nthr, rowmap); break;
-      case 4: fft_stage<T, 4, DIR>(s, sp, sj, npen, ap.L, p, tw, tid, nthr, rowmap); break;
-      case 5: fft_stage<T, 5, DIR>(s, sp, sj, npen, ap.L, p, tw, tid, nthr, rowmap); break;
-      case 7: fft_stage<T, 7, DIR>(s, sp, sj, npen, ap.L, p, tw, tid, nthr, rowmap); break;
-      case 11: fft_stage<T, 11, DIR>(s, sp, sj, npen, ap.L, p, tw, tid, nthr, rowmap); break;
-      case 13: fft_stage<T, 13, DIR>(s, sp, sj, npen, ap.L, p, tw, tid, nthr, rowmap); break;
+      case 2: fft_stage<T, 2, DIR>(s, sp, sj, npen, ap.M, p, tw, tid, nthr, rowmap); break;
+      case 3: fft_stage<T, 3, DIR>(s, sp, sj, npen, ap.M, p, tw, tid, nthr, rowmap); break;
+      case 4: fft_stage<T, 4, DIR>(s, sp, sj, npen, ap.M, p, tw, tid, nthr, rowmap); break;
+      case 5: fft_stage<T, 5, DIR>(s, sp, sj, npen, ap.M, p, tw, tid, nthr, rowmap); break;
+      case 7: fft_stage<T, 7, DIR>(s, sp, sj, npen, ap.M, p, tw, tid, nthr, rowmap); break;
+      case 11: fft_stage<T, 11, DIR>(s, sp, sj, npen, ap.M, p, tw, tid, nthr, rowmap); break;
+      case 13: fft_stage<T, 13, DIR>(s, sp, sj, npen, ap.M, p, tw, tid, nthr, rowmap); break;
       default: break;  // rejected at plan creation
     }
   }
+}
+
+// Length-L DFT (DIR -1 forward, +1 inverse, unnormalised) of npen pencils in LDS, in place;
+// pencil q, element j at s[q*sp + j*sj]. The caller has synchronised the workgroup after writing
+// the input; the output is visible to the whole workgroup on return.
+// Twiddle table tw: W_M^m for m < M; with Bluestein (ap.blue) followed by the chirp w_m (m < L)
+// and the spectra of the convolution kernel for both directions, FFT_M(conj w) / M and
+// FFT_M(w) / M (M values each, the kernel extended periodically: index M - m holds index m).
+template <typename T, int DIR>
+__device__ void fft_lds(cpx<T>* s, int sp, int sj, int npen, const AxisPlan& ap,
+                        const cpx<T>* __restrict__ tw, int tid, int nthr, bool rowmap) {
+  if (!ap.blue) {
+    fft_stages<T, DIR>(s, sp, sj, npen, ap, tw, tid, nthr, rowmap);
+    return;
+  }
+  const int L = ap.L, M = ap.M;
+  const cpx<T>* chirp = tw + M;
+  const cpx<T>* kern = tw + M + L + (DIR < 0 ? 0 : M);
+  // a_j = x_j w_j (conj w_j for the inverse), zero-padded to M
+  for (int e = tid; e < npen * M; e += nthr) {
+    const int q = e / M, j = e - q * M;
+    cpx<T>* a = s + q * sp + j * sj;
+    if (j < L) {
+      cpx<T> w = chirp[j];
+      if (DIR > 0) w.y = -w.y;
+      *a = cmul(*a, w);
+    } else {
+      *a = mk(T(0), T(0));
+    }
+  }
+  __syncthreads();
+  fft_stages<T, -1>(s, sp, sj, npen, ap, tw, tid, nthr, rowmap);
+  for (int e = tid; e < npen * M; e += nthr) {
+    const int q = e / M, k = e - q * M;
+    cpx<T>* a = s + q * sp + k * sj;
+    *a = cmul(*a, kern[k]);
+  }
+  __syncthreads();
+  fft_stages<T, +1>(s, sp, sj, npen, ap, tw, tid, nthr, rowmap);
+  for (int e = tid; e < npen * L; e += nthr) {
+    const int q = e / L, k = e - q * L;
+    cpx<T>* a = s + q * sp + k * sj;
+    cpx<T> w = chirp[k];
+    if (DIR > 0) w.y = -w.y;
+    *a = cmul(*a, w);
+  }
+  __syncthreads();
 }
 
 }  // namespace fh

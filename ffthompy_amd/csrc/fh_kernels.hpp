@@ -288,7 +288,7 @@ k_rows_fwd(T* __restrict__ in, const T* __restrict__ coef, cpx<T>* __restrict__ 
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<T>* s = reinterpret_cast<cpx<T>*>(smem_raw);
   T* sr = reinterpret_cast<T*>(smem_raw);
-  const int L = ap.L, n2h = L / 2 + 1;
+  const int L = ap.L, n2h = L / 2 + 1, M = ap.M;  // M: pencil stride in LDS (>= L)
   const int tid = threadIdx.x, nthr = blockDim.x;
   const long long row0 = (long long)xcd_remap(blockIdx.x, gridDim.x) * rows_pb;
   const int nr = (int)min((long long)rows_pb, nrows - row0);
@@ -322,19 +322,19 @@ k_rows_fwd(T* __restrict__ in, const T* __restrict__ coef, cpx<T>* __restrict__ 
       if (c < nc) {
         const int rr = half * nc + c;
         const int pen = rp * nc + (rr >> 1);
-        sr[2 * (pen * L + x) + (rr & 1)] = y[c];
+        sr[2 * (pen * M + x) + (rr & 1)] = y[c];
       }
     }
   }
   __syncthreads();
-  fft_lds<T, -1>(s, L, 1, npen, ap, tw, tid, nthr, true);
+  fft_lds<T, -1>(s, M, 1, npen, ap, tw, tid, nthr, true);
   const int nrp = rows_pb >> 1;
   for (int e = tid; e < nrp * n2h; e += nthr) {
     const int rp = e / n2h, k = e - rp * n2h;
     const int kn = (k == 0) ? 0 : L - k;
     for (int m = 0; m < nc; ++m) {
-      const cpx<T> Z = s[(rp * nc + m) * L + k];
-      const cpx<T> Zc = conj(s[(rp * nc + m) * L + kn]);
+      const cpx<T> Z = s[(rp * nc + m) * M + k];
+      const cpx<T> Zc = conj(s[(rp * nc + m) * M + kn]);
       const cpx<T> Ya = T(0.5) * (Z + Zc);
       const cpx<T> dz = Z - Zc;
       const cpx<T> Yb = mk(T(0.5) * dz.y, T(-0.5) * dz.x);
@@ -360,7 +360,7 @@ k_rows_inv(const cpx<T>* __restrict__ in, T* __restrict__ out, long long nrows, 
   if (halt && *halt) return;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<T>* s = reinterpret_cast<cpx<T>*>(smem_raw);
-  const int L = ap.L, n2h = L / 2 + 1;
+  const int L = ap.L, n2h = L / 2 + 1, M = ap.M;  // M: pencil stride in LDS (>= L)
   const int tid = threadIdx.x, nthr = blockDim.x;
   const long long row0 = (long long)xcd_remap(blockIdx.x, gridDim.x) * rows_pb;
   const int nr = (int)min((long long)rows_pb, nrows - row0);
@@ -381,13 +381,13 @@ k_rows_inv(const cpx<T>* __restrict__ in, T* __restrict__ out, long long nrows, 
         Ya.y = T(0);
         Yb.y = T(0);
       }
-      cpx<T>* z = s + (rp * nc + m) * L;
+      cpx<T>* z = s + (rp * nc + m) * M;
       z[k] = mk(Ya.x - Yb.y, Ya.y + Yb.x);
       if (!self) z[L - k] = mk(Ya.x + Yb.y, Yb.x - Ya.y);
     }
   }
   __syncthreads();
-  fft_lds<T, +1>(s, L, 1, npen, ap, tw, tid, nthr, true);
+  fft_lds<T, +1>(s, M, 1, npen, ap, tw, tid, nthr, true);
   const T* sr = reinterpret_cast<const T*>(smem_raw);
   double acc = 0.0;
   for (int e = tid; e < rows_pb * L; e += nthr) {
@@ -398,7 +398,7 @@ k_rows_inv(const cpx<T>* __restrict__ in, T* __restrict__ out, long long nrows, 
       for (int c = 0; c < nc; ++c) {
         const int rr = half * nc + c;
         const int pen = rp * nc + (rr >> 1);
-        const T val = scale * sr[2 * (pen * L + x) + (rr & 1)];
+        const T val = scale * sr[2 * (pen * M + x) + (rr & 1)];
         out[c * n + vox] = val;
         if constexpr (EPI == 1) acc += (double)val * (double)w[c * n + vox];
       }

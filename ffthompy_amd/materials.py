@@ -278,3 +278,24 @@ class ElasticTensor:
     @staticmethod
     def from_young(E, nu):
         return ElasticTensor(bulk=E / (3 * (1 - 2 * nu)), mu=E / (2 * (1 + nu)))
+
+
+def lame_compliance(lam, mu):
+    """Lame-form parameters (lambda', mu') of the compliance of C = lambda m m^T + 2 mu I (Mandel,
+    m = (1,1,1,0,0,0)): C^-1 = lambda' m m^T + 2 mu' I with mu' = 1/(4 mu) and
+    lambda' = -lambda / (2 mu (3 lambda + 2 mu)) (Sherman-Morrison on the rank-one term). The dual
+    problem's coefficient is this pointwise inverse (ffthompy/materials.py:121-122,
+    ffthompy/trigpol.py:120-159); works on arrays."""
+    lam = np.asarray(lam, dtype=np.float64)
+    mu = np.asarray(mu, dtype=np.float64)
+    return -lam / (2. * mu * (3. * lam + 2. * mu)), 1. / (4. * mu)
+
+
+# Mandel -> Voigt for a stiffness-like 6x6 matrix (ffthompy/mechanics/matcoef.py:151-207 builds
+# the Mandel form with sqrt(2) shear weights): AH_voigt = T AH_mandel T, T = diag(1,1,1,1/r2,1/r2,1/r2).
+VOIGT_FROM_MANDEL = np.diag([1., 1., 1., 2 ** -0.5, 2 ** -0.5, 2 ** -0.5])
+
+
+def mandel_to_voigt(AH):
+    """Stiffness-like homogenized matrix in Voigt notation (engineering shear strains)."""
+    return VOIGT_FROM_MANDEL @ np.asarray(AH) @ VOIGT_FROM_MANDEL

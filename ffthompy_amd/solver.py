@@ -58,6 +58,10 @@ def linear_solver(Afun, B, ATfun=None, x0=None, par=None, solver=None, callback=
         raise NotImplementedError("This kind (%s) of linear solver is not implemented" % solver)
     t0 = time.perf_counter()
     kind = solver.lower()
+    if callback is not None and not isinstance(callback, CallBack) and not kind.startswith('scipy'):
+        # the reference calls callback(x0) before the solver (solver.py:15-17); a CallBack's log
+        # starts with the initial residual instead
+        callback(B.zeros_like(name='x0') if x0 is None else x0)
     if kind == 'cg':
         x, info = CG(Afun, B, x0=x0, par=par, callback=callback)
     elif kind in ('iterative', 'richardson'):
@@ -72,16 +76,41 @@ def linear_solver(Afun, B, ATfun=None, x0=None, par=None, solver=None, callback=
     return x, info
 
 
+class _IterateCallback:
+    """callback(x) on the device iterate after every iteration of a device solve (the reference's
+    callback(xCG), solver.py:134-136, and callback(x) of richardson / cheby2TERM, solver.py:75-76,
+    277-278): registered on the plan for one solve; x is handed over as a fresh device Tensor (a
+    snapshot, as the reference's functional iterates are). An exception raised by the callback stops
+    the solve and is re-raised here."""
+
+    def __init__(self, Afun, B, callback):
+        self.plan, self.B, self.callback = Afun.plan, B, callback
+
+    def __enter__(self):
+        if self.callback is None or isinstance(self.callback, CallBack):
+            return self
+        B, cb = self.B, self.callback
+        self.plan.set_iterate_callback(lambda kit, x: cb(B._like(x.copy(), name='x')), every=1)
+        return self
+
+    def __exit__(self, etype, exc, tb):
+        if self.callback is None or isinstance(self.callback, CallBack):
+            return False
+        self.plan.set_iterate_callback(None)
+        err = self.plan.pop_callback_error()
+        if err is not None and etype is nat.FhError:
+            raise err
+        return False
+
+
 def _iterate(method, Afun, B, x0, coefs, tol, maxiter, callback):
     B, x = _start(B, x0)
     hist_len = int(maxiter) + 1 if isinstance(callback, CallBack) else 0
-    out = Afun.plan.iter_solve(method, B.val, x.val, coefs, float(tol), int(maxiter), hist_len=hist_len)
+    with _IterateCallback(Afun, B, callback):
+        out = Afun.plan.iter_solve(method, B.val, x.val, coefs, float(tol), int(maxiter), hist_len=hist_len)
     info = {'kit': out['kit'], 'norm_res': out['norm_res']}
-    if callback is not None:
-        if isinstance(callback, CallBack):
-            callback.record(out['res_log'])
-        else:
-            callback(x)
+    if isinstance(callback, CallBack):
+        callback.record(out['res_log'])
     return x, info
 
 
@@ -122,8 +151,19 @@ def scipy_cg(Afun, B, x0=None, par=None, callback=None):
     op = spslin.LinearOperator((n, n), matvec=mv, dtype=np.float64)
     b = B.to_host().val.reshape(-1)
     x0v = np.zeros(n) if x0 is None else x0.to_host().val.reshape(-1)
-    xcol, code = spslin.cg(op, b, x0=x0v, rtol=par['tol'], maxiter=par['maxiter'], M=None,
-                           callback=None if isinstance(callback, CallBack) else callback)
+    cb = callback
+    if isinstance(callback, CallBack):
+        # SciPy hands over the iterate as a flat vector: the CallBack gets it as a device Tensor and
+        # logs ||B - A x|| with its own A and B (this operator and right-hand side if unset)
+        if callback.A is None:
+            callback.A = Afun
+        if callback.B is None:
+            callback.B = B.to_device()
+        Bd = B.to_device()
+
+        def cb(xk):
+            callback(Bd._like(nat.DeviceArray.from_host(np.asarray(xk, dtype=np.float64).reshape(shape)), name='x'))
+    xcol, code = spslin.cg(op, b, x0=x0v, rtol=par['tol'], maxiter=par['maxiter'], M=None, callback=cb)
     x = B.zeros_like(name='x').to_host()
     x.val = xcol.reshape(shape)
     return x.to_device(), {'info': code}
@@ -148,12 +188,13 @@ def CG(Afun, B, x0=None, par=None, callback=None):
     mode = par.get('precision', 'fp64')
     nat.solve_mode(mode)
     B, x = _start(B, x0)
-    hist_len = int(par['maxiter']) + 1 if callback is not None else 0
+    hist_len = int(par['maxiter']) + 1 if isinstance(callback, CallBack) else 0
     every = int(par.get('true_residual_every', 0)) if isinstance(callback, CallBack) else 0
     Afun.plan.set_residual_log(every)
     try:
-        out = Afun.plan.cg_solve(B.val, x.val, tol=float(par['tol']), maxiter=int(par['maxiter']), mode=mode,
-                                 delta=float(par.get('delta', 0.01)), hist_len=hist_len)
+        with _IterateCallback(Afun, B, callback):
+            out = Afun.plan.cg_solve(B.val, x.val, tol=float(par['tol']), maxiter=int(par['maxiter']), mode=mode,
+                                     delta=float(par.get('delta', 0.01)), hist_len=hist_len)
     finally:
         if every:
             Afun.plan.set_residual_log(0)
@@ -162,15 +203,12 @@ def CG(Afun, B, x0=None, par=None, callback=None):
     info = {'kit': out['kit'], 'norm_res': out['norm_res']}
     if mode == 'mixed':
         info['reliable_updates'] = Afun.plan.last_updates()
-    if callback is not None:
-        if isinstance(callback, CallBack):
-            callback.record(out['res_log'])
-            if every:
-                # the reference's CallBack quantity ||B - A x|| (fp64 operator), iterations in res_kit
-                callback.res_norm_recursive = callback.res_norm
-                kits, norms = Afun.plan.residual_log()
-                callback.res_kit = [int(k) for k in kits]
-                callback.res_norm = [float(v) for v in norms]
-        else:
-            callback(x)
+    if isinstance(callback, CallBack):
+        callback.record(out['res_log'])
+        if every:
+            # the reference's CallBack quantity ||B - A x|| (fp64 operator), iterations in res_kit
+            callback.res_norm_recursive = callback.res_norm
+            kits, norms = Afun.plan.residual_log()
+            callback.res_kit = [int(k) for k in kits]
+            callback.res_norm = [float(v) for v in norms]
     return x, info

@@ -33,7 +33,7 @@ enum {
   FH_ERR_ARG = 1,          /* invalid argument (shape, pointer, enum) */
   FH_ERR_HIP = 2,          /* HIP runtime failure */
   FH_ERR_NODEV = 3,        /* no usable GPU */
-  FH_ERR_UNSUPPORTED = 4,  /* grid length with a prime factor above 13, wrong GPU arch */
+  FH_ERR_UNSUPPORTED = 4,  /* grid or tile that does not fit a workgroup, wrong GPU arch */
   FH_ERR_STATE = 5         /* call out of order (e.g. solve before coefficients) */
 };
 
@@ -69,8 +69,10 @@ int fh_memset_zero(void* dst, size_t bytes);
  * Replaces proj.scalar / proj.elasticity (ffthompy/projections.py:9-112, 114-267)
  * and the DFT pair FN, FiN (ffthompy/applications.py:33-37, ffthompy/tensors/operators.py:14-64):
  * G(xi) is never materialised, it is evaluated from the frequency index inside the FFT pass.
- * dim = 2 or 3, N[dim] grid points per axis (any length whose prime factors are <= 13),
- * Y[dim] cell size. */
+ * dim = 2 or 3, N[dim] grid points per axis, Y[dim] cell size. Any length, as numpy.fft / pocketfft
+ * take (ffthompy/tensors/fft.py:39-43): lengths whose prime factors are <= 13 use radix-2..13
+ * stages, others Bluestein's chirp convolution on a 7-smooth length >= 2N-1 (inside the same
+ * LDS kernels; FH_ERR_UNSUPPORTED only if a row of that length does not fit a workgroup's LDS). */
 int fh_plan_create(int dim, const int* N, const double* Y, int physics, int primaldual,
                    fh_plan** out);
 int fh_plan_destroy(fh_plan* plan);
@@ -108,6 +110,16 @@ int fh_plan_use_graphs(fh_plan* plan, int enable);
 int fh_plan_set_residual_log(fh_plan* plan, int every);
 int fh_last_residual_log(const fh_plan* plan, int* kits, double* norms, int cap, int* count);
 int fh_last_solve_graph_batches(const fh_plan* plan, int* n);
+
+/* Host callback on the iterate: the reference's callback(x) after every CG iteration
+ * (ffthompy/general/solver.py:134-136) and after every Richardson / Chebyshev step
+ * (solver.py:75-76, 277-278). With fn set and every >= 1, fh_cg_solve (fp64) and fh_iter_solve
+ * call fn(user, kit, x) after every every-th iteration and after the last one, with x the current
+ * device iterate (fp64, the solve's own x buffer: read it, do not free or modify it); the solve waits
+ * for fn. A nonzero return stops the solve with FH_ERR_STATE. fn = NULL or every = 0: off. Mixed
+ * precision solves reject a plan with a callback (their iterate is split over two precisions). */
+typedef int (*fh_iterate_fn)(void* user, int kit, const void* x);
+int fh_plan_set_iterate_callback(fh_plan* plan, fh_iterate_fn fn, void* user, int every);
 
 /* Coefficient A(x) of the operator, host array in origin-0 voxel order.
  * Replaces Material.get_A_GaNi (ffthompy/materials.py:115-123) as the right factor of
@@ -173,7 +185,10 @@ int fh_apply_projection_hat(fh_plan* plan, int dtype, const void* in_hat, void* 
 /* Conjugate gradients on Afun x = b: CG of ffthompy/general/solver.py:80-139 with the
  * reference's absolute stop sqrt(<r,r>) <= tol, <u,v> = sum(u*v)/prod(N)
  * (ffthompy/tensors/objects.py:634-635). b and x are fp64 fields; x holds x0 on entry.
- * mode FH_SOLVE_FP64: plain fp64 CG.
+ * mode FH_SOLVE_FP64: plain fp64 CG (the reference's recurrence and stop rule); with x0 = 0 the
+ *   converged x is projected once, x <- G x, removing the component outside the range of G that
+ *   the recurrence's rounding accumulates (it would enter the energies at first order and can put
+ *   the GaNi lower bound above the upper one by ~1e-14); with another x0 x is returned as iterated.
  * mode FH_SOLVE_MIXED: fp32 CG with fp64 reliable updates (x accumulated in fp64, the
  *   true residual b - A x recomputed in fp64 whenever the fp32 residual fell by `delta`
  *   or claims convergence; the stop test always uses the fp64 true residual) and a final
@@ -330,8 +345,8 @@ int fh_batch_stats(const fh_batch* batch, int* n_updates, double* norm_res, doub
  * ffthompy/postprocess.py:53-70); the dual matrix is the inverse (postprocess.py:43-46).
  * The quadrature grid is Nbar where its prime factors are <= 13; for odd N with a larger prime
  * in 2N-1 it is the next such length (the integrand is a trigonometric polynomial of degree
- * <= 2N-2 per axis, so every grid of at least 2N-1 points integrates it exactly); for even N
- * without such an Nbar the call fails with FH_ERR_UNSUPPORTED. grid (nullable, dim entries):
+ * <= 2N-2 per axis, so every grid of at least 2N-1 points integrates it exactly); for even N it is
+ * always 2N-1 (transformed with Bluestein's algorithm where needed). grid (nullable, dim entries):
  * an explicit quadrature grid of at least 2N-1 points per axis, exact for fields with
  * frequencies below N/2 (the solutions of a Ga solve); for GaNi solutions of even N only
  * 2N-1 reproduces the reference. */

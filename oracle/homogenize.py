@@ -129,3 +129,38 @@ def run(pb):
         pb.output['res_' + primaldual] = results
         pb.output['mat_' + primaldual] = postprocess(pb, A, mat, solutions, primaldual)
     return pb.output
+
+
+def homogenize_field(A, physics, N, Y, primaldual, tol, maxiter=1000):
+    """GaNi solve of every unit load for an explicit coefficient field A ((k, k)+N, origin 0; for the
+    dual problem already the pointwise inverse): the loop of ffthompy/applications.py:58-90 (scalar)
+    and :143-176 (elasticity) with the driver's materialised projection, B = Afun(-E), CG from
+    x0 = 0 (ffthompy/general/solver.py:80-139) and assembly_matrix (ffthompy/postprocess.py:53-70;
+    the dual matrix inverted, postprocess.py:43-46). Returns (AH, kits, norm_res)."""
+    N = tuple(int(n) for n in N)
+    Y = np.asarray(Y, dtype=np.float64)
+    if physics == 'scalar':
+        _, hG1, hG2 = proj.scalar(N, Y, NyqNul=True)
+    else:
+        _, G1h, G1s, G2h, G2s = proj.elasticity(N, Y, NyqNul=True)
+        hG1, hG2 = G1h + G1s, G2h + G2s
+    nload = A.shape[0]
+    GN = Operator([[DFT(N, inverse=True), hG1 if primaldual == 'primal' else hG2, DFT(N)]], name='G')
+    Af = Field(val=A, order=2, multype=21, name='A_GaNi', Y=Y)
+    Afun = Operator([[GN, Af]], name='FiGFA')
+    sols, kits, norms = [], [], []
+    for iL in range(nload):
+        E = np.zeros(nload)
+        E[iL] = 1.
+        EN = Field(shape=(nload,), N=N, Y=Y)
+        EN.set_mean(E)
+        x0 = Field(shape=(nload,), N=N, Y=Y)
+        B = Afun(-EN)
+        X, info = linear_solver(Afun, B, x0=x0, par={'tol': tol, 'maxiter': maxiter}, solver='CG')
+        kits.append(int(info['kit']))
+        norms.append(float(info['norm_res']))
+        sols.append(add_macro2minimizer(X, E))
+    AH = assembly_matrix(Af, sols)
+    if primaldual == 'dual':
+        AH = np.linalg.inv(AH)
+    return AH, kits, norms

@@ -1,7 +1,7 @@
 """Upper and lower bound matrices of the headline workload's family (random spheres, contrast
 100, CG tol 1e-8, primal with a and dual with 1/a, loads e1-e3) at 45^3 on the GPU against the
 oracle's fp64 solves: fp64 within 1e-10 and fp32-with-fp64-refinement within 1e-8 (relative),
-lower <= upper."""
+lower <= upper within the rounding slack 64 eps max|AH|."""
 import numpy as np
 import pytest
 
@@ -67,4 +67,8 @@ def test_sphere_bounds_match_oracle(case, mode, tol):
     for pd in ('primal', 'dual'):
         err = np.abs(got[pd] - ref[pd]).max() / np.abs(ref[pd]).max()
         assert err < tol, (pd, err)
-    assert np.linalg.eigvalsh(0.5 * ((got['primal'] - got['dual']) + (got['primal'] - got['dual']).T)).min() > -1e-8
+    # GaNi primal and dual are equal analytically and the CG iterates only overestimate both
+    # energies, so lower <= upper up to the rounding of the double-double energies: slack 64 eps |AH|
+    slack = 64 * np.finfo(np.float64).eps * np.abs(got['primal']).max()
+    gap = np.linalg.eigvalsh(0.5 * ((got['primal'] - got['dual']) + (got['primal'] - got['dual']).T)).min()
+    assert gap >= -slack, (gap, slack)

@@ -1,0 +1,137 @@
+"""GPU runs of benchmark configurations 1 and 3 against the float64 oracle.
+
+* Configuration 1 (2-D 45^2, square of side 0.6, contrast 10; GaNi solve + Ga post-processing on
+  2N-1 = 89) through the drop-in Problem, against tests/golden/oracle_configs.json: GaNi and Ga
+  bounds 1e-10 relative with equal CG iteration counts in fp64, 1e-8 in mixed precision; the Ga
+  pair is a guaranteed, separated bracket (lower < upper).
+* Configuration 3 (3-D elasticity, seeded ellipsoids at 25 %, (E, nu) phases, six Mandel loads):
+  at 45^3 against the fixture (primal with the Lame stiffness, dual with the closed-form Lame
+  compliance on the device, the oracle with the pointwise Gauss-Jordan inverse of C): fp64 1e-10
+  and equal iteration counts, mixed 1e-8. At the full 81^3: every solve converges (final residual
+  <= tol), Reuss <= AH <= Voigt in the matrix sense, mixed within 1e-8 of fp64, and
+  lower <= upper within the rounding slack 64 eps |AH| (the GaNi pair is equal analytically).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+nat = pytest.importorskip('ffthompy_amd._native')
+from ffthompy_amd import Problem  # noqa: E402
+from ffthompy_amd import microstructure as ms  # noqa: E402
+from ffthompy_amd.materials import lame_compliance  # noqa: E402
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIX = json.load(open(os.path.join(HERE, 'golden', 'oracle_configs.json')))
+EPS = np.finfo(np.float64).eps
+
+
+def _config1(prec):
+    dim = 2
+    material = {'inclusions': ['square', 'otherwise'], 'positions': [np.zeros(dim), ''],
+                'params': [0.6 * np.ones(dim), ''], 'vals': [10. * np.eye(dim), 1. * np.eye(dim)],
+                'Y': np.ones(dim), 'order': None}
+    return {'name': 'config1', 'physics': 'scalar', 'material': material,
+            'solve': {'kind': 'GaNi', 'N': np.array([45, 45]), 'primaldual': ['primal', 'dual']},
+            'postprocess': [{'kind': 'GaNi'}, {'kind': 'Ga', 'order': None}],
+            'solver': {'kind': 'CG', 'tol': 1e-8, 'maxiter': 1e3, 'precision': prec}}
+
+
+@pytest.mark.parametrize('prec', ['fp64', 'mixed'])
+def test_config1_square_45_matches_oracle(prec):
+    print('case config1', prec, flush=True)
+    pb = Problem(_config1(prec))
+    pb.calculate()
+    fx = FIX['config1']
+    tol = 1e-10 if prec == 'fp64' else 1e-8
+    for pd in ('primal', 'dual'):
+        for key in ('AH_GaNi_' + pd, 'AH_Ga_' + pd):
+            AH, AR = pb.output['mat_' + pd][key], np.asarray(fx[key])
+            assert np.abs(AH - AR).max() < tol * np.abs(AR).max(), (key, AH, AR)
+        if prec == 'fp64':
+            assert [r['info']['kit'] for r in pb.output['res_' + pd]] == fx['kit_' + pd], pd
+    up, lo = pb.output['mat_primal']['AH_Ga_primal'], pb.output['mat_dual']['AH_Ga_dual']
+    assert np.linalg.eigvalsh(up - lo).min() > 0.05, 'the Ga pair must bracket with a gap'
+
+
+def _config3_lame(N):
+    ind, count = ms.ellipsoid_indicator(N, target_fraction=0.25, seed=ms.BASE_SEED)
+    lm0, mu0 = ms.lame_from_young(1.0, 0.3)
+    lm1, mu1 = ms.lame_from_young(10.0, 0.2)
+    return np.where(ind == 1, lm1, lm0), np.where(ind == 1, mu1, mu0), ind
+
+
+def _solve_elastic(N, lam, mu, mode, tol=1e-8):
+    out = {}
+    for pd in ('primal', 'dual'):
+        pl = nat.Plan(N, np.ones(3), 'elasticity', pd)
+        pl.set_coeff('lame', np.stack([lam, mu] if pd == 'primal' else lame_compliance(lam, mu)))
+        sols, kits, norms = [], [], []
+        for iL in range(6):
+            E = np.zeros(6)
+            E[iL] = 1.
+            EN = nat.DeviceArray.zeros((6,) + N)
+            nat.add_constant(EN, -E)
+            B = nat.DeviceArray((6,) + N)
+            pl.matvec(EN, B)
+            x = nat.DeviceArray.zeros((6,) + N)
+            info = pl.cg_solve(B, x, tol=tol, maxiter=3000, mode=mode, delta=1e-4)
+            assert info['status'] == nat.FH_CG_OK
+            kits.append(info['kit'])
+            norms.append(info['norm_res'])
+            nat.add_constant(x, E)
+            sols.append(x)
+        AH = pl.energy_matrix(sols)
+        out[pd] = (AH if pd == 'primal' else np.linalg.inv(AH), kits, norms)
+    return out
+
+
+@pytest.mark.parametrize('mode', ['fp64', 'mixed'])
+def test_config3_elasticity_45_matches_oracle(mode):
+    print('case config3 45', mode, flush=True)
+    fx = FIX['config3']
+    N = tuple(fx['N'])
+    lam, mu, ind = _config3_lame(N)
+    assert abs(ind.mean() - fx['volume_fraction']) < 1e-15
+    res = _solve_elastic(N, lam, mu, mode)
+    tol = 1e-10 if mode == 'fp64' else 1e-8
+    for pd in ('primal', 'dual'):
+        AH, kits, norms = res[pd]
+        AR = np.asarray(fx['AH_' + pd])
+        print(pd, 'rel diff %.2e' % (np.abs(AH - AR).max() / np.abs(AR).max()), kits, fx['kit_' + pd], flush=True)
+        assert np.abs(AH - AR).max() < tol * np.abs(AR).max(), pd
+        if mode == 'fp64':
+            assert kits == fx['kit_' + pd], pd
+        assert max(norms) <= 1e-8
+
+
+def test_config3_elasticity_81_full_size():
+    print('case config3 81', flush=True)
+    N = (81, 81, 81)
+    lam, mu, ind = _config3_lame(N)
+    r64 = _solve_elastic(N, lam, mu, 'fp64')
+    rmx = _solve_elastic(N, lam, mu, 'mixed')
+    m = np.array([1., 1., 1., 0., 0., 0.])
+    C = np.einsum('ij,...->ij...', np.outer(m, m), lam) + 2. * np.einsum('ij,...->ij...', np.eye(6), mu)
+    voigt = C.reshape(6, 6, -1).mean(axis=2)
+    S = np.einsum('ij,...->ij...', np.outer(m, m), lame_compliance(lam, mu)[0]) \
+        + 2. * np.einsum('ij,...->ij...', np.eye(6), lame_compliance(lam, mu)[1])
+    reuss = np.linalg.inv(S.reshape(6, 6, -1).mean(axis=2))
+    for res in (r64, rmx):
+        up, lo = res['primal'][0], res['dual'][0]
+        for pd in ('primal', 'dual'):
+            assert max(res[pd][2]) <= 1e-8, pd
+        slack = 64 * EPS * np.abs(up).max()
+        assert np.linalg.eigvalsh(0.5 * (voigt - up + (voigt - up).T)).min() > -slack
+        assert np.linalg.eigvalsh(0.5 * (lo - reuss + (lo - reuss).T)).min() > -slack
+        # GaNi: primal and dual agree analytically. The CG iterates only overestimate both energies
+        # (any compatible / equilibrated field does), so the computed pair satisfies lower <= upper
+        # up to the rounding of the double-double energies
+        assert np.linalg.eigvalsh(0.5 * (up - lo + (up - lo).T)).min() >= -slack
+        assert np.abs(up - lo).max() < 1e-6 * np.abs(up).max()
+    for pd in ('primal', 'dual'):
+        a, b = rmx[pd][0], r64[pd][0]
+        assert np.abs(a - b).max() < 1e-8 * np.abs(b).max(), pd
+    print('81^3 kits fp64', r64['primal'][1], r64['dual'][1], 'mixed', rmx['primal'][1], rmx['dual'][1], flush=True)

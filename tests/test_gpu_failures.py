@@ -92,7 +92,17 @@ def test_non_finite_coefficient_rejected():
 
 
 @pytest.mark.parametrize('grid', [(17, 15, 15), (15, 34), (19, 19)])
-def test_grid_with_large_prime_factor_rejected(grid):
+def test_grid_with_large_prime_factor_is_planned(grid):
+    """Any length is accepted (Bluestein beyond radix 13); its values are tested against numpy
+    and the oracle in test_gpu_bluestein.py."""
+    pl = nat.Plan(grid, (1.,) * len(grid), 'scalar', 'primal')
+    assert pl.ncomp == len(grid)
+
+
+@pytest.mark.parametrize('grid', [(5, 100003), (3, 65537, 3)])
+def test_row_too_long_for_a_workgroup_rejected(grid):
+    """A prime length whose Bluestein convolution does not fit one workgroup's LDS is refused at
+    plan creation with FH_ERR_UNSUPPORTED (nothing launched)."""
     with pytest.raises(nat.FhError) as e:
         nat.Plan(grid, (1.,) * len(grid), 'scalar', 'primal')
     assert e.value.code == nat.FH_ERR_UNSUPPORTED

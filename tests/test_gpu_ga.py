@@ -178,11 +178,28 @@ def test_ga_energy_on_smooth_quadrature_grid(dim, N, M):
             assert np.abs(got - ref).max() < 1e-11 * np.abs(ref).max(), (order, pd)
 
 
-def test_ga_even_grid_without_smooth_nbar_is_rejected():
-    plan = nat.Plan((10, 10), (1., 1.))  # 2N-1 = 19, prime; even N needs exactly 19
-    with pytest.raises(nat.FhError) as ei:
-        nat.Ga(plan)
-    assert ei.value.code == nat.FH_ERR_UNSUPPORTED
+@pytest.mark.parametrize('N', [(10, 10), (10, 12), (10, 9, 7)])
+def test_ga_even_grid_with_prime_nbar_matches_oracle(N):
+    """Even N with a prime 2N-1 (19, 23): only the reference's own grid 2N-1 reproduces its aliased
+    quadrature, so the device integrates on 19 / 23 points with Bluestein transforms (odd axes keep
+    the next 13-smooth length, exact for them); the energies equal the oracle's (order None, 0, 1)."""
+    print('case ga prime nbar', N, flush=True)
+    dim = len(N)
+    conf = _scalar_material(dim, ['cube' if dim == 3 else 'square', 'otherwise'],
+                            [0.55 * np.ones(dim), ''], [np.zeros(dim), ''])
+    Nbar = tuple(2 * n - 1 for n in N)
+    fields = _random_fields(N, dim, 8)
+    plan = nat.Plan(N, np.ones(dim))
+    ga = nat.Ga(plan)
+    assert ga.M == tuple(2 * n - 1 if n % 2 == 0 else nat.smooth_length(2 * n - 1) for n in N)
+    for order, P in ((None, None), (0, N), (1, N)):
+        for pd in ('primal', 'dual'):
+            c = dict(conf, positions=list(conf['positions']))
+            Aref = OMaterial(dict(c)).get_A_Ga(Nbar, pd, order=order, P=P)
+            ref = o_assembly_matrix(Aref, [Field(val=v.copy(), order=1, Y=np.ones(dim)) for v in fields])
+            Material(dict(c)).get_A_Ga(Nbar, pd, order=order, P=P).load(ga)
+            got = ga.energy_matrix([nat.DeviceArray.from_host(v) for v in fields])
+            assert np.abs(got - ref).max() < 1e-11 * np.abs(ref).max(), (order, pd)
 
 
 GA_SOLVES = [c for c in examples.problems() if c[2]['solve']['kind'] == 'Ga']
