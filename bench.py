@@ -435,11 +435,11 @@ def main():
     if not args.no_autotune:
         for dt, name in ((np.float32, 'fp32'), (np.float64, 'fp64')):
             res = pl.autotune(dt, nrep=5)
-            H.plans['dual'].set_schedule(dt, res['proj_variant'], res['cols_variant'])
+            H.plans['dual'].set_schedule(dt, res['proj_variant'], res['cols_variant'], res['rows_variant'])
             sched[name] = res
-            log('autotune %s: projection tile variants %s ms -> %d; strided %s ms -> %d' % (
+            log('autotune %s: projection tile variants %s ms -> %d; strided %s ms -> %d; rows %s ms -> %d' % (
                 name, np.round(res['proj_ms'], 4).tolist(), res['proj_variant'], np.round(res['cols_ms'], 4).tolist(),
-                res['cols_variant']))
+                res['cols_variant'], np.round(res['rows_ms'], 4).tolist(), res['rows_variant']))
 
     # --- matvec profile (per kernel, HIP events on the plan stream) ---
     prof = {}

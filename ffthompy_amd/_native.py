@@ -62,9 +62,10 @@ SIGNATURES = {
     'fh_last_solve_updates': [_vp, _c_int_p],
     'fh_matvec_profile': [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int, _c_double_p, _c_double_p,
                           _c_int_p],
-    'fh_plan_set_schedule': [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int],
-    'fh_plan_get_schedule': [_vp, ctypes.c_int, _c_int_p, _c_int_p],
-    'fh_plan_autotune': [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int, _c_double_p, _c_double_p, _c_int_p, _c_int_p],
+    'fh_plan_set_schedule': [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int],
+    'fh_plan_get_schedule': [_vp, ctypes.c_int, _c_int_p, _c_int_p, _c_int_p],
+    'fh_plan_autotune': [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int, _c_double_p, _c_double_p, _c_double_p, _c_int_p,
+                         _c_int_p, _c_int_p],
     'fh_batch_create': [ctypes.c_int, _c_int_p, _c_double_p, ctypes.POINTER(ctypes.c_void_p)],
     'fh_batch_destroy': [_vp],
     'fh_batch_set_coeff': [_vp, _c_double_p],
@@ -652,28 +653,32 @@ class Plan:
              ms.ctypes.data_as(_c_double_p), by.ctypes.data_as(_c_double_p), ctypes.byref(nk))
         return ms[:nk.value].copy(), by[:nk.value].copy()
 
-    def set_schedule(self, dtype, proj_variant, cols_variant):
+    def set_schedule(self, dtype, proj_variant, cols_variant, rows_variant=0):
         """Schedule variants of the fixed-length passes for one arithmetic (fh_plan_set_schedule)."""
         call('fh_plan_set_schedule', self._h, ctypes.c_int(dtype_code(dtype)), ctypes.c_int(int(proj_variant)),
-             ctypes.c_int(int(cols_variant)))
+             ctypes.c_int(int(cols_variant)), ctypes.c_int(int(rows_variant)))
 
     def schedule(self, dtype):
-        p, c = ctypes.c_int(), ctypes.c_int()
-        call('fh_plan_get_schedule', self._h, ctypes.c_int(dtype_code(dtype)), ctypes.byref(p), ctypes.byref(c))
-        return p.value, c.value
+        p, c, r = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        call('fh_plan_get_schedule', self._h, ctypes.c_int(dtype_code(dtype)), ctypes.byref(p), ctypes.byref(c),
+             ctypes.byref(r))
+        return p.value, c.value, r.value
 
     def autotune(self, dtype=np.float32, nrep=5, seed=0):
         """Time every schedule variant of the projection and strided passes on this plan (random
         field, nrep matvecs per variant) and keep the fastest: fh_plan_autotune. Returns
-        {'proj_ms': [...], 'cols_ms': [...], 'proj_variant': p, 'cols_variant': c}."""
+        {'proj_ms': [...], 'cols_ms': [...], 'rows_ms': [...], 'proj_variant': p, 'cols_variant': c,
+        'rows_variant': r}."""
         dt = np.dtype(dtype)
         x = DeviceArray.from_host(np.random.default_rng(seed).standard_normal((self.ncomp,) + self.N).astype(dt))
         y = DeviceArray((self.ncomp,) + self.N, dt)
-        tp, tc = np.zeros(4), np.zeros(2)
-        p, c = ctypes.c_int(), ctypes.c_int()
+        tp, tc, tr = np.zeros(4), np.zeros(2), np.zeros(2)
+        p, c, r = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         call('fh_plan_autotune', self._h, ctypes.c_int(dtype_code(dt)), x.ptr, y.ptr, ctypes.c_int(int(nrep)),
-             tp.ctypes.data_as(_c_double_p), tc.ctypes.data_as(_c_double_p), ctypes.byref(p), ctypes.byref(c))
-        return {'proj_ms': tp.tolist(), 'cols_ms': tc.tolist(), 'proj_variant': p.value, 'cols_variant': c.value}
+             tp.ctypes.data_as(_c_double_p), tc.ctypes.data_as(_c_double_p), tr.ctypes.data_as(_c_double_p),
+             ctypes.byref(p), ctypes.byref(c), ctypes.byref(r))
+        return {'proj_ms': tp.tolist(), 'cols_ms': tc.tolist(), 'rows_ms': tr.tolist(), 'proj_variant': p.value,
+                'cols_variant': c.value, 'rows_variant': r.value}
 
     def __del__(self):
         h = getattr(self, '_h', None)

@@ -272,6 +272,7 @@ struct fh_plan {
   int last_graph_batches = 0;    // batches of the last solve replayed from the graph
   int proj_var[2] = {0, 0};      // schedule variants of the fixed-length passes: [fp64, fp32]
   int cols_var[2] = {0, 0};
+  int rows_var[2] = {0, 0};
   int res_every = 0;             // fp64 CG: log ||b - A x|| every res_every iterations (0: off)
   fh_iterate_fn it_fn = nullptr; // host callback on the iterate every it_every iterations (0: off)
   void* it_user = nullptr;
@@ -372,6 +373,7 @@ fh::LaunchCtx make_ctx(const fh_plan* pl) {
   c.t1 = pl->prof_t1;
   c.proj_var = pl->proj_var[sizeof(T) == 8 ? 0 : 1];
   c.cols_var = pl->cols_var[sizeof(T) == 8 ? 0 : 1];
+  c.rows_var = pl->rows_var[sizeof(T) == 8 ? 0 : 1];
   return c;
 }
 
@@ -1730,30 +1732,34 @@ int fh_matvec_profile(fh_plan* pl, int dtype, const void* x, void* y, int nrep, 
   return FH_OK;
 }
 
-int fh_plan_set_schedule(fh_plan* pl, int dtype, int proj_variant, int cols_variant) {
+int fh_plan_set_schedule(fh_plan* pl, int dtype, int proj_variant, int cols_variant, int rows_variant) {
   RETCHK(check_plan(pl));
   RETCHK(check_dtype(dtype));
-  if (proj_variant < 0 || proj_variant >= fh::kProjVariants || cols_variant < 0 || cols_variant >= fh::kColsVariants)
+  if (proj_variant < 0 || proj_variant >= fh::kProjVariants || cols_variant < 0 || cols_variant >= fh::kColsVariants ||
+      rows_variant < 0 || rows_variant >= fh::kRowsVariants)
     return fail(FH_ERR_ARG, "schedule variant out of range");
   const int i = dtype == FH_F64 ? 0 : 1;
-  if (pl->proj_var[i] != proj_variant || pl->cols_var[i] != cols_variant) drop_graphs(pl);
+  if (pl->proj_var[i] != proj_variant || pl->cols_var[i] != cols_variant || pl->rows_var[i] != rows_variant)
+    drop_graphs(pl);
   pl->proj_var[i] = proj_variant;
   pl->cols_var[i] = cols_variant;
+  pl->rows_var[i] = rows_variant;
   return FH_OK;
 }
 
-int fh_plan_get_schedule(const fh_plan* pl, int dtype, int* proj_variant, int* cols_variant) {
+int fh_plan_get_schedule(const fh_plan* pl, int dtype, int* proj_variant, int* cols_variant, int* rows_variant) {
   RETCHK(check_plan(pl));
   RETCHK(check_dtype(dtype));
-  if (!proj_variant || !cols_variant) return fail(FH_ERR_ARG, "null pointer argument");
+  if (!proj_variant || !cols_variant || !rows_variant) return fail(FH_ERR_ARG, "null pointer argument");
   const int i = dtype == FH_F64 ? 0 : 1;
   *proj_variant = pl->proj_var[i];
   *cols_variant = pl->cols_var[i];
+  *rows_variant = pl->rows_var[i];
   return FH_OK;
 }
 
 int fh_plan_autotune(fh_plan* pl, int dtype, const void* x, void* y, int nrep, double* proj_ms, double* cols_ms,
-                     int* proj_variant, int* cols_variant) {
+                     double* rows_ms, int* proj_variant, int* cols_variant, int* rows_variant) {
   RETCHK(check_plan(pl));
   RETCHK(check_dtype(dtype));
   if (!x || !y || nrep < 1) return fail(FH_ERR_ARG, "bad arguments");
@@ -1762,8 +1768,8 @@ int fh_plan_autotune(fh_plan* pl, int dtype, const void* x, void* y, int nrep, d
   const int kproj = pl->n0 > 1 ? 2 : 1;
   double kms[8], kby[8];
   int nkr = 0;
-  const int keep_p = pl->proj_var[i], keep_c = pl->cols_var[i];
-  double tp[fh::kProjVariants], tc[fh::kColsVariants];
+  const int keep_p = pl->proj_var[i], keep_c = pl->cols_var[i], keep_r = pl->rows_var[i];
+  double tp[fh::kProjVariants], tc[fh::kColsVariants], tr[fh::kRowsVariants];
   // each variant of one pass with the other pass at its current choice; the faster wins
   int rc = FH_OK;
   for (int v = 0; v < fh::kProjVariants && rc == FH_OK; ++v) {
@@ -1790,16 +1796,30 @@ int fh_plan_autotune(fh_plan* pl, int dtype, const void* x, void* y, int nrep, d
     for (int v = 1; v < fh::kColsVariants; ++v)
       if (tc[v] < tc[bc]) bc = v;
   }
+  pl->cols_var[i] = bc;
+  // contiguous-axis passes: forward and inverse together
+  for (int v = 0; v < fh::kRowsVariants && rc == FH_OK; ++v) {
+    pl->rows_var[i] = v;
+    rc = fh_matvec_profile(pl, dtype, x, y, nrep, 8, kms, kby, &nkr);
+    tr[v] = rc == FH_OK ? kms[0] + kms[nk - 1] : 0.0;
+  }
+  int br = 0;
+  for (int v = 1; v < fh::kRowsVariants; ++v)
+    if (tr[v] < tr[br]) br = v;
   pl->proj_var[i] = keep_p;
   pl->cols_var[i] = keep_c;
+  pl->rows_var[i] = keep_r;
   if (rc != FH_OK) return rc;
-  RETCHK(fh_plan_set_schedule(pl, dtype, bp, bc));
+  RETCHK(fh_plan_set_schedule(pl, dtype, bp, bc, br));
   if (proj_ms)
     for (int v = 0; v < fh::kProjVariants; ++v) proj_ms[v] = tp[v];
   if (cols_ms)
     for (int v = 0; v < fh::kColsVariants; ++v) cols_ms[v] = tc[v];
+  if (rows_ms)
+    for (int v = 0; v < fh::kRowsVariants; ++v) rows_ms[v] = tr[v];
   if (proj_variant) *proj_variant = bp;
   if (cols_variant) *cols_variant = bc;
+  if (rows_variant) *rows_variant = br;
   return FH_OK;
 }
 

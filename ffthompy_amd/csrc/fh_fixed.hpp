@@ -219,6 +219,10 @@ struct FxShape {
     return FP::TPP >= 16 ? c64a[v] : (FP::TPP >= 8 ? c64b[v] : c64c[v]);
   }
   static constexpr int cols_p(int v) { return v == 0 ? P : (P >= 2 ? P / 2 : P); }
+  // contiguous-axis passes: rows per workgroup (even); variant 1 takes 3/2 as many rows, which
+  // also fills the last wavefront better where TPP * (rpb/2) * NC is not a multiple of 64
+  static constexpr int NROWS = 2;
+  static constexpr int rows_rpb(int v) { return v == 0 ? RPB : RPB / 2 * 3; }
 };
 
 // ---- strided pass: forward (DIR -1) or inverse (DIR +1) along a non-contiguous axis ----

@@ -40,11 +40,15 @@ inline void allow_lds(K kernel, size_t bytes) {
                                                    (int)bytes);
 }
 
-template <typename T, class FP>
-int rows_fwd_fx_launch(const LaunchCtx& c, int nc, const T* in, const T* coef, int kind, cpx<T>* out,
-                       const int* halt, const T* rvec, const CGState* st, T* xc, double* partial,
-                       size_t partial_cap, int* nblocks) {
-  constexpr int rpb = FxShape<FP, T>::RPB;
+// RPB_: rows per workgroup. ALL: every (coefficient, component) combination is instantiated;
+// otherwise only the scalar ones (no / isotropic coefficient, 2 or 3 components) and the others
+// return 0 (the caller then launches the default row count).
+template <typename T, class FP, int RPB_, bool ALL>
+int rows_fwd_fx_launch_r(const LaunchCtx& c, int nc, const T* in, const T* coef, int kind, cpx<T>* out,
+                         const int* halt, const T* rvec, const CGState* st, T* xc, double* partial,
+                         size_t partial_cap, int* nblocks) {
+  constexpr int rpb = RPB_;
+  if (!ALL && !((nc == 2 || nc == 3) && (kind == COEF_NONE || kind == COEF_ISO))) return 0;
   const int npen = (rpb / 2) * nc;
   const int nthr = npen * FP::TPP;
   if (nthr > 1024) return 0;
@@ -82,7 +86,7 @@ int rows_fwd_fx_launch(const LaunchCtx& c, int nc, const T* in, const T* coef, i
   do {                                                                                                        \
     const size_t slds = RowStage<FP, T, NCC>::lds_bytes(rpb, KIND == COEF_ISO ? 1 : (KIND == COEF_LAME ? 2 : 0)); \
     if (slds + FP::L * sizeof(cpx<T>) > 160 * 1024) return 0;                                                 \
-    auto kf = k_rows_fwd_st<T, FP, KIND, NCC, UPDV>;                                                          \
+    auto kf = k_rows_fwd_st<T, FP, KIND, NCC, UPDV, RPB_>;                                                    \
     allow_lds(kf, slds);                                                                                      \
     hipExtLaunchKernelGGL(kf, grid, block, slds, c.stream, c.t0, c.t1, 0, in, coef, out, nrows, c.n, rpb, tw, halt, rvec, st, xc, \
                        partial);                                                                              \
@@ -90,10 +94,10 @@ int rows_fwd_fx_launch(const LaunchCtx& c, int nc, const T* in, const T* coef, i
 #define FH_ST_NC(KIND, UPDV)                 \
   do {                                       \
     switch (nc) {                            \
-      case 1: FH_LAUNCH_ST(KIND, 1, UPDV); break; \
       case 2: FH_LAUNCH_ST(KIND, 2, UPDV); break; \
       case 3: FH_LAUNCH_ST(KIND, 3, UPDV); break; \
-      case 6: FH_LAUNCH_ST(KIND, 6, UPDV); break; \
+      case 1: if constexpr (ALL) { FH_LAUNCH_ST(KIND, 1, UPDV); break; } else return 0; \
+      case 6: if constexpr (ALL) { FH_LAUNCH_ST(KIND, 6, UPDV); break; } else return 0; \
       default: return 0;                     \
     }                                        \
   } while (0)
@@ -102,6 +106,7 @@ int rows_fwd_fx_launch(const LaunchCtx& c, int nc, const T* in, const T* coef, i
       else if (upd == 1) FH_ST_NC(COEF_NONE, 1);
       else return 0;
     } else if (kind == COEF_LAME) {
+      if constexpr (!ALL) return 0;
 #define FH_ST_LAME(UPDV)                                \
   do {                                                  \
     if (nc == 3) FH_LAUNCH_ST(COEF_LAME, 3, UPDV);      \
@@ -157,18 +162,25 @@ int rows_fwd_fx_launch(const LaunchCtx& c, int nc, const T* in, const T* coef, i
   }
   const int key = kind * 8 + nc;
   switch (key) {
-    case COEF_NONE * 8 + 1: FH_LAUNCH_RF(COEF_NONE, 1); break;
     case COEF_NONE * 8 + 2: FH_LAUNCH_RF(COEF_NONE, 2); break;
     case COEF_NONE * 8 + 3: FH_LAUNCH_RF(COEF_NONE, 3); break;
-    case COEF_NONE * 8 + 6: FH_LAUNCH_RF(COEF_NONE, 6); break;
     case COEF_ISO * 8 + 2: FH_LAUNCH_RF(COEF_ISO, 2); break;
     case COEF_ISO * 8 + 3: FH_LAUNCH_RF(COEF_ISO, 3); break;
-    case COEF_LAME * 8 + 3: FH_LAUNCH_RF(COEF_LAME, 3); break;
-    case COEF_LAME * 8 + 6: FH_LAUNCH_RF(COEF_LAME, 6); break;
-    case COEF_FULL * 8 + 2: FH_LAUNCH_RF(COEF_FULL, 2); break;
-    case COEF_FULL * 8 + 3: FH_LAUNCH_RF(COEF_FULL, 3); break;
-    case COEF_FULL * 8 + 6: FH_LAUNCH_RF(COEF_FULL, 6); break;
-    default: return 0;
+    default:
+      if constexpr (ALL) {
+        switch (key) {
+          case COEF_NONE * 8 + 1: FH_LAUNCH_RF(COEF_NONE, 1); break;
+          case COEF_NONE * 8 + 6: FH_LAUNCH_RF(COEF_NONE, 6); break;
+          case COEF_LAME * 8 + 3: FH_LAUNCH_RF(COEF_LAME, 3); break;
+          case COEF_LAME * 8 + 6: FH_LAUNCH_RF(COEF_LAME, 6); break;
+          case COEF_FULL * 8 + 2: FH_LAUNCH_RF(COEF_FULL, 2); break;
+          case COEF_FULL * 8 + 3: FH_LAUNCH_RF(COEF_FULL, 3); break;
+          case COEF_FULL * 8 + 6: FH_LAUNCH_RF(COEF_FULL, 6); break;
+          default: return 0;
+        }
+      } else {
+        return 0;
+      }
   }
 #undef FH_LAUNCH_RF
   const hipError_t e = hipGetLastError();
@@ -176,9 +188,24 @@ int rows_fwd_fx_launch(const LaunchCtx& c, int nc, const T* in, const T* coef, i
 }
 
 template <typename T, class FP>
-int rows_inv_fx_launch(const LaunchCtx& c, int nc, const cpx<T>* in, T* out, T scale, int epi, const T* w,
-                       double* partial, size_t partial_cap, const int* halt, int* nblocks, CGState* st) {
-  constexpr int rpb = FxShape<FP, T>::RPB;
+int rows_fwd_fx_launch(const LaunchCtx& c, int nc, const T* in, const T* coef, int kind, cpx<T>* out,
+                       const int* halt, const T* rvec, const CGState* st, T* xc, double* partial,
+                       size_t partial_cap, int* nblocks) {
+  constexpr int R0 = FxShape<FP, T>::rows_rpb(0), R1 = FxShape<FP, T>::rows_rpb(1);
+  if (c.rows_var == 1) {
+    const int r = rows_fwd_fx_launch_r<T, FP, R1, false>(c, nc, in, coef, kind, out, halt, rvec, st, xc, partial,
+                                                         partial_cap, nblocks);
+    if (r != 0) return r;  // 0: the variant does not fit this case; run the default
+  }
+  return rows_fwd_fx_launch_r<T, FP, R0, true>(c, nc, in, coef, kind, out, halt, rvec, st, xc, partial, partial_cap,
+                                               nblocks);
+}
+
+template <typename T, class FP, int RPB_, bool ALL>
+int rows_inv_fx_launch_r(const LaunchCtx& c, int nc, const cpx<T>* in, T* out, T scale, int epi, const T* w,
+                         double* partial, size_t partial_cap, const int* halt, int* nblocks, CGState* st) {
+  constexpr int rpb = RPB_;
+  if (!ALL && nc != 2 && nc != 3) return 0;
   const int npen = (rpb / 2) * nc;
   const int nthr = npen * FP::TPP;
   if (nthr > 1024) return 0;
@@ -195,31 +222,50 @@ int rows_inv_fx_launch(const LaunchCtx& c, int nc, const cpx<T>* in, T* out, T s
   do {                                                                                                        \
     const size_t slds = RowStage<FP, T, NCC>::lds_bytes(rpb, false);                                          \
     if (slds + FP::L * sizeof(cpx<T>) > 160 * 1024) return 0;                                                 \
-    auto kf = k_rows_inv_st<T, FP, EPI, NCC>;                                                                 \
+    auto kf = k_rows_inv_st<T, FP, EPI, NCC, RPB_>;                                                           \
     allow_lds(kf, slds);                                                                                      \
     hipExtLaunchKernelGGL(kf, grid, block, slds, c.stream, c.t0, c.t1, 0, in, out, nrows, c.n, rpb, tw, scale, w, partial, halt, \
                        st, order_rev(1));                                                                 \
   } while (0)
   const int key = epi * 8 + nc;
   switch (key) {
-    case 1: FH_LAUNCH_RI(0, 1); break;
     case 2: FH_LAUNCH_RI(0, 2); break;
     case 3: FH_LAUNCH_RI(0, 3); break;
-    case 6: FH_LAUNCH_RI(0, 6); break;
-    case 8 + 1: FH_LAUNCH_RI(1, 1); break;
     case 8 + 2: FH_LAUNCH_RI(1, 2); break;
     case 8 + 3: FH_LAUNCH_RI(1, 3); break;
-    case 8 + 6: FH_LAUNCH_RI(1, 6); break;
-    case 16 + 1: FH_LAUNCH_RI(2, 1); break;
     case 16 + 2: FH_LAUNCH_RI(2, 2); break;
     case 16 + 3: FH_LAUNCH_RI(2, 3); break;
-    case 16 + 6: FH_LAUNCH_RI(2, 6); break;
-    default: return 0;
+    default:
+      if constexpr (ALL) {
+        switch (key) {
+          case 1: FH_LAUNCH_RI(0, 1); break;
+          case 6: FH_LAUNCH_RI(0, 6); break;
+          case 8 + 1: FH_LAUNCH_RI(1, 1); break;
+          case 8 + 6: FH_LAUNCH_RI(1, 6); break;
+          case 16 + 1: FH_LAUNCH_RI(2, 1); break;
+          case 16 + 6: FH_LAUNCH_RI(2, 6); break;
+          default: return 0;
+        }
+      } else {
+        return 0;
+      }
   }
 #undef FH_LAUNCH_RI
   if (nblocks) *nblocks = (int)nb;
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? 1 : -(int)e;
+}
+
+template <typename T, class FP>
+int rows_inv_fx_launch(const LaunchCtx& c, int nc, const cpx<T>* in, T* out, T scale, int epi, const T* w,
+                       double* partial, size_t partial_cap, const int* halt, int* nblocks, CGState* st) {
+  constexpr int R0 = FxShape<FP, T>::rows_rpb(0), R1 = FxShape<FP, T>::rows_rpb(1);
+  if (c.rows_var == 1) {
+    const int r = rows_inv_fx_launch_r<T, FP, R1, false>(c, nc, in, out, scale, epi, w, partial, partial_cap, halt,
+                                                         nblocks, st);
+    if (r != 0) return r;
+  }
+  return rows_inv_fx_launch_r<T, FP, R0, true>(c, nc, in, out, scale, epi, w, partial, partial_cap, halt, nblocks, st);
 }
 
 inline void strided_geometry(const LaunchCtx& c, int axis, int nc, long long* outer, long long* ostride,

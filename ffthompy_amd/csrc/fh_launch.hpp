@@ -18,10 +18,11 @@ struct LaunchCtx {
   hipEvent_t t0 = nullptr, t1 = nullptr;  // if set: start/stop time of the launched kernel
   int proj_var = 0;  // projection pass schedule variant (FxShape::proj_f)
   int cols_var = 0;  // strided pass schedule variant (FxShape::cols_p)
+  int rows_var = 0;  // contiguous-axis pass schedule variant (FxShape::rows_rpb)
 };
 
-// Number of schedule variants of the projection and strided passes.
-constexpr int kProjVariants = 4, kColsVariants = 2;
+// Number of schedule variants of the projection, strided and contiguous-axis passes.
+constexpr int kProjVariants = 4, kColsVariants = 2, kRowsVariants = 2;
 
 // Return 1 if a specialised kernel was launched, 0 if the length has none (caller falls
 // back to the generic kernels), or a negative hipError_t on launch failure.

@@ -69,7 +69,7 @@ __device__ __forceinline__ int stage_base(int region, unsigned s0) {
 // ---- forward: A(x) v, R2C along the contiguous axis, AoS half spectrum -------------------
 // UPD 0: v = in. UPD 1: p = r + beta p (in = p, written back), v = p.
 // UPD 2: as UPD 1 and, first, xc += alpha p_old when st->pend; partial sums of <p, a p>.
-template <typename T, class FP, int COEF, int NC, int UPD>
+template <typename T, class FP, int COEF, int NC, int UPD, int RPB = FxShape<FP, T>::RPB>
 __global__ void __launch_bounds__(1024)
 k_rows_fwd_st(const T* __restrict__ in, const T* __restrict__ coef, cpx<T>* __restrict__ out,
               long long nrows, long long n, int rpb, const cpx<T>* __restrict__ twg,
@@ -98,7 +98,7 @@ k_rows_fwd_st(const T* __restrict__ in, const T* __restrict__ coef, cpx<T>* __re
   const __amdgpu_buffer_rsrc_t rin = make_rsrc(in, (unsigned long long)NC * n * SR);
   const unsigned rbase = (unsigned)(row0 * L);
   // every HBM load of the block is issued before the first use (CPT chunks per thread per run)
-  constexpr int RPB = FxShape<FP, T>::RPB;
+  // RPB (template): rows per workgroup; the launch passes rpb == RPB
   constexpr int NT = (RPB / 2) * NC * TPP;
   constexpr int QMAX = (RPB * L + 2 * W - 2) / W + 1;  // chunks touched by one run of RPB rows
   constexpr int CPT = (QMAX + NT - 1) / NT;
@@ -285,7 +285,7 @@ k_rows_fwd_st(const T* __restrict__ in, const T* __restrict__ coef, cpx<T>* __re
 // ---- inverse: C2R along the contiguous axis, then the epilogue --------------------------
 // EPI 0: out = y. EPI 1: out = y, partial sums of <y, w>. EPI 2: out is the CG residual r,
 // r -= alpha y (alpha from st), partial sums of <r, r>.
-template <typename T, class FP, int EPI, int NC>
+template <typename T, class FP, int EPI, int NC, int RPB = FxShape<FP, T>::RPB>
 __global__ void __launch_bounds__(1024)
 k_rows_inv_st(const cpx<T>* __restrict__ in, T* __restrict__ out, long long nrows, long long n, int rpb,
               const cpx<T>* __restrict__ twg, T scale, const T* __restrict__ w, double* __restrict__ partial,
@@ -360,7 +360,7 @@ k_rows_inv_st(const cpx<T>* __restrict__ in, T* __restrict__ out, long long nrow
   }
   __syncthreads();
   // epilogue over each component's run of rows, in 16-byte chunks
-  constexpr int RPB = FxShape<FP, T>::RPB;
+  // RPB (template): rows per workgroup; the launch passes rpb == RPB
   constexpr int NT = (RPB / 2) * NC * TPP;
   constexpr int QMAX = (RPB * L + 2 * W - 2) / W + 1;  // chunks touched by one run of RPB rows
   constexpr int CPT = (QMAX + NT - 1) / NT;

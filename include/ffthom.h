@@ -293,16 +293,18 @@ int fh_matvec_profile(fh_plan* plan, int dtype, const void* x, void* y, int nrep
  * (0..3) sets the frequency points per tile of the projection pass (forward FFT along the
  * outermost axis, G(xi), inverse FFT), which also sets the threads per workgroup (scalar plans;
  * elasticity keeps variant 0's tile); cols_variant (0..1) the columns per tile of the other strided
- * passes (3-D). Every variant computes the same values (the same transforms per pencil); only
- * speed differs. Variant 0 is the default. fh_plan_autotune times every variant on this plan
- * (nrep matvecs of x into y per variant, each pass timed by its own HIP events, as
- * fh_matvec_profile), keeps the fastest of each pass and returns the times in ms
- * (proj_ms[4], cols_ms[2]; either may be NULL) and the choice. FH_ERR_UNSUPPORTED where the plan
- * runs the runtime-radix kernels. No reference counterpart (the reference calls NumPy's FFT). */
-int fh_plan_set_schedule(fh_plan* plan, int dtype, int proj_variant, int cols_variant);
-int fh_plan_get_schedule(const fh_plan* plan, int dtype, int* proj_variant, int* cols_variant);
+ * passes (3-D); rows_variant (0..1) the grid rows per workgroup of the contiguous-axis passes
+ * (variant 1: 3/2 as many; scalar plans, isotropic or no coefficient). Every variant computes the
+ * same values (the same transforms per pencil); only speed differs. Variant 0 is the default.
+ * fh_plan_autotune times every variant on this plan (nrep matvecs of x into y per variant, each
+ * pass timed by its own HIP events, as fh_matvec_profile), keeps the fastest of each pass and
+ * returns the times in ms (proj_ms[4], cols_ms[2], rows_ms[2]; any may be NULL) and the choice.
+ * FH_ERR_UNSUPPORTED where the plan runs the runtime-radix kernels. No reference counterpart (the
+ * reference calls NumPy's FFT). */
+int fh_plan_set_schedule(fh_plan* plan, int dtype, int proj_variant, int cols_variant, int rows_variant);
+int fh_plan_get_schedule(const fh_plan* plan, int dtype, int* proj_variant, int* cols_variant, int* rows_variant);
 int fh_plan_autotune(fh_plan* plan, int dtype, const void* x, void* y, int nrep, double* proj_ms, double* cols_ms,
-                     int* proj_variant, int* cols_variant);
+                     double* rows_ms, int* proj_variant, int* cols_variant, int* rows_variant);
 
 /* ---- Batched 2-D scalar homogenization (many samples, uneven grids, one launch) ----------
  * Each sample s is the reference's scalar GaNi problem on its own grid

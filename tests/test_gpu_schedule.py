@@ -1,8 +1,8 @@
 """Schedule variants of the fixed-length passes and the plan autotuner (fh_plan_set_schedule,
-fh_plan_autotune): every variant computes the same transforms per pencil, so the matvec and the
-mixed CG must be equal to the last bit across variants; the autotuner must pick one of them and
-leave the results unchanged. The matvec itself is pinned to the oracle at variant 0 by
-test_gpu_kernels.py."""
+fh_plan_autotune): every variant computes the same transforms per pencil, so the matvec must be
+equal to the last bit across variants; the autotuner must pick one of them, and a mixed CG after it
+converges alike (the row variant only re-blocks the CG's partial sums). The matvec itself is
+pinned to the oracle at variant 0 by test_gpu_kernels.py."""
 import numpy as np
 import pytest
 
@@ -24,23 +24,26 @@ def test_schedule_variants_are_bitwise_equal(N, dt):
     ref = None
     for pv in range(4):
         for cv in range(2):
-            pl.set_schedule(dt, pv, cv)
-            assert pl.schedule(dt) == (pv, cv)
-            pl.matvec(x, y)
-            got = y.to_host()
-            if ref is None:
-                ref = got
-            else:
-                assert np.array_equal(got, ref), (pv, cv)
-    print('%s %s: 8 schedules equal' % (N, np.dtype(dt).name), flush=True)
+            for rv in range(2):
+                pl.set_schedule(dt, pv, cv, rv)
+                assert pl.schedule(dt) == (pv, cv, rv)
+                pl.matvec(x, y)
+                got = y.to_host()
+                if ref is None:
+                    ref = got
+                else:
+                    assert np.array_equal(got, ref), (pv, cv, rv)
+    print('%s %s: 16 schedules equal' % (N, np.dtype(dt).name), flush=True)
 
 
 def test_schedule_rejects_bad_variants():
     pl = nat.Plan((15, 15, 15), np.ones(3), 'scalar', 'primal')
     with pytest.raises(nat.FhError):
-        pl.set_schedule(np.float32, 4, 0)
+        pl.set_schedule(np.float32, 4, 0, 0)
     with pytest.raises(nat.FhError):
-        pl.set_schedule(np.float64, 0, 2)
+        pl.set_schedule(np.float64, 0, 2, 0)
+    with pytest.raises(nat.FhError):
+        pl.set_schedule(np.float64, 0, 0, 2)
 
 
 @pytest.mark.parametrize('N', [(81, 81, 81), (105, 135)])
@@ -61,11 +64,14 @@ def test_autotune_picks_a_variant_and_keeps_results(N):
     for dt in (np.float32, np.float64):
         res = pl.autotune(dt, nrep=3)
         print(N, np.dtype(dt).name, res, flush=True)
-        assert 0 <= res['proj_variant'] < 4 and 0 <= res['cols_variant'] < 2
-        assert all(t > 0 for t in res['proj_ms'])
-        assert pl.schedule(dt) == (res['proj_variant'], res['cols_variant'])
+        assert 0 <= res['proj_variant'] < 4 and 0 <= res['cols_variant'] < 2 and 0 <= res['rows_variant'] < 2
+        assert all(t > 0 for t in res['proj_ms']) and all(t > 0 for t in res['rows_ms'])
+        assert pl.schedule(dt) == (res['proj_variant'], res['cols_variant'], res['rows_variant'])
         assert res['proj_ms'][res['proj_variant']] == min(res['proj_ms'])
     x1 = nat.DeviceArray.zeros((d,) + N)
     info1 = pl.cg_solve(B, x1, tol=1e-8, maxiter=500, mode='mixed', delta=1e-4)
-    assert info1['kit'] == info0['kit']
-    assert np.array_equal(x1.to_host(), x0.to_host())
+    # the row variant changes how the CG's partial sums are blocked (rounding only): the solve
+    # converges alike, within the solver tolerance
+    assert abs(info1['kit'] - info0['kit']) <= 2 and info1['norm_res'] <= 1e-8
+    a0 = x0.to_host()
+    assert np.abs(x1.to_host() - a0).max() < 1e-6 * np.abs(a0).max()
