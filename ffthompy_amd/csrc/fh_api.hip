@@ -888,7 +888,10 @@ fh::pot::PotParams pot_params(const fh_plan* pl) {
   g.Y[1] = pl->Y[pl->dim == 3 ? 1 : 0];
   g.Y[2] = pl->Y[pl->dim - 1];
   g.banded = pl->banded;
-  for (int a = 0; a < 3; ++a) g.band[a] = pl->band[a];
+  for (int a = 0; a < 3; ++a) {
+    g.band[a] = pl->band[a];
+    g.tpiY[a] = (double)(2.0L * 3.14159265358979323846264338327950288L / (long double)g.Y[a]);
+  }
   return g;
 }
 

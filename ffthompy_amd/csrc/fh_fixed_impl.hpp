@@ -362,9 +362,10 @@ int cols_g_fx_launch(const LaunchCtx& c, int axis, int nc, cpx<T>* data, const G
 
 template <typename T, class FP>
 int rows_rt_fx_launch(const LaunchCtx& c, int nc, cpx<T>* spec, const T* coef, T scale, const int* halt) {
-  // half the rows of the one-way passes: the round trip holds its pencils twice as long, and
-  // the smaller LDS tile lets more workgroups share a CU
-  constexpr int rpb = FxShape<FP, T>::RPB >= 4 ? FxShape<FP, T>::RPB / 2 : 2;
+  // fewer rows than the one-way passes: the round trip holds its pencils twice as long, and the
+  // smaller LDS tile lets more workgroups share a CU; 6 rows for the long pencils (TPP >= 16:
+  // 3 * NC * TPP threads fill the last wavefront better than 4 rows do), half the rows otherwise
+  constexpr int rpb = FP::TPP >= 16 ? 6 : (FxShape<FP, T>::RPB >= 4 ? FxShape<FP, T>::RPB / 2 : 2);
   const int npen = (rpb / 2) * nc;
   const int nthr = npen * FP::TPP;
   if (nthr > 1024) return 0;
@@ -402,10 +403,15 @@ int fixed_rows_rt(const LaunchCtx& c, int nc, cpx<T>* spec, const T* coef, T sca
   }
 }
 
-// frequency points per tile of the potential's strided passes (the default projection tile)
+// frequency points per tile of the potential's strided passes: 6 for the long pencils
+// (TPP >= 16: TPP * 6 * NC threads fill the last wavefront of a workgroup better than 4 do),
+// otherwise the default projection tile; halved until the workgroup fits
 template <typename T, class FP, int NC>
 constexpr int pot_points() {
-  return proj_points<T, FP, NC, 0>();
+  if (FP::TPP < 16) return proj_points<T, FP, NC, 0>();
+  int f = 6;
+  while (f > 1 && FP::TPP * f * NC > 1024) f /= 2;
+  return f;
 }
 
 template <typename T, class FP, int NC>

@@ -21,6 +21,7 @@ struct PotParams {
   int dim;        // gradient components
   int ax0;        // view axis of component 0 (3 - dim)
   double Y[3];    // cell size per view axis
+  double tpiY[3]; // 2 pi / Y per view axis (the symbol is 2 pi k / Y)
   int banded;     // 1: only |k_a| <= band[a] is kept (the coarse grid of a Ga solve)
   int band[3];
 };
@@ -36,21 +37,26 @@ __device__ __forceinline__ void freq_of(long long t, const PotParams& g, int (&k
   k[2] = ((g.n2 & 1) == 0 && 2 * i2 == g.n2) ? -i2 : i2;
 }
 
-// q_j = 2 pi xi_j on the gradient's axes; returns false outside the band.
-__device__ __forceinline__ bool grad_symbol(const int (&k)[3], const PotParams& g, double (&q)[3], double& P) {
+// q_j = 2 pi xi_j = k_j (2 pi / Y_j) on the gradient's axes and P = 1 / |q| (1 at xi = 0), in the
+// arithmetic of T: double with an IEEE square root and division; float with the hardware
+// reciprocal square root (the single-precision CG of the mixed solve); false outside the band.
+template <typename T>
+__device__ __forceinline__ bool grad_symbol(const int (&k)[3], const PotParams& g, T (&q)[3], T& P) {
   if (g.banded)
     for (int a = 0; a < 3; ++a)
       if (abs(k[a]) > g.band[a]) return false;
-  const double twopi = 6.283185307179586;
-  double k2 = 0.0;
+  T k2 = T(0);
   bool zero = true;
   for (int j = 0; j < g.dim; ++j) {
     const int a = g.ax0 + j;
-    q[j] = twopi * ((double)k[a] / g.Y[a]);
+    q[j] = T(k[a]) * T(g.tpiY[a]);
     k2 += q[j] * q[j];
     zero = zero && k[a] == 0;
   }
-  P = zero ? 1.0 : 1.0 / sqrt(k2);
+  if constexpr (sizeof(T) == 4)
+    P = zero ? T(1) : __builtin_amdgcn_rsqf(k2);
+  else
+    P = zero ? T(1) : T(1) / sqrt(k2);
   return true;
 }
 
@@ -72,8 +78,8 @@ k_pot_grad(const cpx<T>* __restrict__ u, cpx<T>* __restrict__ spec, PotParams g,
        t += (long long)gridDim.x * blockDim.x) {
     int k[3];
     freq_of(t, g, k);
-    double q[3] = {0.0, 0.0, 0.0}, P = 1.0;
-    const bool in = grad_symbol(k, g, q, P);
+    T q[3] = {T(0), T(0), T(0)}, P = T(1);
+    const bool in = grad_symbol<T>(k, g, q, P);
     cpx<T> v = u[t];
     if (with_P) v = mk(T(P) * v.x, T(P) * v.y);
     for (int j = 0; j < g.dim; ++j)
@@ -94,9 +100,9 @@ k_pot_div(const cpx<T>* __restrict__ spec, cpx<T>* __restrict__ out, PotParams g
        t += (long long)gridDim.x * blockDim.x) {
     int k[3];
     freq_of(t, g, k);
-    double q[3] = {0.0, 0.0, 0.0}, P = 1.0;
+    T q[3] = {T(0), T(0), T(0)}, P = T(1);
     cpx<T> o = mk(T(0), T(0));
-    if (grad_symbol(k, g, q, P)) {
+    if (grad_symbol<T>(k, g, q, P)) {
       T re = T(0), im = T(0);
       for (int j = 0; j < g.dim; ++j) {
         const cpx<T> s = spec[t * g.dim + j];

@@ -65,11 +65,11 @@ k_pot_cols_in(cpx<T>* __restrict__ p, const cpx<T>* __restrict__ r, const CGStat
       }
       int k[3];
       col_freq(i, fp, g, k);
-      double qv[3] = {0.0, 0.0, 0.0}, Pw = 1.0;
-      if (grad_symbol(k, g, qv, Pw)) {
-        const cpx<T> w = mk(T(Pw) * u.x, T(Pw) * u.y);
+      T qv[3] = {T(0), T(0), T(0)}, Pw = T(1);
+      if (grad_symbol<T>(k, g, qv, Pw)) {
+        const cpx<T> w = mk(Pw * u.x, Pw * u.y);
 #pragma unroll
-        for (int c = 0; c < NC; ++c) v[c] = mk(-T(qv[c]) * w.y, T(qv[c]) * w.x);
+        for (int c = 0; c < NC; ++c) v[c] = mk(-qv[c] * w.y, qv[c] * w.x);
       }
     }
 #pragma unroll
@@ -123,17 +123,17 @@ k_pot_cols_out(const cpx<T>* __restrict__ data, long long inner, long long total
     if (f * NC < ncol) {
       int k[3];
       col_freq(i, fp, g, k);
-      double qv[3] = {0.0, 0.0, 0.0}, Pw = 1.0;
+      T qv[3] = {T(0), T(0), T(0)}, Pw = T(1);
       cpx<T> o = mk(T(0), T(0));
-      if (grad_symbol(k, g, qv, Pw)) {
+      if (grad_symbol<T>(k, g, qv, Pw)) {
         T re = T(0), im = T(0);
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
           const cpx<T> v = s[i * P + f * NC + c];
-          re += -T(qv[c]) * v.y;
-          im += T(qv[c]) * v.x;
+          re += -qv[c] * v.y;
+          im += qv[c] * v.x;
         }
-        const T sc = sgn * T(Pw);
+        const T sc = sgn * Pw;
         o = mk(sc * re, sc * im);
       }
       const long long idx = (long long)i * plane + fp;
