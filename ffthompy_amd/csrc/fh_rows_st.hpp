@@ -421,8 +421,9 @@ k_rows_inv_st(const cpx<T>* __restrict__ in, T* __restrict__ out, long long nrow
 // transformed back (times scale), multiplied by the isotropic coefficient and transformed
 // forward as in k_rows_fwd_st; the result overwrites the block's own rows. A pencil holds the
 // same two real rows (row, component) in both directions, so the product needs no exchange
-// between pencils and the real field never leaves LDS. a(x) is read straight from HBM: the
-// block's rows are one contiguous run, shared by the NC pencils of a row pair.
+// between pencils and the real field never leaves LDS. a(x) of the block's rows (one contiguous
+// run, shared by the NC pencils of a row pair) is staged in LDS behind the pencils by coalesced
+// loads issued with the spectrum's.
 template <typename T, class FP, int NC>
 __global__ void __launch_bounds__(1024)
 k_rows_rt_st(cpx<T>* __restrict__ spec, const T* __restrict__ coef, long long nrows, long long n, int rpb,
@@ -438,6 +439,12 @@ k_rows_rt_st(cpx<T>* __restrict__ spec, const T* __restrict__ coef, long long nr
   const long long row0 = (long long)xcd_remap(blockIdx.x, gridDim.x) * rpb;
   const int nr = (int)min((long long)rpb, nrows - row0);
   for (int e = tid; e < L; e += nthr) twl[e] = twg[e];
+  // the block's coefficient rows behind the pencils (rows past nr: zeros, never used)
+  T* sa = reinterpret_cast<T*>(smem_raw + (size_t)S * L * sizeof(cpx<T>));
+  {
+    const __amdgpu_buffer_rsrc_t rco = make_rsrc(coef + row0 * L, (unsigned long long)nr * L * sizeof(T));
+    for (int e = tid; e < rpb * L; e += nthr) sa[e] = bload_r<T>(rco, (unsigned)e * (unsigned)sizeof(T));
+  }
   // Hermitian completion (rows past nr read 0 through the buffer range)
   const int nrp = rpb >> 1;
   constexpr unsigned CS = sizeof(cpx<T>);
@@ -475,8 +482,8 @@ k_rows_rt_st(cpx<T>* __restrict__ spec, const T* __restrict__ coef, long long nr
     const int rp_ = pen / NC, m_ = pen - rp_ * NC;
     const int rowa = 2 * rp_ + (2 * m_) / NC, rowb = 2 * rp_ + (2 * m_ + 1) / NC;
     const bool va = rowa < nr, vb = rowb < nr;
-    const T* aa = coef + (row0 + (va ? rowa : 0)) * L;
-    const T* ab = coef + (row0 + (vb ? rowb : 0)) * L;
+    const T* aa = sa + rowa * L;
+    const T* ab = sa + rowb * L;
     FxStage<T, FP, FP::NST - 1, +1> sl;
     sl.load(t, [&](int j) { return s[j * S + pen]; });
     __syncthreads();
