@@ -18,6 +18,7 @@ FH_SOLVE_FP64, FH_SOLVE_MIXED = 0, 1
 FH_COEF_ISO, FH_COEF_LAME, FH_COEF_FULL = 1, 2, 3
 FH_ITER_RICHARDSON, FH_ITER_CHEBYSHEV = 1, 2
 FH_CG_OK, FH_CG_BREAKDOWN = 0, 1
+FH_FORM_R, FH_FORM_FULL, FH_FORM_CENTRED = 0, 1, 2
 
 _c_int_p = ctypes.POINTER(ctypes.c_int)
 _c_double_p = ctypes.POINTER(ctypes.c_double)
@@ -58,6 +59,15 @@ SIGNATURES = {
     'fh_axpby': [ctypes.c_int, ctypes.c_size_t, ctypes.c_double, _vp, ctypes.c_double, _vp],
     'fh_dot': [ctypes.c_int, ctypes.c_size_t, _vp, _vp, _c_double_p],
     'fh_add_constant': [ctypes.c_int, ctypes.c_size_t, ctypes.c_int, _c_double_p, _vp],
+    'fh_spectrum_half_to_full': [ctypes.c_int, ctypes.c_int, _c_int_p, ctypes.c_longlong, _vp, _vp, ctypes.c_double,
+                                 ctypes.c_int],
+    'fh_spectrum_full_to_half': [ctypes.c_int, ctypes.c_int, _c_int_p, ctypes.c_longlong, _vp, _vp, ctypes.c_double,
+                                 ctypes.c_int, ctypes.c_int],
+    'fh_fftshift': [ctypes.c_int, ctypes.c_int, _c_int_p, ctypes.c_longlong, _vp, _vp, ctypes.c_int],
+    'fh_spectrum_resize': [ctypes.c_int, ctypes.c_int, _c_int_p, _c_int_p, ctypes.c_longlong, _vp, _vp, ctypes.c_int],
+    'fh_project_spectrum': [ctypes.c_int, ctypes.c_int, _c_int_p, _c_double_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                            ctypes.c_int, _vp, _vp],
+    'fh_pointwise_inverse': [ctypes.c_int, ctypes.c_int, ctypes.c_longlong, _vp, _vp],
     'fh_last_timing': [_vp, _c_double_p, _c_double_p],
     'fh_last_solve_updates': [_vp, _c_int_p],
     'fh_matvec_profile': [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int, _c_double_p, _c_double_p,
@@ -319,6 +329,126 @@ def check_array(arr, size, dtypes, what):
 
 _REAL = (np.float64, np.float32)
 _COMPLEX_OF = {np.dtype(np.float64): np.complex128, np.dtype(np.float32): np.complex64}
+
+
+# ---- spectrum forms and grid changes (ffthompy/tensors/objects.py:135-186, 247-251, 406-511) ----
+
+FORM_CODES = {'r': FH_FORM_R, 0: FH_FORM_FULL, 'c': FH_FORM_CENTRED}
+_REAL_OF = {np.dtype(np.complex128): np.float64, np.dtype(np.complex64): np.float32}
+
+
+def form_code(fft_form):
+    """FH_FORM_* of the reference's fft_form 'r', 0 or 'c'."""
+    if isinstance(fft_form, str) and fft_form in ('r', 'c'):
+        return FORM_CODES[fft_form]
+    if isinstance(fft_form, (int, np.integer)) and not isinstance(fft_form, bool) and fft_form == 0:
+        return FH_FORM_FULL
+    raise FhError("fft_form must be 'r', 0 or 'c', got %r" % (fft_form,), FH_ERR_ARG)
+
+
+def _grid(N):
+    N = [int(n) for n in np.atleast_1d(N)]
+    if not 1 <= len(N) <= 3 or min(N) < 1:
+        raise FhError('grid must have 1 to 3 positive lengths, got %s' % (N,), FH_ERR_ARG)
+    return N, (ctypes.c_int * len(N))(*N)
+
+
+def _half(N):
+    return int(np.prod(N[:-1], dtype=np.int64)) * (N[-1] // 2 + 1)
+
+
+def _cplx_code(arr, what):
+    if arr.dtype not in _REAL_OF:
+        raise FhError('%s must be complex128 or complex64, got %s' % (what, arr.dtype), FH_ERR_ARG)
+    return dtype_code(_REAL_OF[arr.dtype])
+
+
+def spectrum_half_to_full(half, N, count, scale=1., centred=False):
+    """Full spectrum (form 0, or 'c' when centred) of `count` 'r' half spectra, times scale."""
+    N, cN = _grid(N)
+    code = _cplx_code(half, 'half spectrum')
+    check_array(half, count * _half(N), (half.dtype,), 'half spectrum')
+    out = DeviceArray((int(count) * int(np.prod(N)),), half.dtype)
+    call('fh_spectrum_half_to_full', code, len(N), cN, ctypes.c_longlong(count), half.ptr, out.ptr,
+         ctypes.c_double(scale), int(bool(centred)))
+    return out
+
+
+def spectrum_full_to_half(full, N, count, scale=1., centred=False, hermitian=False):
+    """'r' half spectra (times scale) of `count` full spectra (form 0, or 'c' when centred);
+    hermitian takes the Hermitian part first."""
+    N, cN = _grid(N)
+    code = _cplx_code(full, 'full spectrum')
+    check_array(full, count * int(np.prod(N)), (full.dtype,), 'full spectrum')
+    out = DeviceArray((int(count) * _half(N),), full.dtype)
+    call('fh_spectrum_full_to_half', code, len(N), cN, ctypes.c_longlong(count), full.ptr, out.ptr,
+         ctypes.c_double(scale), int(bool(centred)), int(bool(hermitian)))
+    return out
+
+
+def fftshift(x, N, count, inverse=False):
+    """numpy.fft.fftshift (or ifftshift) over the grid axes of `count` fields of grid N."""
+    N, cN = _grid(N)
+    if x.dtype.itemsize not in (4, 8, 16):
+        raise FhError('unsupported dtype %s' % x.dtype, FH_ERR_ARG)
+    check_array(x, count * int(np.prod(N)), (x.dtype,), 'field')
+    out = DeviceArray(x.shape, x.dtype)
+    call('fh_fftshift', x.dtype.itemsize, len(N), cN, ctypes.c_longlong(count), x.ptr, out.ptr, int(bool(inverse)))
+    return out
+
+
+def spectrum_resize(x, N, M, count, enlarge):
+    """Centred spectra of grid N zero-padded (enlarge) or cropped to grid M."""
+    N, cN = _grid(N)
+    M, cM = _grid(M)
+    if len(M) != len(N):
+        raise FhError('grids N and M differ in dimension', FH_ERR_ARG)
+    for n, m in zip(N, M):
+        if (enlarge and m < n + (1 - n % 2)) or (not enlarge and m > n):
+            raise FhError('%s from grid %s to %s: every axis needs %s' % (
+                'enlarging' if enlarge else 'decreasing', tuple(N), tuple(M),
+                'M >= N (odd N) or M >= N + 1 (even N)' if enlarge else 'M <= N'), FH_ERR_ARG)
+    code = _cplx_code(x, 'spectrum')
+    check_array(x, count * int(np.prod(N)), (x.dtype,), 'spectrum')
+    out = DeviceArray((int(count) * int(np.prod(M)),), x.dtype)
+    call('fh_spectrum_resize', code, len(N), cN, cM, ctypes.c_longlong(count), x.ptr, out.ptr, int(bool(enlarge)))
+    return out
+
+
+def project_spectrum(x, N, Y, physics, which, fft_form='r', null_nyq=True):
+    """G1 (which 1) or G2 (which 2) of the scalar or elasticity problem on component spectra of
+    any fft_form, NyqNul as null_nyq (fh_project_spectrum)."""
+    N, cN = _grid(N)
+    d = len(N)
+    ph = {'scalar': FH_SCALAR, 'elasticity': FH_ELASTICITY}.get(physics)
+    if ph is None or d not in (2, 3):
+        raise FhError('physics must be scalar or elasticity on a 2-D or 3-D grid', FH_ERR_ARG)
+    form = form_code(fft_form)
+    nc = d if ph == FH_SCALAR else d * (d + 1) // 2
+    nspec = _half(N) if form == FH_FORM_R else int(np.prod(N))
+    code = _cplx_code(x, 'spectrum')
+    check_array(x, nc * nspec, (x.dtype,), 'spectrum')
+    Yd = np.ascontiguousarray(np.atleast_1d(Y), dtype=np.float64)
+    if Yd.size != d:
+        raise FhError('Y must have %d entries' % d, FH_ERR_ARG)
+    out = DeviceArray(x.shape, x.dtype)
+    call('fh_project_spectrum', code, d, cN, Yd.ctypes.data_as(_c_double_p), ph, int(which), form,
+         int(bool(null_nyq)), x.ptr, out.ptr)
+    return out
+
+
+def pointwise_inverse(A):
+    """Per-voxel inverse of a (k, k) + grid device field (1 <= k <= 6)."""
+    if not isinstance(A, DeviceArray) or len(A.shape) < 3 or A.shape[0] != A.shape[1]:
+        raise FhError('pointwise_inverse needs a (k, k) + grid DeviceArray', FH_ERR_ARG)
+    k = A.shape[0]
+    if k > 6:
+        raise FhError('pointwise_inverse handles k <= 6, got %d' % k, FH_ERR_ARG)
+    dtype_code(A.dtype)
+    nvox = A.size // (k * k)
+    out = DeviceArray(A.shape, A.dtype)
+    call('fh_pointwise_inverse', dtype_code(A.dtype), k, ctypes.c_longlong(nvox), A.ptr, out.ptr)
+    return out
 
 
 def energy_matrix_noalloc(kind, coef, fields):

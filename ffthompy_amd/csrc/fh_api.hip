@@ -14,6 +14,7 @@
 #include "fh_kernels.hpp"
 #include "fh_launch.hpp"
 #include "fh_batch.hpp"
+#include "fh_forms.hpp"
 #include "fh_ga.hpp"
 #include "fh_iter.hpp"
 #include "fh_micro.hpp"
@@ -1656,6 +1657,124 @@ int fh_add_constant(int dtype, size_t nvox, int ncomp, const double* E, void* y)
   HIPCHK(hipGetLastError());
   HIPCHK(hipDeviceSynchronize());
   return FH_OK;
+}
+
+// ---- spectrum forms and grid changes (fh_forms.hpp) ------------------------------------------
+
+namespace {
+
+// 3-D view of a grid of dimension 1..3 (leading axes of length 1); false on bad sizes
+bool view3(int dim, const int* N, fh::forms::Grid3& g) {
+  if (dim < 1 || dim > 3 || !N) return false;
+  int v[3] = {1, 1, 1};
+  for (int a = 0; a < dim; ++a) {
+    if (N[a] < 1) return false;
+    v[3 - dim + a] = N[a];
+  }
+  if ((long long)v[0] * v[1] * v[2] > (1LL << 31)) return false;
+  g = {v[0], v[1], v[2], v[2] / 2 + 1};
+  return true;
+}
+
+int forms_done(hipError_t e, const char* what) {
+  if (e != hipSuccess) return fail(FH_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+  HIPCHK(hipDeviceSynchronize());
+  return FH_OK;
+}
+
+}  // namespace
+
+int fh_spectrum_half_to_full(int dtype, int dim, const int* N, long long count, const void* half, void* full,
+                             double scale, int centred) {
+  RETCHK(check_dtype(dtype));
+  fh::forms::Grid3 g;
+  if (!view3(dim, N, g) || count < 0) return fail(FH_ERR_ARG, "fh_spectrum_half_to_full: bad grid or count");
+  if (count == 0) return FH_OK;
+  if (!half || !full || half == full) return fail(FH_ERR_ARG, "fh_spectrum_half_to_full: bad pointers");
+  RETCHK(ensure_device());
+  return forms_done(fh::forms::half_to_full(dtype == FH_F64, g, count, half, full, scale, centred ? 1 : 0),
+                    "fh_spectrum_half_to_full");
+}
+
+int fh_spectrum_full_to_half(int dtype, int dim, const int* N, long long count, const void* full, void* half,
+                             double scale, int centred, int hermitian) {
+  RETCHK(check_dtype(dtype));
+  fh::forms::Grid3 g;
+  if (!view3(dim, N, g) || count < 0) return fail(FH_ERR_ARG, "fh_spectrum_full_to_half: bad grid or count");
+  if (count == 0) return FH_OK;
+  if (!half || !full || half == full) return fail(FH_ERR_ARG, "fh_spectrum_full_to_half: bad pointers");
+  RETCHK(ensure_device());
+  return forms_done(
+      fh::forms::full_to_half(dtype == FH_F64, g, count, full, half, scale, centred ? 1 : 0, hermitian ? 1 : 0),
+      "fh_spectrum_full_to_half");
+}
+
+int fh_fftshift(int elem_bytes, int dim, const int* N, long long count, const void* in, void* out, int inverse) {
+  fh::forms::Grid3 g;
+  if (!view3(dim, N, g) || count < 0) return fail(FH_ERR_ARG, "fh_fftshift: bad grid or count");
+  if (elem_bytes != 4 && elem_bytes != 8 && elem_bytes != 16) return fail(FH_ERR_ARG, "fh_fftshift: elem_bytes");
+  if (count == 0) return FH_OK;
+  if (!in || !out || in == out) return fail(FH_ERR_ARG, "fh_fftshift: bad pointers");
+  RETCHK(ensure_device());
+  return forms_done(fh::forms::shift(g, count, elem_bytes / 4, in, out, inverse ? 1 : 0), "fh_fftshift");
+}
+
+int fh_spectrum_resize(int dtype, int dim, const int* N, const int* M, long long count, const void* in, void* out,
+                       int enlarge) {
+  RETCHK(check_dtype(dtype));
+  fh::forms::Grid3 a, b;
+  if (!view3(dim, N, a) || !view3(dim, M, b) || count < 0)
+    return fail(FH_ERR_ARG, "fh_spectrum_resize: bad grids or count");
+  for (int ax = 0; ax < dim; ++ax) {
+    const bool ok = enlarge ? M[ax] >= N[ax] + (N[ax] % 2 == 0 ? 1 : 0) : M[ax] <= N[ax];
+    if (!ok)
+      return fail(FH_ERR_ARG, enlarge ? "fh_spectrum_resize: enlarging needs M >= N + 1 (even N) or M >= N (odd N)"
+                                      : "fh_spectrum_resize: decreasing needs M <= N");
+  }
+  if (count == 0) return FH_OK;
+  if (!in || !out || in == out) return fail(FH_ERR_ARG, "fh_spectrum_resize: bad pointers");
+  RETCHK(ensure_device());
+  return forms_done(fh::forms::resize_centred(dtype == FH_F64, a, b, count, in, out, enlarge ? 1 : 0),
+                    "fh_spectrum_resize");
+}
+
+int fh_project_spectrum(int dtype, int dim, const int* N, const double* Y, int physics, int which, int form,
+                        int null_nyq, const void* in, void* out) {
+  RETCHK(check_dtype(dtype));
+  fh::forms::Grid3 v;
+  if (dim < 2 || !view3(dim, N, v)) return fail(FH_ERR_ARG, "fh_project_spectrum: dim must be 2 or 3");
+  if (physics != FH_SCALAR && physics != FH_ELASTICITY) return fail(FH_ERR_ARG, "fh_project_spectrum: physics");
+  if (which != 1 && which != 2) return fail(FH_ERR_ARG, "fh_project_spectrum: which must be 1 or 2");
+  if (form != FH_FORM_R && form != FH_FORM_FULL && form != FH_FORM_CENTRED)
+    return fail(FH_ERR_ARG, "fh_project_spectrum: form");
+  if (!Y) return fail(FH_ERR_ARG, "fh_project_spectrum: null Y");
+  for (int a = 0; a < dim; ++a)
+    if (!(Y[a] > 0.0)) return fail(FH_ERR_ARG, "fh_project_spectrum: Y must be positive");
+  if (!in || !out || in == out) return fail(FH_ERR_ARG, "fh_project_spectrum: bad pointers");
+  RETCHK(ensure_device());
+  GParams g{};
+  g.dim = dim;
+  g.physics = physics;
+  g.dual = which == 2;
+  g.ncomp = physics == FH_SCALAR ? dim : dim * (dim + 1) / 2;
+  g.n0 = v.n0;
+  g.n1 = v.n1;
+  g.n2 = v.n2;
+  for (int a = 0; a < 3; ++a) g.iY[a] = 0.0;
+  for (int a = 0; a < dim; ++a) g.iY[3 - dim + a] = 1.0 / Y[a];
+  g.scale = 1.0;
+  g.banded = 0;
+  return forms_done(fh::forms::project_form(dtype == FH_F64, g, form, null_nyq ? 1 : 0, in, out),
+                    "fh_project_spectrum");
+}
+
+int fh_pointwise_inverse(int dtype, int k, long long nvox, const void* in, void* out) {
+  RETCHK(check_dtype(dtype));
+  if (k < 1 || k > 6 || nvox < 0) return fail(FH_ERR_ARG, "fh_pointwise_inverse: 1 <= k <= 6, nvox >= 0");
+  if (nvox == 0) return FH_OK;
+  if (!in || !out || in == out) return fail(FH_ERR_ARG, "fh_pointwise_inverse: bad pointers");
+  RETCHK(ensure_device());
+  return forms_done(fh::forms::pointwise_inverse(dtype == FH_F64, k, nvox, in, out), "fh_pointwise_inverse");
 }
 
 int fh_last_solve_updates(const fh_plan* pl, int* n_updates) {

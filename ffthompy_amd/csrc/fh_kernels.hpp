@@ -71,15 +71,12 @@ __device__ __forceinline__ bool out_of_band(int s0, int s1, int k2, const GParam
 //  elastic: G1 = Mandel of Gamma_ijkl = sym(delta_ik xi_j xi_l)/|xi|^2 - xi_i xi_j xi_k xi_l/|xi|^4
 //           (ffthompy/tensors/projection.py:46-48, equal to G1h+G1s of
 //           ffthompy/projections.py:114-267); G2 = I[xi != 0] - G1 (projections.py:239-240).
-//  Both vanish at xi = 0 and at Nyquist frequencies.
+//  Both vanish at xi = 0 and where `nyq` is set. apply_G_at takes the signed integer frequencies
+//  (s0, s1, s2) of the 3-D view; apply_G derives them from the indices of the 'r' layout and sets
+//  `nyq` at the Nyquist modes of even axes (the NyqNul projections) and outside a band.
 template <typename T>
-__device__ __forceinline__ void apply_G(cpx<T>* v, int k0, int k1, int k2, const GParams& g) {
-  bool nyq = false;
-  const int s0 = signed_freq(k0, g.n0, nyq);
-  const int s1 = signed_freq(k1, g.n1, nyq);
-  if ((g.n2 & 1) == 0 && 2 * k2 == g.n2) nyq = true;
-  if (g.banded && out_of_band(s0, s1, k2, g)) nyq = true;
-  T xi3[3] = {T(s0 * g.iY[0]), T(s1 * g.iY[1]), T(k2 * g.iY[2])};
+__device__ __forceinline__ void apply_G_at(cpx<T>* v, int s0, int s1, int s2, bool nyq, const GParams& g) {
+  T xi3[3] = {T(s0 * g.iY[0]), T(s1 * g.iY[1]), T(s2 * g.iY[2])};
   const T* xi = (g.dim == 3) ? xi3 : xi3 + 1;
   const int d = g.dim;
   T q = T(0);
@@ -133,6 +130,16 @@ __device__ __forceinline__ void apply_G(cpx<T>* v, int k0, int k1, int k2, const
     o[0] = gam(0, 0); o[1] = gam(1, 1); o[2] = r2 * gam(0, 1);
   }
   for (int c = 0; c < nc; ++c) v[c] = g.dual ? sc * (v[c] - o[c]) : sc * o[c];
+}
+
+template <typename T>
+__device__ __forceinline__ void apply_G(cpx<T>* v, int k0, int k1, int k2, const GParams& g) {
+  bool nyq = false;
+  const int s0 = signed_freq(k0, g.n0, nyq);
+  const int s1 = signed_freq(k1, g.n1, nyq);
+  if ((g.n2 & 1) == 0 && 2 * k2 == g.n2) nyq = true;
+  if (g.banded && out_of_band(s0, s1, k2, g)) nyq = true;
+  apply_G_at<T>(v, s0, s1, k2, nyq, g);
 }
 
 // y = A(x) v at one voxel (ffthompy/tensors/objects.py:223-232, multype 21).

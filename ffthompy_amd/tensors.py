@@ -6,6 +6,8 @@ NumPy array for coefficients held on the host), arithmetic runs as GPU kernels, 
 ``*`` between fields is the grid inner product sum(x*y)/prod(N)
 (ffthompy/tensors/objects.py:618-636).
 """
+import threading
+
 import numpy as np
 
 from . import _native as nat
@@ -32,7 +34,7 @@ class Tensor:
             self.shape = tuple(int(s) for s in shape)
             self.order = len(self.shape)
             if Fourier:
-                nf = self.N[:-1] + (self.N[-1] // 2 + 1,)
+                nf = self.N[:-1] + (self.N[-1] // 2 + 1,) if fft_form == 'r' else self.N
                 cdt = np.complex128 if np.dtype(dtype) == np.float64 else np.complex64
                 self.val = nat.DeviceArray.zeros(self.shape + nf, cdt)
             else:
@@ -123,14 +125,23 @@ class Tensor:
     def norm(self):
         return scalar_product(self, self) ** 0.5
 
+    @property
+    def fft_coef(self):
+        """prod(N) for the unnormalised 'r' form, 1 for forms 0 and 'c' (objects.py:47-66)."""
+        return float(np.prod(self.N)) if self.fft_form == 'r' else 1.
+
+    def mean_index(self):
+        """Index of the zero frequency: 0 for forms 'r' and 0, N//2 for 'c' (ffthompy/trigpol.py:220-224)."""
+        return tuple(n // 2 for n in self.N) if self.fft_form == 'c' else (0,) * len(self.N)
+
     def mean(self):
-        """Mean per component (objects.py:259-272); of an 'r' spectrum: Re(val[0]) / prod(N)."""
+        """Mean per component (objects.py:259-272); of a spectrum: Re(val[mean_index]) / fft_coef."""
         m = np.zeros(self.shape)
         if self.Fourier or not self.on_device:
             val = self.val.to_host() if self.on_device else np.asarray(self.val)
             for di in (np.ndindex(*self.shape) if self.order else [()]):
                 v = val[di]
-                m[di] = np.real(v[(0,) * v.ndim]) / float(np.prod(self.N)) if self.Fourier else np.mean(v)
+                m[di] = np.real(v[self.mean_index()]) / self.fft_coef if self.Fourier else np.mean(v)
             return m
         n = self.pN()
         ones = _ones(n, self.val.dtype)
@@ -153,6 +164,121 @@ class Tensor:
         mean = np.asarray(mean, dtype=np.float64)
         self.add_mean(-self.mean())
         return self.add_mean(mean)
+
+    # -- Fourier forms, origin, inverse and grid changes on device data
+    #    (objects.py:135-186, 247-251, 406-511); host-held values move to the device first
+    @property
+    def axes(self):
+        """Axes of the grid in ``val`` (objects.py:403-405)."""
+        return tuple(range(self.order, self.order + self.dim))
+
+    def _count(self):
+        return int(np.prod(self.shape)) if self.order else 1
+
+    def _dev(self):
+        if not self.on_device:
+            self.val = nat.DeviceArray.from_host(np.ascontiguousarray(self.val))
+        return self.val
+
+    def _layout(self, fft_form):
+        if self.Fourier and fft_form == 'r':
+            return self.shape + self.N[:-1] + (self.N[-1] // 2 + 1,)
+        return self.shape + self.N
+
+    def set_fft_form(self, fft_form='r', copy=False):
+        """Change the layout of a spectrum between the forms 'r', 0 and 'c' (objects.py:135-167):
+        'r' -> 0 / 'c' completes the Hermitian half and divides by prod(N) (then fftshift for 'c');
+        0 / 'c' -> 'r' keeps the first N[-1]//2+1 entries of the last axis times prod(N)."""
+        R = self.copy() if copy else self
+        src, dst = nat.form_code(R.fft_form), nat.form_code(fft_form)
+        if src == dst:
+            return R
+        if R.Fourier:
+            val, N, cnt, pN = R._dev(), R.N, R._count(), float(np.prod(R.N))
+            if src == nat.FH_FORM_R:
+                out = nat.spectrum_half_to_full(val, N, cnt, 1. / pN, centred=dst == nat.FH_FORM_CENTRED)
+            elif dst == nat.FH_FORM_R:
+                out = nat.spectrum_full_to_half(val, N, cnt, pN, centred=src == nat.FH_FORM_CENTRED)
+            else:
+                out = nat.fftshift(val, N, cnt, inverse=src == nat.FH_FORM_CENTRED)
+            R.val = out.reshape(R._layout(fft_form))
+        R.fft_form = fft_form
+        return R
+
+    def shift(self, origin=None):
+        """Move the real-space origin between index 0 and the centre (numpy fftshift / ifftshift
+        over the grid axes, objects.py:169-186)."""
+        assert not self.Fourier
+        if origin == self.origin:
+            return self
+        if origin is not None:
+            raise ValueError()
+        inverse = self.origin == 'c'
+        out = nat.fftshift(self._dev(), self.N, self._count(), inverse=inverse)
+        self.val = out.reshape(self.val.shape)
+        self.origin = 0 if inverse else 'c'
+        return self
+
+    def inv(self):
+        """Pointwise inverse of a square second-order field (objects.py:247-251), on the GPU."""
+        assert self.Fourier is False
+        assert self.order == 2
+        assert self.shape[0] == self.shape[1]
+        src = self.to_device()
+        return self.copy(name='inv({})'.format(self.name), val=nat.pointwise_inverse(src.val))
+
+    def fourier(self, Fourier=None, copy=False):
+        """Forward or inverse DFT of the tensor's fft_form (objects.py:406-426)."""
+        assert self.origin == 0
+        if self.Fourier == Fourier:
+            return self.copy() if copy else self
+        F = DFT(inverse=self.Fourier, N=self.N, fft_form=self.fft_form, Y=self.Y)
+        res = F(self.to_device() if copy else self._like(self._dev()))
+        if copy:
+            return self.copy(val=res.val, Fourier=not self.Fourier)
+        self.val = res.val
+        self.Fourier = not self.Fourier
+        return self
+
+    def _resize(self, M, enlarge):
+        M = tuple(int(m) for m in M)
+        fft_form = self.fft_form
+        self.set_fft_form('c')  # as the reference, the tensor itself is left in form 'c'
+        val = nat.spectrum_resize(self._dev(), self.N, M, self._count(), enlarge)
+        R = self.copy(val=val.reshape(self.shape + M), N=M, fft_form='c')
+        return R.set_fft_form(fft_form)
+
+    def enlarge(self, M):
+        """Zero-pad the spectrum to grid M; the Nyquist entries of even axes are split in halves
+        at -N/2 and +N/2 (objects.py:428-467)."""
+        assert self.Fourier
+        if np.allclose(self.N, M):
+            return self
+        return self._resize(M, True)
+
+    def decrease(self, M):
+        """Drop the frequencies outside grid M (objects.py:469-486)."""
+        assert self.Fourier
+        if np.allclose(self.N, M):
+            return self
+        return self._resize(M, False)
+
+    def project(self, M):
+        """Trigonometric interpolation onto grid M: enlarge when M > N on every axis, decrease
+        when M < N on every axis (objects.py:488-511)."""
+        if np.allclose(self.N, M):
+            return self
+        Fourier = self.Fourier
+        Y = self.copy() if Fourier else self.fourier(copy=True)
+        if np.all(np.greater(M, self.N)):
+            Y = Y.enlarge(M)
+        elif np.all(np.less(M, self.N)):
+            Y = Y.decrease(M)
+        else:
+            raise NotImplementedError()
+        if not Fourier:
+            Y = Y.fourier()
+        return Y
 
     def __repr__(self):
         return 'Tensor(name=%s, shape=%s, N=%s, Fourier=%s, on_device=%s)' % (
@@ -207,32 +333,61 @@ def _host(t):
     return t.val.to_host() if t.on_device else t.val
 
 
+_PLANS = {}
+
+
+def _fft_plan(N, Y):
+    """A plan of grid N for the transforms, kept per host thread (a plan is not re-entrant; a few
+    grids are kept)."""
+    key = (threading.get_ident(), tuple(int(n) for n in N))
+    pl = _PLANS.get(key)
+    if pl is None:
+        if len(_PLANS) >= 8:
+            _PLANS.pop(next(iter(_PLANS)))
+        pl = _PLANS[key] = nat.Plan(key[1], np.ones(len(key[1])) if Y is None else Y)
+    return pl
+
+
 class DFT:
-    """(Inverse) DFT over the grid axes, 'r' form: rfftn unnormalised, irfftn normalised
-    (ffthompy/tensors/operators.py:14-64, ffthompy/tensors/fft.py:39-43), on the GPU."""
+    """(Inverse) DFT over the grid axes on the GPU (ffthompy/tensors/operators.py:14-64,
+    ffthompy/tensors/fft.py:18-43), fft_form
+      'r': rfftn unnormalised / irfftn normalised;
+      0  : fftn / prod(N) / real(ifftn) * prod(N);
+      'c': form 0 with fftshift of the spectrum / ifftshift before the inverse.
+    Forms 0 and 'c' run the real transform and complete or halve the spectrum on the device; the
+    inverse takes the Hermitian part of the spectrum, which gives the real part of ifftn."""
 
     def __init__(self, inverse=False, N=None, fft_form='r', name=None, Y=None):
-        if fft_form != 'r':
-            raise NotImplementedError("the GPU DFT implements fft_form 'r'")
+        self.form = nat.form_code(fft_form)
         self.N = tuple(int(n) for n in N)
         self.inverse = inverse
         self.fft_form = fft_form
         self.name = name or ('iDFT' if inverse else 'DFT')
-        self._plan = nat.Plan(self.N, np.ones(len(self.N)) if Y is None else Y)
+        self._plan = _fft_plan(self.N, Y)
 
     def __call__(self, x):
         assert x.Fourier == self.inverse
         ncomp = int(np.prod(x.shape)) if x.order else 1
         x = x.to_device()
+        N, pN = self.N, float(np.prod(self.N))
+        nh = N[:-1] + (N[-1] // 2 + 1,)
+        centred = self.form == nat.FH_FORM_CENTRED
         if self.inverse:
             real = np.float64 if x.val.dtype == np.complex128 else np.float32
-            out = nat.DeviceArray(x.shape + self.N, real)
-            self._plan.irfftn(x.val, out, ncomp)
+            half = x.val
+            if self.form != nat.FH_FORM_R:
+                half = nat.spectrum_full_to_half(x.val, N, ncomp, pN, centred=centred, hermitian=True)
+            out = nat.DeviceArray(x.shape + N, real)
+            self._plan.irfftn(half.reshape(x.shape + nh), out, ncomp)
             return x._like(out, name='iF({0})'.format(x.name[:10]), Fourier=False)
+        assert nat.form_code(x.fft_form) == self.form
         cdt = np.complex128 if x.val.dtype == np.float64 else np.complex64
-        out = nat.DeviceArray(x.shape + self.N[:-1] + (self.N[-1] // 2 + 1,), cdt)
-        self._plan.rfftn(x.val, out, ncomp)
-        return x._like(out, name='F({0})'.format(x.name[:10]), Fourier=True)
+        half = nat.DeviceArray(x.shape + nh, cdt)
+        self._plan.rfftn(x.val, half, ncomp)
+        if self.form == nat.FH_FORM_R:
+            return x._like(half, name='F({0})'.format(x.name[:10]), Fourier=True)
+        full = nat.spectrum_half_to_full(half, N, ncomp, 1. / pN, centred=centred)
+        return x._like(full.reshape(x.shape + N), name='F({0})'.format(x.name[:10]), Fourier=True)
 
 
 class GpuOperator:

@@ -273,6 +273,41 @@ int fh_axpby(int dtype, size_t n, double a, const void* x, double b, void* y);
 int fh_dot(int dtype, size_t n, const void* x, const void* y, double* out);
 int fh_add_constant(int dtype, size_t nvox, int ncomp, const double* E, void* y);
 
+/* Spectrum forms and grid changes of device fields (no plan needed; dim = 1, 2 or 3 and N the grid,
+ * count fields of the given layout one after another; `in` and `out` must not overlap).
+ * FH_FORM_R is the 'r' half spectrum (unnormalised rfftn), FH_FORM_FULL the reference's fft_form 0
+ * (fftn / prod(N)), FH_FORM_CENTRED its 'c' (form 0 with fftshift). */
+enum { FH_FORM_R = 0, FH_FORM_FULL = 1, FH_FORM_CENTRED = 2 };
+/* full = scale * Hermitian completion of half, fftshifted when centred
+ * (Tensor.set_fft_form from 'r', ffthompy/tensors/objects.py:142-154). */
+int fh_spectrum_half_to_full(int dtype, int dim, const int* N, long long count, const void* half, void* full,
+                             double scale, int centred);
+/* half = scale * the first N[-1]//2+1 entries of the last axis of full (ifftshifted first when
+ * centred; Tensor.set_fft_form to 'r', objects.py:155-163); hermitian = 1 takes the Hermitian part
+ * (full[k] + conj(full[-k])) / 2, so that irfftn of half is the real part of ifftn of full
+ * (the inverse transforms of forms 0 and 'c', ffthompy/tensors/fft.py:24-37). */
+int fh_spectrum_full_to_half(int dtype, int dim, const int* N, long long count, const void* full, void* half,
+                             double scale, int centred, int hermitian);
+/* fftshift (inverse = 0) or ifftshift (inverse = 1) over the grid axes of count fields of
+ * elem_bytes-byte elements (4, 8 or 16): forms 0 <-> 'c' of a spectrum and Tensor.shift of a real
+ * field (objects.py:164-186). */
+int fh_fftshift(int elem_bytes, int dim, const int* N, long long count, const void* in, void* out, int inverse);
+/* Centred spectra (FH_FORM_CENTRED) from grid N to grid M: enlarge = 1 zero-pads, splitting the
+ * Nyquist entries of even axes (Tensor.enlarge, objects.py:428-467; needs M >= N + [N even] on
+ * every axis); enlarge = 0 crops (Tensor.decrease, objects.py:469-486; needs M <= N). */
+int fh_spectrum_resize(int dtype, int dim, const int* N, const int* M, long long count, const void* in, void* out,
+                       int enlarge);
+/* out = G_which in * (which 1: compatible, 2: equilibrated fields) on ncomp component spectra of the
+ * given form (scalar ncomp = dim, elasticity Mandel dim(dim+1)/2; dim 2 or 3): the projections of
+ * ffthompy/projections.py:9-112 (scalar) and 114-267 (elasticity, the sums G1h + G1s, G2h + G2s)
+ * for any fft_form, null_nyq = 1 for NyqNul=True (Nyquist modes of even axes set to zero),
+ * 0 for NyqNul=False (G at frequency -N/2 there). */
+int fh_project_spectrum(int dtype, int dim, const int* N, const double* Y, int physics, int which, int form,
+                        int null_nyq, const void* in, void* out);
+/* Pointwise inverse of a (k, k) + grid matrix field, 1 <= k <= 6, by Gauss-Jordan elimination
+ * without pivoting (Tensor.inv / get_inverse, objects.py:247-251, ffthompy/trigpol.py:120-159). */
+int fh_pointwise_inverse(int dtype, int k, long long nvox, const void* in, void* out);
+
 /* Device-side wall time of the last fh_matvec / fh_cg_solve on this plan, in ms
  * (HIP events on the plan stream), and the number of fp64 residual recomputations
  * of the last mixed-precision solve. Replaces the reference's Timer around a solve

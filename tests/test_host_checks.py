@@ -115,3 +115,39 @@ def test_cg_rejects_bad_precision_and_x0_before_device(monkeypatch):
     monkeypatch.setattr(Tensor, 'to_device', lambda self: self)
     with pytest.raises(ValueError):
         solver.CG(Op(), B, x0=x0, par={'precision': 'fp64'})
+
+
+def test_form_and_grid_changes_check_on_the_host(no_library_calls):
+    """Spectrum-form entry points reject bad forms, grids, sizes and dtypes before any call."""
+    for bad in ('x', 1, None, True):
+        with pytest.raises(nat.FhError):
+            nat.form_code(bad)
+    assert nat.form_code(np.int64(0)) == nat.FH_FORM_FULL and nat.form_code('c') == nat.FH_FORM_CENTRED
+    half = fake_array((2, 6, 4), np.complex128)  # N = (6, 7): half (6, 4)
+    with pytest.raises(nat.FhError):
+        nat.spectrum_half_to_full(half, (6, 7), 3)  # wrong count
+    with pytest.raises(nat.FhError):
+        nat.spectrum_half_to_full(fake_array((2, 6, 4)), (6, 7), 2)  # real, not complex
+    with pytest.raises(nat.FhError):
+        nat.spectrum_full_to_half(half, (6, 7), 2)  # a half spectrum is not a full one
+    with pytest.raises(nat.FhError):
+        nat.fftshift(fake_array((2, 5, 5), np.int16), (5, 5), 2)
+    full = fake_array((2, 6, 7), np.complex128)
+    with pytest.raises(nat.FhError):
+        nat.spectrum_resize(full, (6, 7), (6, 9), 2, enlarge=True)  # even axis needs M >= N + 1
+    with pytest.raises(nat.FhError):
+        nat.spectrum_resize(full, (6, 7), (7, 8), 2, enlarge=False)  # decreasing needs M <= N
+    with pytest.raises(nat.FhError):
+        nat.spectrum_resize(full, (6, 7), (7, 9, 3), 2, enlarge=True)
+    with pytest.raises(nat.FhError):
+        nat.project_spectrum(fake_array((3, 6, 4), np.complex128), (6, 7), (1., 1.), 'scalar', 1, 'r')
+    with pytest.raises(nat.FhError):
+        nat.project_spectrum(fake_array((2, 6, 7), np.complex128), (6, 7), (1., 1.), 'scalar', 1, 'r')
+    with pytest.raises(nat.FhError):
+        nat.project_spectrum(fake_array((2, 6, 4), np.complex128), (6, 7), (1.,), 'scalar', 1, 'r')
+    with pytest.raises(nat.FhError):
+        nat.project_spectrum(fake_array((1, 9), np.complex128), (9,), (1.,), 'scalar', 1, 0)
+    with pytest.raises(nat.FhError):
+        nat.pointwise_inverse(fake_array((7, 7, 4, 4)))
+    with pytest.raises(nat.FhError):
+        nat.pointwise_inverse(fake_array((3, 3, 4, 4), np.int32))

@@ -62,6 +62,39 @@ def test_elastic_projection_is_mandel_gamma(fft_form):
     assert np.abs((G2h.val + G2s.val) - (I0 - M)).max() < 1e-13
 
 
+def _nyquist_mask(N, fft_form):
+    """True where some even axis sits at its Nyquist frequency -N/2 (layout of fft_form)."""
+    xi = grid.frequencies(N, np.ones(len(N)), fft_form=fft_form)
+    m = np.zeros(tuple(len(x) for x in xi), dtype=bool)
+    for a, (x, n) in enumerate(zip(xi, N)):
+        if n % 2 == 0:
+            shp = [1] * len(N)
+            shp[a] = len(x)
+            m |= (x == -n // 2).reshape(shp)
+    return m
+
+
+@pytest.mark.parametrize('N', [(6, 8, 4), (5, 7, 9), (8, 6), (4, 7)])
+@pytest.mark.parametrize('fft_form', ['r', 0, 'c'])
+def test_projections_of_every_form_equal_closed_forms(N, fft_form):
+    """Scalar G1 = xi xi^T/|xi|^2 and elastic G1h+G1s = Mandel(Gamma) (ffthompy/tensors/projection.py)
+    on the frequencies of each layout, -N/2 at the Nyquist index: exactly with NyqNul=False, and
+    with NyqNul=True away from the Nyquist modes, which are zero."""
+    d = len(N)
+    Y = np.array([1., 2., 0.5])[:d]
+    mask = _nyquist_mask(N, fft_form)
+    C0, C1, C2 = projections.scalar_closed_form(N, Y, fft_form=fft_form)
+    Gam = projections.mandel_matrix_of(projections.gamma_small_strain(N, Y, fft_form=fft_form))
+    for nyq in (False, True):
+        _, G1, G2 = projections.scalar(N, Y, NyqNul=nyq, fft_form=fft_form)
+        _, a, b, c, e = projections.elasticity(N, Y, NyqNul=nyq, fft_form=fft_form)
+        for got, ref in ((G1.val, C1), (G2.val, C2), (a.val + b.val, Gam)):
+            if nyq:
+                assert np.abs(got[..., mask]).max(initial=0.) == 0.
+                got, ref = got[..., ~mask], ref[..., ~mask]
+            assert np.abs(got - ref).max() < 1e-13
+
+
 @pytest.mark.parametrize('N', [(5, 5), (5, 4), (3, 5, 7), (4, 4, 4)])
 def test_fft_forms_round_trip_and_dft_matrix(N):
     """ffthompy/tensors/unittest_operators.py:23-100 and matvecs/unittest_matvec.py:14-28."""
