@@ -47,10 +47,12 @@ struct RowStage {
   // starts W + (s0 mod W) values into its region and may be read up to W - 1 values past
   // its end), a whole number of chunks
   __host__ __device__ static constexpr int comp_stride(int rpb) { return (rpb * FP::L + 3 * W + W - 1) / W * W; }
-  // pencils are interleaved, element j of pencil q at j*S + q, with an odd stride S >= the
-  // pencil count: lanes on consecutive pencils hit consecutive banks, lanes walking j of one
-  // pencil hit distinct bank pairs
-  __host__ __device__ static constexpr int pen_stride(int rpb) { return ((rpb / 2) * NC) | 1; }
+  // pencils are interleaved, element j of pencil q at j*S + q, with S the pencil count: thread
+  // (t, q) = (tid / S, tid % S) of a stage reads and writes element i*S + q, so the lanes of a
+  // wavefront touch consecutive LDS words whenever their indices i advance with t; the
+  // Stockham index patterns of every fixed length then issue without bank conflicts (an odd
+  // stride doubled the LDS cycles of the 243-point passes through 2-way conflicts)
+  __host__ __device__ static constexpr int pen_stride(int rpb) { return (rpb / 2) * NC; }
   // bytes: max(pencils, staging) + staging of ncoef coefficient runs (forward pass only)
   __host__ __device__ static constexpr size_t lds_bytes(int rpb, int ncoef) {
     const size_t pen = (size_t)pen_stride(rpb) * FP::L * sizeof(cpx<T>);
@@ -254,7 +256,8 @@ k_rows_fwd_st(const T* __restrict__ in, const T* __restrict__ coef, cpx<T>* __re
   cpx<T>* obp = out + row0 * (n2h * NC);
   const int nrp_v = (nr + 1) >> 1;
   for (int e = tid; e < nrp_v * n2h; e += nthr) {
-    const int rp = e / n2h, k = e - rp * n2h;
+    // row pair fastest: the lanes' reads s[k*S + rp*NC + m] are NC elements apart
+    const int k = e / nrp_v, rp = e - k * nrp_v;
     const int kc = k ? L - k : 0;
     cpx<T> Y[2 * NC];
 #pragma unroll
@@ -316,7 +319,8 @@ k_rows_inv_st(const cpx<T>* __restrict__ in, T* __restrict__ out, long long nrow
   const __amdgpu_buffer_rsrc_t rsp =
       make_rsrc(in + row0 * (long long)n2h * NC, (unsigned long long)nr * n2h * NC * CS);
   for (int e = tid; e < nrp * n2h; e += nthr) {
-    const int rp = e / n2h, k = e - rp * n2h;
+    // row pair fastest: the lanes' writes s[k*S + rp*NC + m] are NC elements apart
+    const int k = e / nrp, rp = e - k * nrp;
     const bool self = (k == 0) || (2 * k == L);
     const unsigned o0 = (unsigned)(((2 * rp) * n2h + k) * NC) * CS;
     const unsigned o1 = o0 + (unsigned)(n2h * NC) * CS;
@@ -451,7 +455,8 @@ k_rows_rt_st(cpx<T>* __restrict__ spec, const T* __restrict__ coef, long long nr
   cpx<T>* blk = spec + row0 * (long long)n2h * NC;
   const __amdgpu_buffer_rsrc_t rsp = make_rsrc(blk, (unsigned long long)nr * n2h * NC * CS);
   for (int e = tid; e < nrp * n2h; e += nthr) {
-    const int rp = e / n2h, k = e - rp * n2h;
+    // row pair fastest: the lanes' writes s[k*S + rp*NC + m] are NC elements apart
+    const int k = e / nrp, rp = e - k * nrp;
     const bool self = (k == 0) || (2 * k == L);
     const unsigned o0 = (unsigned)(((2 * rp) * n2h + k) * NC) * CS;
     const unsigned o1 = o0 + (unsigned)(n2h * NC) * CS;
@@ -505,7 +510,8 @@ k_rows_rt_st(cpx<T>* __restrict__ spec, const T* __restrict__ coef, long long nr
   // untangle the paired rows into the AoS half spectrum (as k_rows_fwd_st)
   const int nrp_v = (nr + 1) >> 1;
   for (int e = tid; e < nrp_v * n2h; e += nthr) {
-    const int rp = e / n2h, k = e - rp * n2h;
+    // row pair fastest: the lanes' reads s[k*S + rp*NC + m] are NC elements apart
+    const int k = e / nrp_v, rp = e - k * nrp_v;
     const int kc = k ? L - k : 0;
     cpx<T> Y[2 * NC];
 #pragma unroll

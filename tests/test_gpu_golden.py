@@ -68,3 +68,37 @@ def test_gani_golden_on_gpu(case):
                 i, j = map(int, ij.split(','))
                 assert abs(AH[i, j] - v) < 1e-9
         assert np.abs(AH - ref.output['mat_' + pd][key]).max() < 1e-12
+
+
+@pytest.mark.parametrize('case', GANI, ids=['%s_%s' % (c[0], c[1]) for c in GANI])
+def test_device_residual_log_matches_oracle_callback_on_goldens(case):
+    """The device's true-residual log (fh_plan_set_residual_log, every = 1) against the oracle's
+    CallBack log (ffthompy/general/solver_pp.py:13-24, x0 first) of the same golden problem, at the
+    reference's 1e-9 absolute. The oracle's logs are pinned to the Krylov-Galerkin residuals by
+    tests/test_oracle_histories.py."""
+    f, name, conf, mats = case
+    print('case residual log golden', f, name, flush=True)
+    conf = dict(conf)
+    conf['postprocess'] = [{'kind': 'GaNi'}]
+    ref = OracleProblem(dict(conf), mats())
+    ref.calculate()
+    N = tuple(int(n) for n in conf['solve']['N'])
+    for pd in conf['solve']['primaldual']:
+        A = ref.output['A_' + pd].val
+        plan = nat.Plan(N, ref.Y, conf['physics'], pd)
+        nc = plan.ncomp
+        plan.set_coeff('full', A.reshape((nc * nc,) + N))
+        plan.set_residual_log(1)
+        for iL, res in enumerate(ref.output['res_' + pd]):
+            E = np.zeros(nc)
+            E[iL] = 1.
+            EN = nat.DeviceArray.zeros((nc,) + N)
+            nat.add_constant(EN, -E)
+            B = nat.DeviceArray((nc,) + N)
+            plan.matvec(EN, B)
+            x = nat.DeviceArray.zeros((nc,) + N)
+            info = plan.cg_solve(B, x, tol=conf['solver']['tol'], maxiter=1000, mode='fp64')
+            kits, norms = plan.residual_log()
+            cb = np.asarray(res['res_norm'])
+            assert info['kit'] == res['info']['kit'] and list(kits) == list(range(info['kit'] + 1))
+            assert np.abs(norms - cb).max() < 1e-9, (pd, iL, norms, cb)
