@@ -68,6 +68,7 @@ SIGNATURES = {
     'fh_project_spectrum': [ctypes.c_int, ctypes.c_int, _c_int_p, _c_double_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                             ctypes.c_int, _vp, _vp],
     'fh_pointwise_inverse': [ctypes.c_int, ctypes.c_int, ctypes.c_longlong, _vp, _vp],
+    'fh_spectrum_grad': [ctypes.c_int, ctypes.c_int, _c_int_p, _c_double_p, _vp, _vp],
     'fh_last_timing': [_vp, _c_double_p, _c_double_p],
     'fh_last_solve_updates': [_vp, _c_int_p],
     'fh_matvec_profile': [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int, _c_double_p, _c_double_p,
@@ -434,6 +435,19 @@ def project_spectrum(x, N, Y, physics, which, fft_form='r', null_nyq=True):
     out = DeviceArray(x.shape, x.dtype)
     call('fh_project_spectrum', code, d, cN, Yd.ctypes.data_as(_c_double_p), ph, int(which), form,
          int(bool(null_nyq)), x.ptr, out.ptr)
+    return out
+
+
+def spectrum_grad(x, N, Y):
+    """Gradient (d,) + half grid of a scalar 'r' spectrum of grid N (fh_spectrum_grad)."""
+    N, cN = _grid(N)
+    code = _cplx_code(x, 'spectrum')
+    check_array(x, _half(N), (x.dtype,), 'spectrum')
+    Yd = np.ascontiguousarray(np.atleast_1d(Y), dtype=np.float64)
+    if Yd.size != len(N):
+        raise FhError('Y must have %d entries' % len(N), FH_ERR_ARG)
+    out = DeviceArray((len(N),) + tuple(N[:-1]) + (N[-1] // 2 + 1,), x.dtype)
+    call('fh_spectrum_grad', code, len(N), cN, Yd.ctypes.data_as(_c_double_p), x.ptr, out.ptr)
     return out
 
 

@@ -1768,6 +1768,21 @@ int fh_project_spectrum(int dtype, int dim, const int* N, const double* Y, int p
                     "fh_project_spectrum");
 }
 
+int fh_spectrum_grad(int dtype, int dim, const int* N, const double* Y, const void* in, void* out) {
+  RETCHK(check_dtype(dtype));
+  fh::forms::Grid3 v;
+  if (!view3(dim, N, v)) return fail(FH_ERR_ARG, "fh_spectrum_grad: bad grid");
+  if (!Y) return fail(FH_ERR_ARG, "fh_spectrum_grad: null Y");
+  double tp[3] = {0.0, 0.0, 0.0};
+  for (int a = 0; a < dim; ++a) {
+    if (!(Y[a] > 0.0)) return fail(FH_ERR_ARG, "fh_spectrum_grad: Y must be positive");
+    tp[3 - dim + a] = (double)(2.0L * 3.14159265358979323846264338327950288L / (long double)Y[a]);
+  }
+  if (!in || !out || in == out) return fail(FH_ERR_ARG, "fh_spectrum_grad: bad pointers");
+  RETCHK(ensure_device());
+  return forms_done(fh::forms::spectrum_grad(dtype == FH_F64, v, dim, tp, in, out), "fh_spectrum_grad");
+}
+
 int fh_pointwise_inverse(int dtype, int k, long long nvox, const void* in, void* out) {
   RETCHK(check_dtype(dtype));
   if (k < 1 || k > 6 || nvox < 0) return fail(FH_ERR_ARG, "fh_pointwise_inverse: 1 <= k <= 6, nvox >= 0");
@@ -2830,10 +2845,6 @@ int fh_potential_solve(fh_plan* pl, int mode, const double* E, double tol, int m
   pl->res_norm.clear();
   if (pl->res_every > 0)
     return fail(FH_ERR_UNSUPPORTED, "the residual log (fh_plan_set_residual_log) is kept by fh_cg_solve only");
-  if (!pl->banded)
-    for (int a = 0; a < pl->dim; ++a)
-      if ((pl->Nu[a] & 1) == 0)
-        return fail(FH_ERR_UNSUPPORTED, "the potential formulation needs odd grids (even: Nyquist modes)");
   const bool mixed = mode == FH_SOLVE_MIXED;
   pl->last_graph_batches = 0;
   const fh::pot::PotParams g = pot_params(pl);

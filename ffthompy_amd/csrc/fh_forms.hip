@@ -199,6 +199,32 @@ k_project_form(const cpx<T>* __restrict__ in, cpx<T>* __restrict__ out, long lon
   }
 }
 
+// Gradient of a scalar 'r' spectrum, out[j][t] = 2 pi i xi_j in[t] for the d grid axes
+// (Tensor grad of a scalar, ffthompy/tensors/operators.py:227-260, with the frequencies of
+// ffthompy/trigpol.py:27-40: the Nyquist index of an even axis stands for -n/2). Component-major
+// output in the reference layout (d,) + half grid. tpiY[a] = 2 pi / Y of view axis a.
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_spectrum_grad(const cpx<T>* __restrict__ in, cpx<T>* __restrict__ out, Grid3 g, int dim, double t0, double t1,
+                double t2) {
+  const long long nh = (long long)g.n0 * g.n1 * g.n2h;
+  const double tp[3] = {t0, t1, t2};
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < nh;
+       t += (long long)gridDim.x * blockDim.x) {
+    int i0, i1, i2;
+    unflatten(t, g.n1, g.n2h, i0, i1, i2);
+    bool nyq = false;
+    const int k[3] = {form_freq(i0, g.n0, FORM_FULL, nyq), form_freq(i1, g.n1, FORM_FULL, nyq),
+                      form_freq(i2, g.n2, FORM_FULL, nyq)};
+    const cpx<T> v = in[t];
+    for (int j = 0; j < dim; ++j) {
+      const int a = 3 - dim + j;
+      const T q = T((double)k[a] * tp[a]);
+      out[j * nh + t] = mk(-q * v.y, q * v.x);
+    }
+  }
+}
+
 // Pointwise inverse of a (K, K) matrix field, entry (i, j) of voxel v at [(i*K + j)*nvox + v]:
 // Gauss-Jordan elimination without pivoting in the order of ffthompy/trigpol.py:120-159 (row m
 // normalised by its pivot and subtracted from the rows below, then a backward sweep).
@@ -311,6 +337,17 @@ hipError_t project_form(int dbl, const GParams& g, int form, int null_nyq, const
   else
     hipLaunchKernelGGL(k_project_form<float>, grid, dim3(256), 0, 0, (const cpx<float>*)in, (cpx<float>*)out, nspec,
                        l2, form, null_nyq, g);
+  return hipGetLastError();
+}
+
+hipError_t spectrum_grad(int dbl, Grid3 g, int dim, const double* tpiY, const void* in, void* out) {
+  const dim3 grid = blocks_for((long long)g.n0 * g.n1 * g.n2h);
+  if (dbl)
+    hipLaunchKernelGGL(k_spectrum_grad<double>, grid, dim3(256), 0, 0, (const cpx<double>*)in, (cpx<double>*)out, g,
+                       dim, tpiY[0], tpiY[1], tpiY[2]);
+  else
+    hipLaunchKernelGGL(k_spectrum_grad<float>, grid, dim3(256), 0, 0, (const cpx<float>*)in, (cpx<float>*)out, g,
+                       dim, tpiY[0], tpiY[1], tpiY[2]);
   return hipGetLastError();
 }
 

@@ -32,6 +32,8 @@ hipError_t shift(Grid3 g, long long count, int words, const void* in, void* out,
 hipError_t resize_centred(int dbl, Grid3 a, Grid3 b, long long count, const void* in, void* out, int enlarge);
 hipError_t project_form(int dbl, const GParams& g, int form, int null_nyq, const void* in, void* out);
 hipError_t pointwise_inverse(int dbl, int k, long long nvox, const void* in, void* out);
+// tpiY: 2 pi / Y per view axis (3 entries)
+hipError_t spectrum_grad(int dbl, Grid3 g, int dim, const double* tpiY, const void* in, void* out);
 
 }  // namespace forms
 }  // namespace fh

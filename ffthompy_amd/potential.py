@@ -99,20 +99,44 @@ def homog_GaNi_full_potential(Agani, Aga, pars):
     # the Gram matrix of d copies of one field has every entry <A XEN, XEN>
     if Aga is None:
         AH = plan.energy_matrix([X] * d)[0, 0]
-    else:
+    elif all(n % 2 for n in N):
+        # odd N: grad Fu is Hermitian, so enlarging it is interpolating the real field X
         ga = Aga.load(nat.Ga(plan))
         AH = ga.energy_matrix([X] * d)[0, 0]
+    else:
+        # even N: the Nyquist entries of grad Fu (frequency -N/2) are not Hermitian; as the
+        # reference (homogenisation.py:123-126) the gradient spectrum itself is enlarged to 2N-1
+        # (Tensor.project, Nyquist entries split in halves) and transformed there
+        ga = Aga.load(nat.Ga(plan))
+        M = ga.M
+        XM = _enlarged_gradient(Fu, N, M, Y)
+        nat.add_constant(XM, E)
+        AH = ga.energy_matrix([XM] * d, on_grid=True)[0, 0]
     return Struct(AH=AH, Fu=_fourier_tensor('Fu', Fu, N, Y), info=info, time=elapsed, pars=pars)
+
+
+def _enlarged_gradient(Fu, N, M, Y):
+    """F^-1 on grid M of grad(Fu) enlarged from N to M (Tensor.enlarge of the 'r' spectrum,
+    ffthompy/tensors/objects.py:428-467), on the device."""
+    d = len(N)
+    G = nat.spectrum_grad(Fu, N, Y)
+    full = nat.spectrum_half_to_full(G, N, d, 1. / float(np.prod(N)), centred=True)
+    big = nat.spectrum_resize(full, N, M, d, enlarge=True)
+    half = nat.spectrum_full_to_half(big, M, d, float(np.prod(M)), centred=True)
+    X = nat.DeviceArray((d,) + tuple(M))
+    Mh = tuple(M[:-1]) + (M[-1] // 2 + 1,)
+    nat.Plan(M, Y, 'scalar', 'primal').irfftn(half.reshape((d,) + Mh), X, d)
+    return X
 
 
 def _crop_spectrum(val, M, N):
     """'r' spectrum on grid M of a trigonometric polynomial with frequencies of grid N (odd) ->
-    its 'r' spectrum on N (the reference's Tensor.project/decrease, values times prod(N)/prod(M))."""
-    idx = []
-    for a, (m, n) in enumerate(zip(M, N)):
-        k = np.arange(n // 2 + 1) if a == len(N) - 1 else np.fft.ifftshift(np.arange(-(n // 2), n // 2 + 1))
-        idx.append(k % m)
-    return val[np.ix_(*idx)] * (float(np.prod(N)) / float(np.prod(M)))
+    its 'r' spectrum on N (the reference's Tensor.decrease of an 'r' spectrum, values times
+    prod(N)/prod(M)), on the device."""
+    full = nat.spectrum_half_to_full(val, M, 1, 1. / float(np.prod(M)), centred=True)
+    small = nat.spectrum_resize(full, M, N, 1, enlarge=False)
+    half = nat.spectrum_full_to_half(small, N, 1, float(np.prod(N)), centred=True)
+    return half.reshape(tuple(N[:-1]) + (N[-1] // 2 + 1,))
 
 
 def homog_Ga_full_potential(Aga, pars):
@@ -142,6 +166,6 @@ def homog_Ga_full_potential(Aga, pars):
     XEN = X.copy()
     nat.add_constant(XEN, E)
     AH = ga.energy_matrix([XEN] * d, on_grid=True)[0, 0]
-    Fu = _crop_spectrum(FuM.to_host(), M, N)
+    Fu = _crop_spectrum(FuM, M, N)
     return Struct(AH=AH, e=Tensor(name='e', val=X, order=1, Y=Y), Fu=_fourier_tensor('Fu', Fu, N, Y), info=info,
                   time=elapsed, pars=pars)

@@ -307,6 +307,11 @@ int fh_project_spectrum(int dtype, int dim, const int* N, const double* Y, int p
 /* Pointwise inverse of a (k, k) + grid matrix field, 1 <= k <= 6, by Gauss-Jordan elimination
  * without pivoting (Tensor.inv / get_inverse, objects.py:247-251, ffthompy/trigpol.py:120-159). */
 int fh_pointwise_inverse(int dtype, int k, long long nvox, const void* in, void* out);
+/* out = grad in: the dim-component 'r' spectrum (dim,) + half grid of 2 pi i xi_j times the scalar 'r'
+ * spectrum in, xi_j = k_j / Y_j with the Nyquist index of an even axis at k = -N/2 (the reference's
+ * grad of a scalar Tensor, ffthompy/tensors/operators.py:227-260; frequencies
+ * ffthompy/trigpol.py:27-40). dim 1 to 3. */
+int fh_spectrum_grad(int dtype, int dim, const int* N, const double* Y, const void* in, void* out);
 
 /* Device-side wall time of the last fh_matvec / fh_cg_solve on this plan, in ms
  * (HIP events on the plan stream), and the number of fp64 residual recomputations

@@ -10,7 +10,7 @@ the oracle (oracle/potential.py), fp64 both:
 * At 45^3 (random spheres, contrast 10) the potential value equals the gradient formulation's
   AH_11 from the device CG (both to tol 1e-10): size-independent check of the same Galerkin
   problem.
-* Even grids without a band are rejected.
+* Even grids (GaNi, with and without Ga post-processing), fp64 and mixed, against the oracle.
 """
 import numpy as np
 import pytest
@@ -157,11 +157,39 @@ def test_potential_equals_gradient_formulation_45():
     assert abs(res.AH - ah_grad) < 1e-9 * ah_grad, (res.AH, ah_grad)
 
 
-def test_even_grid_without_band_rejected():
-    plan = nat.Plan((6, 6), (1., 1.), 'scalar', 'primal')
-    plan.set_coeff('iso', np.ones((6, 6)))
-    with pytest.raises(nat.FhError):
-        plan.potential_solve(np.eye(2)[0], 1e-8, 10)
+@pytest.mark.parametrize('N,ga', [((6, 8), None), ((8, 8), 'exact'), ((44, 46), None), ((6, 5, 4), None),
+                                  ((16, 16, 16), None), ((6, 6, 6), 'exact')])
+def test_even_grid_gani_potential_matches_oracle(N, ga):
+    """Even grids as the reference runs them (homogenisation.py:86-128): the Nyquist index of an
+    even axis carries the frequency -N/2 in grad, div and P, the inverse real transform drops the
+    imaginary parts of the last axis' self-conjugate planes, and the inner product counts the
+    Nyquist plane of an even last axis once. The result differs from the gradient formulation's
+    (whose projection nulls the Nyquist modes); the oracle restates the reference's steps."""
+    print('case even-grid potential', N, ga, flush=True)
+    d = len(N)
+    rng = np.random.default_rng(23)
+    a = 1. + 9. * rng.random(N)
+    A = np.einsum('ij,...->ij...', np.eye(d), a)
+    Agani = gpot.Tensor(name='A', val=A, order=2, multype=21)
+    oAgani = Field(val=A, order=2, N=N, Y=np.ones(d), multype=21)
+    Aga = oAga = None
+    if ga:
+        mat, omat = _materials(examples.scalar_2d_materials if d == 2 else examples.scalar_3d_materials,
+                               'square' if d == 2 else 'cube')
+        Nbar = tuple(2 * n - 1 for n in N)
+        Aga, oAga = mat.get_A_Ga(Nbar, 'primal'), omat.get_A_Ga(Nbar, 'primal')
+        Agani, oAgani = mat.get_A_GaNi(N, 'primal'), omat.get_A_GaNi(N, 'primal')
+    ref = opot.homog_GaNi_full_potential(oAgani, oAga, tol=1e-10, maxiter=500)
+    for prec in ('fp64', 'mixed'):
+        res = gpot.homog_GaNi_full_potential(Agani, Aga, {'solver': {'tol': 1e-10, 'maxiter': 500,
+                                                                      'precision': prec}})
+        assert res.info['status'] == 0
+        if prec == 'fp64':
+            assert res.info['kit'] == ref['info']['kit']
+            assert abs(res.AH - ref['AH']) < 1e-12 * abs(ref['AH'])
+            _check_fu(res.Fu, ref['Fu'], 1e-10)
+        else:
+            assert abs(res.AH - ref['AH']) < 1e-8 * abs(ref['AH'])
 
 
 @pytest.mark.parametrize('N', [(27, 27, 27), (45, 45), (15, 27, 9)])
@@ -323,3 +351,16 @@ def test_potential_on_new_fixed_lengths_equals_generic():
         out.append((info['kit'], Fu.to_host()))
     assert out[0][0] == out[1][0]
     assert np.abs(out[0][1] - out[1][1]).max() <= 1e-12 * np.abs(out[1][1]).max()
+
+
+def test_device_crop_of_spectra_matches_oracle():
+    """The crop of a Ga solve's potential spectrum from the quadrature grid M back to N (the
+    reference's Tensor.decrease of an 'r' spectrum) against the oracle's project."""
+    from ffthompy_amd.potential import _crop_spectrum
+    rng = np.random.default_rng(9)
+    for N, M in (((5, 7), (10, 15)), ((5, 5, 3), (9, 12, 5))):
+        x = rng.standard_normal(N)
+        X = opot.DFT(N)(Field(val=x, order=0, N=N, Y=np.ones(len(N))))
+        big = X.project(M)
+        got = _crop_spectrum(nat.DeviceArray.from_host(big.val), M, N).to_host()
+        assert np.abs(got - X.val).max() < 1e-12 * np.abs(X.val).max()

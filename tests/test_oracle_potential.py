@@ -103,17 +103,11 @@ def test_operator_symmetric_positive(N):
     assert op(X) * X > 0
 
 
-def test_host_preconditioner_and_crop_match_oracle():
-    from ffthompy_amd.potential import _crop_spectrum, get_preconditioner
+def test_host_preconditioner_matches_oracle():
+    from ffthompy_amd.potential import get_preconditioner
     for N, Y in (((5, 7), (1., 2.)), ((3, 5, 9), (1., 1., 0.5))):
         P = get_preconditioner(N, {'Y': np.asarray(Y)})
         assert np.abs(P.val - opot.preconditioner(N, np.asarray(Y))).max() < 1e-15
-    rng = np.random.default_rng(9)
-    N, M = (5, 7), (10, 15)
-    X = _random_spectrum(N, rng)
-    big = X.project(M)
-    got = _crop_spectrum(big.val, M, N)
-    assert np.abs(got - X.val).max() < 1e-12 * np.abs(X.val).max()
 
 
 def test_host_inner_product_of_r_spectra_matches_oracle():
