@@ -207,25 +207,35 @@ def bench_elastic(args, nat, ms, dev):
             # the dual coefficient: the pointwise compliance C^-1, again of Lame form
             pl.set_coeff('lame', np.stack(lame_compliance(lam, mu)))
         plans[pd] = pl
+    schedule = {}
+    if not args.no_autotune:
+        for pd, pl in plans.items():
+            for dt in (np.float32, np.float64):
+                r = pl.autotune(dt, nrep=5)
+                schedule['%s_%s' % (pd, np.dtype(dt).name)] = (r['proj_variant'], r['cols_variant'], r['rows_variant'])
+        log('elastic81: schedules %s' % schedule)
+    # right-hand sides, solutions and the macroscopic field are allocated before the timed region
+    EN = nat.DeviceArray((6,) + N)
+    B = nat.DeviceArray((6,) + N)
+    X = {pd: [nat.DeviceArray((6,) + N) for _ in range(6)] for pd in plans}
 
     def run(mode):
         res = {}
         for pd in ('primal', 'dual'):
             pl = plans[pd]
-            sols, kits = [], []
+            kits = []
             for iL in range(6):
                 E = np.zeros(6)
                 E[iL] = 1.
-                EN = nat.DeviceArray.zeros((6,) + N)
+                nat.axpby(0., EN, 0., EN)
                 nat.add_constant(EN, -E)
-                B = nat.DeviceArray((6,) + N)
                 pl.matvec(EN, B)
-                x = nat.DeviceArray.zeros((6,) + N)
+                x = X[pd][iL]
+                nat.axpby(0., x, 0., x)
                 info = pl.cg_solve(B, x, tol=1e-8, maxiter=5000, mode=mode, delta=args.delta)
                 kits.append(info['kit'])
                 nat.add_constant(x, E)
-                sols.append(x)
-            AH = pl.energy_matrix(sols)
+            AH = pl.energy_matrix(X[pd])
             res[pd] = (AH if pd == 'primal' else np.linalg.inv(AH), kits)
         return res
 
@@ -246,7 +256,7 @@ def bench_elastic(args, nat, ms, dev):
            'data': 'synthetic (seeded ellipsoids, seed %d)' % ms.BASE_SEED,
            'config': {'workload': '3-D elasticity 81^3, ellipsoids %.3f, (E,nu) (1,0.3) vs (10,0.2), 6 Mandel '
                                   'loads, primal+dual, CG tol 1e-8' % ind.mean()},
-           'fp64_time_s': round(t64, 4), 'speedup_mixed_vs_fp64': round(t64 / tm, 3),
+           'fp64_time_s': round(t64, 4), 'speedup_mixed_vs_fp64': round(t64 / tm, 3), 'schedule': schedule,
            'kit_mixed': {k: v[1] for k, v in rm.items()}, 'kit_fp64': {k: v[1] for k, v in r64.items()},
            'AH_upper_mandel_diag': np.round(np.diag(up), 6).tolist(),
            'AH_lower_mandel_diag': np.round(np.diag(lo), 6).tolist(),

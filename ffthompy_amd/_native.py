@@ -525,9 +525,11 @@ class Plan:
         call('fh_plan_set_primaldual', self._h, ctypes.c_int(FH_PRIMAL if primaldual == 'primal' else FH_DUAL))
         self.primaldual = primaldual
 
-    def set_cg_variant(self, fused=True):
-        """Mixed-precision CG: fused iteration (default) or the operator-form denominator."""
-        call('fh_plan_set_cg_variant', self._h, ctypes.c_int(1 if fused else 0))
+    def set_cg_variant(self, fused=True, defer=True):
+        """Mixed-precision CG: fused iteration (default) or the operator-form denominator; with
+        defer=False the fused iteration keeps its one-workgroup alpha/beta kernels on small grids
+        too (fh_plan_set_cg_variant 1 or 2)."""
+        call('fh_plan_set_cg_variant', self._h, ctypes.c_int((1 if defer else 2) if fused else 0))
 
     def set_band(self, band, scale=1.0):
         """Projection restricted to |k_a| <= band[a] (None: unrestricted); `scale` multiplies

@@ -261,6 +261,9 @@ struct fh_plan {
   int last_updates = 0;
   int generic = 0;               // 1: never use the fixed-length kernels
   bool fused_cg = true;          // mixed CG: fused iteration where available
+  bool defer_ok = true;          // fused iteration: alpha/beta inside the passes on small grids
+  bool cg_defer = false;         // this solve runs the deferred alpha/beta
+  int defer_nb2 = 0;             // workgroups of the fused inverse row pass (its partials)
   hipEvent_t prof_t0 = nullptr, prof_t1 = nullptr;  // matvec profile: events of the next launch
   int banded = 0;                // 1: projection restricted to |k_a| <= band[a] (view axes)
   int band[3] = {0, 0, 0};
@@ -717,36 +720,87 @@ static int ensure_hist(fh_plan* pl, int len) {
 //   rows_inv : r -= alpha G(A p), partial <r, r>
 //   beta     : beta, stop / reliable-update test
 // Returns 0 when the fused kernels do not cover this plan (nothing was enqueued).
+// Partials per pass that the deferred alpha/beta of a small grid sums (the forward pass's go to
+// pl->partial[0..kDeferMax), the inverse pass's to pl->partial[kDeferMax..2 kDeferMax)).
+constexpr int kDeferMax = 1024;
+
+// Deferred alpha/beta: rows of at most 8192 (every row pass has at least 8 rows per workgroup, so
+// at most kDeferMax workgroups) and room for both partial runs.
+static bool defer_applies(const fh_plan* pl) {
+  return pl->defer_ok && (long long)pl->n0 * pl->n1 <= 8LL * kDeferMax && pl->partial_cap >= 2 * (size_t)kDeferMax;
+}
+
+// One fused fp32 CG iteration. With pl->cg_defer (small grids) alpha and beta have no kernels of
+// their own: the forward row pass takes the previous beta from the inverse pass's partials, block 0
+// of the first strided pass finalises beta and alpha, and the inverse row pass marks its beta as
+// deferred; k_cg_flush finalises the last beta of a batch.
 static int enqueue_fused_iteration(fh_plan* pl, float* xc, float* r, float* p, const float* coef, bool* ok) {
   *ok = false;
   if (pl->generic || (pl->coef_kind != fh::COEF_ISO && pl->coef_kind != fh::COEF_LAME)) return FH_OK;
   const int nc = pl->ncomp;
   const int* halt = &pl->st->halt;
   cpx<float>* spec = reinterpret_cast<cpx<float>*>(pl->spec);
-  const fh::LaunchCtx ctx = make_ctx<float>(pl);
+  const bool defer = pl->cg_defer;
+  double* part1 = pl->partial;
+  double* part2 = defer ? pl->partial + kDeferMax : pl->partial;
+  fh::LaunchCtx ctx = make_ctx<float>(pl);
+  if (defer) {
+    ctx.bpart = part2;
+    ctx.bnp = pl->defer_nb2;
+  }
   int nb = 0;
-  int rr = fh::fixed_rows_fwd<float>(ctx, nc, p, coef, pl->coef_kind, spec, halt, r, pl->st, xc, pl->partial,
+  int rr = fh::fixed_rows_fwd<float>(ctx, nc, p, coef, pl->coef_kind, spec, halt, r, pl->st, xc, part1,
                                      pl->partial_cap, &nb);
   if (rr == 0) return FH_OK;
   RETCHK(fixed_result(rr, "rows forward (fused)"));
   *ok = true;
-  hipLaunchKernelGGL(fh::k_cg_alpha, dim3(1), dim3(kFinThreads), 0, pl->stream, pl->st, pl->partial, nb);
+  if (defer && nb > kDeferMax) return fail(FH_ERR_STATE, "deferred alpha: too many row workgroups");
+  if (!defer)
+    hipLaunchKernelGGL(fh::k_cg_alpha, dim3(1), dim3(kFinThreads), 0, pl->stream, pl->st, part1, nb);
   const GParams g = make_gparams(pl, 1.0 / (double)pl->n);
+  // first strided pass: axis 1 forward (3-D) or the projection pass (2-D)
+  bool first_done = false;
+  if (defer) {
+    fh::LaunchCtx cf = make_ctx<float>(pl);
+    cf.fin = fh::CgFin{pl->st, part1, part2, nb, pl->defer_nb2};
+    const int rf = pl->n0 > 1 ? fh::fixed_cols<float>(cf, -1, 1, nc, spec, halt)
+                              : fh::fixed_cols_g<float>(cf, 1, nc, spec, g, halt);
+    if (rf != 0) {
+      RETCHK(fixed_result(rf, "strided pass (fused, finalising)"));
+      first_done = true;
+    } else {
+      // no fixed kernel on this axis: finalise with the one-workgroup kernels
+      hipLaunchKernelGGL(fh::k_cg_flush, dim3(1), dim3(32), 0, pl->stream, pl->st, part2, pl->defer_nb2);
+      hipLaunchKernelGGL(fh::k_cg_alpha, dim3(1), dim3(kFinThreads), 0, pl->stream, pl->st, part1, nb);
+    }
+  }
   if (pl->n0 > 1) {
-    RETCHK(launch_cols<float>(pl, 0, 1, nc, spec, g, halt));
+    if (!first_done) RETCHK(launch_cols<float>(pl, 0, 1, nc, spec, g, halt));
     RETCHK(launch_cols<float>(pl, 2, 0, nc, spec, g, halt));
     RETCHK(launch_cols<float>(pl, 1, 1, nc, spec, g, halt));
-  } else {
+  } else if (!first_done) {
     RETCHK(launch_cols<float>(pl, 2, 1, nc, spec, g, halt));
   }
   int nb2 = 0;
-  rr = fh::fixed_rows_inv<float>(ctx, nc, spec, r, 1.0f, 2, nullptr, pl->partial, pl->partial_cap, halt, &nb2,
-                                 pl->st);
+  fh::LaunchCtx ci = make_ctx<float>(pl);
+  ci.defer_beta = defer ? 1 : 0;
+  rr = fh::fixed_rows_inv<float>(ci, nc, spec, r, 1.0f, 2, nullptr, part2, defer ? (size_t)kDeferMax : pl->partial_cap,
+                                 halt, &nb2, pl->st);
   if (rr == 0) return fail(FH_ERR_STATE, "fused rows inverse unavailable");
   RETCHK(fixed_result(rr, "rows inverse (fused)"));
-  hipLaunchKernelGGL(fh::k_cg_beta, dim3(1), dim3(kFinThreads), 0, pl->stream, pl->st, pl->partial, nb2);
+  if (defer)
+    pl->defer_nb2 = nb2;
+  else
+    hipLaunchKernelGGL(fh::k_cg_beta, dim3(1), dim3(kFinThreads), 0, pl->stream, pl->st, part2, nb2);
   HIPCHK(hipGetLastError());
   return FH_OK;
+}
+
+// End of a batch of fused iterations with deferred beta: finalise it for the host.
+static void enqueue_flush(fh_plan* pl) {
+  if (pl->cg_defer)
+    hipLaunchKernelGGL(fh::k_cg_flush, dim3(1), dim3(32), 0, pl->stream, pl->st, pl->partial + kDeferMax,
+                       pl->defer_nb2);
 }
 
 // One CG iteration in precision T on vectors (x, r, p, Ap); everything halts on st->halt.
@@ -1148,8 +1202,10 @@ int fh_plan_use_generic(fh_plan* pl, int generic) {
 
 int fh_plan_set_cg_variant(fh_plan* pl, int fused) {
   RETCHK(check_plan(pl));
+  if (fused < 0 || fused > 2) return fail(FH_ERR_ARG, "fh_plan_set_cg_variant: 0, 1 or 2");
   drop_graphs(pl);
   pl->fused_cg = fused != 0;
+  pl->defer_ok = fused == 1;
   return FH_OK;
 }
 
@@ -1408,6 +1464,8 @@ int fh_cg_solve(fh_plan* pl, int mode, const void* b_, void* x_, double tol, int
     HIPCHK(hipMemcpyAsync(p, r, (size_t)nv * sizeof(float), hipMemcpyDeviceToDevice, pl->stream));
     // fused iterations where the fixed-length kernels cover the plan (isotropic coefficient)
     bool fused = pl->fused_cg && pl->op_scale == 1.0;  // the fused step assumes an unscaled G
+    pl->cg_defer = fused && defer_applies(pl);
+    pl->defer_nb2 = 0;
     const int gb = grid_blocks(nv);
     auto fold = [&]() {
       if (fused)
@@ -1435,6 +1493,7 @@ int fh_cg_solve(fh_plan* pl, int mode, const void* b_, void* x_, double tol, int
             rc = enqueue_fused_iteration(pl, xc, r, p, pl->coef32, &ok);
             if (rc != FH_OK || !ok) all = false;
           }
+          if (all) enqueue_flush(pl);
           const hipError_t ec = hipStreamEndCapture(pl->stream, &graph);
           if (all && rc == FH_OK && ec == hipSuccess && graph &&
               hipGraphInstantiate(&pl->it_exec, graph, nullptr, nullptr, 0) == hipSuccess) {
@@ -1466,9 +1525,11 @@ int fh_cg_solve(fh_plan* pl, int mode, const void* b_, void* x_, double tol, int
           RETCHK(enqueue_fused_iteration(pl, xc, r, p, pl->coef32, &ok));
           if (ok) continue;
           fused = false;  // not covered: nothing was enqueued, use the general iteration
+          pl->cg_defer = false;
         }
         RETCHK(enqueue_iteration<float>(pl, xc, r, p, Ap, pl->coef32));
       }
+      if (fused) enqueue_flush(pl);
       HIPCHK(hipMemcpyAsync(&s, pl->st, sizeof(s), hipMemcpyDeviceToHost, pl->stream));
       HIPCHK(hipStreamSynchronize(pl->stream));
       if (s.done) break;

@@ -231,8 +231,9 @@ struct FxShape {
 template <typename T, class FP, int P, int DIR>
 __global__ void __launch_bounds__(FP::TPP * P)
 k_cols_fx(cpx<T>* __restrict__ data, long long ostride, long long inner, int ntiles, long long total,
-          const cpx<T>* __restrict__ twg, const int* __restrict__ halt, int rev) {
+          const cpx<T>* __restrict__ twg, const int* __restrict__ halt, int rev, CgFin fin) {
   if (halt && *halt) return;
+  if (fin.st && blockIdx.x == 0) cg_fin_block(fin);
   constexpr int L = FP::L;
   constexpr unsigned CS = sizeof(cpx<T>);
   __shared__ cpx<T> s[L * P];
@@ -377,8 +378,9 @@ __device__ __forceinline__ void apply_G_fx(cpx<T> (&v)[NC], int k0, int k1, int 
 template <typename T, class FP, class FPR, int F, int NC>
 __global__ void __launch_bounds__(FP::TPP * F * NC)
 k_cols_g_sw(cpx<T>* __restrict__ data, long long inner, int ntiles, long long total,
-            const cpx<T>* __restrict__ twg, GParams g, int n2h, const int* __restrict__ halt) {
+            const cpx<T>* __restrict__ twg, GParams g, int n2h, const int* __restrict__ halt, CgFin fin) {
   if (halt && *halt) return;
+  if (fin.st && blockIdx.x == 0) cg_fin_block(fin);
   constexpr int L = FP::L, P = F * NC, TPP = FP::TPP, NT = TPP * P;
   constexpr int NI = FPR::NST;
   constexpr unsigned CS = sizeof(cpx<T>);

@@ -89,7 +89,7 @@ int rows_fwd_fx_launch_r(const LaunchCtx& c, int nc, const T* in, const T* coef,
     auto kf = k_rows_fwd_st<T, FP, KIND, NCC, UPDV, RPB_>;                                                    \
     allow_lds(kf, slds);                                                                                      \
     hipExtLaunchKernelGGL(kf, grid, block, slds, c.stream, c.t0, c.t1, 0, in, coef, out, nrows, c.n, rpb, tw, halt, rvec, st, xc, \
-                       partial);                                                                              \
+                       partial, c.bpart, c.bnp);                                                              \
   } while (0)
 #define FH_ST_NC(KIND, UPDV)                 \
   do {                                       \
@@ -137,9 +137,9 @@ int rows_fwd_fx_launch_r(const LaunchCtx& c, int nc, const T* in, const T* coef,
   // own component
   if (rvec && kind != COEF_ISO && kind != COEF_NONE) return 0;
   if (xc) {
-    // fused iteration (k_rows_fwd_fx UPD 2), single precision only
+    // fused iteration (k_rows_fwd_fx UPD 2), single precision only; not with a deferred beta
     if constexpr (sizeof(T) == 4) {
-      if (!rvec || kind != COEF_ISO || (size_t)nb > partial_cap) return 0;
+      if (!rvec || kind != COEF_ISO || (size_t)nb > partial_cap || c.bpart) return 0;
 #define FH_LAUNCH_RF2(NCC)                                                                                \
   do {                                                                                                    \
     auto kf = k_rows_fwd_fx<T, FP, COEF_ISO, NCC, 2>;                                                     \
@@ -225,7 +225,7 @@ int rows_inv_fx_launch_r(const LaunchCtx& c, int nc, const cpx<T>* in, T* out, T
     auto kf = k_rows_inv_st<T, FP, EPI, NCC, RPB_>;                                                           \
     allow_lds(kf, slds);                                                                                      \
     hipExtLaunchKernelGGL(kf, grid, block, slds, c.stream, c.t0, c.t1, 0, in, out, nrows, c.n, rpb, tw, scale, w, partial, halt, \
-                       st, order_rev(1));                                                                 \
+                       st, order_rev(1), c.defer_beta);                                                   \
   } while (0)
   const int key = epi * 8 + nc;
   switch (key) {
@@ -294,10 +294,10 @@ int cols_fx_launch_p(const LaunchCtx& c, int dir, int axis, int nc, cpx<T>* data
   const cpx<T>* tw = reinterpret_cast<const cpx<T>*>(c.tw[axis]);
   if (dir < 0)
     hipExtLaunchKernelGGL((k_cols_fx<T, FP, P, -1>), grid, block, 0, c.stream, c.t0, c.t1, 0, data, ostride, inner, (int)ntiles,
-                       total, tw, halt, order_rev(0));
+                       total, tw, halt, order_rev(0), c.fin);
   else
     hipExtLaunchKernelGGL((k_cols_fx<T, FP, P, +1>), grid, block, 0, c.stream, c.t0, c.t1, 0, data, ostride, inner, (int)ntiles,
-                       total, tw, halt, order_rev(2));
+                       total, tw, halt, order_rev(2), c.fin);
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? 1 : -(int)e;
 }
@@ -331,21 +331,20 @@ int cols_g_fx_launch_v(const LaunchCtx& c, int axis, cpx<T>* data, const GParams
   const dim3 grid((unsigned)ntiles), block(FP::TPP * F * NC);
   const cpx<T>* tw = reinterpret_cast<const cpx<T>*>(c.tw[axis]);
   hipExtLaunchKernelGGL((k_cols_g_sw<T, FP, FPR, F, NC>), grid, block, 0, c.stream, c.t0, c.t1, 0, data, inner, (int)ntiles, total,
-                     tw, g, c.n2h, halt);
+                     tw, g, c.n2h, halt, c.fin);
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? 1 : -(int)e;
 }
 
 template <typename T, class FP, class FPR, int NC>
 int cols_g_fx_launch_nc(const LaunchCtx& c, int axis, cpx<T>* data, const GParams& g, const int* halt) {
-  // the scalar problems (2 or 3 components) get every schedule variant, elasticity the default
-  if constexpr (NC == 2 || NC == 3) {
-    switch (c.proj_var) {
-      case 1: return cols_g_fx_launch_v<T, FP, FPR, NC, 1>(c, axis, data, g, halt);
-      case 2: return cols_g_fx_launch_v<T, FP, FPR, NC, 2>(c, axis, data, g, halt);
-      case 3: return cols_g_fx_launch_v<T, FP, FPR, NC, 3>(c, axis, data, g, halt);
-      default: break;
-    }
+  // every schedule variant for 2, 3 and 6 components (the tile's points are halved until the
+  // workgroup fits; at small grids fewer points per tile give more workgroups than CUs)
+  switch (c.proj_var) {
+    case 1: return cols_g_fx_launch_v<T, FP, FPR, NC, 1>(c, axis, data, g, halt);
+    case 2: return cols_g_fx_launch_v<T, FP, FPR, NC, 2>(c, axis, data, g, halt);
+    case 3: return cols_g_fx_launch_v<T, FP, FPR, NC, 3>(c, axis, data, g, halt);
+    default: break;
   }
   return cols_g_fx_launch_v<T, FP, FPR, NC, 0>(c, axis, data, g, halt);
 }

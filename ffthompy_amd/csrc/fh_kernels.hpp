@@ -26,6 +26,7 @@ struct CGState {
   double rr, rr_old, pAp, alpha, beta;
   double tol, delta, rmax, inv_nvox;
   int kit, maxiter, done, pause, halt, status, mixed, hist_len, nupdates;
+  int bpend;     // fused iteration, small grids: the beta of the last inverse pass is not finalised yet
   int pend;      // fused iteration: alpha * p of the last iteration not yet added to x
   double* hist;  // device residual log (nullable)
 };
@@ -261,23 +262,103 @@ __device__ __forceinline__ void finalize_alpha(CGState* st, double psum) {
   }
 }
 
+// The step that <r,r> = rsum (unscaled) makes: rr_new, beta = rr_new / rr, the iteration count,
+// the norm and the stop decisions. One function, so that every kernel that evaluates it from the
+// same sum takes the same decision.
+struct BetaStep {
+  double rr, beta, nr;
+  int kit;
+  bool conv, maxed, pause_mixed;
+};
+__device__ __forceinline__ BetaStep beta_step(const CGState* st, double rsum) {
+  BetaStep b;
+  b.rr = rsum * st->inv_nvox;
+  b.beta = b.rr / st->rr;
+  b.kit = st->kit + 1;
+  b.nr = sqrt(b.rr);
+  b.conv = !(b.nr > st->tol);
+  b.maxed = b.kit >= st->maxiter;
+  b.pause_mixed = b.conv || b.nr < st->delta * st->rmax || b.maxed;
+  return b;
+}
+
 // rr <- <r,r>; beta = rr_new / rr_old; iteration count, residual log and stop test.
 __device__ __forceinline__ void finalize_beta(CGState* st, double rsum) {
-  const double rr = rsum * st->inv_nvox;
+  const BetaStep b = beta_step(st, rsum);
   st->rr_old = st->rr;
-  st->rr = rr;
-  st->beta = rr / st->rr_old;
-  st->kit += 1;
-  const double nr = sqrt(rr);
-  if (st->hist && st->kit < st->hist_len) st->hist[st->kit] = nr;
-  const bool conv = !(nr > st->tol);
-  const bool maxed = st->kit >= st->maxiter;
+  st->rr = b.rr;
+  st->beta = b.beta;
+  st->kit = b.kit;
+  if (st->hist && st->kit < st->hist_len) st->hist[st->kit] = b.nr;
   if (st->mixed) {
-    if (conv || nr < st->delta * st->rmax || maxed) st->pause = 1;
+    if (b.pause_mixed) st->pause = 1;
   } else {
-    st->done = conv || maxed;
+    st->done = b.conv || b.maxed;
   }
   st->halt = st->done | st->pause;
+}
+
+// Sum of np per-workgroup partials in an order that does not depend on the block size: the first
+// 32 lanes add lane-strided runs and combine them by an xor butterfly (every lane ends with the same
+// bits). Every kernel that calls it has at least 32 threads per block. Returned to every thread of
+// the block; call from uniform control flow.
+__device__ __forceinline__ double fixed_sum_partials(const double* __restrict__ p, int np) {
+  __shared__ double res;
+  if (threadIdx.x < 32) {
+    // 16 loads in flight per lane before the additions (which keep their order): the partials
+    // were just written by workgroups on every XCD, so each load is a trip to memory
+    double a = 0.0;
+    for (int i0 = 0; i0 < np; i0 += 16 * 32) {
+      double v[16];
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        const int i = i0 + k * 32 + (int)threadIdx.x;
+        v[k] = i < np ? p[i] : 0.0;
+      }
+#pragma unroll
+      for (int k = 0; k < 16; ++k) a += v[k];
+    }
+#pragma unroll
+    for (int o = 16; o > 0; o >>= 1) a += __shfl_xor(a, o, 32);
+    if (threadIdx.x == 0) res = a;
+  }
+  __syncthreads();
+  const double r = res;
+  __syncthreads();
+  return r;
+}
+
+// Finalisation carried by block 0 of the first strided pass of a fused iteration on a small grid
+// (in place of the one-workgroup alpha and beta kernels): the deferred beta of the previous
+// iteration from the inverse pass's partials p2 (when st->bpend), then alpha of this iteration
+// from the forward pass's partials p1. Null st: nothing to do.
+struct CgFin {
+  CGState* st;
+  const double* p1;
+  const double* p2;
+  int n1, n2;
+};
+// The caller has already returned on a halted state (read before this kernel wrote anything), so
+// every thread of the block enters; the decisions are shared through LDS, never re-read from the
+// state this block writes, so the block's control flow stays uniform.
+__device__ __forceinline__ void cg_fin_block(const CgFin& f) {
+  CGState* st = f.st;
+  __shared__ int stop;
+  if (st->bpend) {
+    const double rs = fixed_sum_partials(f.p2, f.n2);
+    if (threadIdx.x == 0) {
+      finalize_beta(st, rs);
+      st->bpend = 0;
+      stop = st->halt;
+    }
+    __syncthreads();
+    if (stop) return;
+  }
+  const double ps = fixed_sum_partials(f.p1, f.n1);
+  if (threadIdx.x == 0) {
+    finalize_alpha(st, ps);
+    st->pend = st->status == 0 ? 1 : 0;
+  }
 }
 
 // ---- pass 1: coefficient multiply + real-to-complex FFT along the contiguous axis ----
@@ -590,6 +671,7 @@ static __global__ void k_cg_restart(CGState* st, const double* __restrict__ part
     st->rmax = nr;
     st->pause = 0;
     st->pend = 0;
+    st->bpend = 0;
     if (st->kit == 0 && hist && st->hist_len > 0) hist[0] = nr;
     st->done = (!(nr > st->tol)) || (st->kit >= st->maxiter);
     st->halt = st->done;
@@ -616,6 +698,17 @@ static __global__ void k_cg_beta(CGState* st, const double* __restrict__ partial
   for (int i = threadIdx.x; i < np; i += blockDim.x) acc += partial[i];
   const double b = block_sum(acc);
   if (threadIdx.x == 0) finalize_beta(st, b);
+}
+
+// The deferred beta of a fused iteration on a small grid, at the end of a batch of iterations, so
+// that the host reads a finalised state (the same sum and decision as cg_fin_block).
+static __global__ void k_cg_flush(CGState* st, const double* __restrict__ partial, int np) {
+  if (st->halt || !st->bpend) return;
+  const double rs = fixed_sum_partials(partial, np);
+  if (threadIdx.x == 0) {
+    finalize_beta(st, rs);
+    st->bpend = 0;
+  }
 }
 
 // 16-byte vector of T (4 floats or 2 doubles) for the streaming CG kernels.

@@ -19,6 +19,14 @@ struct LaunchCtx {
   int proj_var = 0;  // projection pass schedule variant (FxShape::proj_f)
   int cols_var = 0;  // strided pass schedule variant (FxShape::cols_p)
   int rows_var = 0;  // contiguous-axis pass schedule variant (FxShape::rows_rpb)
+  // fused CG iteration on a small grid (alpha and beta without their one-workgroup kernels):
+  // the forward row pass takes the deferred beta from the inverse pass's partials bpart[0..bnp),
+  // the inverse row pass marks beta as deferred, and block 0 of the strided pass given `fin`
+  // finalises beta and alpha (cg_fin_block)
+  const double* bpart = nullptr;
+  int bnp = 0;
+  int defer_beta = 0;
+  CgFin fin{nullptr, nullptr, nullptr, 0, 0};
 };
 
 // Number of schedule variants of the projection, strided and contiguous-axis passes.

@@ -76,7 +76,7 @@ __global__ void __launch_bounds__(1024)
 k_rows_fwd_st(const T* __restrict__ in, const T* __restrict__ coef, cpx<T>* __restrict__ out,
               long long nrows, long long n, int rpb, const cpx<T>* __restrict__ twg,
               const int* __restrict__ halt, const T* __restrict__ rvec, const CGState* __restrict__ st,
-              T* __restrict__ xc, double* __restrict__ partial) {
+              T* __restrict__ xc, double* __restrict__ partial, const double* __restrict__ bpart, int bnp) {
   static_assert(COEF == COEF_NONE || COEF == COEF_ISO || (COEF == COEF_LAME && (NC == 3 || NC == 6)),
                 "staged pass: no coefficient, isotropic a, or Lame (lambda, mu) in Mandel form");
   constexpr int KV = COEF == COEF_NONE ? 0 : (COEF == COEF_ISO ? 1 : 2);  // coefficient runs
@@ -104,7 +104,7 @@ k_rows_fwd_st(const T* __restrict__ in, const T* __restrict__ coef, cpx<T>* __re
   constexpr int NT = (RPB / 2) * NC * TPP;
   constexpr int QMAX = (RPB * L + 2 * W - 2) / W + 1;  // chunks touched by one run of RPB rows
   constexpr int CPT = (QMAX + NT - 1) / NT;
-  const T bta = UPD ? T(st->beta) : T(0);
+  T bta = UPD ? T(st->beta) : T(0);
   const bool fold = UPD == 2 && st->pend != 0;
   const T alp = UPD == 2 ? T(st->alpha) : T(0);
   const __amdgpu_buffer_rsrc_t rr = make_rsrc(UPD ? rvec : in, (unsigned long long)NC * n * SR);
@@ -149,6 +149,17 @@ k_rows_fwd_st(const T* __restrict__ in, const T* __restrict__ coef, cpx<T>* __re
         const unsigned q = q0 + tid + i * NT;
         if (q < q1) *reinterpret_cast<Chunk<T>*>(sa + ab + (int)(q * W - s0)) = A[k][i];
       }
+    }
+  }
+  if constexpr (UPD == 2) {
+    // small grids: the beta of the previous iteration is not finalised yet; every workgroup sums
+    // the inverse pass's partials in one fixed order (its loads overlap the block's own, issued
+    // above) and takes the decision the finalising block of the next pass takes: on a stop or a
+    // reliable update nothing is written, as when the beta kernel halts the iteration
+    if (bpart && st->bpend) {
+      const BetaStep b = beta_step(st, fixed_sum_partials(bpart, bnp));
+      if (b.pause_mixed) return;
+      bta = T(b.beta);
     }
   }
   // CG vector updates, the operand v -> real-row staging
@@ -292,7 +303,7 @@ template <typename T, class FP, int EPI, int NC, int RPB = FxShape<FP, T>::RPB>
 __global__ void __launch_bounds__(1024)
 k_rows_inv_st(const cpx<T>* __restrict__ in, T* __restrict__ out, long long nrows, long long n, int rpb,
               const cpx<T>* __restrict__ twg, T scale, const T* __restrict__ w, double* __restrict__ partial,
-              const int* __restrict__ halt, CGState* __restrict__ st, int rev) {
+              const int* __restrict__ halt, CGState* __restrict__ st, int rev, int defer_beta) {
   if (halt && *halt) return;
   constexpr int L = FP::L, n2h = L / 2 + 1, TPP = FP::TPP, W = Chunk<T>::W;
   constexpr unsigned SR = sizeof(T);
@@ -417,6 +428,8 @@ k_rows_inv_st(const cpx<T>* __restrict__ in, T* __restrict__ out, long long nrow
     const double bsum = block_sum(acc);
     if (tid == 0) partial[blockIdx.x] = bsum;
   }
+  // EPI 2 on small grids: beta is finalised by the next forward pass's readers (st->bpend)
+  if (EPI == 2 && defer_beta && blockIdx.x == 0 && tid == 0) st->bpend = 1;
 }
 
 // ---- round trip: C2R, a(x) v, R2C along the contiguous axis, in place ---------------------

@@ -91,7 +91,10 @@ int fh_plan_use_generic(fh_plan* plan, int generic);
  * using <p, a p> = <p, G(a p)> for p in the range of G; it applies to isotropic
  * coefficients on the fixed-length kernels and falls back otherwise. fused = 0 keeps the
  * reference's operator-form denominator <p, Afun(p)> (ffthompy/general/solver.py:125-126).
- * Plain FP64 solves always use the operator form. */
+ * fused = 1 (default): on grids of at most 8192 rows (n0 * n1) alpha and beta are formed inside
+ * the passes (each forward-pass workgroup sums the inverse pass's partials in one fixed order,
+ * block 0 of the first strided pass finalises the state) instead of by two one-workgroup kernels
+ * per iteration; fused = 2 keeps those kernels. Plain FP64 solves always use the operator form. */
 int fh_plan_set_cg_variant(fh_plan* plan, int fused);
 
 /* Mixed-precision CG with the fused iteration: enable = 1 (default) replays batches of
@@ -331,8 +334,7 @@ int fh_matvec_profile(fh_plan* plan, int dtype, const void* x, void* y, int nrep
 
 /* Schedule of the fixed-length kernels, per arithmetic (dtype FH_F64 / FH_F32). proj_variant
  * (0..3) sets the frequency points per tile of the projection pass (forward FFT along the
- * outermost axis, G(xi), inverse FFT), which also sets the threads per workgroup (scalar plans;
- * elasticity keeps variant 0's tile); cols_variant (0..1) the columns per tile of the other strided
+ * outermost axis, G(xi), inverse FFT), which also sets the threads per workgroup; cols_variant (0..1) the columns per tile of the other strided
  * passes (3-D); rows_variant (0..1) the grid rows per workgroup of the contiguous-axis passes
  * (variant 1: 3/2 as many; scalar plans, isotropic or no coefficient). Every variant computes the
  * same values (the same transforms per pencil); only speed differs. Variant 0 is the default.

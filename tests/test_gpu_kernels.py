@@ -325,6 +325,46 @@ def test_fused_mixed_iteration_matches_operator_form(N):
     assert abs(out[True][1] - out[False][1]) / out[False][1] < 1e-10, out
 
 
+@pytest.mark.parametrize('N,physics', [((27, 27, 27), 'scalar'), ((45, 45), 'scalar'), ((81, 81, 81), 'elasticity'),
+                                       ((45, 45, 45), 'elasticity'), ((63, 81), 'elasticity'), ((5, 9, 15), 'scalar')])
+def test_deferred_alpha_beta_match_kernels(N, physics):
+    """Small grids: alpha and beta formed inside the passes (every forward-pass workgroup sums the
+    inverse pass's partials in one fixed order, block 0 of the first strided pass finalises)
+    against the fused iteration with its one-workgroup alpha/beta kernels: the sums differ in
+    their order only, so the iteration counts agree within 1 and the energies to 1e-10; both
+    converge to the tolerance, and a plan switches between the two without stale state."""
+    print('case deferred alpha/beta', N, physics, flush=True)
+    rng = np.random.default_rng(33)
+    ph = rng.random(N) < 0.3
+    plan = nat.Plan(N, np.ones(len(N)), physics, 'primal')
+    nc = plan.ncomp
+    if physics == 'scalar':
+        a = np.where(ph, 100., 1.)
+        plan.set_coeff('iso', a)
+    else:
+        lam, mu = np.where(ph, 5.0, 0.5), np.where(ph, 4.0, 0.4)
+        plan.set_coeff('lame', np.stack([lam, mu]))
+    E = np.zeros(nc)
+    E[0] = 1.
+    EN = nat.DeviceArray.zeros((nc,) + N)
+    nat.add_constant(EN, -E)
+    B = nat.DeviceArray((nc,) + N)
+    plan.matvec(EN, B)
+    out = {}
+    for defer in (True, False, True):
+        plan.set_cg_variant(True, defer=defer)
+        x = nat.DeviceArray.zeros((nc,) + N)
+        info = plan.cg_solve(B, x, tol=1e-8, maxiter=2000, mode='mixed', delta=1e-4)
+        assert info['status'] == nat.FH_CG_OK and info['norm_res'] <= 1e-8
+        nat.add_constant(x, E)
+        AH = plan.energy_matrix([x] * nc)[0, 0]
+        if defer in out:
+            assert out[defer] == (info['kit'], AH), 'deferred path not reproducible'
+        out[defer] = (info['kit'], AH)
+    assert abs(out[True][0] - out[False][0]) <= 1, out
+    assert abs(out[True][1] - out[False][1]) < 1e-10 * abs(out[False][1]), out
+
+
 @pytest.mark.parametrize('N', [(27, 27, 27), (45, 45, 45), (63, 81)])
 def test_fused_mixed_iteration_lame(N):
     """Fused single-precision iteration with a Lame coefficient (staged row pass computes

@@ -10,17 +10,24 @@ from ffthompy_amd import _native as nat
 
 pytestmark = pytest.mark.gpu
 
-CASES = [((45, 45, 45), np.float32), ((45, 45, 45), np.float64), ((81, 81, 81), np.float32),
-         ((243, 27, 243), np.float32), ((63, 135), np.float32), ((99, 99), np.float64)]
+CASES = [((45, 45, 45), np.float32, 'scalar'), ((45, 45, 45), np.float64, 'scalar'),
+         ((81, 81, 81), np.float32, 'scalar'), ((243, 27, 243), np.float32, 'scalar'),
+         ((63, 135), np.float32, 'scalar'), ((99, 99), np.float64, 'scalar'),
+         ((81, 81, 81), np.float32, 'elasticity'), ((45, 27, 45), np.float64, 'elasticity'),
+         ((81, 45), np.float32, 'elasticity')]
 
 
-@pytest.mark.parametrize('N,dt', CASES, ids=['%s-%s' % (c[0], np.dtype(c[1]).name) for c in CASES])
-def test_schedule_variants_are_bitwise_equal(N, dt):
+@pytest.mark.parametrize('N,dt,physics', CASES, ids=['%s-%s-%s' % (c[0], np.dtype(c[1]).name, c[2]) for c in CASES])
+def test_schedule_variants_are_bitwise_equal(N, dt, physics):
     rng = np.random.default_rng(11)
-    pl = nat.Plan(N, np.ones(len(N)), 'scalar', 'primal')
-    pl.set_coeff('iso', 1. + 9. * rng.random(N))
-    x = nat.DeviceArray.from_host(rng.standard_normal((len(N),) + N).astype(dt))
-    y = nat.DeviceArray((len(N),) + N, dt)
+    pl = nat.Plan(N, np.ones(len(N)), physics, 'primal')
+    if physics == 'scalar':
+        pl.set_coeff('iso', 1. + 9. * rng.random(N))
+    else:
+        pl.set_coeff('lame', np.stack([1. + rng.random(N), 0.5 + rng.random(N)]))
+    nc = pl.ncomp
+    x = nat.DeviceArray.from_host(rng.standard_normal((nc,) + N).astype(dt))
+    y = nat.DeviceArray((nc,) + N, dt)
     ref = None
     for pv in range(4):
         for cv in range(2):
@@ -33,7 +40,7 @@ def test_schedule_variants_are_bitwise_equal(N, dt):
                     ref = got
                 else:
                     assert np.array_equal(got, ref), (pv, cv, rv)
-    print('%s %s: 16 schedules equal' % (N, np.dtype(dt).name), flush=True)
+    print('%s %s %s: 16 schedules equal' % (N, np.dtype(dt).name, physics), flush=True)
 
 
 def test_schedule_rejects_bad_variants():
