@@ -69,6 +69,8 @@ SIGNATURES = {
                             ctypes.c_int, _vp, _vp],
     'fh_pointwise_inverse': [ctypes.c_int, ctypes.c_int, ctypes.c_longlong, _vp, _vp],
     'fh_spectrum_grad': [ctypes.c_int, ctypes.c_int, _c_int_p, _c_double_p, _vp, _vp],
+    'fh_pointwise_product': [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_longlong,
+                             _vp, _vp, _vp],
     'fh_last_timing': [_vp, _c_double_p, _c_double_p],
     'fh_last_solve_updates': [_vp, _c_int_p],
     'fh_matvec_profile': [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int, _c_double_p, _c_double_p,
@@ -448,6 +450,26 @@ def spectrum_grad(x, N, Y):
         raise FhError('Y must have %d entries' % len(N), FH_ERR_ARG)
     out = DeviceArray((len(N),) + tuple(N[:-1]) + (N[-1] // 2 + 1,), x.dtype)
     call('fh_spectrum_grad', code, len(N), cN, Yd.ctypes.data_as(_c_double_p), x.ptr, out.ptr)
+    return out
+
+
+_ELEM = {np.dtype(np.float64): (FH_F64, 0), np.dtype(np.float32): (FH_F32, 0),
+         np.dtype(np.complex128): (FH_F64, 1), np.dtype(np.complex64): (FH_F32, 1)}
+
+
+def pointwise_product(X, Y, A, B, nvox, mode=0):
+    """out[a] = sum_b X[a, b] Y[b] (mode 0) or X[a] Y[a] (mode 1) at every grid point, on the
+    device (fh_pointwise_product). X, Y: DeviceArrays of one dtype, component-major."""
+    if not isinstance(X, DeviceArray) or not isinstance(Y, DeviceArray) or X.dtype != Y.dtype:
+        raise FhError('pointwise_product needs two DeviceArrays of one dtype', FH_ERR_ARG)
+    if X.dtype not in _ELEM:
+        raise FhError('unsupported dtype %s' % X.dtype, FH_ERR_ARG)
+    code, cplx = _ELEM[X.dtype]
+    A, B, nvox = int(A), int(B), int(nvox)
+    check_array(X, A * (B if mode == 0 else 1) * nvox, (X.dtype,), 'X')
+    check_array(Y, (B if mode == 0 else A) * nvox, (X.dtype,), 'Y')
+    out = DeviceArray((A * nvox,), X.dtype)
+    call('fh_pointwise_product', code, cplx, int(mode), A, B, ctypes.c_longlong(nvox), X.ptr, Y.ptr, out.ptr)
     return out
 
 

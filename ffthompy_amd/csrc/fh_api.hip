@@ -1844,6 +1844,19 @@ int fh_spectrum_grad(int dtype, int dim, const int* N, const double* Y, const vo
   return forms_done(fh::forms::spectrum_grad(dtype == FH_F64, v, dim, tp, in, out), "fh_spectrum_grad");
 }
 
+int fh_pointwise_product(int dtype, int is_complex, int mode, int A, int B, long long nvox, const void* X,
+                         const void* Y, void* out) {
+  RETCHK(check_dtype(dtype));
+  if ((mode != 0 && mode != 1) || A < 1 || A > 4096 || (mode == 0 && (B < 1 || B > 4096)) || nvox < 0)
+    return fail(FH_ERR_ARG, "fh_pointwise_product: bad mode or sizes");
+  if (nvox == 0) return FH_OK;
+  if (!X || !Y || !out || out == X || out == Y) return fail(FH_ERR_ARG, "fh_pointwise_product: bad pointers");
+  RETCHK(ensure_device());
+  return forms_done(fh::forms::pointwise_product(dtype == FH_F64, is_complex ? 1 : 0, mode, A, mode == 0 ? B : 1,
+                                                 nvox, X, Y, out),
+                    "fh_pointwise_product");
+}
+
 int fh_pointwise_inverse(int dtype, int k, long long nvox, const void* in, void* out) {
   RETCHK(check_dtype(dtype));
   if (k < 1 || k > 6 || nvox < 0) return fail(FH_ERR_ARG, "fh_pointwise_inverse: 1 <= k <= 6, nvox >= 0");

@@ -225,6 +225,43 @@ k_spectrum_grad(const cpx<T>* __restrict__ in, cpx<T>* __restrict__ out, Grid3 g
   }
 }
 
+// Pointwise products of Tensor.__mul__ (ffthompy/tensors/objects.py:223-245, the einsums of
+// objects.py:599-604) for fields of nvox grid points, component-major:
+//  mode 0: out[a][v] = sum_b X[a][b][v] * Y[b][v]   (multype 21: A = i, B = j; 42: A = ij, B = kl;
+//          'grad': B = 1; 'div': A = 1)
+//  mode 1: out[a][v] = X[a][v] * Y[a][v]             (multype 00, elementwise)
+// V is the element type (real or complex of either precision).
+template <typename V>
+__device__ __forceinline__ V vmul(V a, V b) {
+  return a * b;
+}
+template <>
+__device__ __forceinline__ cpx<float> vmul(cpx<float> a, cpx<float> b) {
+  return mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <>
+__device__ __forceinline__ cpx<double> vmul(cpx<double> a, cpx<double> b) {
+  return mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+template <typename V>
+__global__ void __launch_bounds__(256)
+k_pointwise_product(const V* __restrict__ X, const V* __restrict__ Y, V* __restrict__ out, int A, int B,
+                    long long nvox, int mode) {
+  for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < nvox;
+       v += (long long)gridDim.x * blockDim.x) {
+    for (int a = 0; a < A; ++a) {
+      if (mode == 1) {
+        out[(long long)a * nvox + v] = vmul(X[(long long)a * nvox + v], Y[(long long)a * nvox + v]);
+        continue;
+      }
+      V acc = vmul(X[(long long)a * B * nvox + v], Y[v]);
+      for (int b = 1; b < B; ++b) acc = acc + vmul(X[((long long)a * B + b) * nvox + v], Y[(long long)b * nvox + v]);
+      out[(long long)a * nvox + v] = acc;
+    }
+  }
+}
+
 // Pointwise inverse of a (K, K) matrix field, entry (i, j) of voxel v at [(i*K + j)*nvox + v]:
 // Gauss-Jordan elimination without pivoting in the order of ffthompy/trigpol.py:120-159 (row m
 // normalised by its pivot and subtracted from the rows below, then a backward sweep).
@@ -337,6 +374,24 @@ hipError_t project_form(int dbl, const GParams& g, int form, int null_nyq, const
   else
     hipLaunchKernelGGL(k_project_form<float>, grid, dim3(256), 0, 0, (const cpx<float>*)in, (cpx<float>*)out, nspec,
                        l2, form, null_nyq, g);
+  return hipGetLastError();
+}
+
+hipError_t pointwise_product(int dbl, int cplx, int mode, int A, int B, long long nvox, const void* X, const void* Y,
+                             void* out) {
+  const dim3 grid = blocks_for(nvox);
+  if (cplx && dbl)
+    hipLaunchKernelGGL(k_pointwise_product<cpx<double>>, grid, dim3(256), 0, 0, (const cpx<double>*)X,
+                       (const cpx<double>*)Y, (cpx<double>*)out, A, B, nvox, mode);
+  else if (cplx)
+    hipLaunchKernelGGL(k_pointwise_product<cpx<float>>, grid, dim3(256), 0, 0, (const cpx<float>*)X,
+                       (const cpx<float>*)Y, (cpx<float>*)out, A, B, nvox, mode);
+  else if (dbl)
+    hipLaunchKernelGGL(k_pointwise_product<double>, grid, dim3(256), 0, 0, (const double*)X, (const double*)Y,
+                       (double*)out, A, B, nvox, mode);
+  else
+    hipLaunchKernelGGL(k_pointwise_product<float>, grid, dim3(256), 0, 0, (const float*)X, (const float*)Y,
+                       (float*)out, A, B, nvox, mode);
   return hipGetLastError();
 }
 

@@ -34,6 +34,8 @@ hipError_t project_form(int dbl, const GParams& g, int form, int null_nyq, const
 hipError_t pointwise_inverse(int dbl, int k, long long nvox, const void* in, void* out);
 // tpiY: 2 pi / Y per view axis (3 entries)
 hipError_t spectrum_grad(int dbl, Grid3 g, int dim, const double* tpiY, const void* in, void* out);
+hipError_t pointwise_product(int dbl, int cplx, int mode, int A, int B, long long nvox, const void* X, const void* Y,
+                             void* out);
 
 }  // namespace forms
 }  // namespace fh

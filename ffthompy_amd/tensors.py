@@ -117,10 +117,43 @@ class Tensor:
     def __call__(self, x):
         return self.__mul__(x)
 
-    def __mul__(self, Y):
-        if self.multype in ('scal', 'scalar'):
+    def __mul__(self, Y, multype=None):
+        """The reference's products (objects.py:223-245): 'scal' the grid inner product; 21
+        'ij...,j...->i...', 42 'ijkl...,kl...->ij...', 00 elementwise, 'grad' 'i...,...->i...',
+        'div' 'i...,i...->...' as pointwise kernels on the device (fh_pointwise_product)."""
+        if multype is None:
+            multype = self.multype
+        if multype in ('scal', 'scalar'):
             return scalar_product(self, Y)
-        raise NotImplementedError('only the grid inner product is defined between device fields')
+        X = self
+        assert X.Fourier == Y.Fourier
+        assert nat.form_code(X.fft_form) == nat.form_code(Y.fft_form)
+        grid = X.val.shape[X.order:]
+        assert tuple(grid) == tuple(Y.val.shape[Y.order:])
+        nvox = int(np.prod(grid))
+        xs, ys = tuple(X.shape), tuple(Y.shape)
+        if multype in (21, '21'):
+            assert len(xs) == 2 and ys == xs[1:]
+            A, B, mode, out_shape = xs[0], xs[1], 0, xs[:1]
+        elif multype in (42, '42'):
+            assert len(xs) == 4 and ys == xs[2:]
+            A, B, mode, out_shape = xs[0] * xs[1], xs[2] * xs[3], 0, xs[:2]
+        elif multype in (0, '00', 'elementwise', 'hadamard'):
+            assert xs == ys
+            A, B, mode, out_shape = int(np.prod(xs)) if xs else 1, 1, 1, xs
+        elif multype == 'grad':
+            assert len(xs) == 1 and ys == ()
+            A, B, mode, out_shape = xs[0], 1, 0, xs
+        elif multype == 'div':
+            assert len(xs) == 1 and ys == xs
+            A, B, mode, out_shape = 1, xs[0], 0, ()
+        else:
+            raise NotImplementedError('multype %r' % (multype,))
+        Xd, Yd = X.to_device().val, Y.to_device().val
+        if Xd.dtype != Yd.dtype:
+            raise ValueError('Tensor product of %s and %s values' % (Xd.dtype, Yd.dtype))
+        val = nat.pointwise_product(Xd, Yd, A, B, nvox, mode).reshape(out_shape + tuple(grid))
+        return Y._like(val, name='{0}({1})'.format(X.name, Y.name), order=len(out_shape))
 
     def norm(self):
         return scalar_product(self, self) ** 0.5

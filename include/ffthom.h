@@ -315,6 +315,14 @@ int fh_pointwise_inverse(int dtype, int k, long long nvox, const void* in, void*
  * grad of a scalar Tensor, ffthompy/tensors/operators.py:227-260; frequencies
  * ffthompy/trigpol.py:27-40). dim 1 to 3. */
 int fh_spectrum_grad(int dtype, int dim, const int* N, const double* Y, const void* in, void* out);
+/* Pointwise products of device Tensors (Tensor.__mul__ for multype 21, 42, 00, 'grad' and 'div',
+ * ffthompy/tensors/objects.py:223-245 and the einsums of objects.py:599-604), nvox grid points,
+ * component-major, real (is_complex 0) or complex values of dtype's precision:
+ *   mode 0: out[a][v] = sum_b X[a][b][v] * Y[b][v]   (X: A x B components, Y: B, out: A)
+ *   mode 1: out[a][v] = X[a][v] * Y[a][v]             (A components each; B unused)
+ * out must not overlap X or Y; 1 <= A, B <= 4096. */
+int fh_pointwise_product(int dtype, int is_complex, int mode, int A, int B, long long nvox, const void* X,
+                         const void* Y, void* out);
 
 /* Device-side wall time of the last fh_matvec / fh_cg_solve on this plan, in ms
  * (HIP events on the plan stream), and the number of fp64 residual recomputations

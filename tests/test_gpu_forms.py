@@ -204,3 +204,64 @@ def test_projections_every_form_match_oracle(N, physics, nyq, form):
         assert _rel(G(T).to_host().val, ref) < 1e-13, G
     T32 = _dev(X, 1, N, Fourier=True, fft_form=form, dtype=np.complex64)
     assert _rel(G1(T32).to_host().val, np.einsum('ij...,j...->i...', M1, X)) < 1e-5
+
+
+PRODUCTS = [(21, (3, 3), (3,)), (42, (3, 3, 3, 3), (3, 3)), (0, (2, 3), (2, 3)), ('grad', (3,), ()),
+            ('div', (3,), (3,)), (21, (6, 6), (6,))]
+
+
+@pytest.mark.parametrize('multype,xs,ys', PRODUCTS, ids=['%s-%s' % (p[0], 'x'.join(map(str, p[1]))) for p in PRODUCTS])
+def test_tensor_products_match_einsum(multype, xs, ys):
+    """Tensor.__mul__ with the reference's multypes (objects.py:223-245) on device fields, real
+    fp64/fp32 and complex spectra, against numpy.einsum."""
+    sub = {21: 'ij...,j...->i...', 42: 'ijkl...,kl...->ij...', 0: '...,...->...', 'grad': 'i...,...->i...',
+           'div': 'i...,i...->...'}[multype]
+    N = (6, 5, 7)
+    rng = np.random.default_rng(15)
+    X = rng.standard_normal(xs + N)
+    Y = rng.standard_normal(ys + N)
+    ref = np.einsum(sub, X, Y)
+    for dt, tol in ((np.float64, 1e-14), (np.float32, 1e-5)):
+        tx = Tensor(val=nat.DeviceArray.from_host(X.astype(dt)), order=len(xs), multype=multype)
+        ty = Tensor(val=nat.DeviceArray.from_host(Y.astype(dt)), order=len(ys))
+        got = tx(ty)
+        assert got.order == ref.ndim - len(N) and got.N == N
+        assert _rel(got.to_host().val, ref) < tol
+    # complex spectra in form 0
+    Xc = X + 1j * rng.standard_normal(X.shape)
+    Yc = Y + 1j * rng.standard_normal(Y.shape)
+    tx = Tensor(val=nat.DeviceArray.from_host(Xc), order=len(xs), multype=multype, Fourier=True, fft_form=0)
+    ty = Tensor(val=nat.DeviceArray.from_host(Yc), order=len(ys), Fourier=True, fft_form=0)
+    assert _rel(tx(ty).to_host().val, np.einsum(sub, Xc, Yc)) < 1e-14
+    # a coefficient held on the host acts on a device field
+    th = Tensor(val=X, order=len(xs), multype=multype)
+    assert _rel(th(Tensor(val=nat.DeviceArray.from_host(Y), order=len(ys))).to_host().val, ref) < 1e-14
+
+
+@pytest.mark.parametrize('N,physics', [((5, 7, 9), 'scalar'), ((6, 8), 'scalar'), ((5, 5, 7), 'elasticity')])
+def test_operator_composition_matches_oracle(N, physics):
+    """The reference driver's composition Afun = Operator([[Operator([[iF, hG1, F]]), A]])
+    (ffthompy/applications.py:54-58) built from the drop-in's DFT, projection and coefficient Tensor,
+    against the oracle's materialised operator."""
+    from ffthompy_amd.projections import Operator
+    from oracle.fields import Operator as OOperator
+    d = len(N)
+    rng = np.random.default_rng(16)
+    Y = np.ones(d)
+    nc = d if physics == 'scalar' else d * (d + 1) // 2
+    a = 1. + rng.random(N)
+    A = np.einsum('ij,...->ij...', np.eye(nc), a) + 0.1 * rng.standard_normal((nc, nc) + N)
+    x = rng.standard_normal((nc,) + N)
+    if physics == 'scalar':
+        _, oG1, _ = oproj.scalar(N, Y)
+        _, G1, _ = projections.scalar(N, Y)
+        MG = oG1
+    else:
+        _, a1, b1, _, _ = oproj.elasticity(N, Y)
+        _, G1, _ = projections.elasticity(N, Y)
+        MG = a1 + b1
+    oAfun = OOperator([[OOperator([[ODFT(N, inverse=True), MG, ODFT(N)]]), Field(val=A, order=2, multype=21)]])
+    ref = oAfun(Field(val=x, order=1)).val
+    Afun = Operator(mat=[[Operator(mat=[[DFT(N=N, inverse=True), G1, DFT(N=N)]]), Tensor(val=A, order=2, multype=21)]])
+    got = Afun(_dev(x, 1, N)).to_host().val
+    assert _rel(got, ref) < 1e-12
