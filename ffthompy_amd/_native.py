@@ -840,7 +840,7 @@ class Plan:
         dt = np.dtype(dtype)
         x = DeviceArray.from_host(np.random.default_rng(seed).standard_normal((self.ncomp,) + self.N).astype(dt))
         y = DeviceArray((self.ncomp,) + self.N, dt)
-        tp, tc, tr = np.zeros(4), np.zeros(2), np.zeros(2)
+        tp, tc, tr = np.zeros(4), np.zeros(2), np.zeros(3)
         p, c, r = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         call('fh_plan_autotune', self._h, ctypes.c_int(dtype_code(dt)), x.ptr, y.ptr, ctypes.c_int(int(nrep)),
              tp.ctypes.data_as(_c_double_p), tc.ctypes.data_as(_c_double_p), tr.ctypes.data_as(_c_double_p),

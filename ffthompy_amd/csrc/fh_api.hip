@@ -722,12 +722,12 @@ static int ensure_hist(fh_plan* pl, int len) {
 // Returns 0 when the fused kernels do not cover this plan (nothing was enqueued).
 // Partials per pass that the deferred alpha/beta of a small grid sums (the forward pass's go to
 // pl->partial[0..kDeferMax), the inverse pass's to pl->partial[kDeferMax..2 kDeferMax)).
-constexpr int kDeferMax = 1024;
+constexpr int kDeferMax = 2048;
 
-// Deferred alpha/beta: rows of at most 8192 (every row pass has at least 8 rows per workgroup, so
-// at most kDeferMax workgroups) and room for both partial runs.
+// Deferred alpha/beta: at most 6 * kDeferMax grid rows (every fixed-length row pass has at least 6
+// rows per workgroup, so at most kDeferMax workgroups) and room for both partial runs.
 static bool defer_applies(const fh_plan* pl) {
-  return pl->defer_ok && (long long)pl->n0 * pl->n1 <= 8LL * kDeferMax && pl->partial_cap >= 2 * (size_t)kDeferMax;
+  return pl->defer_ok && (long long)pl->n0 * pl->n1 <= 6LL * kDeferMax && pl->partial_cap >= 2 * (size_t)kDeferMax;
 }
 
 // One fused fp32 CG iteration. With pl->cg_defer (small grids) alpha and beta have no kernels of

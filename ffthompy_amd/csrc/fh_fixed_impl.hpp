@@ -192,10 +192,15 @@ int rows_fwd_fx_launch(const LaunchCtx& c, int nc, const T* in, const T* coef, i
                        const int* halt, const T* rvec, const CGState* st, T* xc, double* partial,
                        size_t partial_cap, int* nblocks) {
   constexpr int R0 = FxShape<FP, T>::rows_rpb(0), R1 = FxShape<FP, T>::rows_rpb(1);
+  constexpr int R2 = FxShape<FP, T>::rows_rpb(2);
   if (c.rows_var == 1) {
     const int r = rows_fwd_fx_launch_r<T, FP, R1, false>(c, nc, in, coef, kind, out, halt, rvec, st, xc, partial,
                                                          partial_cap, nblocks);
     if (r != 0) return r;  // 0: the variant does not fit this case; run the default
+  } else if (c.rows_var == 2) {
+    const int r = rows_fwd_fx_launch_r<T, FP, R2, false>(c, nc, in, coef, kind, out, halt, rvec, st, xc, partial,
+                                                         partial_cap, nblocks);
+    if (r != 0) return r;
   }
   return rows_fwd_fx_launch_r<T, FP, R0, true>(c, nc, in, coef, kind, out, halt, rvec, st, xc, partial, partial_cap,
                                                nblocks);
@@ -260,8 +265,13 @@ template <typename T, class FP>
 int rows_inv_fx_launch(const LaunchCtx& c, int nc, const cpx<T>* in, T* out, T scale, int epi, const T* w,
                        double* partial, size_t partial_cap, const int* halt, int* nblocks, CGState* st) {
   constexpr int R0 = FxShape<FP, T>::rows_rpb(0), R1 = FxShape<FP, T>::rows_rpb(1);
+  constexpr int R2 = FxShape<FP, T>::rows_rpb(2);
   if (c.rows_var == 1) {
     const int r = rows_inv_fx_launch_r<T, FP, R1, false>(c, nc, in, out, scale, epi, w, partial, partial_cap, halt,
+                                                         nblocks, st);
+    if (r != 0) return r;
+  } else if (c.rows_var == 2) {
+    const int r = rows_inv_fx_launch_r<T, FP, R2, false>(c, nc, in, out, scale, epi, w, partial, partial_cap, halt,
                                                          nblocks, st);
     if (r != 0) return r;
   }

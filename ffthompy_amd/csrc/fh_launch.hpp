@@ -30,7 +30,7 @@ struct LaunchCtx {
 };
 
 // Number of schedule variants of the projection, strided and contiguous-axis passes.
-constexpr int kProjVariants = 4, kColsVariants = 2, kRowsVariants = 2;
+constexpr int kProjVariants = 4, kColsVariants = 2, kRowsVariants = 3;
 
 // Return 1 if a specialised kernel was launched, 0 if the length has none (caller falls
 // back to the generic kernels), or a negative hipError_t on launch failure.

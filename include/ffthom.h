@@ -91,7 +91,7 @@ int fh_plan_use_generic(fh_plan* plan, int generic);
  * using <p, a p> = <p, G(a p)> for p in the range of G; it applies to isotropic
  * coefficients on the fixed-length kernels and falls back otherwise. fused = 0 keeps the
  * reference's operator-form denominator <p, Afun(p)> (ffthompy/general/solver.py:125-126).
- * fused = 1 (default): on grids of at most 8192 rows (n0 * n1) alpha and beta are formed inside
+ * fused = 1 (default): on grids of at most 12288 rows (n0 * n1) alpha and beta are formed inside
  * the passes (each forward-pass workgroup sums the inverse pass's partials in one fixed order,
  * block 0 of the first strided pass finalises the state) instead of by two one-workgroup kernels
  * per iteration; fused = 2 keeps those kernels. Plain FP64 solves always use the operator form. */
@@ -343,12 +343,12 @@ int fh_matvec_profile(fh_plan* plan, int dtype, const void* x, void* y, int nrep
 /* Schedule of the fixed-length kernels, per arithmetic (dtype FH_F64 / FH_F32). proj_variant
  * (0..3) sets the frequency points per tile of the projection pass (forward FFT along the
  * outermost axis, G(xi), inverse FFT), which also sets the threads per workgroup; cols_variant (0..1) the columns per tile of the other strided
- * passes (3-D); rows_variant (0..1) the grid rows per workgroup of the contiguous-axis passes
- * (variant 1: 3/2 as many; scalar plans, isotropic or no coefficient). Every variant computes the
+ * passes (3-D); rows_variant (0..2) the grid rows per workgroup of the contiguous-axis passes
+ * (variant 1: 3/2 as many, variant 2: 3/4 as many; scalar plans, isotropic or no coefficient). Every variant computes the
  * same values (the same transforms per pencil); only speed differs. Variant 0 is the default.
  * fh_plan_autotune times every variant on this plan (nrep matvecs of x into y per variant, each
  * pass timed by its own HIP events, as fh_matvec_profile), keeps the fastest of each pass and
- * returns the times in ms (proj_ms[4], cols_ms[2], rows_ms[2]; any may be NULL) and the choice.
+ * returns the times in ms (proj_ms[4], cols_ms[2], rows_ms[3]; any may be NULL) and the choice.
  * FH_ERR_UNSUPPORTED where the plan runs the runtime-radix kernels. No reference counterpart (the
  * reference calls NumPy's FFT). */
 int fh_plan_set_schedule(fh_plan* plan, int dtype, int proj_variant, int cols_variant, int rows_variant);
