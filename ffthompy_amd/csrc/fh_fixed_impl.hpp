@@ -86,6 +86,7 @@ int rows_fwd_fx_launch_r(const LaunchCtx& c, int nc, const T* in, const T* coef,
   do {                                                                                                        \
     const size_t slds = RowStage<FP, T, NCC>::lds_bytes(rpb, KIND == COEF_ISO ? 1 : (KIND == COEF_LAME ? 2 : 0)); \
     if (slds + FP::L * sizeof(cpx<T>) > 160 * 1024) return 0;                                                 \
+    if (!offsets_fit(nrows * (FP::L / 2 + 1) * NCC * 2, sizeof(T))) return 0;                                 \
     auto kf = k_rows_fwd_st<T, FP, KIND, NCC, UPDV, RPB_>;                                                    \
     allow_lds(kf, slds);                                                                                      \
     hipExtLaunchKernelGGL(kf, grid, block, slds, c.stream, c.t0, c.t1, 0, in, coef, out, nrows, c.n, rpb, tw, halt, rvec, st, xc, \

@@ -53,11 +53,19 @@ struct RowStage {
   // Stockham index patterns of every fixed length then issue without bank conflicts (an odd
   // stride doubled the LDS cycles of the 243-point passes through 2-way conflicts)
   __host__ __device__ static constexpr int pen_stride(int rpb) { return (rpb / 2) * NC; }
-  // bytes: max(pencils, staging) + staging of ncoef coefficient runs (forward pass only)
+  // the forward pass's half spectrum of rpb rows in HBM order, plus one 16-byte chunk of
+  // alignment slack (the run starts at its HBM complex index modulo the chunk's complex count)
+  __host__ __device__ static constexpr size_t spec_bytes(int rpb) {
+    return (size_t)rpb * (FP::L / 2 + 1) * NC * sizeof(cpx<T>) + 16;
+  }
+  // bytes: max(pencils, staging, spectrum run) + staging of ncoef coefficient runs (forward
+  // pass only)
   __host__ __device__ static constexpr size_t lds_bytes(int rpb, int ncoef) {
     const size_t pen = (size_t)pen_stride(rpb) * FP::L * sizeof(cpx<T>);
     const size_t stg = (size_t)NC * comp_stride(rpb) * sizeof(T);
-    return (pen > stg ? pen : stg) + (size_t)ncoef * comp_stride(rpb) * sizeof(T);
+    const size_t spc = spec_bytes(rpb);
+    const size_t mx = pen > stg ? pen : stg;
+    return (mx > spc ? mx : spc) + (size_t)ncoef * comp_stride(rpb) * sizeof(T);
   }
 };
 
@@ -261,33 +269,66 @@ k_rows_fwd_st(const T* __restrict__ in, const T* __restrict__ coef, cpx<T>* __re
     auto at = [&](int j, int) -> cpx<T>& { return s[j * S + pen]; };
     fx_lds_stages<T, FP, -1, 1, FP::NST, 1>(t, at, twl);
   }
-  // untangle the paired rows and store the AoS half spectrum (contiguous per block): a
-  // thread takes one row pair and one k; pencil m holds the real rows 2m and 2m+1 of the
-  // pair's 2*NC (row, component) rows, so two LDS reads give two outputs
-  cpx<T>* obp = out + row0 * (n2h * NC);
+  // untangle the paired rows into registers: a thread takes one row pair and one k; pencil m
+  // holds the real rows 2m and 2m+1 of the pair's 2*NC (row, component) rows, so two LDS
+  // reads give two outputs. Row pair fastest: the lanes' reads s[k*S + rp*NC + m] are NC
+  // elements apart.
   const int nrp_v = (nr + 1) >> 1;
-  for (int e = tid; e < nrp_v * n2h; e += nthr) {
-    // row pair fastest: the lanes' reads s[k*S + rp*NC + m] are NC elements apart
+  constexpr int UIT = ((RPB / 2) * n2h + NT - 1) / NT;
+  cpx<T> Y[UIT][2 * NC];  // Y[u][h*NC + c] is (row 2rp+h, component c)
+#pragma unroll
+  for (int u = 0; u < UIT; ++u) {
+    const int e = tid + u * NT;
+    if (e >= nrp_v * n2h) continue;
     const int k = e / nrp_v, rp = e - k * nrp_v;
     const int kc = k ? L - k : 0;
-    cpx<T> Y[2 * NC];
 #pragma unroll
     for (int m = 0; m < NC; ++m) {
       const int q = rp * NC + m;
       const cpx<T> Z = s[k * S + q];
       const cpx<T> Zc = conj(s[kc * S + q]);
       const cpx<T> d = Z - Zc;
-      Y[2 * m] = T(0.5) * (Z + Zc);
-      Y[2 * m + 1] = mk(T(0.5) * d.y, T(-0.5) * d.x);
+      Y[u][2 * m] = T(0.5) * (Z + Zc);
+      Y[u][2 * m + 1] = mk(T(0.5) * d.y, T(-0.5) * d.x);
     }
-    // Y[h*NC + c] is (row 2rp+h, component c)
-    cpx<T>* o0 = obp + ((long long)(2 * rp) * n2h + k) * NC;
+  }
+  // the block's half spectrum is one contiguous AoS run of nr * n2h * NC values in HBM: it is
+  // written over the pencils in HBM order (starting at its HBM index modulo the chunk's count
+  // CW, so HBM chunk q sits on a 16-byte LDS boundary) and stored in 16-byte chunks, instead of
+  // each lane storing its NC values at a row pair's stride
+  constexpr int CW = 16 / (int)sizeof(cpx<T>);
+  const long long c0 = row0 * (long long)(n2h * NC);
+  const int so = (int)(c0 % CW);
+  __syncthreads();
 #pragma unroll
-    for (int c = 0; c < NC; ++c) o0[c] = Y[c];
+  for (int u = 0; u < UIT; ++u) {
+    const int e = tid + u * NT;
+    if (e >= nrp_v * n2h) continue;
+    const int k = e / nrp_v, rp = e - k * nrp_v;
+    cpx<T>* o0 = s + so + ((2 * rp) * n2h + k) * NC;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) o0[c] = Y[u][c];
     if (2 * rp + 1 < nr) {
-      cpx<T>* o1 = o0 + (long long)n2h * NC;
 #pragma unroll
-      for (int c = 0; c < NC; ++c) o1[c] = Y[NC + c];
+      for (int c = 0; c < NC; ++c) o0[n2h * NC + c] = Y[u][NC + c];
+    }
+  }
+  __syncthreads();
+  {
+    const __amdgpu_buffer_rsrc_t rsp = make_rsrc(out, (unsigned long long)nrows * n2h * NC * sizeof(cpx<T>));
+    const long long c1 = c0 + (long long)nr * n2h * NC;  // one past the run's last value
+    const unsigned q0 = (unsigned)(c0 / CW), q1 = (unsigned)((c1 + CW - 1) / CW);
+    for (unsigned q = q0 + tid; q < q1; q += nthr) {
+      const long long g = (long long)q * CW;  // the chunk's first complex index
+      const int l = so + (int)(g - c0);       // its LDS index (a multiple of CW)
+      if (g >= c0 && g + CW <= c1) {
+        bstore_chunk<T>(rsp, q * 16u, *reinterpret_cast<const Chunk<T>*>(s + l));
+      } else {
+#pragma unroll
+        for (int e = 0; e < CW; ++e) {
+          if (g + e >= c0 && g + e < c1) bstore_c<T>(rsp, (unsigned)((g + e) * sizeof(cpx<T>)), s[l + e]);
+        }
+      }
     }
   }
   if constexpr (UPD == 2) {
