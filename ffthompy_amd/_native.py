@@ -834,7 +834,8 @@ class Plan:
 
     def autotune(self, dtype=np.float32, nrep=5, seed=0):
         """Time every schedule variant of the projection and strided passes on this plan (random
-        field, nrep matvecs per variant) and keep the fastest: fh_plan_autotune. Returns
+        field, 3 interleaved rounds of nrep matvecs per variant, best round) and keep the fastest:
+        fh_plan_autotune. Returns
         {'proj_ms': [...], 'cols_ms': [...], 'rows_ms': [...], 'proj_variant': p, 'cols_variant': c,
         'rows_variant': r}."""
         dt = np.dtype(dtype)

@@ -1981,26 +1981,31 @@ int fh_plan_autotune(fh_plan* pl, int dtype, const void* x, void* y, int nrep, d
   int nkr = 0;
   const int keep_p = pl->proj_var[i], keep_c = pl->cols_var[i], keep_r = pl->rows_var[i];
   double tp[fh::kProjVariants], tc[fh::kColsVariants], tr[fh::kRowsVariants];
-  // each variant of one pass with the other pass at its current choice; the faster wins
+  // each variant of one pass with the other passes at their current choice; the faster wins.
+  // kRounds interleaved rounds over the variants, each variant keeping its best round, so a
+  // slow spell of the device does not fall on one variant only
+  constexpr int kRounds = 3;
   int rc = FH_OK;
-  for (int v = 0; v < fh::kProjVariants && rc == FH_OK; ++v) {
-    pl->proj_var[i] = v;
-    rc = fh_matvec_profile(pl, dtype, x, y, nrep, 8, kms, kby, &nkr);
-    tp[v] = rc == FH_OK ? kms[kproj] : 0.0;
+  for (int v = 0; v < fh::kProjVariants; ++v) tp[v] = 1e300;
+  for (int r = 0; r < kRounds; ++r) {
+    for (int v = 0; v < fh::kProjVariants && rc == FH_OK; ++v) {
+      pl->proj_var[i] = v;
+      rc = fh_matvec_profile(pl, dtype, x, y, nrep, 8, kms, kby, &nkr);
+      if (rc == FH_OK) tp[v] = std::min(tp[v], kms[kproj]);
+    }
   }
   int bp = 0;
   for (int v = 1; v < fh::kProjVariants; ++v)
     if (tp[v] < tp[bp]) bp = v;
   pl->proj_var[i] = bp;
   int bc = keep_c;
-  for (int v = 0; v < fh::kColsVariants && rc == FH_OK; ++v) {
-    if (nk == 3) {
-      tc[v] = 0.0;
-      continue;
+  for (int v = 0; v < fh::kColsVariants; ++v) tc[v] = nk == 3 ? 0.0 : 1e300;
+  for (int r = 0; r < kRounds && nk == 5; ++r) {
+    for (int v = 0; v < fh::kColsVariants && rc == FH_OK; ++v) {
+      pl->cols_var[i] = v;
+      rc = fh_matvec_profile(pl, dtype, x, y, nrep, 8, kms, kby, &nkr);
+      if (rc == FH_OK) tc[v] = std::min(tc[v], kms[1] + kms[3]);
     }
-    pl->cols_var[i] = v;
-    rc = fh_matvec_profile(pl, dtype, x, y, nrep, 8, kms, kby, &nkr);
-    tc[v] = rc == FH_OK ? kms[1] + kms[3] : 0.0;
   }
   if (nk == 5) {
     bc = 0;
@@ -2009,10 +2014,13 @@ int fh_plan_autotune(fh_plan* pl, int dtype, const void* x, void* y, int nrep, d
   }
   pl->cols_var[i] = bc;
   // contiguous-axis passes: forward and inverse together
-  for (int v = 0; v < fh::kRowsVariants && rc == FH_OK; ++v) {
-    pl->rows_var[i] = v;
-    rc = fh_matvec_profile(pl, dtype, x, y, nrep, 8, kms, kby, &nkr);
-    tr[v] = rc == FH_OK ? kms[0] + kms[nk - 1] : 0.0;
+  for (int v = 0; v < fh::kRowsVariants; ++v) tr[v] = 1e300;
+  for (int r = 0; r < kRounds; ++r) {
+    for (int v = 0; v < fh::kRowsVariants && rc == FH_OK; ++v) {
+      pl->rows_var[i] = v;
+      rc = fh_matvec_profile(pl, dtype, x, y, nrep, 8, kms, kby, &nkr);
+      if (rc == FH_OK) tr[v] = std::min(tr[v], kms[0] + kms[nk - 1]);
+    }
   }
   int br = 0;
   for (int v = 1; v < fh::kRowsVariants; ++v)

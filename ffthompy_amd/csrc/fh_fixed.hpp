@@ -219,12 +219,15 @@ struct FxShape {
     return FP::TPP >= 16 ? c64a[v] : (FP::TPP >= 8 ? c64b[v] : c64c[v]);
   }
   static constexpr int cols_p(int v) { return v == 0 ? P : (P >= 2 ? P / 2 : P); }
-  // contiguous-axis passes: rows per workgroup (even); variant 1 takes 3/2 as many rows and
-  // variant 2 3/4 as many, which also changes how full the last wavefront is (TPP * (rpb/2) * NC
-  // threads: 243 points, 3 components: 8 rows 324 threads in 6 wavefronts, 12 rows 486 in 8,
-  // 6 rows 243 in 4) and how many workgroups share a CU's LDS
+  // contiguous-axis passes: rows per workgroup (even); variant 1 takes 3/2 as many rows in single
+  // precision and 1/2 as many in double, variant 2 3/4 as many, which also changes how full the
+  // last wavefront is (TPP * (rpb/2) * NC threads: 243 points, 3 components: 8 rows 324 threads
+  // in 6 wavefronts, 12 rows 486 in 8, 6 rows 243 in 4, 4 rows 162 in 3) and how many workgroups
+  // share a CU's LDS
   static constexpr int NROWS = 3;
-  static constexpr int rows_rpb(int v) { return v == 0 ? RPB : (v == 1 ? RPB / 2 * 3 : RPB / 4 * 3); }
+  static constexpr int rows_rpb(int v) {
+    return v == 0 ? RPB : (v == 1 ? (sizeof(T) == 4 ? RPB / 2 * 3 : RPB / 2) : RPB / 4 * 3);
+  }
 };
 
 // ---- strided pass: forward (DIR -1) or inverse (DIR +1) along a non-contiguous axis ----

@@ -74,12 +74,13 @@ int rows_fwd_fx_launch_r(const LaunchCtx& c, int nc, const T* in, const T* coef,
                          st, xc, partial);                                                            \
     }                                                                                                 \
   } while (0)
-  // staged kernels (16-byte HBM chunks through LDS) in single precision; in double precision
-  // their LDS footprint halves the occupancy and the direct kernel is faster. The staged
-  // kernels finish every direction update before any coefficient product, so the fused
-  // update also serves the Lame coefficient there.
+  // staged kernels (16-byte HBM chunks through LDS) in single precision, and in double
+  // precision for the smaller row counts of variants 1 and 2 (at the default row count their
+  // doubled LDS footprint halves the occupancy and the direct kernel, which also serves every
+  // coefficient kind, is as fast). The staged kernels finish every direction update before any
+  // coefficient product, so the fused update also serves the Lame coefficient there.
   const bool lame_ok = kind == COEF_LAME && (nc == 3 || nc == 6);
-  if ((kind == COEF_ISO || kind == COEF_NONE || lame_ok) && sizeof(T) == 4) {
+  if ((kind == COEF_ISO || kind == COEF_NONE || lame_ok) && (sizeof(T) == 4 || !ALL)) {
     const int upd = xc ? 2 : (rvec ? 1 : 0);
     if (upd == 2 && (!rvec || kind == COEF_NONE || (size_t)nb > partial_cap)) return 0;
 #define FH_LAUNCH_ST(KIND, NCC, UPDV)                                                                         \
@@ -217,7 +218,7 @@ int rows_inv_fx_launch_r(const LaunchCtx& c, int nc, const cpx<T>* in, T* out, T
   if (nthr > 1024) return 0;
   const size_t lds = (size_t)npen * FP::L * sizeof(cpx<T>);
   if (lds + FP::L * sizeof(cpx<T>) > 160 * 1024) return 0;
-  if (!offsets_fit(c.n * nc, sizeof(T))) return 0;
+  if (!offsets_fit(c.n * nc, sizeof(T)) || !offsets_fit(c.nh * nc, sizeof(cpx<T>))) return 0;
   const long long nrows = (long long)c.n0 * c.n1;
   const long long nb = (nrows + rpb - 1) / rpb;
   if (epi && (size_t)nb > partial_cap) return 0;

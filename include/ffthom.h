@@ -344,10 +344,12 @@ int fh_matvec_profile(fh_plan* plan, int dtype, const void* x, void* y, int nrep
  * (0..3) sets the frequency points per tile of the projection pass (forward FFT along the
  * outermost axis, G(xi), inverse FFT), which also sets the threads per workgroup; cols_variant (0..1) the columns per tile of the other strided
  * passes (3-D); rows_variant (0..2) the grid rows per workgroup of the contiguous-axis passes
- * (variant 1: 3/2 as many, variant 2: 3/4 as many; scalar plans, isotropic or no coefficient). Every variant computes the
- * same values (the same transforms per pencil); only speed differs. Variant 0 is the default.
- * fh_plan_autotune times every variant on this plan (nrep matvecs of x into y per variant, each
- * pass timed by its own HIP events, as fh_matvec_profile), keeps the fastest of each pass and
+ * (variant 1: 3/2 as many in single precision, 1/2 as many in double; variant 2: 3/4 as many;
+ * scalar plans, isotropic or no coefficient). Every variant computes the same values (the same
+ * transforms per pencil); only speed differs. Variant 0 is the default.
+ * fh_plan_autotune times every variant on this plan (3 interleaved rounds of nrep matvecs of x
+ * into y per variant, each pass timed by its own HIP events, as fh_matvec_profile; a variant's
+ * time is its best round), keeps the fastest of each pass and
  * returns the times in ms (proj_ms[4], cols_ms[2], rows_ms[3]; any may be NULL) and the choice.
  * FH_ERR_UNSUPPORTED where the plan runs the runtime-radix kernels. No reference counterpart (the
  * reference calls NumPy's FFT). */

@@ -12,6 +12,7 @@ pytestmark = pytest.mark.gpu
 
 CASES = [((45, 45, 45), np.float32, 'scalar'), ((45, 45, 45), np.float64, 'scalar'),
          ((81, 81, 81), np.float32, 'scalar'), ((243, 27, 243), np.float32, 'scalar'),
+         ((27, 45, 243), np.float64, 'scalar'),
          ((63, 135), np.float32, 'scalar'), ((99, 99), np.float64, 'scalar'),
          ((81, 81, 81), np.float32, 'elasticity'), ((45, 27, 45), np.float64, 'elasticity'),
          ((81, 45), np.float32, 'elasticity')]
