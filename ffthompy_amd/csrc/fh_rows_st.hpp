@@ -370,22 +370,35 @@ k_rows_inv_st(const cpx<T>* __restrict__ in, T* __restrict__ out, long long nrow
   constexpr unsigned CS = sizeof(cpx<T>);
   const __amdgpu_buffer_rsrc_t rsp =
       make_rsrc(in + row0 * (long long)n2h * NC, (unsigned long long)nr * n2h * NC * CS);
-  for (int e = tid; e < nrp * n2h; e += nthr) {
+  // every spectrum load of the thread is issued before the first LDS write (the launch passes
+  // rpb == RPB and NTH threads)
+  constexpr int NTH = (RPB / 2) * NC * TPP;
+  constexpr int UIT = ((RPB / 2) * n2h + NTH - 1) / NTH;  // (row pair, k) items per thread
+  cpx<T> Y[UIT][2 * NC];
+#pragma unroll
+  for (int u = 0; u < UIT; ++u) {
+    const int e = tid + u * NTH;
+    if (e >= nrp * n2h) continue;
+    const int k = e / nrp, rp = e - k * nrp;
+    const unsigned o0 = (unsigned)(((2 * rp) * n2h + k) * NC) * CS;
+    const unsigned o1 = o0 + (unsigned)(n2h * NC) * CS;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      Y[u][c] = bload_c<T>(rsp, o0 + c * CS);
+      Y[u][NC + c] = bload_c<T>(rsp, o1 + c * CS);
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < UIT; ++u) {
+    const int e = tid + u * NTH;
+    if (e >= nrp * n2h) continue;
     // row pair fastest: the lanes' writes s[k*S + rp*NC + m] are NC elements apart
     const int k = e / nrp, rp = e - k * nrp;
     const bool self = (k == 0) || (2 * k == L);
-    const unsigned o0 = (unsigned)(((2 * rp) * n2h + k) * NC) * CS;
-    const unsigned o1 = o0 + (unsigned)(n2h * NC) * CS;
-    cpx<T> Y[2 * NC];
-#pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      Y[c] = bload_c<T>(rsp, o0 + c * CS);
-      Y[NC + c] = bload_c<T>(rsp, o1 + c * CS);
-    }
     const int kc = k ? L - k : 0;
 #pragma unroll
     for (int m = 0; m < NC; ++m) {
-      cpx<T> Ya = Y[2 * m], Yb = Y[2 * m + 1];
+      cpx<T> Ya = Y[u][2 * m], Yb = Y[u][2 * m + 1];
       if (self) {
         Ya.y = T(0);
         Yb.y = T(0);
