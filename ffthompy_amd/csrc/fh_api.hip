@@ -536,9 +536,10 @@ int launch_cols(fh_plan* pl, int mode, int axis, int nc, cpx<T>* data, const GPa
 // y = F^-1 G F (A x); kind COEF_NONE gives the bare projection. With w != null the
 // last pass also writes per-workgroup partial sums of y.w into pl->partial.
 // rvec != null: x is the CG direction, updated in place to rvec + beta x before use.
+// passes 1 .. 4 of y = F^-1 G F (A x): the plan's spectrum then holds G F (A x) along all but the
+// contiguous axis
 template <typename T>
-int run_operator(fh_plan* pl, const T* x, T* y, int kind, const T* coef, const T* w, const int* halt,
-                 int* nblocks_out, const T* rvec = nullptr) {
+int run_operator_front(fh_plan* pl, const T* x, int kind, const T* coef, const int* halt, const T* rvec) {
   const int nc = pl->ncomp;
   cpx<T>* spec = reinterpret_cast<cpx<T>*>(pl->spec);
   const GParams g = make_gparams(pl, (kind != fh::COEF_NONE ? pl->op_scale : 1.0) / (double)pl->n);
@@ -550,7 +551,42 @@ int run_operator(fh_plan* pl, const T* x, T* y, int kind, const T* coef, const T
   } else {
     RETCHK(launch_cols<T>(pl, 2, 1, nc, spec, g, halt));
   }
-  RETCHK(launch_rows_inv<T>(pl, nc, spec, y, T(1), w, pl->partial, halt, nblocks_out));
+  return FH_OK;
+}
+
+template <typename T>
+int run_operator(fh_plan* pl, const T* x, T* y, int kind, const T* coef, const T* w, const int* halt,
+                 int* nblocks_out, const T* rvec = nullptr) {
+  RETCHK(run_operator_front<T>(pl, x, kind, coef, halt, rvec));
+  cpx<T>* spec = reinterpret_cast<cpx<T>*>(pl->spec);
+  RETCHK(launch_rows_inv<T>(pl, pl->ncomp, spec, y, T(1), w, pl->partial, halt, nblocks_out));
+  return FH_OK;
+}
+
+// r = b - A x with the fp64 operator and a single-precision r. Where the fixed-length inverse
+// row pass covers the plan it forms the residual in its epilogue (A x never goes to HBM; *fused
+// = true, the partial sums of <r, r> come from its *np workgroups); otherwise Ax = A x is stored
+// for the separate residual kernel (*fused = false).
+int run_operator_residual(fh_plan* pl, const double* x, const double* b, double* Ax, float* r, bool* fused,
+                          int* np) {
+  RETCHK(run_operator_front<double>(pl, x, pl->coef_kind, reinterpret_cast<const double*>(pl->coef64), nullptr,
+                                    nullptr));
+  cpx<double>* spec = reinterpret_cast<cpx<double>*>(pl->spec);
+  *fused = false;
+  if (!pl->generic) {
+    fh::LaunchCtx c = make_ctx<double>(pl);
+    c.r32 = r;
+    int nb = 0;
+    const int rr = fh::fixed_rows_inv<double>(c, pl->ncomp, spec, nullptr, 1.0, 3, b, pl->partial, pl->partial_cap,
+                                              nullptr, &nb, pl->st);
+    if (rr != 0) {
+      RETCHK(fixed_result(rr, "rows inverse with residual (fixed)"));
+      *fused = true;
+      *np = nb;
+      return FH_OK;
+    }
+  }
+  RETCHK(launch_rows_inv<double>(pl, pl->ncomp, spec, Ax, 1.0, nullptr, pl->partial, nullptr, nullptr));
   return FH_OK;
 }
 
@@ -831,12 +867,19 @@ template <typename TR>
 static int enqueue_true_residual(fh_plan* pl, const double* b, const double* x, double* Ax, TR* r,
                                  bool x_zero = false) {
   const long long nv = (long long)pl->ncomp * pl->n;
+  int np = grid_blocks(nv);
+  bool fused = false;
   // x = 0: A x = 0 exactly, so r = b without the operator (the same bits as b - A 0)
-  if (!x_zero) RETCHK(run_operator<double>(pl, x, Ax, pl->coef_kind, pl->coef64, nullptr, nullptr, nullptr));
-  const int gb = grid_blocks(nv);
-  hipLaunchKernelGGL((fh::k_residual<TR>), dim3(gb), dim3(256), 0, pl->stream, nv, b, x_zero ? nullptr : Ax, r,
-                     pl->partial);
-  hipLaunchKernelGGL(fh::k_cg_restart, dim3(1), dim3(256), 0, pl->stream, pl->st, pl->partial, gb, pl->hist);
+  if (!x_zero) {
+    if constexpr (sizeof(TR) == 4)
+      RETCHK(run_operator_residual(pl, x, b, Ax, r, &fused, &np));
+    else
+      RETCHK(run_operator<double>(pl, x, Ax, pl->coef_kind, pl->coef64, nullptr, nullptr, nullptr));
+  }
+  if (!fused)
+    hipLaunchKernelGGL((fh::k_residual<TR>), dim3(np), dim3(256), 0, pl->stream, nv, b, x_zero ? nullptr : Ax, r,
+                       pl->partial);
+  hipLaunchKernelGGL(fh::k_cg_restart, dim3(1), dim3(256), 0, pl->stream, pl->st, pl->partial, np, pl->hist);
   HIPCHK(hipGetLastError());
   return FH_OK;
 }

@@ -223,6 +223,7 @@ int rows_inv_fx_launch_r(const LaunchCtx& c, int nc, const cpx<T>* in, T* out, T
   const long long nb = (nrows + rpb - 1) / rpb;
   if (epi && (size_t)nb > partial_cap) return 0;
   if (epi == 1 && !w) return 0;
+  if (epi == 3 && (!w || !c.r32 || sizeof(T) != 8)) return 0;
   const dim3 grid((unsigned)nb), block(nthr);
   const cpx<T>* tw = reinterpret_cast<const cpx<T>*>(c.tw[2]);
 #define FH_LAUNCH_RI(EPI, NCC)                                                                                \
@@ -232,7 +233,7 @@ int rows_inv_fx_launch_r(const LaunchCtx& c, int nc, const cpx<T>* in, T* out, T
     auto kf = k_rows_inv_st<T, FP, EPI, NCC, RPB_>;                                                           \
     allow_lds(kf, slds);                                                                                      \
     hipExtLaunchKernelGGL(kf, grid, block, slds, c.stream, c.t0, c.t1, 0, in, out, nrows, c.n, rpb, tw, scale, w, partial, halt, \
-                       st, order_rev(1), c.defer_beta);                                                   \
+                       st, order_rev(1), c.defer_beta, c.r32);                                            \
   } while (0)
   const int key = epi * 8 + nc;
   switch (key) {
@@ -242,6 +243,10 @@ int rows_inv_fx_launch_r(const LaunchCtx& c, int nc, const cpx<T>* in, T* out, T
     case 8 + 3: FH_LAUNCH_RI(1, 3); break;
     case 16 + 2: FH_LAUNCH_RI(2, 2); break;
     case 16 + 3: FH_LAUNCH_RI(2, 3); break;
+    case 24 + 2:
+      if constexpr (sizeof(T) == 8) { FH_LAUNCH_RI(3, 2); break; } else { return 0; }
+    case 24 + 3:
+      if constexpr (sizeof(T) == 8) { FH_LAUNCH_RI(3, 3); break; } else { return 0; }
     default:
       if constexpr (ALL) {
         switch (key) {
@@ -251,6 +256,10 @@ int rows_inv_fx_launch_r(const LaunchCtx& c, int nc, const cpx<T>* in, T* out, T
           case 8 + 6: FH_LAUNCH_RI(1, 6); break;
           case 16 + 1: FH_LAUNCH_RI(2, 1); break;
           case 16 + 6: FH_LAUNCH_RI(2, 6); break;
+          case 24 + 1:
+            if constexpr (sizeof(T) == 8) { FH_LAUNCH_RI(3, 1); break; } else { return 0; }
+          case 24 + 6:
+            if constexpr (sizeof(T) == 8) { FH_LAUNCH_RI(3, 6); break; } else { return 0; }
           default: return 0;
         }
       } else {

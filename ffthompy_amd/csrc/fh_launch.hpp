@@ -27,6 +27,8 @@ struct LaunchCtx {
   int bnp = 0;
   int defer_beta = 0;
   CgFin fin{nullptr, nullptr, nullptr, 0, 0};
+  // inverse row pass epilogue 3 (double precision): the single-precision residual it writes
+  float* r32 = nullptr;
 };
 
 // Number of schedule variants of the projection, strided and contiguous-axis passes.
@@ -42,7 +44,8 @@ int fixed_rows_fwd(const LaunchCtx& c, int nc, const T* in, const T* coef, int k
                    const int* halt, const T* rvec, const CGState* st, T* xc, double* partial,
                    size_t partial_cap, int* nblocks);
 // epi 0: out = y; epi 1: also partial sums of <out, w>; epi 2: out is r, r -= alpha y and
-// partial sums of <r, r> (k_rows_inv_st).
+// partial sums of <r, r>; epi 3 (double precision; out unused): c.r32 = float(w - y), the
+// residual of b = w, and partial sums of <w - y, w - y> in double (k_rows_inv_st).
 template <typename T>
 int fixed_rows_inv(const LaunchCtx& c, int nc, const cpx<T>* in, T* out, T scale, int epi, const T* w,
                    double* partial, size_t partial_cap, const int* halt, int* nblocks, CGState* st);

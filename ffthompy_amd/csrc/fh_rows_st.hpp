@@ -339,12 +339,16 @@ k_rows_fwd_st(const T* __restrict__ in, const T* __restrict__ coef, cpx<T>* __re
 
 // ---- inverse: C2R along the contiguous axis, then the epilogue --------------------------
 // EPI 0: out = y. EPI 1: out = y, partial sums of <y, w>. EPI 2: out is the CG residual r,
-// r -= alpha y (alpha from st), partial sums of <r, r>.
+// r -= alpha y (alpha from st), partial sums of <r, r>. EPI 3 (double precision): the
+// residual of the right-hand side b = w, r32 = float(b - y) (out unused), partial sums of
+// <b - y, b - y> in double: the operator's output never goes to HBM.
 template <typename T, class FP, int EPI, int NC, int RPB = FxShape<FP, T>::RPB>
 __global__ void __launch_bounds__(1024)
 k_rows_inv_st(const cpx<T>* __restrict__ in, T* __restrict__ out, long long nrows, long long n, int rpb,
               const cpx<T>* __restrict__ twg, T scale, const T* __restrict__ w, double* __restrict__ partial,
-              const int* __restrict__ halt, CGState* __restrict__ st, int rev, int defer_beta) {
+              const int* __restrict__ halt, CGState* __restrict__ st, int rev, int defer_beta,
+              float* __restrict__ r32) {
+  static_assert(EPI != 3 || sizeof(T) == 8, "residual epilogue: double precision operator");
   if (halt && *halt) return;
   constexpr int L = FP::L, n2h = L / 2 + 1, TPP = FP::TPP, W = Chunk<T>::W;
   constexpr unsigned SR = sizeof(T);
@@ -433,11 +437,12 @@ k_rows_inv_st(const cpx<T>* __restrict__ in, T* __restrict__ out, long long nrow
   constexpr int NT = (RPB / 2) * NC * TPP;
   constexpr int QMAX = (RPB * L + 2 * W - 2) / W + 1;  // chunks touched by one run of RPB rows
   constexpr int CPT = (QMAX + NT - 1) / NT;
-  const __amdgpu_buffer_rsrc_t rout = make_rsrc(out, (unsigned long long)NC * n * SR);
+  const __amdgpu_buffer_rsrc_t rout = make_rsrc(EPI == 3 ? (void*)r32 : (void*)out,
+                                                (unsigned long long)NC * n * (EPI == 3 ? 4u : SR));
   const T alp = EPI == 2 ? T(st->alpha) : T(0);
-  // the epilogue operand (w or r) is read here, not prefetched before the transform: holding
+  // the epilogue operand (w, b or r) is read here, not prefetched before the transform: holding
   // it in registers through the transform costs more occupancy than the early load gains
-  const __amdgpu_buffer_rsrc_t rv = make_rsrc(EPI == 1 ? w : out, (unsigned long long)NC * n * SR);
+  const __amdgpu_buffer_rsrc_t rv = make_rsrc((EPI == 1 || EPI == 3) ? w : out, (unsigned long long)NC * n * SR);
   double acc = 0.0;
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
@@ -461,11 +466,27 @@ k_rows_inv_st(const cpx<T>* __restrict__ in, T* __restrict__ out, long long nrow
           const bool inr = full || (f >= s0 && f < s0 + len);
           if constexpr (EPI == 1) {
             if (inr) acc += (double)Y.e[e] * (double)V.e[e];
-          } else {
+          } else if constexpr (EPI == 2) {
             O.e[e] = V.e[e] - alp * Y.e[e];
+            if (inr) acc += (double)O.e[e] * (double)O.e[e];
+          } else {
+            O.e[e] = V.e[e] - Y.e[e];
             if (inr) acc += (double)O.e[e] * (double)O.e[e];
           }
         }
+      }
+      if constexpr (EPI == 3) {
+        // W = 2 doubles -> two floats (8 bytes) at the same element offset of r32
+        if (full) {
+          bstore_c<float>(rout, q * 8u, mk((float)O.e[0], (float)O.e[1]));
+        } else {
+#pragma unroll
+          for (int e = 0; e < W; ++e) {
+            const unsigned f = q * W + e;
+            if (f >= s0 && f < s0 + len) bstore_r<float>(rout, f * 4u, (float)O.e[e]);
+          }
+        }
+        continue;
       }
       if (full) {
         bstore_chunk<T>(rout, ob, O);
