@@ -1483,8 +1483,8 @@ int fh_cg_solve(fh_plan* pl, int mode, const void* b_, void* x_, double tol, int
         // rounding, which enters the energy (and the bound ordering) at first order; the projection
         // removes it (x changes by that rounding only). With a nonzero x0 its component outside the
         // range of G belongs to the reference's result and is kept.
-        RETCHK(run_operator<double>(pl, x, Ap, fh::COEF_NONE, nullptr, nullptr, nullptr, nullptr));
-        HIPCHK(hipMemcpyAsync(x, Ap, (size_t)nv * sizeof(double), hipMemcpyDeviceToDevice, pl->stream));
+        // (in place: the forward row pass has read all of x before the inverse one writes it)
+        RETCHK(run_operator<double>(pl, x, x, fh::COEF_NONE, nullptr, nullptr, nullptr, nullptr));
         HIPCHK(hipStreamSynchronize(pl->stream));
       }
       if (every > 0 && s.kit > logged && (s.kit % every == 0 || s.done)) RETCHK(log_true(s.kit));
@@ -1584,9 +1584,9 @@ int fh_cg_solve(fh_plan* pl, int mode, const void* b_, void* x_, double tol, int
     }
     fold();
     HIPCHK(hipGetLastError());
-    // final fp64 projection x <- G x removes the non-conforming part left by fp32 rounding
-    RETCHK(run_operator<double>(pl, x, Ax, fh::COEF_NONE, nullptr, nullptr, nullptr, nullptr));
-    HIPCHK(hipMemcpyAsync(x, Ax, (size_t)nv * sizeof(double), hipMemcpyDeviceToDevice, pl->stream));
+    // final fp64 projection x <- G x removes the non-conforming part left by fp32 rounding (in
+    // place: the forward row pass has read all of x before the inverse one writes it)
+    RETCHK(run_operator<double>(pl, x, x, fh::COEF_NONE, nullptr, nullptr, nullptr, nullptr));
   }
   HIPCHK(hipEventRecord(pl->ev1, pl->stream));
   HIPCHK(hipMemcpyAsync(&s, pl->st, sizeof(s), hipMemcpyDeviceToHost, pl->stream));
