@@ -269,7 +269,7 @@ struct fh_plan {
   int band[3] = {0, 0, 0};
   double op_scale = 1.0;         // factor of the operator G A (not of the bare projection)
   // fused mixed CG: a batch of iterations captured once as a HIP graph, replayed per batch
-  hipGraphExec_t it_exec = nullptr;
+  hipGraphExec_t it_exec[9] = {};  // captured batches of 1, 2, 4, 8 fused iterations, by length
   const void* it_key[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   bool it_graph_off = false;     // capture failed on this plan: direct launches from then on
   bool use_graphs = true;        // fh_plan_use_graphs
@@ -791,14 +791,14 @@ static int enqueue_fused_iteration(fh_plan* pl, float* xc, float* r, float* p, c
   RETCHK(fixed_result(rr, "rows forward (fused)"));
   *ok = true;
   if (defer && nb > kDeferMax) return fail(FH_ERR_STATE, "deferred alpha: too many row workgroups");
-  if (!defer)
-    hipLaunchKernelGGL(fh::k_cg_alpha, dim3(1), dim3(kFinThreads), 0, pl->stream, pl->st, part1, nb);
   const GParams g = make_gparams(pl, 1.0 / (double)pl->n);
-  // first strided pass: axis 1 forward (3-D) or the projection pass (2-D)
+  // first strided pass: axis 1 forward (3-D) or the projection pass (2-D); its block 0 finalises
+  // alpha (needed two passes later, by the inverse row pass) while the others transform, and on
+  // small grids first the deferred beta (the partials of the last inverse pass)
   bool first_done = false;
-  if (defer) {
+  {
     fh::LaunchCtx cf = make_ctx<float>(pl);
-    cf.fin = fh::CgFin{pl->st, part1, part2, nb, pl->defer_nb2};
+    cf.fin = fh::CgFin{pl->st, part1, defer ? part2 : nullptr, nb, defer ? pl->defer_nb2 : 0};
     const int rf = pl->n0 > 1 ? fh::fixed_cols<float>(cf, -1, 1, nc, spec, halt)
                               : fh::fixed_cols_g<float>(cf, 1, nc, spec, g, halt);
     if (rf != 0) {
@@ -806,7 +806,8 @@ static int enqueue_fused_iteration(fh_plan* pl, float* xc, float* r, float* p, c
       first_done = true;
     } else {
       // no fixed kernel on this axis: finalise with the one-workgroup kernels
-      hipLaunchKernelGGL(fh::k_cg_flush, dim3(1), dim3(32), 0, pl->stream, pl->st, part2, pl->defer_nb2);
+      if (defer)
+        hipLaunchKernelGGL(fh::k_cg_flush, dim3(1), dim3(32), 0, pl->stream, pl->st, part2, pl->defer_nb2);
       hipLaunchKernelGGL(fh::k_cg_alpha, dim3(1), dim3(kFinThreads), 0, pl->stream, pl->st, part1, nb);
     }
   }
@@ -1056,9 +1057,38 @@ int pot_operator(fh_plan* pl, cpx<T>* p, const cpx<T>* r, cpx<T>* out, T* re, co
 // plan's projection, band, kernel family and coefficient at capture time. Called by every entry
 // point that changes one of those (all entry points have finished their GPU work on return).
 void drop_graphs(fh_plan* pl) {
-  if (pl->it_exec) (void)hipGraphExecDestroy(pl->it_exec);
-  pl->it_exec = nullptr;
+  for (auto& e : pl->it_exec) {
+    if (e) (void)hipGraphExecDestroy(e);
+    e = nullptr;
+  }
 }
+
+// Length of the next batch of CG iterations between two host synchronisations: the iterations
+// that the convergence rate since the last synchronisation predicts until the residual norm falls
+// below `target` (the next stop or reliable-update threshold), plus one, rounded up to a power of
+// two within [1, cap] (so that at most four captured batch graphs exist per plan). An iteration
+// enqueued past the stop returns at once on the halt flag but still dispatches its grids (the
+// waste of a fixed batch of 8 is up to 7 iterations at every stop and reliable update); a batch
+// one short costs one more synchronisation.
+struct BatchPredictor {
+  double nr_prev = -1.0;
+  int kit_prev = 0;
+  double rate = -1.0;
+  int next(const CGState& s, double target, int cap) {
+    const double nr = std::sqrt(s.rr);
+    if (nr_prev > 0.0 && s.kit > kit_prev && nr > 0.0)
+      rate = nr < nr_prev ? std::pow(nr / nr_prev, 1.0 / (double)(s.kit - kit_prev)) : -1.0;
+    nr_prev = nr;
+    kit_prev = s.kit;
+    if (!(rate > 0.0 && rate < 1.0) || !(target > 0.0) || !(nr > target)) return cap;
+    const double n = std::log(target / nr) / std::log(rate);
+    if (!(n < (double)cap)) return cap;
+    const int want = (int)std::ceil(n) + 1;
+    int len = 1;
+    while (len < want && len < cap) len *= 2;
+    return std::min(len, cap);
+  }
+};
 
 }  // namespace
 
@@ -1136,7 +1166,7 @@ int fh_plan_destroy(fh_plan* pl) {
   if (pl->st) (void)hipFree(pl->st);
   if (pl->hist) (void)hipFree(pl->hist);
   if (pl->work) (void)hipFree(pl->work);
-  if (pl->it_exec) (void)hipGraphExecDestroy(pl->it_exec);
+  drop_graphs(pl);
   if (pl->ev0) (void)hipEventDestroy(pl->ev0);
   if (pl->ev1) (void)hipEventDestroy(pl->ev1);
   if (pl->stream) (void)hipStreamDestroy(pl->stream);
@@ -1468,16 +1498,19 @@ int fh_cg_solve(fh_plan* pl, int mode, const void* b_, void* x_, double tol, int
     const int cb_every = pl->it_fn ? pl->it_every : 0;
     int cb_last = 0;
     int kit_host = 0;
+    BatchPredictor pred;
+    int next_len = per_check;
     for (;;) {
       // batches end on the logged / called-back iterations (multiples of every, cb_every), at
-      // most per_check long
-      int batch = per_check;
+      // most next_len long
+      int batch = next_len;
       if (every > 0) batch = std::min(batch, every - kit_host % every);
       if (cb_every > 0) batch = std::min(batch, cb_every - kit_host % cb_every);
       for (int k = 0; k < batch; ++k) RETCHK(enqueue_iteration<double>(pl, x, r, p, Ap, pl->coef64));
       HIPCHK(hipMemcpyAsync(&s, pl->st, sizeof(s), hipMemcpyDeviceToHost, pl->stream));
       HIPCHK(hipStreamSynchronize(pl->stream));
       kit_host = s.kit;
+      next_len = pred.next(s, s.tol, per_check);
       if (s.done && x0_zero) {
         // x <- G x once at the end: the recurrence leaves x outside the range of G by its accumulated
         // rounding, which enters the energy (and the bound ordering) at first order; the projection
@@ -1521,17 +1554,25 @@ int fh_cg_solve(fh_plan* pl, int mode, const void* b_, void* x_, double tol, int
     // launch per batch instead of seven per iteration. The graph is rebuilt when a buffer moves.
     int batch = 0;
     pl->last_graph_batches = 0;
+    BatchPredictor pred;
+    int len = per_check;
+    // after a synchronisation: the next threshold is the stop test or the reliable update
+    // (nr < delta * rmax, rmax the last true residual: after a pause, about the current norm)
+    auto next_len = [&]() {
+      const double rmax = s.pause ? std::sqrt(s.rr) : s.rmax;
+      len = pred.next(s, std::max(s.tol, s.delta * rmax), per_check);
+    };
     for (;; ++batch) {
       const void* key[5] = {xc, r, p, pl->coef32, pl->spec};
       if (fused && batch > 0 && pl->use_graphs && !pl->it_graph_off) {
-        if (!pl->it_exec || std::memcmp(key, pl->it_key, sizeof(key)) != 0) {
-          if (pl->it_exec) (void)hipGraphExecDestroy(pl->it_exec);
-          pl->it_exec = nullptr;
+        if (std::memcmp(key, pl->it_key, sizeof(key)) != 0) drop_graphs(pl);
+        hipGraphExec_t& ex = pl->it_exec[len];
+        if (!ex) {
           hipGraph_t graph = nullptr;
           bool all = true;
           int rc = FH_OK;
           if (hipStreamBeginCapture(pl->stream, hipStreamCaptureModeRelaxed) != hipSuccess) all = false;
-          for (int k = 0; all && k < per_check; ++k) {
+          for (int k = 0; all && k < len; ++k) {
             bool ok = false;
             rc = enqueue_fused_iteration(pl, xc, r, p, pl->coef32, &ok);
             if (rc != FH_OK || !ok) all = false;
@@ -1539,21 +1580,22 @@ int fh_cg_solve(fh_plan* pl, int mode, const void* b_, void* x_, double tol, int
           if (all) enqueue_flush(pl);
           const hipError_t ec = hipStreamEndCapture(pl->stream, &graph);
           if (all && rc == FH_OK && ec == hipSuccess && graph &&
-              hipGraphInstantiate(&pl->it_exec, graph, nullptr, nullptr, 0) == hipSuccess) {
+              hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0) == hipSuccess) {
             std::memcpy(pl->it_key, key, sizeof(key));
           } else {
-            pl->it_exec = nullptr;
+            ex = nullptr;
             pl->it_graph_off = true;  // capture not available here: enqueue the kernels directly
           }
           if (graph) (void)hipGraphDestroy(graph);
           (void)hipGetLastError();
         }
-        if (pl->it_exec) {
-          HIPCHK(hipGraphLaunch(pl->it_exec, pl->stream));
+        if (ex) {
+          HIPCHK(hipGraphLaunch(ex, pl->stream));
           ++pl->last_graph_batches;
           HIPCHK(hipMemcpyAsync(&s, pl->st, sizeof(s), hipMemcpyDeviceToHost, pl->stream));
           HIPCHK(hipStreamSynchronize(pl->stream));
           if (s.done) break;
+          next_len();
           if (s.pause) {
             fold();
             RETCHK(enqueue_true_residual<float>(pl, b, x, Ax, r));
@@ -1562,7 +1604,7 @@ int fh_cg_solve(fh_plan* pl, int mode, const void* b_, void* x_, double tol, int
           continue;
         }
       }
-      for (int k = 0; k < per_check; ++k) {
+      for (int k = 0; k < len; ++k) {
         if (fused) {
           bool ok = false;
           RETCHK(enqueue_fused_iteration(pl, xc, r, p, pl->coef32, &ok));
@@ -1576,6 +1618,7 @@ int fh_cg_solve(fh_plan* pl, int mode, const void* b_, void* x_, double tol, int
       HIPCHK(hipMemcpyAsync(&s, pl->st, sizeof(s), hipMemcpyDeviceToHost, pl->stream));
       HIPCHK(hipStreamSynchronize(pl->stream));
       if (s.done) break;
+      next_len();
       if (s.pause) {
         fold();
         RETCHK(enqueue_true_residual<float>(pl, b, x, Ax, r));

@@ -328,10 +328,10 @@ __device__ __forceinline__ double fixed_sum_partials(const double* __restrict__ 
   return r;
 }
 
-// Finalisation carried by block 0 of the first strided pass of a fused iteration on a small grid
-// (in place of the one-workgroup alpha and beta kernels): the deferred beta of the previous
-// iteration from the inverse pass's partials p2 (when st->bpend), then alpha of this iteration
-// from the forward pass's partials p1. Null st: nothing to do.
+// Finalisation carried by block 0 of the first strided pass of a fused iteration (in place of the
+// one-workgroup alpha kernel, and on a small grid of the beta kernel too): the deferred beta of the
+// previous iteration from the inverse pass's partials p2 (p2 set and st->bpend), then alpha of
+// this iteration from the forward pass's partials p1. Null st: nothing to do.
 struct CgFin {
   CGState* st;
   const double* p1;
@@ -344,7 +344,7 @@ struct CgFin {
 __device__ __forceinline__ void cg_fin_block(const CgFin& f) {
   CGState* st = f.st;
   __shared__ int stop;
-  if (st->bpend) {
+  if (f.p2 && st->bpend) {
     const double rs = fixed_sum_partials(f.p2, f.n2);
     if (threadIdx.x == 0) {
       finalize_beta(st, rs);
