@@ -28,6 +28,8 @@ struct CGState {
   int kit, maxiter, done, pause, halt, status, mixed, hist_len, nupdates;
   int bpend;     // fused iteration, small grids: the beta of the last inverse pass is not finalised yet
   int pend;      // fused iteration: alpha * p of the last iteration not yet added to x
+  int xfresh;    // fused iteration: the single-precision correction holds no value since the last
+                 // (re)start, so its stale contents are read as zero (no clearing pass)
   double* hist;  // device residual log (nullable)
 };
 
@@ -357,6 +359,7 @@ __device__ __forceinline__ void cg_fin_block(const CgFin& f) {
   const double ps = fixed_sum_partials(f.p1, f.n1);
   if (threadIdx.x == 0) {
     finalize_alpha(st, ps);
+    if (st->pend) st->xfresh = 0;  // this iteration's forward pass wrote the correction
     st->pend = st->status == 0 ? 1 : 0;
   }
 }
@@ -672,6 +675,7 @@ static __global__ void k_cg_restart(CGState* st, const double* __restrict__ part
     st->pause = 0;
     st->pend = 0;
     st->bpend = 0;
+    st->xfresh = 1;
     if (st->kit == 0 && hist && st->hist_len > 0) hist[0] = nr;
     st->done = (!(nr > st->tol)) || (st->kit >= st->maxiter);
     st->halt = st->done;
@@ -687,6 +691,7 @@ static __global__ void k_cg_alpha(CGState* st, const double* __restrict__ partia
   const double b = block_sum(acc);
   if (threadIdx.x == 0) {
     finalize_alpha(st, b);
+    if (st->pend) st->xfresh = 0;  // this iteration's forward pass wrote the correction
     st->pend = st->status == 0 ? 1 : 0;
   }
 }
@@ -776,15 +781,15 @@ __global__ void k_cg_pupdate(long long n, T* __restrict__ p, const T* __restrict
   for (long long i = nv * W + gid; i < n; i += stride) p[i] = r[i] + bta * p[i];
 }
 
-// Fused iteration: x64 += x32 + alpha p when a step is pending (st->pend), x32 = 0.
-static __global__ void k_fold_pending(long long n, double* __restrict__ x64, float* __restrict__ x32,
+// Fused iteration: x64 += x32 + alpha p when a step is pending (st->pend); x32 is read as zero
+// when it holds no value (st->xfresh) and left as it is: the restart that follows marks it fresh.
+static __global__ void k_fold_pending(long long n, double* __restrict__ x64, const float* __restrict__ x32,
                                       const float* __restrict__ p, const CGState* __restrict__ st) {
   const float a = st->pend ? (float)st->alpha : 0.0f;
+  const bool fresh = st->xfresh != 0;
   const long long stride = (long long)gridDim.x * blockDim.x;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    x64[i] += (double)(x32[i] + a * p[i]);
-    x32[i] = 0.0f;
-  }
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    x64[i] += (double)((fresh ? 0.0f : x32[i]) + a * p[i]);
 }
 
 // x64 += x32, x32 = 0 (mixed precision: fold the single-precision correction)

@@ -78,7 +78,8 @@ __device__ __forceinline__ int stage_base(int region, unsigned s0) {
 
 // ---- forward: A(x) v, R2C along the contiguous axis, AoS half spectrum -------------------
 // UPD 0: v = in. UPD 1: p = r + beta p (in = p, written back), v = p.
-// UPD 2: as UPD 1 and, first, xc += alpha p_old when st->pend; partial sums of <p, a p>.
+// UPD 2: as UPD 1 and, first, xc += alpha p_old when st->pend (xc = alpha p_old, without reading
+// it, while it holds no value since the last restart: st->xfresh); partial sums of <p, a p>.
 template <typename T, class FP, int COEF, int NC, int UPD, int RPB = FxShape<FP, T>::RPB>
 __global__ void __launch_bounds__(1024)
 k_rows_fwd_st(const T* __restrict__ in, const T* __restrict__ coef, cpx<T>* __restrict__ out,
@@ -114,6 +115,7 @@ k_rows_fwd_st(const T* __restrict__ in, const T* __restrict__ coef, cpx<T>* __re
   constexpr int CPT = (QMAX + NT - 1) / NT;
   T bta = UPD ? T(st->beta) : T(0);
   const bool fold = UPD == 2 && st->pend != 0;
+  const bool fresh = UPD == 2 && st->xfresh != 0;  // xc read as zero
   const T alp = UPD == 2 ? T(st->alpha) : T(0);
   const __amdgpu_buffer_rsrc_t rr = make_rsrc(UPD ? rvec : in, (unsigned long long)NC * n * SR);
   const __amdgpu_buffer_rsrc_t rx = make_rsrc(UPD == 2 ? xc : in, (unsigned long long)NC * n * SR);
@@ -129,7 +131,7 @@ k_rows_fwd_st(const T* __restrict__ in, const T* __restrict__ coef, cpx<T>* __re
         P[c][i] = bload_chunk<T>(rin, q * 16u);
         if constexpr (UPD >= 1) R[c][i] = bload_chunk<T>(rr, q * 16u);
         if constexpr (UPD == 2) {
-          if (fold) X[c][i] = bload_chunk<T>(rx, q * 16u);
+          if (fold && !fresh) X[c][i] = bload_chunk<T>(rx, q * 16u);
         }
       }
     }
@@ -187,7 +189,7 @@ k_rows_fwd_st(const T* __restrict__ in, const T* __restrict__ coef, cpx<T>* __re
 #pragma unroll
         for (int e = 0; e < W; ++e) {
           if constexpr (UPD == 2) {
-            if (fold) X[c][i].e[e] += alp * Pv.e[e];
+            if (fold) X[c][i].e[e] = (fresh ? T(0) : X[c][i].e[e]) + alp * Pv.e[e];
           }
           Pn.e[e] = R[c][i].e[e] + bta * Pv.e[e];
         }
