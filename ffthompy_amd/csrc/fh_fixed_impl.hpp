@@ -396,8 +396,7 @@ int rows_rt_fx_launch(const LaunchCtx& c, int nc, cpx<T>* spec, const T* coef, T
   const cpx<T>* tw = reinterpret_cast<const cpx<T>*>(c.tw[2]);
 #define FH_LAUNCH_RT(NCC)                                                                                     \
   do {                                                                                                        \
-    const size_t slds = (size_t)RowStage<FP, T, NCC>::pen_stride(rpb) * FP::L * sizeof(cpx<T>) +               \
-                        (size_t)rpb * FP::L * sizeof(T); /* pencils, then the block's coefficient rows */      \
+    const size_t slds = (size_t)RowStage<FP, T, NCC>::pen_stride(rpb) * FP::L * sizeof(cpx<T>); /* pencils */ \
     if (slds + FP::L * sizeof(cpx<T>) > 160 * 1024) return 0;                                                 \
     auto kf = k_rows_rt_st<T, FP, NCC>;                                                                       \
     allow_lds(kf, slds);                                                                                      \

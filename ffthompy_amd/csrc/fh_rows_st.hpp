@@ -515,9 +515,8 @@ k_rows_inv_st(const cpx<T>* __restrict__ in, T* __restrict__ out, long long nrow
 // transformed back (times scale), multiplied by the isotropic coefficient and transformed
 // forward as in k_rows_fwd_st; the result overwrites the block's own rows. A pencil holds the
 // same two real rows (row, component) in both directions, so the product needs no exchange
-// between pencils and the real field never leaves LDS. a(x) of the block's rows (one contiguous
-// run, shared by the NC pencils of a row pair) is staged in LDS behind the pencils by coalesced
-// loads issued with the spectrum's.
+// between pencils and the real field never leaves LDS. Each thread loads the a(x) values of its
+// last inverse stage's outputs from HBM before that stage, so the tile holds only the pencils.
 template <typename T, class FP, int NC>
 __global__ void __launch_bounds__(1024)
 k_rows_rt_st(cpx<T>* __restrict__ spec, const T* __restrict__ coef, long long nrows, long long n, int rpb,
@@ -533,12 +532,6 @@ k_rows_rt_st(cpx<T>* __restrict__ spec, const T* __restrict__ coef, long long nr
   const long long row0 = (long long)xcd_remap(blockIdx.x, gridDim.x) * rpb;
   const int nr = (int)min((long long)rpb, nrows - row0);
   for (int e = tid; e < L; e += nthr) twl[e] = twg[e];
-  // the block's coefficient rows behind the pencils (rows past nr: zeros, never used)
-  T* sa = reinterpret_cast<T*>(smem_raw + (size_t)S * L * sizeof(cpx<T>));
-  {
-    const __amdgpu_buffer_rsrc_t rco = make_rsrc(coef + row0 * L, (unsigned long long)nr * L * sizeof(T));
-    for (int e = tid; e < rpb * L; e += nthr) sa[e] = bload_r<T>(rco, (unsigned)e * (unsigned)sizeof(T));
-  }
   // Hermitian completion (rows past nr read 0 through the buffer range)
   const int nrp = rpb >> 1;
   constexpr unsigned CS = sizeof(cpx<T>);
@@ -573,19 +566,47 @@ k_rows_rt_st(cpx<T>* __restrict__ spec, const T* __restrict__ coef, long long nr
   auto at = [&](int j, int) -> cpx<T>& { return s[j * S + pen]; };
   fx_lds_stages<T, FP, +1, 0, FP::NST - 1, 1>(t, at, twl);
   {
-    // last inverse stage, then a(x) times the two real rows, back into the pencil
+    // last inverse stage, then a(x) times the two real rows, back into the pencil. The thread's
+    // a(x) values (its outputs' indices of the rows rowa, rowb) are loaded from HBM before the
+    // stage (the NC pencils of a row pair read the same values, from cache after the first), so
+    // the tile holds only the pencils in LDS
+    using SL = FxStage<T, FP, FP::NST - 1, +1>;
     const int rp_ = pen / NC, m_ = pen - rp_ * NC;
     const int rowa = 2 * rp_ + (2 * m_) / NC, rowb = 2 * rp_ + (2 * m_ + 1) / NC;
     const bool va = rowa < nr, vb = rowb < nr;
-    const T* aa = sa + rowa * L;
-    const T* ab = sa + rowb * L;
-    FxStage<T, FP, FP::NST - 1, +1> sl;
+    const __amdgpu_buffer_rsrc_t rco = make_rsrc(coef + row0 * L, (unsigned long long)nr * L * sizeof(T));
+    T ka[SL::M][SL::R], kb[SL::M][SL::R];
+#pragma unroll
+    for (int m = 0; m < SL::M; ++m) {
+      const int i = t + m * SL::TPP;
+      if (SL::valid(i)) {
+        const int k = i % SL::p, base = (i - k) * SL::R + k;
+#pragma unroll
+        for (int j = 0; j < SL::R; ++j) {
+          const int x = base + j * SL::p;
+          ka[m][j] = bload_r<T>(rco, (unsigned)(rowa * L + x) * (unsigned)sizeof(T));
+          kb[m][j] = bload_r<T>(rco, (unsigned)(rowb * L + x) * (unsigned)sizeof(T));
+        }
+      }
+    }
+    SL sl;
     sl.load(t, [&](int j) { return s[j * S + pen]; });
     __syncthreads();
     sl.compute(t, twl);
-    sl.store(t, [&](int j, cpx<T> v) {
-      s[j * S + pen] = mk(va ? scale * v.x * aa[j] : T(0), vb ? scale * v.y * ab[j] : T(0));
-    });
+    // the outputs in the order of FxStage::store
+#pragma unroll
+    for (int m = 0; m < SL::M; ++m) {
+      const int i = t + m * SL::TPP;
+      if (SL::valid(i)) {
+        const int k = i % SL::p, base = (i - k) * SL::R + k;
+#pragma unroll
+        for (int j = 0; j < SL::R; ++j) {
+          const cpx<T> v = sl.v[m][j];
+          s[(base + j * SL::p) * S + pen] =
+              mk(va ? scale * v.x * ka[m][j] : T(0), vb ? scale * v.y * kb[m][j] : T(0));
+        }
+      }
+    }
   }
   __syncthreads();
   {
