@@ -8,12 +8,14 @@
 //    (16-byte LDS writes); the coefficient rows are staged the same way; the first FFT
 //    stage reads a(x) v(x) from the staging, pairs two real rows into one complex pencil,
 //    and overwrites the staging with pencils (after a barrier); stages run in LDS; the
-//    half spectra are untangled into the AoS spectrum.
+//    half spectra are untangled in registers, written back over the pencils in HBM order
+//    (the block's AoS half spectrum is one contiguous run) and stored in 16-byte chunks.
 //  * FFT threads: lane q = tid % npen owns pencil q (consecutive lanes, consecutive
 //    pencils at the same element: conflict-free LDS access, shared twiddles).
-//  * inverse: AoS spectrum -> Hermitian-completed complex pencils, stages in LDS, the
-//    last stage writes its two real rows into the real-row staging; the epilogue (store,
-//    dot product or residual update) reads the staging and HBM in 16-byte chunks.
+//  * inverse: AoS spectrum (every load of a thread issued first) -> Hermitian-completed
+//    complex pencils, stages in LDS, the last stage writes its two real rows into the
+//    real-row staging; the epilogue (store, dot product, residual update, or the residual
+//    of a right-hand side in single precision) reads the staging and HBM in 16-byte chunks.
 // A staging row run of component c starts at an LDS float index congruent to its HBM
 // element index modulo the chunk width, so chunk k of HBM lands on a 16-byte boundary in
 // LDS; chunks straddling the run's ends are loaded whole (the extra values are never used)
