@@ -109,6 +109,19 @@ k_pot_cols_out(const cpx<T>* __restrict__ data, long long inner, long long total
   const unsigned rowb = (unsigned)inner * CS, qb = (unsigned)q * CS;
   FxStage<T, FP, 0, -1> s0;
   s0.load(t, [=](int j) { return bload_c<T>(rs, (unsigned)j * rowb + qb); });
+  // the output sweep's points (i, f): thread tid takes f = tid % F and i = tid / F + u * STEP;
+  // the dot-product operand w of those points is loaded here, in flight during the transform
+  constexpr int STEP = NT / F, IT = (L + STEP - 1) / STEP;
+  const int f = tid % F;
+  const long long fp = col0 / NC + f;
+  cpx<T> wv[IT];
+  if (w) {
+#pragma unroll
+    for (int u = 0; u < IT; ++u) {
+      const int i = tid / F + u * STEP;
+      if (i < L && f * NC < ncol) wv[u] = w[(long long)i * plane + fp];
+    }
+  }
   for (int e = tid; e < L; e += NT) twl[e] = twg[e];
   __syncthreads();
   s0.compute(t, twl);
@@ -116,11 +129,11 @@ k_pot_cols_out(const cpx<T>* __restrict__ data, long long inner, long long total
   s0.store(t, [&](int j, cpx<T> v) { s[j * P + q] = v; });
   __syncthreads();
   fx_lds_stages<T, FP, -1, 1, FP::NST, 1>(t, at, twl);
-  const int f = tid % F;
-  const long long fp = col0 / NC + f;
   double acc = 0.0;
-  for (int i = tid / F; i < L; i += NT / F) {
-    if (f * NC < ncol) {
+#pragma unroll
+  for (int u = 0; u < IT; ++u) {
+    const int i = tid / F + u * STEP;
+    if (i < L && f * NC < ncol) {
       int k[3];
       col_freq(i, fp, g, k);
       T qv[3] = {T(0), T(0), T(0)}, Pw = T(1);
@@ -139,9 +152,8 @@ k_pot_cols_out(const cpx<T>* __restrict__ data, long long inner, long long total
       const long long idx = (long long)i * plane + fp;
       out[idx] = o;
       if (w) {
-        const cpx<T> wv = w[idx];
         const double wt = (k[2] == 0 || ((g.n2 & 1) == 0 && 2 * k[2] == -g.n2)) ? 1.0 : 2.0;
-        acc += wt * ((double)o.x * (double)wv.x + (double)o.y * (double)wv.y);
+        acc += wt * ((double)o.x * (double)wv[u].x + (double)o.y * (double)wv[u].y);
       }
     }
   }
