@@ -2055,6 +2055,82 @@ int fh_plan_get_schedule(const fh_plan* pl, int dtype, int* proj_variant, int* c
   return FH_OK;
 }
 
+// Average time (ms) of the two row passes of the fused fp32 CG iteration (forward with the direction
+// update and the pending step, inverse with the residual update), nrep launches each after one
+// warm-up, on the plan's current schedule: what the mixed solve runs, where the matvec's row passes
+// move other vectors. The vectors are the mixed solve's own (plan work memory, p and r copied from
+// x and y); the CG state is a benign one (no halt, one pending step) that the next solve replaces.
+// *ms < 0 where the fused kernels do not cover the plan.
+static int time_fused_rows(fh_plan* pl, const float* x, const float* y, int nrep, double* ms) {
+  *ms = -1.0;
+  if (pl->generic || (pl->coef_kind != fh::COEF_ISO && pl->coef_kind != fh::COEF_LAME)) return FH_OK;
+  const long long nv = (long long)pl->ncomp * pl->n;
+  const long long nvp = (nv + 63) / 64 * 64;  // the mixed solve's layout of its work vectors
+  RETCHK(ensure_work(pl, 4 * (size_t)nvp * sizeof(float) + (size_t)nvp * sizeof(double)));
+  float* r = (float*)((double*)pl->work + nvp);
+  float* p = r + nvp;
+  float* xc = p + 2 * nvp;
+  HIPCHK(hipMemcpyAsync(p, x, (size_t)nv * sizeof(float), hipMemcpyDeviceToDevice, pl->stream));
+  HIPCHK(hipMemcpyAsync(r, y, (size_t)nv * sizeof(float), hipMemcpyDeviceToDevice, pl->stream));
+  HIPCHK(hipMemsetAsync(xc, 0, (size_t)nv * sizeof(float), pl->stream));
+  CGState s;
+  std::memset(&s, 0, sizeof(s));
+  s.rr = 1.0;
+  s.alpha = 1e-3;
+  s.beta = 0.5;
+  s.inv_nvox = 1.0 / (double)pl->n;
+  s.maxiter = 1 << 30;
+  s.mixed = 1;
+  s.pend = 1;
+  HIPCHK(hipMemcpyAsync(pl->st, &s, sizeof(s), hipMemcpyHostToDevice, pl->stream));
+  hipEvent_t ev[4] = {};
+  for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+  const int nc = pl->ncomp;
+  cpx<float>* spec = reinterpret_cast<cpx<float>*>(pl->spec);
+  int rc = FH_OK;
+  double acc = 0.0;
+  bool covered = true;
+  for (int it = -1; it < nrep && rc == FH_OK && covered; ++it) {
+    fh::LaunchCtx cf = make_ctx<float>(pl);
+    cf.t0 = ev[0];
+    cf.t1 = ev[1];
+    int nb = 0, nb2 = 0;
+    int k = fh::fixed_rows_fwd<float>(cf, nc, p, pl->coef32, pl->coef_kind, spec, &pl->st->halt, r, pl->st, xc,
+                                      pl->partial, pl->partial_cap, &nb);
+    if (k == 0) {
+      covered = false;
+      break;
+    }
+    rc = fixed_result(k, "rows forward (fused, timed)");
+    if (rc != FH_OK) break;
+    fh::LaunchCtx ci = make_ctx<float>(pl);
+    ci.t0 = ev[2];
+    ci.t1 = ev[3];
+    k = fh::fixed_rows_inv<float>(ci, nc, spec, r, 1.0f, 2, nullptr, pl->partial, pl->partial_cap, &pl->st->halt,
+                                  &nb2, pl->st);
+    if (k == 0) {
+      covered = false;
+      break;
+    }
+    rc = fixed_result(k, "rows inverse (fused, timed)");
+    if (rc != FH_OK) break;
+    if (hipStreamSynchronize(pl->stream) != hipSuccess) {
+      rc = fail(FH_ERR_HIP, "hipStreamSynchronize");
+      break;
+    }
+    float a = 0.f, b = 0.f;
+    if (hipEventElapsedTime(&a, ev[0], ev[1]) != hipSuccess || hipEventElapsedTime(&b, ev[2], ev[3]) != hipSuccess) {
+      rc = fail(FH_ERR_HIP, "hipEventElapsedTime");
+      break;
+    }
+    if (it >= 0) acc += (double)a + (double)b;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  if (rc != FH_OK) return rc;
+  if (covered) *ms = acc / nrep;
+  return FH_OK;
+}
+
 int fh_plan_autotune(fh_plan* pl, int dtype, const void* x, void* y, int nrep, double* proj_ms, double* cols_ms,
                      double* rows_ms, int* proj_variant, int* cols_variant, int* rows_variant) {
   RETCHK(check_plan(pl));
@@ -2099,11 +2175,26 @@ int fh_plan_autotune(fh_plan* pl, int dtype, const void* x, void* y, int nrep, d
       if (tc[v] < tc[bc]) bc = v;
   }
   pl->cols_var[i] = bc;
-  // contiguous-axis passes: forward and inverse together
+  // contiguous-axis passes: forward and inverse together; in single precision, where the mixed
+  // solve runs the fused iteration, its row passes (which also stream r, p and the correction)
   for (int v = 0; v < fh::kRowsVariants; ++v) tr[v] = 1e300;
+  bool fused_rows = dtype == FH_F32 && pl->fused_cg && pl->op_scale == 1.0;
   for (int r = 0; r < kRounds; ++r) {
     for (int v = 0; v < fh::kRowsVariants && rc == FH_OK; ++v) {
       pl->rows_var[i] = v;
+      if (fused_rows) {
+        double fms = -1.0;
+        rc = time_fused_rows(pl, (const float*)x, (const float*)y, nrep, &fms);
+        if (rc == FH_OK && fms >= 0.0) {
+          tr[v] = std::min(tr[v], fms);
+          continue;
+        }
+        // not covered: every variant by the matvec's row passes
+        fused_rows = false;
+        for (int u = 0; u < fh::kRowsVariants; ++u) tr[u] = 1e300;
+        r = -1;
+        break;
+      }
       rc = fh_matvec_profile(pl, dtype, x, y, nrep, 8, kms, kby, &nkr);
       if (rc == FH_OK) tr[v] = std::min(tr[v], kms[0] + kms[nk - 1]);
     }

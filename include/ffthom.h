@@ -349,7 +349,9 @@ int fh_matvec_profile(fh_plan* plan, int dtype, const void* x, void* y, int nrep
  * transforms per pencil); only speed differs. Variant 0 is the default.
  * fh_plan_autotune times every variant on this plan (3 interleaved rounds of nrep matvecs of x
  * into y per variant, each pass timed by its own HIP events, as fh_matvec_profile; a variant's
- * time is its best round), keeps the fastest of each pass and
+ * time is its best round; in single precision, where the mixed solve runs the fused CG iteration,
+ * the row counts are timed on that iteration's two row passes, on the plan's work vectors and a
+ * placeholder CG state that the next solve replaces), keeps the fastest of each pass and
  * returns the times in ms (proj_ms[4], cols_ms[2], rows_ms[3]; any may be NULL) and the choice.
  * FH_ERR_UNSUPPORTED where the plan runs the runtime-radix kernels. No reference counterpart (the
  * reference calls NumPy's FFT). */
