@@ -833,9 +833,10 @@ class Plan:
         return p.value, c.value, r.value
 
     def autotune(self, dtype=np.float32, nrep=5, seed=0):
-        """Time every schedule variant of the projection and strided passes on this plan (random
-        field, 3 interleaved rounds of nrep matvecs per variant, best round) and keep the fastest:
-        fh_plan_autotune. Returns
+        """Time every schedule variant of the projection, strided and row passes on this plan (random
+        field, 3 interleaved rounds of nrep matvecs per variant, best round; in single precision the
+        row counts on the fused CG iteration's row passes) and keep the fastest: fh_plan_autotune.
+        Returns
         {'proj_ms': [...], 'cols_ms': [...], 'rows_ms': [...], 'proj_variant': p, 'cols_variant': c,
         'rows_variant': r}."""
         dt = np.dtype(dtype)
