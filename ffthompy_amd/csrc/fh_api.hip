@@ -866,7 +866,7 @@ static int enqueue_iteration(fh_plan* pl, T* x, T* r, T* p, T* Ap, const T* coef
 // r = b - A x (fp64 operator); rr, beta, done via k_cg_restart
 template <typename TR>
 static int enqueue_true_residual(fh_plan* pl, const double* b, const double* x, double* Ax, TR* r,
-                                 bool x_zero = false) {
+                                 bool x_zero = false, bool newdir = false) {
   const long long nv = (long long)pl->ncomp * pl->n;
   int np = grid_blocks(nv);
   bool fused = false;
@@ -880,7 +880,8 @@ static int enqueue_true_residual(fh_plan* pl, const double* b, const double* x, 
   if (!fused)
     hipLaunchKernelGGL((fh::k_residual<TR>), dim3(np), dim3(256), 0, pl->stream, nv, b, x_zero ? nullptr : Ax, r,
                        pl->partial);
-  hipLaunchKernelGGL(fh::k_cg_restart, dim3(1), dim3(256), 0, pl->stream, pl->st, pl->partial, np, pl->hist);
+  hipLaunchKernelGGL(fh::k_cg_restart, dim3(1), dim3(256), 0, pl->stream, pl->st, pl->partial, np, pl->hist,
+                     newdir);
   HIPCHK(hipGetLastError());
   return FH_OK;
 }
@@ -1543,11 +1544,28 @@ int fh_cg_solve(fh_plan* pl, int mode, const void* b_, void* x_, double tol, int
     pl->cg_defer = fused && defer_applies(pl);
     pl->defer_nb2 = 0;
     const int gb = grid_blocks(nv);
-    auto fold = [&]() {
+    // The single-precision correction c is folded as x += G c, c projected in fp64 (in Ax, free
+    // until the true residual overwrites it): rounding in the fp32 recurrence leaves c outside the
+    // range of G by about 1e-5 of its size. Folded as it is, that part would enter every later
+    // true residual, the iteration would offset it with an error inside the range, and a final
+    // projection of x would leave that error in the solution (its residual about |A| times the
+    // drift); left in x instead, it would enter the energies at first order. The true residual
+    // then differs from the recurrence by more than rounding, so the iteration restarts its search
+    // direction (p = r).
+    auto fold = [&]() -> int {
       if (fused)
-        hipLaunchKernelGGL(fh::k_fold_pending, dim3(gb), dim3(256), 0, pl->stream, nv, x, xc, p, pl->st);
+        hipLaunchKernelGGL(fh::k_fold_pending, dim3(gb), dim3(256), 0, pl->stream, nv, Ax, xc, p, pl->st, true);
       else
-        hipLaunchKernelGGL(fh::k_fold_correction, dim3(gb), dim3(256), 0, pl->stream, nv, x, xc);
+        hipLaunchKernelGGL(fh::k_fold_correction, dim3(gb), dim3(256), 0, pl->stream, nv, Ax, xc, true);
+      HIPCHK(hipGetLastError());
+      RETCHK(run_operator<double>(pl, Ax, Ax, fh::COEF_NONE, nullptr, nullptr, nullptr, nullptr));
+      hipLaunchKernelGGL(fh::k_axpby<double>, dim3(gb), dim3(256), 0, pl->stream, nv, 1.0, (const double*)Ax, 1.0, x);
+      HIPCHK(hipGetLastError());
+      return FH_OK;
+    };
+    auto update = [&]() -> int {
+      RETCHK(fold());
+      return enqueue_true_residual<float>(pl, b, x, Ax, r, false, true);
     };
     // After the first batch (which establishes that the fused kernels cover the plan), a batch of
     // fused iterations is captured once as a HIP graph (kernel nodes only) and replayed: one
@@ -1596,11 +1614,7 @@ int fh_cg_solve(fh_plan* pl, int mode, const void* b_, void* x_, double tol, int
           HIPCHK(hipStreamSynchronize(pl->stream));
           if (s.done) break;
           next_len();
-          if (s.pause) {
-            fold();
-            RETCHK(enqueue_true_residual<float>(pl, b, x, Ax, r));
-            HIPCHK(hipGetLastError());
-          }
+          if (s.pause) RETCHK(update());
           continue;
         }
       }
@@ -1619,17 +1633,11 @@ int fh_cg_solve(fh_plan* pl, int mode, const void* b_, void* x_, double tol, int
       HIPCHK(hipStreamSynchronize(pl->stream));
       if (s.done) break;
       next_len();
-      if (s.pause) {
-        fold();
-        RETCHK(enqueue_true_residual<float>(pl, b, x, Ax, r));
-        HIPCHK(hipGetLastError());
-      }
+      if (s.pause) RETCHK(update());
     }
-    fold();
-    HIPCHK(hipGetLastError());
-    // final fp64 projection x <- G x removes the non-conforming part left by fp32 rounding (in
-    // place: the forward row pass has read all of x before the inverse one writes it)
-    RETCHK(run_operator<double>(pl, x, x, fh::COEF_NONE, nullptr, nullptr, nullptr, nullptr));
+    // the correction since the last true residual (none when the solve stopped at one): x0 keeps
+    // any part outside the range of G, as in the double-precision solve
+    if (!(s.xfresh && !s.pend)) RETCHK(fold());
   }
   HIPCHK(hipEventRecord(pl->ev1, pl->stream));
   HIPCHK(hipMemcpyAsync(&s, pl->st, sizeof(s), hipMemcpyDeviceToHost, pl->stream));
@@ -3153,7 +3161,7 @@ int fh_potential_solve(fh_plan* pl, int mode, const double* E, double tol, int m
     cpx<double>* p = r + nhp;
     cpx<double>* Ap = p + nhp;
     hipLaunchKernelGGL((fh::pot::k_pot_residual<double>), grid, block, 0, pl->stream, b, nullptr, r, g, pl->partial);
-    hipLaunchKernelGGL(fh::k_cg_restart, dim3(1), dim3(256), 0, pl->stream, pl->st, pl->partial, gb, pl->hist);
+    hipLaunchKernelGGL(fh::k_cg_restart, dim3(1), dim3(256), 0, pl->stream, pl->st, pl->partial, gb, pl->hist, false);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(p, r, (size_t)nh * sizeof(cpx<double>), hipMemcpyDeviceToDevice, pl->stream));
     auto batch64 = [&]() -> int {
@@ -3185,7 +3193,7 @@ int fh_potential_solve(fh_plan* pl, int mode, const double* E, double tol, int m
     float* re32 = reinterpret_cast<float*>(re);
     HIPCHK(hipMemsetAsync(xc, 0, (size_t)nh * sizeof(cpx<float>), pl->stream));
     hipLaunchKernelGGL((fh::pot::k_pot_residual<float>), grid, block, 0, pl->stream, b, nullptr, r, g, pl->partial);
-    hipLaunchKernelGGL(fh::k_cg_restart, dim3(1), dim3(256), 0, pl->stream, pl->st, pl->partial, gb, pl->hist);
+    hipLaunchKernelGGL(fh::k_cg_restart, dim3(1), dim3(256), 0, pl->stream, pl->st, pl->partial, gb, pl->hist, false);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(p, r, (size_t)nh * sizeof(cpx<float>), hipMemcpyDeviceToDevice, pl->stream));
     const int fb = grid_blocks(2 * nh);
@@ -3207,14 +3215,14 @@ int fh_potential_solve(fh_plan* pl, int mode, const double* E, double tol, int m
       HIPCHK(hipStreamSynchronize(pl->stream));
       if (s.done) break;
       if (s.pause) {
-        hipLaunchKernelGGL(fh::k_fold_correction, dim3(fb), block, 0, pl->stream, 2 * nh, (double*)x, (float*)xc);
+        hipLaunchKernelGGL(fh::k_fold_correction, dim3(fb), block, 0, pl->stream, 2 * nh, (double*)x, (float*)xc, false);
         RETCHK(pot_operator<double>(pl, x, nullptr, Ax, re, nullptr, nullptr, nullptr));
         hipLaunchKernelGGL((fh::pot::k_pot_residual<float>), grid, block, 0, pl->stream, b, Ax, r, g, pl->partial);
-        hipLaunchKernelGGL(fh::k_cg_restart, dim3(1), dim3(256), 0, pl->stream, pl->st, pl->partial, gb, pl->hist);
+        hipLaunchKernelGGL(fh::k_cg_restart, dim3(1), dim3(256), 0, pl->stream, pl->st, pl->partial, gb, pl->hist, false);
         HIPCHK(hipGetLastError());
       }
     }
-    hipLaunchKernelGGL(fh::k_fold_correction, dim3(fb), block, 0, pl->stream, 2 * nh, (double*)x, (float*)xc);
+    hipLaunchKernelGGL(fh::k_fold_correction, dim3(fb), block, 0, pl->stream, 2 * nh, (double*)x, (float*)xc, false);
     HIPCHK(hipGetLastError());
   }
   // Fu = P x (homogenisation.py:117); e = F^-1 grad Fu

@@ -659,16 +659,17 @@ __global__ void k_residual(long long n, const double* __restrict__ b, const doub
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-// (Re)start: rr = <r,r> of the true residual; beta for continuing the direction.
+// (Re)start: rr = <r,r> of the true residual; beta for continuing the direction, or 0 (p = r) with
+// newdir, when the true residual differs from the recurrence by more than rounding.
 static __global__ void k_cg_restart(CGState* st, const double* __restrict__ partial, int np,
-                             double* __restrict__ hist) {
+                             double* __restrict__ hist, bool newdir) {
   double acc = 0.0;
 #pragma unroll 8  // loads issued together; the additions keep their order
   for (int i = threadIdx.x; i < np; i += blockDim.x) acc += partial[i];
   const double b = block_sum(acc);
   if (threadIdx.x == 0) {
     const double rr = b * st->inv_nvox;
-    st->beta = (st->kit > 0 && st->rr_old > 0.0) ? rr / st->rr_old : 0.0;
+    st->beta = (st->kit > 0 && st->rr_old > 0.0 && !newdir) ? rr / st->rr_old : 0.0;
     st->rr = rr;
     const double nr = sqrt(rr);
     st->rmax = nr;
@@ -783,20 +784,21 @@ __global__ void k_cg_pupdate(long long n, T* __restrict__ p, const T* __restrict
 
 // Fused iteration: x64 += x32 + alpha p when a step is pending (st->pend); x32 is read as zero
 // when it holds no value (st->xfresh) and left as it is: the restart that follows marks it fresh.
+// With set, x64 = x32 + alpha p (the correction alone, for a projection before it is added).
 static __global__ void k_fold_pending(long long n, double* __restrict__ x64, const float* __restrict__ x32,
-                                      const float* __restrict__ p, const CGState* __restrict__ st) {
+                                      const float* __restrict__ p, const CGState* __restrict__ st, bool set) {
   const float a = st->pend ? (float)st->alpha : 0.0f;
   const bool fresh = st->xfresh != 0;
   const long long stride = (long long)gridDim.x * blockDim.x;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-    x64[i] += (double)((fresh ? 0.0f : x32[i]) + a * p[i]);
+    x64[i] = (set ? 0.0 : x64[i]) + (double)((fresh ? 0.0f : x32[i]) + a * p[i]);
 }
 
-// x64 += x32, x32 = 0 (mixed precision: fold the single-precision correction)
-static __global__ void k_fold_correction(long long n, double* __restrict__ x64, float* __restrict__ x32) {
+// x64 += x32 (x64 = x32 with set), x32 = 0 (mixed precision: fold the single-precision correction)
+static __global__ void k_fold_correction(long long n, double* __restrict__ x64, float* __restrict__ x32, bool set) {
   const long long stride = (long long)gridDim.x * blockDim.x;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    x64[i] += (double)x32[i];
+    x64[i] = (set ? 0.0 : x64[i]) + (double)x32[i];
     x32[i] = 0.0f;
   }
 }

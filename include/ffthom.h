@@ -192,10 +192,12 @@ int fh_apply_projection_hat(fh_plan* plan, int dtype, const void* in_hat, void* 
  *   converged x is projected once, x <- G x, removing the component outside the range of G that
  *   the recurrence's rounding accumulates (it would enter the energies at first order and can put
  *   the GaNi lower bound above the upper one by ~1e-14); with another x0 x is returned as iterated.
- * mode FH_SOLVE_MIXED: fp32 CG with fp64 reliable updates (x accumulated in fp64, the
- *   true residual b - A x recomputed in fp64 whenever the fp32 residual fell by `delta`
- *   or claims convergence; the stop test always uses the fp64 true residual) and a final
- *   fp64 projection x <- G x.
+ * mode FH_SOLVE_MIXED: fp32 CG with fp64 reliable updates: whenever the fp32 residual fell by
+ *   `delta` or claims convergence, the fp32 correction c is projected in fp64 and added,
+ *   x <- x + G c, the true residual b - A x is recomputed in fp64 and the search direction
+ *   restarts from it (p = r); the stop test always uses the fp64 true residual, so norm_res is
+ *   the residual of the returned x. The projection removes the part of c outside the range of G
+ *   that fp32 rounding leaves (about 1e-5 of c); x0 keeps its own part, as in mode FH_SOLVE_FP64.
  * kit, norm_res: iterations and final residual norm, as info['kit'], info['norm_res'].
  * res_hist (host, nullable): sqrt(<r,r>) after every iteration (entry 0: initial),
  * hist_len entries at most. cg_status: FH_CG_OK or FH_CG_BREAKDOWN (<p,Ap> <= 0). */

@@ -5,7 +5,9 @@ the CPU by tests/golden/make_oracle_scalar243.py with the oracle's own operators
 
 * The microstructure is the one the fixture was made from (coefficient sum, volume fraction).
 * fp64: CG iteration counts equal the oracle's for every load; the bound matrices agree to 1e-10.
-* mixed (fp32 CG + fp64 reliable updates + fp64 projection): counts equal, bounds within 1e-8.
+* mixed (fp32 CG + fp64 reliable updates, each fp32 correction projected in fp64): the counts exceed
+  the oracle's by at most 5 (the iteration restarts its search direction after a projected update),
+  bounds within 1e-10.
 * lower <= upper holds within the rounding slack 64 eps max|AH| in both modes.
 """
 import json
@@ -35,7 +37,7 @@ def test_scalar243_spheres_matches_fp64_oracle():
     EN = nat.DeviceArray((3,) + N)
     B = nat.DeviceArray((3,) + N)
     sols = [nat.DeviceArray((3,) + N) for _ in range(3)]
-    for mode, tol in (('fp64', 1e-10), ('mixed', 1e-8)):
+    for mode, tol in (('fp64', 1e-10), ('mixed', 1e-10)):
         AH = {}
         for pd, pl in plans.items():
             kits = []
@@ -56,7 +58,10 @@ def test_scalar243_spheres_matches_fp64_oracle():
             ref = np.asarray(FIX['AH_' + pd])
             rel = np.abs(AH[pd] - ref).max() / np.abs(ref).max()
             print('%s %s kits %s (oracle %s) rel diff %.2e' % (mode, pd, kits, FIX['kit_' + pd], rel), flush=True)
-            assert kits == FIX['kit_' + pd], (mode, pd)
+            if mode == 'fp64':
+                assert kits == FIX['kit_' + pd], (mode, pd)
+            else:
+                assert all(0 <= k - r <= 5 for k, r in zip(kits, FIX['kit_' + pd])), (mode, pd)
             assert rel < tol, (mode, pd, rel)
         up, lo = AH['primal'], AH['dual']
         slack = 64 * EPS * np.abs(up).max()
