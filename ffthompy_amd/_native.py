@@ -691,7 +691,7 @@ class Plan:
         call('fh_apply_projection_hat', self._h, dtype_code(real), xin.ptr, xout.ptr)
         return xout
 
-    def cg_solve(self, b, x, tol, maxiter, mode='fp64', delta=0.01, hist_len=0):
+    def cg_solve(self, b, x, tol, maxiter, mode='fp64', delta=1e-4, hist_len=0):
         code = solve_mode(mode)
         self._field(b, 'b', (np.float64,))
         self._field(x, 'x', (np.float64,))
@@ -886,7 +886,7 @@ class Batch:
         flat = np.ascontiguousarray(flat)
         call('fh_batch_set_coeff', self._h, flat.ctypes.data_as(_c_double_p))
 
-    def solve(self, tol=1e-8, maxiter=10000, mode='mixed', delta=0.01):
+    def solve(self, tol=1e-8, maxiter=10000, mode='mixed', delta=1e-4):
         up = np.zeros((self.S, 2, 2))
         lo = np.zeros((self.S, 2, 2))
         kit = np.zeros((self.S, 4), dtype=np.int32)
@@ -900,7 +900,7 @@ class Batch:
     def set_schedule(self, variant, order_mode=0):
         call('fh_batch_set_schedule', self._h, ctypes.c_int(variant), ctypes.c_int(order_mode))
 
-    def autotune(self, mode='mixed', delta=0.01, iters=12):
+    def autotune(self, mode='mixed', delta=1e-4, iters=12):
         """Time every kernel schedule on this batch; keep the fastest. Returns (times_ms[order][variant],
         variant, order_mode)."""
         t = np.zeros(4)

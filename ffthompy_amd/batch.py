@@ -11,7 +11,7 @@ import numpy as np
 from . import _native as nat
 
 
-def homogenize_batch(fields, Y=None, precision='mixed', tol=1e-8, maxiter=10000, delta=0.01, schedule=None):
+def homogenize_batch(fields, Y=None, precision='mixed', tol=1e-8, maxiter=10000, delta=1e-4, schedule=None):
     """Upper and lower bounds of the homogenized conductivity of every sample.
 
     fields: sequence of 2-D arrays a_s (positive, origin-0 voxel values, axis lengths 27, 45,

@@ -70,6 +70,7 @@ enum Phase : int {
   PH_ITER,      // q = G(c p), <p, q>                    (fp32 mixed, fp64 plain)
   PH_TRUE,      // r = b - G(c x)                        (fp64, mixed only)
   PH_PROJ,      // x = G x                               (fp64, mixed only)
+  PH_FOLD,      // d = G d, x += d: the fp32 correction   (fp64, mixed only)
   PH_DONE
 };
 
@@ -170,6 +171,10 @@ __global__ void __launch_bounds__(NT) k_batch_solve(Args A) {
             out.mode = 2;
             out.b = s.b;
             out.r32 = s.r32;
+          } else if (ph == PH_FOLD) {
+            in.coef = 0;
+            in.v = s.q64;
+            out.y = s.q64;
           } else {  // PH_PROJ
             in.coef = 0;
             in.v = x;
@@ -197,7 +202,7 @@ __global__ void __launch_bounds__(NT) k_batch_solve(Args A) {
           cg.rr = wg_sum(a2) * inv_n;
           cg.rmax = sqrt(cg.rr);
           const bool done = !(cg.rmax > P.tol) || P.maxiter <= 0;
-          ph = done ? (P.mixed ? PH_PROJ : PH_DONE) : PH_ITER;
+          ph = done ? PH_DONE : PH_ITER;
         } else if (ph == PH_ITER) {
           cg.first = 0;
           const double pAp = wg_sum(acc) * inv_n;
@@ -236,23 +241,32 @@ __global__ void __launch_bounds__(NT) k_batch_solve(Args A) {
           const bool conv = !(nr > P.tol), maxed = cg.kit >= P.maxiter;
           if (P.mixed) {
             if (conv || nr < P.delta * cg.rmax || maxed) {
-              // reliable update: fold the fp32 correction; b - A x follows in fp64
+              // reliable update: the fp32 correction, projected in fp64 (q64 is free in mixed
+              // precision), is added to x; b - A x follows in fp64. Rounding leaves the correction
+              // outside the range of G by about 1e-5 of its size: added unprojected, that part
+              // would enter the later true residuals and be offset by an error inside the range
               for (int e = tid; e < N; e += NT) {
-                x[e] += (double)s.xc32[e];
+                s.q64[e] = (double)s.xc32[e];
                 s.xc32[e] = 0.0f;
               }
-              ph = PH_TRUE;
+              ph = PH_FOLD;
             }
           } else if (conv || maxed) {
             ph = PH_DONE;
           }
+        } else if (ph == PH_FOLD) {
+          __syncthreads();  // the projected correction of every thread is stored
+          for (int e = tid; e < N; e += NT) x[e] += s.q64[e];
+          ph = PH_TRUE;
         } else if (ph == PH_TRUE) {
+          // the true residual differs from the recurrence by more than rounding (the projection
+          // removed a part of the correction): the search direction restarts from it (p = r)
           const double rt = wg_sum(acc) * inv_n;
-          cg.beta = rt / cg.rr_old;
+          cg.beta = 0.0;
           cg.rr = rt;
           cg.rmax = sqrt(rt);
           cg.nupd += 1;
-          ph = (!(cg.rmax > P.tol) || cg.kit >= P.maxiter) ? PH_PROJ : PH_ITER;
+          ph = (!(cg.rmax > P.tol) || cg.kit >= P.maxiter) ? PH_DONE : PH_ITER;
         } else {  // PH_PROJ done
           ph = PH_DONE;
         }

@@ -24,8 +24,9 @@
 //
 // Mixed precision: the inner CG runs in fp32; when the recursive residual has dropped by
 // `delta` against the last true residual, or the stop test fires, the fp32 correction is
-// folded into the fp64 solution and the true residual b - A x is recomputed with the
-// fp64 operator (reliable update); the solve ends with the fp64 projection x <- G x.
+// projected with the fp64 operator and added to the fp64 solution, and the true residual
+// b - A x is recomputed in fp64 (reliable update); the search direction restarts from it. The
+// solution stays in the range of G; a breakdown ends with the fp64 projection x <- G x.
 #pragma once
 #include "fh_fixed.hpp"
 #include "fh_kernels.hpp"

@@ -194,7 +194,7 @@ def CG(Afun, B, x0=None, par=None, callback=None):
     try:
         with _IterateCallback(Afun, B, callback):
             out = Afun.plan.cg_solve(B.val, x.val, tol=float(par['tol']), maxiter=int(par['maxiter']), mode=mode,
-                                     delta=float(par.get('delta', 0.01)), hist_len=hist_len)
+                                     delta=float(par.get('delta', 1e-4)), hist_len=hist_len)
     finally:
         if every:
             Afun.plan.set_residual_log(0)

@@ -376,8 +376,8 @@ int fh_batch_destroy(fh_batch* batch);
 /* a: host fp64, the samples' voxel values concatenated, each in C order with origin at
  * index 0 (the layout of Material.get_A_GaNi, ffthompy/materials.py:115-123). */
 int fh_batch_set_coeff(fh_batch* batch, const double* a);
-/* mode FH_SOLVE_FP64 or FH_SOLVE_MIXED (fp32 inner CG, fp64 reliable updates, fp64
- * projection and energies). Host outputs (any may be NULL): upper[4s..4s+3], lower[4s..],
+/* mode FH_SOLVE_FP64 or FH_SOLVE_MIXED (fp32 inner CG, fp64 reliable updates as fh_cg_solve:
+ * each fp32 correction projected in fp64 before it is added; fp64 energies). Host outputs (any may be NULL): upper[4s..4s+3], lower[4s..],
  * kit[4s..4s+3] = iterations of (primal e1, primal e2, dual e1, dual e2), status[s] =
  * FH_CG_OK or FH_CG_BREAKDOWN. Stop rule as fh_cg_solve. */
 int fh_batch_solve(fh_batch* batch, int mode, double tol, int maxiter, double delta, double* upper,
