@@ -253,6 +253,7 @@ struct fh_plan {
   size_t partial_cap = 0;
   double* aux = nullptr;         // kAuxDoubles after the partial sums: small results (energies, sums)
   CGState* st = nullptr;
+  CGState* st_add = nullptr;  // alpha = -1: the inverse pass's epilogue 2 adds (x += G c)
   double* hist = nullptr;
   int hist_cap = 0;
   void* work = nullptr;          // CG vectors
@@ -603,6 +604,29 @@ int grid_blocks(long long n, int per = 256, int cap = 2048) {
   if (b < 1) b = 1;
   if (b > cap) b = cap;
   return (int)b;
+}
+
+// x += G c with the fp64 projection (c is overwritten where the fixed-length kernels do not cover
+// the plan). The fixed-length inverse row pass adds G c into x in its epilogue: epilogue 2,
+// out -= alpha y, with the constant state alpha = -1 (its partial sums land in pl->partial, which
+// the caller's next residual overwrites); otherwise G c goes to c and a separate pass adds it.
+static int run_project_add(fh_plan* pl, double* c, double* x) {
+  RETCHK(run_operator_front<double>(pl, c, fh::COEF_NONE, nullptr, nullptr, nullptr));
+  cpx<double>* spec = reinterpret_cast<cpx<double>*>(pl->spec);
+  if (!pl->generic) {
+    fh::LaunchCtx cx = make_ctx<double>(pl);
+    cx.defer_beta = 0;
+    int nb = 0;
+    const int rr = fh::fixed_rows_inv<double>(cx, pl->ncomp, spec, x, 1.0, 2, nullptr, pl->partial, pl->partial_cap,
+                                              nullptr, &nb, pl->st_add);
+    if (rr != 0) return fixed_result(rr, "rows inverse adding the projection (fixed)");
+  }
+  RETCHK(launch_rows_inv<double>(pl, pl->ncomp, spec, c, 1.0, nullptr, pl->partial, nullptr, nullptr));
+  const long long nv = (long long)pl->ncomp * pl->n;
+  hipLaunchKernelGGL(fh::k_axpby<double>, dim3(grid_blocks(nv)), dim3(256), 0, pl->stream, nv, 1.0, (const double*)c,
+                     1.0, x);
+  HIPCHK(hipGetLastError());
+  return FH_OK;
 }
 
 int check_plan(const fh_plan* pl) {
@@ -1258,9 +1282,17 @@ int fh_plan_create(int dim, const int* N, const double* Y, int physics, int prim
   const size_t spec_bytes = (size_t)pl->ncomp * pl->nh * sizeof(cpx<double>);
   if (hipMalloc(&pl->spec, spec_bytes) != hipSuccess ||
       hipMalloc(&pl->partial, (pl->partial_cap + kAuxDoubles) * sizeof(double)) != hipSuccess ||
-      hipMalloc(&pl->st, sizeof(CGState)) != hipSuccess)
+      hipMalloc(&pl->st, 2 * sizeof(CGState)) != hipSuccess)
     return bail(fail(FH_ERR_HIP, "hipMalloc plan buffers failed (out of device memory?)"));
   pl->aux = pl->partial + pl->partial_cap;
+  pl->st_add = pl->st + 1;
+  {
+    CGState u;
+    std::memset(&u, 0, sizeof(u));
+    u.alpha = -1.0;
+    if (hipMemcpy(pl->st_add, &u, sizeof(u), hipMemcpyHostToDevice) != hipSuccess)
+      return bail(fail(FH_ERR_HIP, "plan state initialisation failed"));
+  }
   *out = pl;
   return FH_OK;
 }
@@ -1558,10 +1590,7 @@ int fh_cg_solve(fh_plan* pl, int mode, const void* b_, void* x_, double tol, int
       else
         hipLaunchKernelGGL(fh::k_fold_correction, dim3(gb), dim3(256), 0, pl->stream, nv, Ax, xc, true);
       HIPCHK(hipGetLastError());
-      RETCHK(run_operator<double>(pl, Ax, Ax, fh::COEF_NONE, nullptr, nullptr, nullptr, nullptr));
-      hipLaunchKernelGGL(fh::k_axpby<double>, dim3(gb), dim3(256), 0, pl->stream, nv, 1.0, (const double*)Ax, 1.0, x);
-      HIPCHK(hipGetLastError());
-      return FH_OK;
+      return run_project_add(pl, Ax, x);
     };
     auto update = [&]() -> int {
       RETCHK(fold());
