@@ -35,7 +35,7 @@ def grf_samples(shapes, corr=0.1, std=1.0, seed=BASE_SEED + 11):
 SMALL = [(27, 27), (27, 45), (45, 27), (63, 81), (81, 63), (45, 135), (105, 27), (135, 63)]
 
 
-@pytest.mark.parametrize('precision,tolerance', [('fp64', 1e-10), ('mixed', 1e-8)])
+@pytest.mark.parametrize('precision,tolerance', [('fp64', 1e-10), ('mixed', 1e-10)])
 def test_batch_matches_oracle(precision, tolerance):
     print('case batch vs oracle', precision, SMALL, flush=True)
     fields = grf_samples(SMALL)
@@ -94,7 +94,9 @@ def test_batch_matches_single_plan_solve():
 def test_uq_workload_512():
     """512 samples, grids from {63,81,105,135}^2, exp of a GRF (correlation 0.1, log-std 1):
     mixed precision to tol 1e-8; every sample converged, lower <= upper, and eight samples
-    (the largest grids included) against the oracle within 1e-8."""
+    (the largest grids included) against the oracle within 1e-10 (each reliable update projects the
+    fp32 correction, so the mixed solutions stay in the range of G and their bounds agree with the
+    fp64 ones to rounding; with unprojected corrections they differed by 6.6e-11)."""
     print('case batch uq 512', flush=True)
     dims, fields = uq_samples()
     out = homogenize_batch(fields, precision='mixed', tol=1e-8)
@@ -109,8 +111,8 @@ def test_uq_workload_512():
     pick = sorted(set([big] + list(np.random.default_rng(5).choice(len(dims), 7, replace=False))))
     for s in pick:
         up, lo = oracle_bounds(fields[s])
-        assert rel(out['upper'][s], up) < 1e-8, (s, dims[s], rel(out['upper'][s], up))
-        assert rel(out['lower'][s], lo) < 1e-8, (s, dims[s], rel(out['lower'][s], lo))
+        assert rel(out['upper'][s], up) < 1e-10, (s, dims[s], rel(out['upper'][s], up))
+        assert rel(out['lower'][s], lo) < 1e-10, (s, dims[s], rel(out['lower'][s], lo))
 
 
 def test_schedules_agree_and_autotune_picks_fastest():

@@ -2,13 +2,15 @@
 
 * Configuration 1 (2-D 45^2, square of side 0.6, contrast 10; GaNi solve + Ga post-processing on
   2N-1 = 89) through the drop-in Problem, against tests/golden/oracle_configs.json: GaNi and Ga
-  bounds 1e-10 relative with equal CG iteration counts in fp64, 1e-8 in mixed precision; the Ga
+  bounds 1e-10 relative with equal CG iteration counts in fp64; in mixed precision GaNi 1e-10, Ga
+  1e-8 (first order in the solver tolerance there); the Ga
   pair is a guaranteed, separated bracket (lower < upper).
 * Configuration 3 (3-D elasticity, seeded ellipsoids at 25 %, (E, nu) phases, six Mandel loads):
   at 45^3 against the fixture (primal with the Lame stiffness, dual with the closed-form Lame
   compliance on the device, the oracle with the pointwise Gauss-Jordan inverse of C): fp64 1e-10
-  and equal iteration counts, mixed 1e-8. At the full 81^3: every solve converges (final residual
-  <= tol), Reuss <= AH <= Voigt in the matrix sense, mixed within 1e-8 of fp64, and
+  and equal iteration counts, mixed 1e-10. At the full 81^3: every solve converges (final residual
+  <= tol), Reuss <= AH <= Voigt in the matrix sense, mixed within 1e-12 of fp64 (each reliable update
+  projects the fp32 correction, so both solutions lie in the range of G), and
   lower <= upper within the rounding slack 64 eps |AH| (the GaNi pair is equal analytically).
 """
 import json
@@ -45,9 +47,14 @@ def test_config1_square_45_matches_oracle(prec):
     pb = Problem(_config1(prec))
     pb.calculate()
     fx = FIX['config1']
-    tol = 1e-10 if prec == 'fp64' else 1e-8
+    # GaNi bounds are energies at the stationary point of their own functional: the solver's
+    # tolerance enters them at second order, so mixed and fp64 agree to 1e-10. The Ga bounds
+    # evaluate another functional at the GaNi solution: the field's tolerance-level difference
+    # between the two solves (1e-9) enters them at first order
+    tols = {'AH_GaNi_': 1e-10, 'AH_Ga_': 1e-10 if prec == 'fp64' else 1e-8}
     for pd in ('primal', 'dual'):
-        for key in ('AH_GaNi_' + pd, 'AH_Ga_' + pd):
+        for pre, tol in tols.items():
+            key = pre + pd
             AH, AR = pb.output['mat_' + pd][key], np.asarray(fx[key])
             assert np.abs(AH - AR).max() < tol * np.abs(AR).max(), (key, AH, AR)
         if prec == 'fp64':
@@ -96,7 +103,7 @@ def test_config3_elasticity_45_matches_oracle(mode):
     lam, mu, ind = _config3_lame(N)
     assert abs(ind.mean() - fx['volume_fraction']) < 1e-15
     res = _solve_elastic(N, lam, mu, mode)
-    tol = 1e-10 if mode == 'fp64' else 1e-8
+    tol = 1e-10
     for pd in ('primal', 'dual'):
         AH, kits, norms = res[pd]
         AR = np.asarray(fx['AH_' + pd])
@@ -133,5 +140,5 @@ def test_config3_elasticity_81_full_size():
         assert np.abs(up - lo).max() < 1e-6 * np.abs(up).max()
     for pd in ('primal', 'dual'):
         a, b = rmx[pd][0], r64[pd][0]
-        assert np.abs(a - b).max() < 1e-8 * np.abs(b).max(), pd
+        assert np.abs(a - b).max() < 1e-12 * np.abs(b).max(), pd
     print('81^3 kits fp64', r64['primal'][1], r64['dual'][1], 'mixed', rmx['primal'][1], rmx['dual'][1], flush=True)

@@ -245,12 +245,12 @@ MIXED_CASES = [('scalar', 'iso', (45, 45, 45), 100.), ('scalar', 'iso', (63, 81)
 
 @pytest.mark.parametrize('physics,kind,N,contrast', MIXED_CASES, ids=['%s-%s-%s' % c[:3] for c in MIXED_CASES])
 def test_mixed_precision_solve_matches_fp64(physics, kind, N, contrast):
-    """fp32 CG with fp64 reliable updates and a final fp64 projection, against the plain
-    fp64 solve. The fp32 corrections carry a non-conforming part of relative size ~1e-7;
-    the final projection removes it, which perturbs the field at first order (about
-    1e-7 times the contrast) but the energy only at second order, so the field is checked
-    to 1e-4 and the energy (AH entry) to 1e-8. The fp64 solve also matches the oracle's
-    CG iteration count."""
+    """fp32 CG with fp64 reliable updates, against the plain fp64 solve. Each fp32 correction is
+    projected in fp64 before it is added, so both solutions lie in the range of G and differ by
+    what the tolerance leaves: the field is checked to 1e-6 and the energy (AH entry) to 1e-12
+    (with the corrections folded unprojected and x projected once at the end the field differed
+    at first order in the fp32 drift, 1e-5 here). The fp64 solve also matches the oracle's CG
+    iteration count."""
     print('case mixed', physics, kind, N, flush=True)
     rng = np.random.default_rng(30)
     Y = np.ones(len(N))
@@ -285,10 +285,10 @@ def test_mixed_precision_solve_matches_fp64(physics, kind, N, contrast):
         nat.add_constant(x, E)
         sols[mode] = x
     x64, xmx = sols['fp64'].to_host(), sols['mixed'].to_host()
-    assert rel(xmx, x64) < 1e-4, rel(xmx, x64)
+    assert rel(xmx, x64) < 1e-6, rel(xmx, x64)
     e64 = np.einsum('i...,i...', np.einsum('ij...,j...->i...', A, x64), x64).mean()
     emx = np.einsum('i...,i...', np.einsum('ij...,j...->i...', A, xmx), xmx).mean()
-    assert abs(emx - e64) / abs(e64) < 1e-8, (emx, e64)
+    assert abs(emx - e64) / abs(e64) < 1e-12, (emx, e64)
     assert infos['mixed']['norm_res'] <= 1e-8
     if int(np.prod(N)) <= 30000:
         Bh, xref, info = _oracle_solve(N, Y, physics, 'primal', A, E, 1e-8)
