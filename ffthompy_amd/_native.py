@@ -891,7 +891,7 @@ class Batch:
         lo = np.zeros((self.S, 2, 2))
         kit = np.zeros((self.S, 4), dtype=np.int32)
         st = np.zeros(self.S, dtype=np.int32)
-        call('fh_batch_solve', self._h, ctypes.c_int(FH_SOLVE_FP64 if mode == 'fp64' else FH_SOLVE_MIXED),
+        call('fh_batch_solve', self._h, ctypes.c_int(solve_mode(mode)),
              ctypes.c_double(tol), ctypes.c_int(int(maxiter)), ctypes.c_double(delta),
              up.ctypes.data_as(_c_double_p), lo.ctypes.data_as(_c_double_p), kit.ctypes.data_as(_c_int_p),
              st.ctypes.data_as(_c_int_p))
@@ -906,7 +906,7 @@ class Batch:
         t = np.zeros(4)
         v = ctypes.c_int()
         o = ctypes.c_int()
-        call('fh_batch_autotune', self._h, ctypes.c_int(FH_SOLVE_FP64 if mode == 'fp64' else FH_SOLVE_MIXED),
+        call('fh_batch_autotune', self._h, ctypes.c_int(solve_mode(mode)),
              ctypes.c_double(delta), ctypes.c_int(iters), t.ctypes.data_as(_c_double_p), ctypes.byref(v),
              ctypes.byref(o))
         return t.reshape(2, 2), v.value, o.value
