@@ -22,6 +22,7 @@ def homogenize_batch(fields, Y=None, precision='mixed', tol=1e-8, maxiter=10000,
     'time' device seconds of the batch; 'schedule' (variant, task order) of the solve kernel and
     'tune_ms' the autotuner's timings (schedule=None: tuned on this batch; or a fixed pair).
     """
+    nat.solve_mode(precision)  # 'fp64' or 'mixed'; anything else raises before any device call
     dims = [np.shape(a) for a in fields]
     if any(len(d) != 2 for d in dims):
         raise ValueError('every sample must be a 2-D array')

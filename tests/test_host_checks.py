@@ -101,6 +101,18 @@ def test_precision_strings():
             nat.solve_mode(bad)
 
 
+def test_batch_rejects_bad_precision_before_device(monkeypatch):
+    from ffthompy_amd import batch
+
+    def no_device(*a, **k):
+        raise AssertionError('the batch was created before the precision was checked')
+
+    monkeypatch.setattr(nat, 'Batch', no_device)
+    for bad in ('FP64', 'double', 'fp32'):
+        with pytest.raises(nat.FhError):
+            batch.homogenize_batch([np.ones((27, 27))], precision=bad)
+
+
 def test_cg_rejects_bad_precision_and_x0_before_device(monkeypatch):
     from ffthompy_amd.tensors import Tensor
 
